@@ -1,0 +1,141 @@
+/*
+ * w2w_raster.h -- C ABI of the MI355X (gfx950) Gaussian tile rasterizer.
+ *
+ * Drop-in boundary for the reference's pybind module `_C`
+ * (cudaForwardBackwardRenderer/ext.cpp:5-7):
+ *   rasterize_gaussians           rasterize_points.h:8-20,  rasterize_points.cu:31-100
+ *   rasterize_gaussians_backward  rasterize_points.h:24-48, rasterize_points.cu:101-189
+ * The Python shim word2world_amd.raster keeps those positional signatures and
+ * calls the functions below through ctypes.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Every tensor pointer is device memory
+ *    (hipMalloc / torch CUDA tensor), float32 unless stated, C-contiguous.
+ *  - The caller allocates every buffer (outputs and the three workspace
+ *    buffers sized by w2w_workspace_sizes); no entry point allocates, frees
+ *    or synchronises the device, so all of them are legal under stream
+ *    capture.  All work goes on `stream` (a hipStream_t; NULL = default).
+ *  - Every entry point returns a W2W_* status and never throws.
+ *  - The number of rendered instances (K) stays on the device: it is the
+ *    uint32 at byte offset 0 of the geometry buffer; the uint32 at byte 4 is
+ *    nonzero when K exceeded the binning capacity (the outputs are then
+ *    incomplete and the caller re-runs with a larger capacity).
+ */
+#ifndef W2W_RASTER_H
+#define W2W_RASTER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define W2W_ABI_VERSION 1
+
+enum {
+  W2W_OK = 0,
+  W2W_ERR_INVALID = 1,   /* bad size, null pointer or unsupported tile size */
+  W2W_ERR_HIP = 2,       /* a launch failed */
+  W2W_ERR_CAPACITY = 3,  /* workspace smaller than w2w_workspace_sizes asks */
+};
+
+/* One view of one scene: the inputs of rasterize_gaussians
+ * (rasterize_points.cu:33-45) plus the image and tile size the reference
+ * hard-coded (800x800, 16; PD-1). */
+typedef struct W2WFrame {
+  int32_t P;            /* Gaussians */
+  int32_t M;            /* SH coefficients per Gaussian (row stride of sh) */
+  int32_t D;            /* active SH degree, 0..3, (D+1)^2 <= M */
+  int32_t W, H;         /* image size in pixels */
+  int32_t tile;         /* tile edge: 8, 16 or 32 */
+  float focal_x, focal_y;
+  const float* means;       /* [P,3] */
+  const float* scales;      /* [P,3] activated (exp) */
+  const float* rotations;   /* [P,4] (w,x,y,z), normalised */
+  const float* opacities;   /* [P] activated (sigmoid) */
+  const float* sh;          /* [P,M,3] */
+  const float* viewmatrix;  /* [16] column-major W2C (Renderer.py:135) */
+  const float* projmatrix;  /* [16] column-major P*W2C (Renderer.py:136) */
+  const float* campos;      /* [3] */
+  const float* background;  /* [3] */
+} W2WFrame;
+
+/* Caller-owned workspace: geometry (per Gaussian), binning (per instance,
+ * `capacity` instances) and image (per pixel / tile) state, kept between the
+ * forward and the backward of one view (the reference's geomBuffer,
+ * binningBuffer and imgBuffer, rasterize_points.cu:61-66). */
+typedef struct W2WWorkspace {
+  void* geom;
+  size_t geom_bytes;
+  void* binning;
+  size_t binning_bytes;
+  void* image;
+  size_t image_bytes;
+  int64_t capacity;
+} W2WWorkspace;
+
+/* Byte offsets of the arrays inside the workspace buffers (tests and
+ * debugging read them; the layout is a pure function of the sizes). */
+typedef struct W2WLayout {
+  size_t geom_bytes, binning_bytes, image_bytes;
+  size_t splats;     /* geom: 48-byte records (px,py,a,b | c,o,depth,0 | r,g,b,0) */
+  size_t radii;      /* geom: int32[P] */
+  size_t rect;       /* geom: uint32x2[P] (x0|y0<<16, x1|y1<<16) */
+  size_t counts;     /* geom: uint32[P] tiles touched */
+  size_t perm;       /* geom: uint32[P] Gaussian ids in (depth, id) order */
+  size_t offsets;    /* geom: uint32[P] exclusive scan of counts in depth order */
+  size_t gacc;       /* geom: float[P][12] backward accumulators */
+  size_t sorted_ids; /* binning: uint32[capacity] Gaussian id per sorted instance */
+  size_t sorted_tiles; /* binning: uint32[capacity] tile id per sorted instance */
+  size_t ranges;     /* image: uint32x2[T] per-tile [start,end) */
+  size_t n_contrib;  /* image: uint32[H*W] 1-based last contributor */
+  size_t final_T;    /* image: float[H*W] final transmittance */
+  int32_t tiles_x, tiles_y;
+} W2WLayout;
+
+int w2w_version(void);
+
+int w2w_workspace_layout(int32_t P, int32_t W, int32_t H, int32_t tile, int64_t capacity,
+                         W2WLayout* out);
+
+/* rasterize_gaussians: preprocess, depth sort, scan, emit, tile sort, ranges,
+ * blend.  Outputs: color [3,H,W], depth [H,W] (sum z_ndc alpha T), alpha
+ * [H,W] (sum alpha T), radii [P] int32. */
+int w2w_forward(const W2WFrame* f, const W2WWorkspace* ws, float* out_color, float* out_depth,
+                float* out_alpha, int32_t* out_radii, void* stream);
+
+/* rasterize_gaussians_backward: gradients of the forward above w.r.t. the
+ * same inputs, in the reference's tuple order (rasterize_points.cu:188):
+ * dL_dmeans2D [P,3] (NDC units, z = 0), dL_dcolors [P,3], dL_dopacity [P],
+ * dL_dmeans3D [P,3], dL_dcov3D [P,6], dL_dsh [P,M,3], dL_dscales [P,3],
+ * dL_drotations [P,4].  `ws` must hold this view's forward state. */
+int w2w_backward(const W2WFrame* f, const W2WWorkspace* ws, const float* dL_dcolor,
+                 const float* dL_ddepth, const float* dL_dalpha, float* dL_dmeans2D,
+                 float* dL_dcolors, float* dL_dopacity, float* dL_dmeans3D, float* dL_dcov3D,
+                 float* dL_dsh, float* dL_dscales, float* dL_drotations, void* stream);
+
+/* 64-bit keys (tile << 32 | depth bits) of the sorted instances, i.e. the
+ * reference's sorted keysArray (prepareSort.cu:431-434), for parity checks. */
+int w2w_sorted_keys(const W2WFrame* f, const W2WWorkspace* ws, uint64_t* out_keys, void* stream);
+
+/* Stable radix sort of (u64 key, u32 value) pairs over bits [0, end_bit):
+ * the semantics of radix_sort_2bit_coalesced (prepareSort.cu:303-360).
+ * `temp` must hold w2w_sort_temp_bytes(n, end_bit) bytes.  Output buffers
+ * may not alias the inputs. */
+size_t w2w_sort_temp_bytes(int64_t n, int32_t end_bit);
+int w2w_sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
+                       uint32_t* vals_out, int64_t n, int32_t end_bit, void* temp,
+                       size_t temp_bytes, void* stream);
+
+/* Exclusive scan of uint32 values (the reference's kogge-stone scan,
+ * gaussianRasterizer.cu:77-117, is inclusive: inclusive[i] = out[i] + in[i]).
+ * Writes the total to *total (device).  temp: w2w_scan_temp_bytes(n). */
+size_t w2w_scan_temp_bytes(int64_t n);
+int w2w_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n,
+                           void* temp, size_t temp_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* W2W_RASTER_H */

@@ -1,0 +1,48 @@
+"""The C-ABI library loads without a GPU and exports every function that
+include/w2w_raster.h declares; layout queries are pure host logic."""
+import os
+import re
+
+import pytest
+
+from word2world_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "w2w_raster.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(w2w_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    names = _declared()
+    assert "w2w_forward" in names and "w2w_backward" in names
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(_lib.EXPORTS) <= set(names)
+    assert L.w2w_version() == 1
+
+
+def test_layout_is_pure_host_logic():
+    L1 = _lib.layout(4096, 256, 256, 16, 100000)
+    assert L1.tiles_x == 16 and L1.tiles_y == 16
+    assert L1.splats % 256 == 0 and L1.sorted_ids % 256 == 0
+    L2 = _lib.layout(4096, 256, 256, 16, 200000)
+    assert L2.binning_bytes > L1.binning_bytes and L2.geom_bytes == L1.geom_bytes
+    assert _lib.layout(0, 64, 64, 8, 0).geom_bytes > 0
+    # 16-bit tile coordinates and 30-bit look-back words bound the sizes
+    with pytest.raises(RuntimeError):
+        _lib.layout(10, 64, 64, 12, 100)  # unsupported tile size
+    with pytest.raises(RuntimeError):
+        _lib.layout(10, 64, 64, 16, 1 << 30)
+
+
+def test_sort_and_scan_temp_sizes():
+    L = _lib.load()
+    assert L.w2w_sort_temp_bytes(0, 64) > 0
+    assert L.w2w_sort_temp_bytes(1 << 20, 64) >= 12 * (1 << 20)
+    assert L.w2w_sort_temp_bytes(100, 0) == 0
+    assert L.w2w_scan_temp_bytes(1 << 20) >= 4 * 256

@@ -1,0 +1,92 @@
+"""ctypes binding of ``libw2w_raster.so`` (C ABI: include/w2w_raster.h).
+
+The library is the only compute path: if it is missing or fails to load, every
+entry point raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libw2w_raster.so")
+
+W2W_OK, W2W_ERR_INVALID, W2W_ERR_HIP, W2W_ERR_CAPACITY = 0, 1, 2, 3
+_STATUS_TEXT = {W2W_ERR_INVALID: "invalid argument", W2W_ERR_HIP: "HIP launch failure",
+                W2W_ERR_CAPACITY: "workspace too small"}
+
+_vp = ctypes.c_void_p
+
+
+class W2WFrame(ctypes.Structure):
+    _fields_ = [("P", ctypes.c_int32), ("M", ctypes.c_int32), ("D", ctypes.c_int32),
+                ("W", ctypes.c_int32), ("H", ctypes.c_int32), ("tile", ctypes.c_int32),
+                ("focal_x", ctypes.c_float), ("focal_y", ctypes.c_float),
+                ("means", _vp), ("scales", _vp), ("rotations", _vp), ("opacities", _vp),
+                ("sh", _vp), ("viewmatrix", _vp), ("projmatrix", _vp), ("campos", _vp),
+                ("background", _vp)]
+
+
+class W2WWorkspace(ctypes.Structure):
+    _fields_ = [("geom", _vp), ("geom_bytes", ctypes.c_size_t), ("binning", _vp),
+                ("binning_bytes", ctypes.c_size_t), ("image", _vp), ("image_bytes", ctypes.c_size_t),
+                ("capacity", ctypes.c_int64)]
+
+
+class W2WLayout(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_size_t) for n in (
+        "geom_bytes", "binning_bytes", "image_bytes", "splats", "radii", "rect", "counts", "perm",
+        "offsets", "gacc", "sorted_ids", "sorted_tiles", "ranges", "n_contrib", "final_T")] + [
+        ("tiles_x", ctypes.c_int32), ("tiles_y", ctypes.c_int32)]
+
+
+EXPORTS = ["w2w_version", "w2w_workspace_layout", "w2w_forward", "w2w_backward", "w2w_sorted_keys",
+           "w2w_sort_temp_bytes", "w2w_sort_pairs_u64", "w2w_scan_temp_bytes",
+           "w2w_exclusive_scan_u32"]
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the library (does not touch the GPU)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with word2world_amd.build.build() "
+                           "(the rasterizer has no fallback path)")
+    L = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    L.w2w_version.restype = ctypes.c_int
+    L.w2w_version.argtypes = []
+    L.w2w_workspace_layout.restype = ctypes.c_int
+    L.w2w_workspace_layout.argtypes = [ctypes.c_int32] * 4 + [ctypes.c_int64, P(W2WLayout)]
+    L.w2w_forward.restype = ctypes.c_int
+    L.w2w_forward.argtypes = [P(W2WFrame), P(W2WWorkspace), _vp, _vp, _vp, _vp, _vp]
+    L.w2w_backward.restype = ctypes.c_int
+    L.w2w_backward.argtypes = [P(W2WFrame), P(W2WWorkspace)] + [_vp] * 12
+    L.w2w_sorted_keys.restype = ctypes.c_int
+    L.w2w_sorted_keys.argtypes = [P(W2WFrame), P(W2WWorkspace), _vp, _vp]
+    L.w2w_sort_temp_bytes.restype = ctypes.c_size_t
+    L.w2w_sort_temp_bytes.argtypes = [ctypes.c_int64, ctypes.c_int32]
+    L.w2w_sort_pairs_u64.restype = ctypes.c_int
+    L.w2w_sort_pairs_u64.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp,
+                                     ctypes.c_size_t, _vp]
+    L.w2w_scan_temp_bytes.restype = ctypes.c_size_t
+    L.w2w_scan_temp_bytes.argtypes = [ctypes.c_int64]
+    L.w2w_exclusive_scan_u32.restype = ctypes.c_int
+    L.w2w_exclusive_scan_u32.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp, ctypes.c_size_t, _vp]
+    _lib = L
+    return L
+
+
+def check(status: int, what: str) -> None:
+    if status != W2W_OK:
+        raise RuntimeError(f"{what} failed: {_STATUS_TEXT.get(status, status)} (status {status})")
+
+
+def layout(P: int, W: int, H: int, tile: int, capacity: int) -> W2WLayout:
+    out = W2WLayout()
+    check(load().w2w_workspace_layout(P, W, H, tile, capacity, ctypes.byref(out)),
+          "w2w_workspace_layout")
+    return out

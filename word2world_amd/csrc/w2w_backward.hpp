@@ -1,0 +1,380 @@
+// Backward rasterizer kernels for gfx950.
+//
+//   k_blend_bwd       per-pixel reverse traversal (the role of backward.cu:521-665):
+//                     starts from the stored final T and last-contributor index,
+//                     recomputes alpha with the forward's exact arithmetic and
+//                     T <- T / (1 - alpha); no per-(pixel, Gaussian) state.  The
+//                     64 lanes of a wave hold 64 pixels of one tile and visit the
+//                     same instance together, so each gradient field is reduced
+//                     across the wave with DPP and issued as one atomic per wave.
+//   k_preprocess_bwd  per-Gaussian chain rule (the role of backward.cu:27-227 and
+//                     :229-519), differentiating the forward exactly as defined
+//                     in k_preprocess (radial clamp, degree-aware SH with the 0.3
+//                     coefficient, NDC depth, no dilation).
+#pragma once
+#include "w2w_device.hpp"
+#include "w2w_forward.hpp"
+
+namespace w2w {
+
+// gradient accumulator record per Gaussian (floats)
+enum GaccField : int {
+  GA_MX = 0, GA_MY = 1,            // dL/dmean2D in NDC units
+  GA_CA = 2, GA_CB = 3, GA_CC = 4, // dL/dconic (a, b, c of the power form)
+  GA_OP = 5,                       // dL/dopacity
+  GA_R = 6, GA_G = 7, GA_B = 8,    // dL/drgb
+  GA_DEPTH = 9,                    // dL/d(NDC depth)
+  GA_STRIDE = 12,
+};
+
+// Sum over the 64 lanes of a wave, result valid in lane 63: quad_perm
+// butterflies, row rotations, then the GFX9 row broadcasts.  All lanes must
+// be active.
+W2W_DEV float wave_reduce_dpp(float v) {
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x124, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x128, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x142, 0xA, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x143, 0xC, 0xF, false));
+  return v;
+}
+
+W2W_DEV float lane63(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+
+struct BlendBwdArgs {
+  const uint2* ranges;
+  const uint32_t* ids;
+  const Splat* splats;
+  const float* bg;
+  int W, H, tiles_x, T;
+  uint32_t P;
+  const uint32_t* n_contrib;
+  const float* final_T;
+  const float* dL_dcolor;   // [3,H,W]
+  const float* dL_dalpha;   // [H,W]
+  const float* dL_ddepth;   // [H,W]
+  float* gacc;              // [P][GA_STRIDE], pre-zeroed
+};
+
+template <int TS>
+__global__ __launch_bounds__(TS* TS) void k_blend_bwd(BlendBwdArgs a) {
+  constexpr int B = TS * TS;
+  __shared__ Splat s_sp[B];
+  __shared__ uint32_t s_id[B];
+  __shared__ uint32_t s_max;
+  const int tile = xcd_tile(blockIdx.x, a.T);
+  if (tile >= a.T) return;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+  const int px = tx * TS + (tid % TS), py = ty * TS + (tid / TS);
+  const bool inside = px < a.W && py < a.H;
+  const float pxf = (float)px, pyf = (float)py;
+  const uint2 range = a.ranges[tile];
+  const size_t pid = (size_t)py * a.W + px;
+  const size_t HW = (size_t)a.W * a.H;
+
+  const int last = inside ? (int)a.n_contrib[pid] : 0;
+  const float Tf = inside ? a.final_T[pid] : 0.0f;
+  float gC0 = 0.0f, gC1 = 0.0f, gC2 = 0.0f, gA = 0.0f, gD = 0.0f;
+  if (inside) {
+    gC0 = a.dL_dcolor[pid];
+    gC1 = a.dL_dcolor[HW + pid];
+    gC2 = a.dL_dcolor[2 * HW + pid];
+    gA = a.dL_dalpha[pid];
+    gD = a.dL_ddepth[pid];
+  }
+  const float bgdot = (a.bg[0] * gC0 + a.bg[1] * gC1) + a.bg[2] * gC2;
+  const float hw = 0.5f * (float)a.W, hh = 0.5f * (float)a.H;
+  const float a_min = 0x1.010102p-8f;
+
+  if (tid == 0) s_max = 0;
+  __syncthreads();
+  // wave-level max of last, then block max
+  int wmax = last;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
+  if (lane == 0) atomicMax(&s_max, (uint32_t)wmax);
+  __syncthreads();
+  const int nproc = (int)s_max;
+
+  float T = Tf;
+  float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, accD = 0.0f, accA = 0.0f;
+  float la = 0.0f, lc0 = 0.0f, lc1 = 0.0f, lc2 = 0.0f, ld = 0.0f;
+
+  for (int bend = nproc; bend > 0; bend -= B) {
+    const int bstart = bend > B ? bend - B : 0;
+    __syncthreads();
+    const int k = bstart + tid;
+    if (k < bend) {
+      uint32_t id = a.ids[range.x + k];
+      id = id < a.P ? id : 0u;
+      s_id[tid] = id;
+      s_sp[tid] = a.splats[id];
+    }
+    __syncthreads();
+    const int jtop = min(bend, wmax) - 1 - bstart;  // wave-uniform
+    for (int j = jtop; j >= 0; j--) {
+      const int kk = bstart + j;
+      float g_mx = 0.0f, g_my = 0.0f, g_ca = 0.0f, g_cb = 0.0f, g_cc = 0.0f, g_op = 0.0f;
+      float g_r = 0.0f, g_g = 0.0f, g_b = 0.0f, g_d = 0.0f;
+      bool contrib = false;
+      if (kk < last) {
+        const float4 sa = s_sp[j].a;
+        const float4 sb = s_sp[j].b;
+        const float dx = sa.x - pxf, dy = sa.y - pyf;
+        const float pw = power_of(sa.z, sa.w, sb.x, dx, dy);
+        if (pw <= 0.0f) {
+          const float G = exp_neg(pw);
+          const float o = sb.y;
+          const float oG = o * G;
+          const float al = fminf(0.99f, oG);
+          if (al >= a_min) {
+            contrib = true;
+            T = T / (1.0f - al);
+            const float aT = al * T;
+            const float4 sc = s_sp[j].c;
+            acc0 = la * lc0 + (1.0f - la) * acc0;
+            acc1 = la * lc1 + (1.0f - la) * acc1;
+            acc2 = la * lc2 + (1.0f - la) * acc2;
+            accD = la * ld + (1.0f - la) * accD;
+            accA = la + (1.0f - la) * accA;
+            float dLda = ((sc.x - acc0) * gC0 + (sc.y - acc1) * gC1) + (sc.z - acc2) * gC2;
+            dLda = dLda + (sb.z - accD) * gD;
+            dLda = dLda + (1.0f - accA) * gA;
+            dLda = dLda * T;
+            dLda = dLda - (Tf / (1.0f - al)) * bgdot;
+            lc0 = sc.x; lc1 = sc.y; lc2 = sc.z; ld = sb.z; la = al;
+            g_r = aT * gC0;
+            g_g = aT * gC1;
+            g_b = aT * gC2;
+            g_d = aT * gD;
+            if (oG <= 0.99f) {
+              g_op = G * dLda;
+              const float dLdp = (o * dLda) * G;
+              g_ca = (-0.5f * dx * dx) * dLdp;
+              g_cb = (-dx * dy) * dLdp;
+              g_cc = (-0.5f * dy * dy) * dLdp;
+              g_mx = (-(sa.z * dx + sa.w * dy) * dLdp) * hw;
+              g_my = (-(sa.w * dx + sb.x * dy) * dLdp) * hh;
+            }
+          }
+        }
+      }
+      if (__ballot(contrib) != 0ull) {
+        g_mx = wave_reduce_dpp(g_mx);
+        g_my = wave_reduce_dpp(g_my);
+        g_ca = wave_reduce_dpp(g_ca);
+        g_cb = wave_reduce_dpp(g_cb);
+        g_cc = wave_reduce_dpp(g_cc);
+        g_op = wave_reduce_dpp(g_op);
+        g_r = wave_reduce_dpp(g_r);
+        g_g = wave_reduce_dpp(g_g);
+        g_b = wave_reduce_dpp(g_b);
+        g_d = wave_reduce_dpp(g_d);
+        if (lane == 63) {
+          float* dst = a.gacc + (size_t)s_id[j] * GA_STRIDE;
+          atomicAdd(dst + GA_MX, g_mx);
+          atomicAdd(dst + GA_MY, g_my);
+          atomicAdd(dst + GA_CA, g_ca);
+          atomicAdd(dst + GA_CB, g_cb);
+          atomicAdd(dst + GA_CC, g_cc);
+          atomicAdd(dst + GA_OP, g_op);
+          atomicAdd(dst + GA_R, g_r);
+          atomicAdd(dst + GA_G, g_g);
+          atomicAdd(dst + GA_B, g_b);
+          atomicAdd(dst + GA_DEPTH, g_d);
+        }
+      }
+    }
+  }
+}
+
+struct PreBwdArgs {
+  int P, M, D, W, H;
+  float fx, fy;
+  const float* means;
+  const float* scales;
+  const float* rots;
+  const float* sh;
+  const float* view;
+  const float* proj;
+  const float* campos;
+  const int32_t* radii;
+  const float* gacc;
+  float* dL_dmeans2D;   // [P,3]
+  float* dL_dcolors;    // [P,3]
+  float* dL_dopacity;   // [P]
+  float* dL_dmeans3D;   // [P,3]
+  float* dL_dcov3D;     // [P,6]
+  float* dL_dsh;        // [P,M,3]
+  float* dL_dscales;    // [P,3]
+  float* dL_drots;      // [P,4]
+};
+
+__global__ __launch_bounds__(256) void k_preprocess_bwd(PreBwdArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.P) return;
+  const float* ga = a.gacc + (size_t)i * GA_STRIDE;
+  float gm[3] = {0.0f, 0.0f, 0.0f};
+  float dcov[6] = {0, 0, 0, 0, 0, 0};
+  float dsc[3] = {0, 0, 0};
+  float dq[4] = {0, 0, 0, 0};
+  float* dsh = a.dL_dsh + (size_t)i * a.M * 3;
+  const bool vis = a.radii[i] > 0;
+  a.dL_dmeans2D[3 * i] = vis ? ga[GA_MX] : 0.0f;
+  a.dL_dmeans2D[3 * i + 1] = vis ? ga[GA_MY] : 0.0f;
+  a.dL_dmeans2D[3 * i + 2] = 0.0f;
+  a.dL_dcolors[3 * i] = vis ? ga[GA_R] : 0.0f;
+  a.dL_dcolors[3 * i + 1] = vis ? ga[GA_G] : 0.0f;
+  a.dL_dcolors[3 * i + 2] = vis ? ga[GA_B] : 0.0f;
+  a.dL_dopacity[i] = vis ? ga[GA_OP] : 0.0f;
+  for (int k = 0; k < a.M * 3; k++) dsh[k] = 0.0f;
+  if (vis) {
+    Cam c;
+    load_cam(c, a.view, a.proj, a.campos, a.W, a.H, a.fx, a.fy);
+    const float p[3] = {a.means[3 * i], a.means[3 * i + 1], a.means[3 * i + 2]};
+    const float s[3] = {a.scales[3 * i], a.scales[3 * i + 1], a.scales[3 * i + 2]};
+    const float q[4] = {a.rots[4 * i], a.rots[4 * i + 1], a.rots[4 * i + 2], a.rots[4 * i + 3]};
+    Geom g;
+    geometry(c, p, s, q, g);
+    if (g.visible) {
+      const float* V = c.V;
+      const float* F = c.F;
+      // colour: SH coefficients and view direction
+      {
+        float dx = p[0] - c.cp[0], dy = p[1] - c.cp[1], dz = p[2] - c.cp[2];
+        float len = __builtin_sqrtf((dx * dx + dy * dy) + dz * dz);
+        float x = dx / len, y = dy / len, z = dz / len;
+        float Y[16], gx[16], gy[16], gz[16];
+        sh_basis(a.D, x, y, z, Y);
+        sh_basis_grad(a.D, x, y, z, gx, gy, gz);
+        const int nb = (a.D + 1) * (a.D + 1);
+        const float* cf = a.sh + (size_t)i * a.M * 3;
+        const float gr0 = ga[GA_R], gr1 = ga[GA_G], gr2 = ga[GA_B];
+        float gdx = 0.0f, gdy = 0.0f, gdz = 0.0f;
+        for (int k = 0; k < nb; k++) {
+          dsh[3 * k] = Y[k] * gr0;
+          dsh[3 * k + 1] = Y[k] * gr1;
+          dsh[3 * k + 2] = Y[k] * gr2;
+          float dot = (cf[3 * k] * gr0 + cf[3 * k + 1] * gr1) + cf[3 * k + 2] * gr2;
+          gdx = __builtin_fmaf(gx[k], dot, gdx);
+          gdy = __builtin_fmaf(gy[k], dot, gdy);
+          gdz = __builtin_fmaf(gz[k], dot, gdz);
+        }
+        float pd = (gdx * x + gdy * y) + gdz * z;
+        gm[0] += (gdx - pd * x) / len;
+        gm[1] += (gdy - pd * y) / len;
+        gm[2] += (gdz - pd * z) / len;
+      }
+      // NDC mean and depth through the full projection
+      {
+        float gnx = ga[GA_MX], gny = ga[GA_MY], gnz = ga[GA_DEPTH];
+        float gcx = gnx * g.invW, gcy = gny * g.invW, gcz = gnz * g.invW;
+        float ginvW = (gnx * g.cx + gny * g.cy) + gnz * g.cz;
+        float gcw = -ginvW * g.invW * g.invW;
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+          gm[j] += ((F[4 * j + 0] * gcx + F[4 * j + 1] * gcy) + F[4 * j + 2] * gcz) + F[4 * j + 3] * gcw;
+      }
+      // conic -> cov2D
+      const float gca = ga[GA_CA], gcb = ga[GA_CB], gcc = ga[GA_CC];
+      const float ca = g.ca, cb = g.cb, cc = g.cc, h = 0.5f * gcb;
+      const float q00 = ca * gca + cb * h, q01 = ca * h + cb * gcc;
+      const float q10 = cb * gca + cc * h, q11 = cb * h + cc * gcc;
+      const float g_cxx = -(q00 * ca + q01 * cb);
+      const float g_cxy = -2.0f * (q00 * cb + q01 * cc);
+      const float g_cyy = -(q10 * cb + q11 * cc);
+      // cov2D -> Sigma and J
+      const float* S6 = g.Sig;
+      float v[3], w[3];
+      v[0] = (S6[0] * g.jx[0] + S6[1] * g.jx[1]) + S6[2] * g.jx[2];
+      v[1] = (S6[1] * g.jx[0] + S6[3] * g.jx[1]) + S6[4] * g.jx[2];
+      v[2] = (S6[2] * g.jx[0] + S6[4] * g.jx[1]) + S6[5] * g.jx[2];
+      w[0] = (S6[0] * g.jy[0] + S6[1] * g.jy[1]) + S6[2] * g.jy[2];
+      w[1] = (S6[1] * g.jy[0] + S6[3] * g.jy[1]) + S6[4] * g.jy[2];
+      w[2] = (S6[2] * g.jy[0] + S6[4] * g.jy[1]) + S6[5] * g.jy[2];
+      const float hxy = 0.5f * g_cxy;
+      float GS[9];
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int l = 0; l < 3; l++)
+          GS[3 * k + l] = (g_cxx * g.jx[k] * g.jx[l] + hxy * (g.jx[k] * g.jy[l] + g.jy[k] * g.jx[l])) +
+                          g_cyy * g.jy[k] * g.jy[l];
+      dcov[0] = GS[0]; dcov[1] = 2.0f * GS[1]; dcov[2] = 2.0f * GS[2];
+      dcov[3] = GS[4]; dcov[4] = 2.0f * GS[5]; dcov[5] = GS[8];
+      float gjx[3], gjy[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        gjx[k] = 2.0f * g_cxx * v[k] + g_cxy * w[k];
+        gjy[k] = g_cxy * v[k] + 2.0f * g_cyy * w[k];
+      }
+      // J -> view-space point
+      const float zi = 1.0f / g.vz, zi2 = zi * zi;
+      float g_zi = 0.0f, g_zi2 = 0.0f, g_txc = 0.0f, g_tyc = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const float right = V[4 * k + 0], up = V[4 * k + 1], fwd = V[4 * k + 2];
+        g_zi += gjx[k] * c.fx * right + gjy[k] * c.fy * up;
+        g_zi2 -= gjx[k] * c.fx * g.txc * fwd + gjy[k] * c.fy * g.tyc * fwd;
+        g_txc -= gjx[k] * c.fx * fwd * zi2;
+        g_tyc -= gjy[k] * c.fy * fwd * zi2;
+      }
+      float g_vz = g_zi * (-zi * zi) + g_zi2 * (-2.0f * zi2 * zi);
+      const float g_rxc = g_txc * g.vz, g_ryc = g_tyc * g.vz;
+      g_vz += g_txc * g.rxc + g_tyc * g.ryc;
+      float g_rx, g_ry;
+      if (g.clamped) {
+        const float s3 = g.s * g.s * g.s;
+        const float lx2 = c.limx * c.limx, ly2 = c.limy * c.limy;
+        g_rx = g_rxc * (g.s - s3 * g.rx * g.rx / lx2) + g_ryc * (-s3 * g.rx * g.ry / lx2);
+        g_ry = g_rxc * (-s3 * g.rx * g.ry / ly2) + g_ryc * (g.s - s3 * g.ry * g.ry / ly2);
+      } else {
+        g_rx = g_rxc;
+        g_ry = g_ryc;
+      }
+      const float g_vx = g_rx / g.vz, g_vy = g_ry / g.vz;
+      g_vz -= (g_rx * g.rx + g_ry * g.ry) / g.vz;
+      const float g_tx = g_vx / g.tw, g_ty = g_vy / g.tw, g_tz = g_vz / g.tw;
+      const float g_tw = -((g_vx * g.vx + g_vy * g.vy) + g_vz * g.vz) / g.tw;
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+        gm[j] += ((V[4 * j + 0] * g_tx + V[4 * j + 1] * g_ty) + V[4 * j + 2] * g_tz) + V[4 * j + 3] * g_tw;
+      // Sigma -> scales and quaternion
+      const float* Mm = g.M;
+      const float* R = g.R;
+      float gM[9], gR[9];
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int col = 0; col < 3; col++)
+          gM[3 * r + col] = 2.0f * ((GS[3 * r + 0] * Mm[col] + GS[3 * r + 1] * Mm[3 + col]) + GS[3 * r + 2] * Mm[6 + col]);
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int col = 0; col < 3; col++) gR[3 * r + col] = gM[3 * r + col] * s[col];
+#pragma unroll
+      for (int col = 0; col < 3; col++)
+        dsc[col] = (gM[col] * R[col] + gM[3 + col] * R[3 + col]) + gM[6 + col] * R[6 + col];
+      const float qw = q[0], qx = q[1], qy = q[2], qz = q[3];
+      dq[0] = 2.0f * (((-qz * gR[1] + qy * gR[2]) + (qz * gR[3] - qx * gR[5])) + (-qy * gR[6] + qx * gR[7]));
+      dq[1] = 2.0f * ((((qy * gR[1] + qz * gR[2]) + (qy * gR[3] - 2.0f * qx * gR[4])) + (-qw * gR[5] + qz * gR[6])) + (qw * gR[7] - 2.0f * qx * gR[8]));
+      dq[2] = 2.0f * ((((-2.0f * qy * gR[0] + qx * gR[1]) + (qw * gR[2] + qx * gR[3])) + (qz * gR[5] - qw * gR[6])) + (qz * gR[7] - 2.0f * qy * gR[8]));
+      dq[3] = 2.0f * ((((-2.0f * qz * gR[0] - qw * gR[1]) + (qx * gR[2] + qw * gR[3])) + (-2.0f * qz * gR[4] + qy * gR[5])) + (qx * gR[6] + qy * gR[7]));
+    }
+  }
+  a.dL_dmeans3D[3 * i] = gm[0];
+  a.dL_dmeans3D[3 * i + 1] = gm[1];
+  a.dL_dmeans3D[3 * i + 2] = gm[2];
+#pragma unroll
+  for (int k = 0; k < 6; k++) a.dL_dcov3D[6 * i + k] = dcov[k];
+#pragma unroll
+  for (int k = 0; k < 3; k++) a.dL_dscales[3 * i + k] = dsc[k];
+#pragma unroll
+  for (int k = 0; k < 4; k++) a.dL_drots[4 * i + k] = dq[k];
+}
+
+}  // namespace w2w

@@ -1,0 +1,399 @@
+// C ABI of the rasterizer (declarations and contract: include/w2w_raster.h).
+//
+// Host side only sequences launches on the caller's stream: it carves the
+// caller-owned workspace, never allocates, frees or synchronises, and never
+// reads K back (kernels that depend on K read it from the device status
+// word), so a whole forward or backward can be captured into a HIP graph.
+#include <hip/hip_runtime.h>
+
+#include "../../include/w2w_raster.h"
+#include "w2w_backward.hpp"
+#include "w2w_device.hpp"
+#include "w2w_forward.hpp"
+#include "w2w_sort.hpp"
+
+using namespace w2w;
+
+namespace {
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+struct Plan {
+  W2WLayout L;
+  int32_t P, W, H, ts;
+  int64_t cap;
+  uint32_t T, dtiles, stiles, ctiles;
+  int tpasses, tlast_bits;
+  size_t status, dkey0, dkey1, id1, dhist, dstatus, slb;  // geom
+  size_t tkey0, tkey1, tid0, tid1, thist, tstatus;        // binning
+};
+
+int tile_bits(uint32_t T) {
+  int b = 0;
+  while ((1u << b) < T) b++;
+  return b < 1 ? 1 : b;
+}
+
+bool make_plan(int32_t P, int32_t W, int32_t H, int32_t ts, int64_t cap, Plan& p) {
+  if (P < 0 || W <= 0 || H <= 0 || cap < 0 || cap >= (int64_t)LB_VAL) return false;
+  if (ts != 8 && ts != 16 && ts != 32) return false;
+  if ((int64_t)P >= (int64_t)LB_VAL) return false;
+  p.P = P; p.W = W; p.H = H; p.ts = ts; p.cap = cap;
+  W2WLayout& L = p.L;
+  L.tiles_x = (W + ts - 1) / ts;
+  L.tiles_y = (H + ts - 1) / ts;
+  if (L.tiles_x > 65535 || L.tiles_y > 65535) return false;
+  p.T = (uint32_t)L.tiles_x * (uint32_t)L.tiles_y;
+  p.dtiles = cdiv((uint64_t)P, SORT_TILE);
+  p.stiles = cdiv((uint64_t)P, SCAN_TILE);
+  p.ctiles = cdiv((uint64_t)cap, SORT_TILE);
+  int tb = tile_bits(p.T);
+  p.tpasses = (tb + 7) / 8;
+  p.tlast_bits = tb - 8 * (p.tpasses - 1);
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t at = o; o = align_up(o + bytes); return at; };
+  const size_t Pz = (size_t)P;
+  p.status = take(ST_WORDS * 4);
+  L.splats = take(sizeof(Splat) * Pz);
+  L.radii = take(4 * Pz);
+  L.rect = take(8 * Pz);
+  L.counts = take(4 * Pz);
+  p.dkey0 = take(4 * Pz);
+  p.dkey1 = take(4 * Pz);
+  L.perm = take(4 * Pz);
+  p.id1 = take(4 * Pz);
+  L.offsets = take(4 * Pz);
+  p.dhist = take(4 * 4 * RADIX);
+  p.dstatus = take((size_t)4 * p.dtiles * RADIX * 4);
+  p.slb = take((size_t)p.stiles * 4 + 4);
+  L.gacc = take(4 * GA_STRIDE * Pz);
+  L.geom_bytes = o;
+  o = 0;
+  const size_t C = (size_t)cap;
+  p.tkey0 = take(4 * C);
+  p.tkey1 = take(4 * C);
+  p.tid0 = take(4 * C);
+  p.tid1 = take(4 * C);
+  p.thist = take(4 * 4 * RADIX);
+  p.tstatus = take((size_t)p.tpasses * p.ctiles * RADIX * 4);
+  L.binning_bytes = o;
+  L.sorted_tiles = (p.tpasses & 1) ? p.tkey1 : p.tkey0;
+  L.sorted_ids = (p.tpasses & 1) ? p.tid1 : p.tid0;
+  o = 0;
+  L.ranges = take(8 * (size_t)p.T);
+  L.n_contrib = take(4 * (size_t)W * H);
+  L.final_T = take(4 * (size_t)W * H);
+  L.image_bytes = o;
+  return true;
+}
+
+template <typename T>
+T* at(void* base, size_t off) { return reinterpret_cast<T*>(static_cast<char*>(base) + off); }
+
+bool ws_ok(const W2WWorkspace* ws, const Plan& p) {
+  return ws && ws->geom && ws->binning && ws->image && ws->geom_bytes >= p.L.geom_bytes &&
+         ws->binning_bytes >= p.L.binning_bytes && ws->image_bytes >= p.L.image_bytes &&
+         ws->capacity >= p.cap;
+}
+
+int launch_status() {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? W2W_OK : W2W_ERR_HIP;
+}
+
+void fill_u32(uint32_t* ptr, uint32_t v, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, ptr, v, n);
+}
+
+bool frame_ok(const W2WFrame* f) {
+  if (!f) return false;
+  if (f->D < 0 || f->D > 3 || f->M < (f->D + 1) * (f->D + 1)) return false;
+  if (!(f->focal_x > 0.0f) || !(f->focal_y > 0.0f)) return false;
+  if (!f->viewmatrix || !f->projmatrix || !f->campos || !f->background) return false;
+  if (f->P > 0 && (!f->means || !f->scales || !f->rotations || !f->opacities || !f->sh)) return false;
+  return true;
+}
+
+template <int TS>
+void launch_blend(const BlendArgs& a, hipStream_t s) {
+  const uint32_t grid = 8 * cdiv(a.T, 8);
+  hipLaunchKernelGGL(k_blend<TS>, dim3(grid), dim3(TS * TS), 0, s, a);
+}
+
+template <int TS>
+void launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s) {
+  const uint32_t grid = 8 * cdiv(a.T, 8);
+  hipLaunchKernelGGL(k_blend_bwd<TS>, dim3(grid), dim3(TS * TS), 0, s, a);
+}
+
+template <typename KT>
+void sort_passes(const KT* kin, const uint32_t* vin, KT* kA, uint32_t* vA, KT* kB, uint32_t* vB,
+                 const uint32_t* d_n, uint32_t n_cap, int npasses, int last_bits, bool write_last_keys,
+                 uint32_t* hist, uint32_t* status, uint32_t* tickets, hipStream_t s) {
+  const uint32_t ntiles = cdiv(n_cap, SORT_TILE);
+  if (ntiles == 0) return;
+  const uint32_t hgrid = std::min<uint32_t>(cdiv(n_cap, 256), 2048);
+  hipLaunchKernelGGL(k_sort_hist<KT>, dim3(hgrid), dim3(256), 0, s, kin, d_n, n_cap, 0, npasses,
+                     last_bits, hist, status, ntiles, tickets);
+  const uint32_t pgrid = std::min<uint32_t>(ntiles, 2048);
+  const KT* ks = kin;
+  const uint32_t* vs = vin;
+  for (int p = 0; p < npasses; p++) {
+    const bool toA = ((npasses - 1 - p) % 2) == 0;
+    KT* kd = toA ? kA : kB;
+    uint32_t* vd = toA ? vA : vB;
+    const int nbits = (p == npasses - 1) ? last_bits : 8;
+    uint32_t* st = status + (size_t)p * ntiles * RADIX;
+    if (p < npasses - 1 || write_last_keys)
+      hipLaunchKernelGGL((k_sort_pass<KT, true>), dim3(pgrid), dim3(SORT_THREADS), 0, s, ks, vs, kd,
+                         vd, d_n, n_cap, 8 * p, nbits, hist + p * RADIX, st, tickets + p);
+    else
+      hipLaunchKernelGGL((k_sort_pass<KT, false>), dim3(pgrid), dim3(SORT_THREADS), 0, s, ks, vs,
+                         kd, vd, d_n, n_cap, 8 * p, nbits, hist + p * RADIX, st, tickets + p);
+    ks = kd;
+    vs = vd;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int w2w_version(void) { return W2W_ABI_VERSION; }
+
+int w2w_workspace_layout(int32_t P, int32_t W, int32_t H, int32_t tile, int64_t capacity,
+                         W2WLayout* out) {
+  Plan p;
+  if (!out || !make_plan(P, W, H, tile, capacity, p)) return W2W_ERR_INVALID;
+  *out = p.L;
+  return W2W_OK;
+}
+
+int w2w_forward(const W2WFrame* f, const W2WWorkspace* ws, float* out_color, float* out_depth,
+                float* out_alpha, int32_t* out_radii, void* stream) {
+  if (!frame_ok(f) || !out_color || !out_depth || !out_alpha || (f->P > 0 && !out_radii))
+    return W2W_ERR_INVALID;
+  Plan p;
+  if (!ws || !make_plan(f->P, f->W, f->H, f->tile, ws->capacity, p)) return W2W_ERR_INVALID;
+  if (!ws_ok(ws, p)) return W2W_ERR_CAPACITY;
+  hipStream_t s = (hipStream_t)stream;
+  hipGetLastError();
+  void* G = ws->geom;
+  void* B = ws->binning;
+  void* I = ws->image;
+  uint32_t* status = at<uint32_t>(G, p.status);
+  const uint32_t P = (uint32_t)f->P;
+  const uint32_t cap = (uint32_t)p.cap;
+
+  fill_u32(status, 0u, ST_WORDS, s);
+  fill_u32(at<uint32_t>(I, p.L.ranges), 0u, 2 * (size_t)p.T, s);
+  if (P > 0) {
+    fill_u32(at<uint32_t>(G, p.dhist), 0u, 4 * RADIX, s);
+    fill_u32(at<uint32_t>(G, p.slb), 0u, p.stiles + 1, s);
+    fill_u32(at<uint32_t>(B, p.thist), 0u, 4 * RADIX, s);
+
+    PreArgs pa;
+    pa.P = f->P; pa.M = f->M; pa.D = f->D; pa.W = f->W; pa.H = f->H; pa.ts = f->tile;
+    pa.tiles_x = p.L.tiles_x; pa.tiles_y = p.L.tiles_y;
+    pa.fx = f->focal_x; pa.fy = f->focal_y;
+    pa.means = f->means; pa.scales = f->scales; pa.rots = f->rotations; pa.opac = f->opacities;
+    pa.sh = f->sh; pa.view = f->viewmatrix; pa.proj = f->projmatrix; pa.campos = f->campos;
+    pa.splats = at<Splat>(G, p.L.splats);
+    pa.radii = at<int32_t>(G, p.L.radii);
+    pa.rect = at<uint2>(G, p.L.rect);
+    pa.counts = at<uint32_t>(G, p.L.counts);
+    pa.dkey = at<uint32_t>(G, p.dkey0);
+    pa.ids = at<uint32_t>(G, p.L.perm);
+    hipLaunchKernelGGL(k_preprocess, dim3(cdiv(P, 256)), dim3(256), 0, s, pa);
+
+    // depth sort: (depth bits, id) over P, 4 passes, result in dkey0 / perm
+    sort_passes<uint32_t>(at<uint32_t>(G, p.dkey0), at<uint32_t>(G, p.L.perm),
+                          at<uint32_t>(G, p.dkey0), at<uint32_t>(G, p.L.perm),
+                          at<uint32_t>(G, p.dkey1), at<uint32_t>(G, p.id1), nullptr, P, 4, 8,
+                          false, at<uint32_t>(G, p.dhist), at<uint32_t>(G, p.dstatus),
+                          status + ST_DTICKET, s);
+
+    hipLaunchKernelGGL(k_scan<true>, dim3(p.stiles), dim3(SCAN_THREADS), 0, s,
+                       at<uint32_t>(G, p.L.counts), at<uint32_t>(G, p.L.perm),
+                       at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.slb) + 1,
+                       at<uint32_t>(G, p.slb), status + ST_K, status + ST_OVERFLOW, P, cap);
+    if (cap > 0) {
+      hipLaunchKernelGGL(k_emit, dim3(cdiv(P, 256)), dim3(256), 0, s, at<uint32_t>(G, p.L.perm),
+                         at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.L.counts),
+                         at<uint2>(G, p.L.rect), P, (uint32_t)p.L.tiles_x, cap,
+                         at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0));
+      sort_passes<uint32_t>(at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0),
+                            at<uint32_t>(B, p.L.sorted_tiles), at<uint32_t>(B, p.L.sorted_ids),
+                            at<uint32_t>(B, (p.tpasses & 1) ? p.tkey0 : p.tkey1),
+                            at<uint32_t>(B, (p.tpasses & 1) ? p.tid0 : p.tid1), status + ST_K, cap,
+                            p.tpasses, p.tlast_bits, true, at<uint32_t>(B, p.thist),
+                            at<uint32_t>(B, p.tstatus), status + ST_TTICKET, s);
+      const uint32_t rgrid = std::min<uint32_t>(cdiv(cap, 256), 4096);
+      hipLaunchKernelGGL(k_ranges, dim3(rgrid), dim3(256), 0, s, at<uint32_t>(B, p.L.sorted_tiles),
+                         status, cap, p.T, at<uint2>(I, p.L.ranges));
+    }
+    hipMemcpyAsync(out_radii, at<int32_t>(G, p.L.radii), 4 * (size_t)P, hipMemcpyDeviceToDevice, s);
+  }
+  BlendArgs ba;
+  ba.ranges = at<uint2>(I, p.L.ranges);
+  ba.ids = at<uint32_t>(B, p.L.sorted_ids);
+  ba.splats = at<Splat>(G, p.L.splats);
+  ba.bg = f->background;
+  ba.W = f->W; ba.H = f->H; ba.tiles_x = p.L.tiles_x; ba.T = (int)p.T;
+  ba.P = (uint32_t)f->P;
+  ba.color = out_color; ba.alpha = out_alpha; ba.depth = out_depth;
+  ba.n_contrib = at<uint32_t>(I, p.L.n_contrib);
+  ba.final_T = at<float>(I, p.L.final_T);
+  switch (f->tile) {
+    case 8: launch_blend<8>(ba, s); break;
+    case 16: launch_blend<16>(ba, s); break;
+    default: launch_blend<32>(ba, s); break;
+  }
+  return launch_status();
+}
+
+int w2w_backward(const W2WFrame* f, const W2WWorkspace* ws, const float* dL_dcolor,
+                 const float* dL_ddepth, const float* dL_dalpha, float* dL_dmeans2D,
+                 float* dL_dcolors, float* dL_dopacity, float* dL_dmeans3D, float* dL_dcov3D,
+                 float* dL_dsh, float* dL_dscales, float* dL_drotations, void* stream) {
+  if (!frame_ok(f) || !dL_dcolor || !dL_ddepth || !dL_dalpha) return W2W_ERR_INVALID;
+  if (f->P > 0 && (!dL_dmeans2D || !dL_dcolors || !dL_dopacity || !dL_dmeans3D || !dL_dcov3D ||
+                   !dL_dsh || !dL_dscales || !dL_drotations))
+    return W2W_ERR_INVALID;
+  Plan p;
+  if (!ws || !make_plan(f->P, f->W, f->H, f->tile, ws->capacity, p)) return W2W_ERR_INVALID;
+  if (!ws_ok(ws, p)) return W2W_ERR_CAPACITY;
+  if (f->P == 0) return W2W_OK;
+  hipStream_t s = (hipStream_t)stream;
+  hipGetLastError();
+  void* G = ws->geom;
+  void* B = ws->binning;
+  void* I = ws->image;
+  const uint32_t P = (uint32_t)f->P;
+  float* gacc = at<float>(G, p.L.gacc);
+  fill_u32(reinterpret_cast<uint32_t*>(gacc), 0u, (size_t)GA_STRIDE * P, s);
+  BlendBwdArgs ba;
+  ba.ranges = at<uint2>(I, p.L.ranges);
+  ba.ids = at<uint32_t>(B, p.L.sorted_ids);
+  ba.splats = at<Splat>(G, p.L.splats);
+  ba.bg = f->background;
+  ba.W = f->W; ba.H = f->H; ba.tiles_x = p.L.tiles_x; ba.T = (int)p.T;
+  ba.P = (uint32_t)f->P;
+  ba.n_contrib = at<uint32_t>(I, p.L.n_contrib);
+  ba.final_T = at<float>(I, p.L.final_T);
+  ba.dL_dcolor = dL_dcolor; ba.dL_dalpha = dL_dalpha; ba.dL_ddepth = dL_ddepth;
+  ba.gacc = gacc;
+  switch (f->tile) {
+    case 8: launch_blend_bwd<8>(ba, s); break;
+    case 16: launch_blend_bwd<16>(ba, s); break;
+    default: launch_blend_bwd<32>(ba, s); break;
+  }
+  PreBwdArgs pa;
+  pa.P = f->P; pa.M = f->M; pa.D = f->D; pa.W = f->W; pa.H = f->H;
+  pa.fx = f->focal_x; pa.fy = f->focal_y;
+  pa.means = f->means; pa.scales = f->scales; pa.rots = f->rotations; pa.sh = f->sh;
+  pa.view = f->viewmatrix; pa.proj = f->projmatrix; pa.campos = f->campos;
+  pa.radii = at<int32_t>(G, p.L.radii);
+  pa.gacc = gacc;
+  pa.dL_dmeans2D = dL_dmeans2D; pa.dL_dcolors = dL_dcolors; pa.dL_dopacity = dL_dopacity;
+  pa.dL_dmeans3D = dL_dmeans3D; pa.dL_dcov3D = dL_dcov3D; pa.dL_dsh = dL_dsh;
+  pa.dL_dscales = dL_dscales; pa.dL_drots = dL_drotations;
+  hipLaunchKernelGGL(k_preprocess_bwd, dim3(cdiv(P, 256)), dim3(256), 0, s, pa);
+  return launch_status();
+}
+
+__global__ void k_sorted_keys(const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ ids,
+                              const Splat* __restrict__ splats, const uint32_t* __restrict__ status,
+                              uint32_t cap, uint32_t P, uint64_t* __restrict__ out) {
+  uint32_t n = status[ST_K];
+  n = n < cap ? n : cap;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    uint32_t id = ids[i];
+    float z = id < P ? splats[id].b.z : 0.0f;
+    out[i] = ((uint64_t)tiles[i] << 32) | (uint64_t)__float_as_uint(z);
+  }
+}
+
+int w2w_sorted_keys(const W2WFrame* f, const W2WWorkspace* ws, uint64_t* out_keys, void* stream) {
+  if (!frame_ok(f) || !out_keys) return W2W_ERR_INVALID;
+  Plan p;
+  if (!ws || !make_plan(f->P, f->W, f->H, f->tile, ws->capacity, p)) return W2W_ERR_INVALID;
+  if (!ws_ok(ws, p)) return W2W_ERR_CAPACITY;
+  if (p.cap == 0) return W2W_OK;
+  hipStream_t s = (hipStream_t)stream;
+  hipGetLastError();
+  const uint32_t grid = std::min<uint32_t>(cdiv((uint64_t)p.cap, 256), 4096);
+  hipLaunchKernelGGL(k_sorted_keys, dim3(grid), dim3(256), 0, s,
+                     at<uint32_t>(ws->binning, p.L.sorted_tiles),
+                     at<uint32_t>(ws->binning, p.L.sorted_ids), at<Splat>(ws->geom, p.L.splats),
+                     at<uint32_t>(ws->geom, p.status), (uint32_t)p.cap, (uint32_t)f->P, out_keys);
+  return launch_status();
+}
+
+size_t w2w_sort_temp_bytes(int64_t n, int32_t end_bit) {
+  if (n < 0 || end_bit < 1 || end_bit > 64) return 0;
+  const int npasses = (end_bit + 7) / 8;
+  const size_t tiles = cdiv((uint64_t)n, SORT_TILE);
+  size_t o = 0;
+  o = align_up(o + 8 * 4);                                  // tickets
+  o = align_up(o + 4 * (size_t)npasses * RADIX);            // histogram
+  o = align_up(o + 4 * (size_t)npasses * tiles * RADIX);    // look-back words
+  o = align_up(o + 8 * (size_t)n);                          // key ping-pong
+  o = align_up(o + 4 * (size_t)n);                          // value ping-pong
+  return o;
+}
+
+int w2w_sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, uint64_t* keys_out,
+                       uint32_t* vals_out, int64_t n, int32_t end_bit, void* temp,
+                       size_t temp_bytes, void* stream) {
+  if (n < 0 || n >= (int64_t)LB_VAL || end_bit < 1 || end_bit > 64) return W2W_ERR_INVALID;
+  if (n == 0) return W2W_OK;
+  if (!keys_in || !vals_in || !keys_out || !vals_out || !temp) return W2W_ERR_INVALID;
+  if (temp_bytes < w2w_sort_temp_bytes(n, end_bit)) return W2W_ERR_CAPACITY;
+  hipStream_t s = (hipStream_t)stream;
+  hipGetLastError();
+  const int npasses = (end_bit + 7) / 8;
+  const int last_bits = end_bit - 8 * (npasses - 1);
+  const size_t tiles = cdiv((uint64_t)n, SORT_TILE);
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t a = o; o = align_up(o + bytes); return a; };
+  uint32_t* tickets = at<uint32_t>(temp, take(8 * 4));
+  uint32_t* hist = at<uint32_t>(temp, take(4 * (size_t)npasses * RADIX));
+  uint32_t* status = at<uint32_t>(temp, take(4 * (size_t)npasses * tiles * RADIX));
+  uint64_t* kB = at<uint64_t>(temp, take(8 * (size_t)n));
+  uint32_t* vB = at<uint32_t>(temp, take(4 * (size_t)n));
+  fill_u32(hist, 0u, (size_t)npasses * RADIX, s);
+  sort_passes<uint64_t>(keys_in, vals_in, keys_out, vals_out, kB, vB, nullptr, (uint32_t)n, npasses,
+                        last_bits, true, hist, status, tickets, s);
+  return launch_status();
+}
+
+size_t w2w_scan_temp_bytes(int64_t n) {
+  if (n < 0) return 0;
+  return align_up(4 * ((size_t)cdiv((uint64_t)n, SCAN_TILE) + 2));
+}
+
+int w2w_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n,
+                           void* temp, size_t temp_bytes, void* stream) {
+  if (n < 0 || n >= (int64_t)LB_VAL || !total) return W2W_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  hipGetLastError();
+  if (n == 0) {
+    fill_u32(total, 0u, 1, s);
+    return launch_status();
+  }
+  if (!in || !out || !temp) return W2W_ERR_INVALID;
+  if (temp_bytes < w2w_scan_temp_bytes(n)) return W2W_ERR_CAPACITY;
+  const uint32_t tiles = cdiv((uint64_t)n, SCAN_TILE);
+  uint32_t* t = static_cast<uint32_t*>(temp);
+  fill_u32(t, 0u, (size_t)tiles + 2, s);
+  hipLaunchKernelGGL(k_scan<false>, dim3(tiles), dim3(SCAN_THREADS), 0, s, in, nullptr, out, t + 2,
+                     t, total, nullptr, (uint32_t)n, 0xffffffffu);
+  return launch_status();
+}
+
+}  // extern "C"

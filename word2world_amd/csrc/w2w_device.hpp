@@ -1,0 +1,289 @@
+// Device-side math shared by the rasterizer kernels (gfx950, wave64).
+//
+// Compiled with -ffp-contract=off: every multiply and add rounds on its own,
+// and fused multiply-adds are explicit __builtin_fmaf.  The operation order
+// of the geometry, radius and blend arithmetic is fixed so that tile counts,
+// keys, ranges, n_contrib and the final transmittance are reproducible bit
+// for bit on any IEEE float32 implementation that follows the same order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define W2W_DEV __device__ __forceinline__
+
+namespace w2w {
+
+// exp(x) for x in [-87, 0]: Cody-Waite split x = k ln2 + r, degree-6 Taylor
+// polynomial for e^r, scaled by 2^k.  ~1 ulp; deterministic bits.
+W2W_DEV float exp_neg(float x) {
+  float kf = __builtin_rintf(x * 0x1.715476p+0f);
+  float r = __builtin_fmaf(kf, -0x1.62e4p-1f, x);
+  r = __builtin_fmaf(kf, -0x1.7f7d1cp-20f, r);
+  float p = 0x1.6c16c2p-10f;
+  p = __builtin_fmaf(p, r, 0x1.111112p-7f);
+  p = __builtin_fmaf(p, r, 0x1.555556p-5f);
+  p = __builtin_fmaf(p, r, 0x1.555556p-3f);
+  p = __builtin_fmaf(p, r, 0x1p-1f);
+  p = __builtin_fmaf(p, r, 1.0f);
+  p = __builtin_fmaf(p, r, 1.0f);
+  return __builtin_ldexpf(p, (int)kf);
+}
+
+// ln(x) for finite normal x > 0: x = m 2^e, m in [sqrt(1/2), sqrt(2)),
+// ln m = 2 atanh((m-1)/(m+1)) by its odd series to s^11.
+W2W_DEV float log_pos(float x) {
+  uint32_t ix = __float_as_uint(x);
+  int e = (int)((ix >> 23) & 0xffu) - 127;
+  float m = __uint_as_float((ix & 0x7fffffu) | 0x3f800000u);
+  if (m > 0x1.6a09e6p+0f) {
+    m = m * 0.5f;
+    e = e + 1;
+  }
+  float f = m - 1.0f;
+  float s = f / (2.0f + f);
+  float z = s * s;
+  float p = 0x1.745d18p-4f;
+  p = __builtin_fmaf(p, z, 0x1.c71c72p-4f);
+  p = __builtin_fmaf(p, z, 0x1.24924ap-3f);
+  p = __builtin_fmaf(p, z, 0x1.99999ap-3f);
+  p = __builtin_fmaf(p, z, 0x1.555556p-2f);
+  p = __builtin_fmaf(p, z, 1.0f);
+  float lnm = (2.0f * s) * p;
+  float ef = (float)e;
+  return __builtin_fmaf(ef, 0x1.62e4p-1f, __builtin_fmaf(ef, 0x1.7f7d1cp-20f, lnm));
+}
+
+// Gaussian exponent -0.5 (a dx^2 + c dy^2) - b dx dy in a fixed order.
+W2W_DEV float power_of(float ca, float cb, float cc, float dx, float dy) {
+  float q = __builtin_fmaf(ca * dx, dx, (cc * dy) * dy);
+  return __builtin_fmaf(-0.5f, q, -((cb * dx) * dy));
+}
+
+// Reference SH constants (gaussianForwardPass.cu:11-39; coefficient 6 = 0.3).
+constexpr float SH_C0 = 0.28209479177387814f;
+constexpr float SH_C1 = 0.4886025119029199f;
+constexpr float SH_C2A = 1.0925484305920792f;
+constexpr float SH_C2E = 0.5462742152960396f;
+constexpr float SH_C26 = 0.3f;
+constexpr float SH_K9 = -0.5900435899266435f;
+constexpr float SH_K10 = 2.890611442640554f;
+constexpr float SH_K11 = -0.4570457994644658f;
+constexpr float SH_K12 = 0.3731763325901154f;
+constexpr float SH_K13 = -0.4570457994644658f;
+constexpr float SH_K14 = 1.445305721320277f;
+constexpr float SH_K15 = -0.5900435899266435f;
+
+W2W_DEV void sh_basis(int D, float x, float y, float z, float* Y) {
+  Y[0] = SH_C0;
+  if (D < 1) return;
+  Y[1] = -SH_C1 * y;
+  Y[2] = SH_C1 * z;
+  Y[3] = -SH_C1 * x;
+  if (D < 2) return;
+  float xx = x * x, yy = y * y, zz = z * z;
+  float xy = x * y, yz = y * z, xz = x * z;
+  Y[4] = SH_C2A * xy;
+  Y[5] = -SH_C2A * yz;
+  Y[6] = SH_C26 * ((2.0f * zz - xx) - yy);
+  Y[7] = -SH_C2A * xz;
+  Y[8] = SH_C2E * (xx - yy);
+  if (D < 3) return;
+  Y[9] = (SH_K9 * y) * (3.0f * xx - yy);
+  Y[10] = ((SH_K10 * x) * y) * z;
+  Y[11] = (SH_K11 * y) * ((4.0f * zz - xx) - yy);
+  Y[12] = (SH_K12 * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy);
+  Y[13] = (SH_K13 * x) * ((4.0f * zz - xx) - yy);
+  Y[14] = (SH_K14 * z) * (xx - yy);
+  Y[15] = (SH_K15 * x) * (xx - 3.0f * yy);
+}
+
+W2W_DEV void sh_basis_grad(int D, float x, float y, float z, float* gx, float* gy, float* gz) {
+#pragma unroll
+  for (int k = 0; k < 16; k++) gx[k] = gy[k] = gz[k] = 0.0f;
+  if (D < 1) return;
+  gy[1] = -SH_C1;
+  gz[2] = SH_C1;
+  gx[3] = -SH_C1;
+  if (D < 2) return;
+  float xx = x * x, yy = y * y, zz = z * z;
+  gx[4] = SH_C2A * y; gy[4] = SH_C2A * x;
+  gy[5] = -SH_C2A * z; gz[5] = -SH_C2A * y;
+  gx[6] = SH_C26 * (-2.0f * x); gy[6] = SH_C26 * (-2.0f * y); gz[6] = SH_C26 * (4.0f * z);
+  gx[7] = -SH_C2A * z; gz[7] = -SH_C2A * x;
+  gx[8] = SH_C2E * (2.0f * x); gy[8] = SH_C2E * (-2.0f * y);
+  if (D < 3) return;
+  gx[9] = SH_K9 * (6.0f * x * y); gy[9] = SH_K9 * (3.0f * xx - 3.0f * yy);
+  gx[10] = SH_K10 * (y * z); gy[10] = SH_K10 * (x * z); gz[10] = SH_K10 * (x * y);
+  gx[11] = SH_K11 * (-2.0f * x * y); gy[11] = SH_K11 * ((4.0f * zz - xx) - 3.0f * yy); gz[11] = SH_K11 * (8.0f * y * z);
+  gx[12] = SH_K12 * (-6.0f * x * z); gy[12] = SH_K12 * (-6.0f * y * z); gz[12] = SH_K12 * ((6.0f * zz - 3.0f * xx) - 3.0f * yy);
+  gx[13] = SH_K13 * ((4.0f * zz - 3.0f * xx) - yy); gy[13] = SH_K13 * (-2.0f * x * y); gz[13] = SH_K13 * (8.0f * x * z);
+  gx[14] = SH_K14 * (2.0f * x * z); gy[14] = SH_K14 * (-2.0f * y * z); gz[14] = SH_K14 * (xx - yy);
+  gx[15] = SH_K15 * (3.0f * xx - 3.0f * yy); gy[15] = SH_K15 * (-6.0f * x * y);
+}
+
+// Camera constants read once per thread from device memory (uniform
+// addresses: scalar loads).  view/proj use the column-major storage the
+// reference's caller passes (Renderer.py:135-136).
+struct Cam {
+  float V[16];
+  float F[16];
+  float cp[3];
+  float fx, fy, limx, limy;
+  int W, H;
+};
+
+W2W_DEV void load_cam(Cam& c, const float* view, const float* proj, const float* campos, int W,
+                      int H, float fx, float fy) {
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    c.V[k] = view[k];
+    c.F[k] = proj[k];
+  }
+  c.cp[0] = campos[0];
+  c.cp[1] = campos[1];
+  c.cp[2] = campos[2];
+  c.fx = fx;
+  c.fy = fy;
+  c.W = W;
+  c.H = H;
+  c.limx = (float)W / (2.0f * fx);
+  c.limy = (float)H / (2.0f * fy);
+}
+
+// Per-Gaussian projection state: the EWA projection of projection.cu:7-154
+// with PD-1/PD-2/PD-12/PD-13 applied.
+struct Geom {
+  bool visible;
+  bool clamped;
+  float tw, vx, vy, vz;
+  float rx, ry, rxc, ryc, s;
+  float txc, tyc;
+  float jx[3], jy[3];
+  float R[9], M[9], Sig[6];
+  float cxx, cxy, cyy, det;
+  float ca, cb, cc;
+  float cx, cy, cz, invW;
+  float ndcz;
+  float px, py;
+};
+
+W2W_DEV void geometry(const Cam& c, const float p[3], const float s[3], const float q[4], Geom& g) {
+  const float* V = c.V;
+  g.visible = false;
+  float tx = ((V[0] * p[0] + V[4] * p[1]) + V[8] * p[2]) + V[12];
+  float ty = ((V[1] * p[0] + V[5] * p[1]) + V[9] * p[2]) + V[13];
+  float tz = ((V[2] * p[0] + V[6] * p[1]) + V[10] * p[2]) + V[14];
+  g.tw = ((V[3] * p[0] + V[7] * p[1]) + V[11] * p[2]) + V[15];
+  g.vx = tx / g.tw;
+  g.vy = ty / g.tw;
+  g.vz = tz / g.tw;
+  if (!(g.vz > 1.0f)) return;
+  g.rx = g.vx / g.vz;
+  g.ry = g.vy / g.vz;
+  float nx = g.rx / c.limx, ny = g.ry / c.limy;
+  float sq = nx * nx + ny * ny;
+  g.clamped = sq > 1.0f;
+  g.s = 1.0f;
+  g.rxc = g.rx;
+  g.ryc = g.ry;
+  if (g.clamped) {
+    g.s = 1.0f / __builtin_sqrtf(sq);
+    g.rxc = g.rx * g.s;
+    g.ryc = g.ry * g.s;
+  }
+  g.txc = g.rxc * g.vz;
+  g.tyc = g.ryc * g.vz;
+  float zi = 1.0f / g.vz;
+  float zi2 = zi * zi;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float right = V[4 * k + 0], up = V[4 * k + 1], fwd = V[4 * k + 2];
+    g.jx[k] = c.fx * (right * zi - (g.txc * fwd) * zi2);
+    g.jy[k] = c.fy * (up * zi - (g.tyc * fwd) * zi2);
+  }
+  float w = q[0], x = q[1], y = q[2], z = q[3];
+  float xx = x * x, yy = y * y, zz = z * z;
+  float xy = x * y, xz = x * z, yz = y * z;
+  float wx = w * x, wy = w * y, wz = w * z;
+  float* R = g.R;
+  R[0] = 1.0f - 2.0f * (yy + zz); R[1] = 2.0f * (xy - wz);        R[2] = 2.0f * (xz + wy);
+  R[3] = 2.0f * (xy + wz);        R[4] = 1.0f - 2.0f * (xx + zz); R[5] = 2.0f * (yz - wx);
+  R[6] = 2.0f * (xz - wy);        R[7] = 2.0f * (yz + wx);        R[8] = 1.0f - 2.0f * (xx + yy);
+  float* Mm = g.M;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) Mm[3 * i + j] = R[3 * i + j] * s[j];
+#define W2W_SIGE(i, j) ((Mm[3 * (i) + 0] * Mm[3 * (j) + 0] + Mm[3 * (i) + 1] * Mm[3 * (j) + 1]) + Mm[3 * (i) + 2] * Mm[3 * (j) + 2])
+  g.Sig[0] = W2W_SIGE(0, 0); g.Sig[1] = W2W_SIGE(0, 1); g.Sig[2] = W2W_SIGE(0, 2);
+  g.Sig[3] = W2W_SIGE(1, 1); g.Sig[4] = W2W_SIGE(1, 2); g.Sig[5] = W2W_SIGE(2, 2);
+#undef W2W_SIGE
+  const float* S6 = g.Sig;
+  float v0 = (S6[0] * g.jx[0] + S6[1] * g.jx[1]) + S6[2] * g.jx[2];
+  float v1 = (S6[1] * g.jx[0] + S6[3] * g.jx[1]) + S6[4] * g.jx[2];
+  float v2 = (S6[2] * g.jx[0] + S6[4] * g.jx[1]) + S6[5] * g.jx[2];
+  float w0 = (S6[0] * g.jy[0] + S6[1] * g.jy[1]) + S6[2] * g.jy[2];
+  float w1 = (S6[1] * g.jy[0] + S6[3] * g.jy[1]) + S6[4] * g.jy[2];
+  float w2 = (S6[2] * g.jy[0] + S6[4] * g.jy[1]) + S6[5] * g.jy[2];
+  g.cxx = (g.jx[0] * v0 + g.jx[1] * v1) + g.jx[2] * v2;
+  g.cxy = (g.jx[0] * w0 + g.jx[1] * w1) + g.jx[2] * w2;
+  g.cyy = (g.jy[0] * w0 + g.jy[1] * w1) + g.jy[2] * w2;
+  g.det = g.cxx * g.cyy - g.cxy * g.cxy;
+  if (!(__builtin_fabsf(g.det) >= 1e-8f)) return;
+  float dinv = 1.0f / g.det;
+  g.ca = g.cyy * dinv;
+  g.cb = -g.cxy * dinv;
+  g.cc = g.cxx * dinv;
+  const float* F = c.F;
+  g.cx = ((F[0] * p[0] + F[4] * p[1]) + F[8] * p[2]) + F[12];
+  g.cy = ((F[1] * p[0] + F[5] * p[1]) + F[9] * p[2]) + F[13];
+  g.cz = ((F[2] * p[0] + F[6] * p[1]) + F[10] * p[2]) + F[14];
+  float cw = ((F[3] * p[0] + F[7] * p[1]) + F[11] * p[2]) + F[15];
+  g.invW = 1.0f / (cw + 1e-5f);
+  float ndcx = g.cx * g.invW;
+  float ndcy = g.cy * g.invW;
+  g.ndcz = g.cz * g.invW;
+  g.px = ((ndcx + 1.0f) * 0.5f) * (float)c.W - 0.5f;
+  g.py = ((ndcy + 1.0f) * 0.5f) * (float)c.H - 0.5f;
+  g.visible = true;
+}
+
+W2W_DEV uint32_t tile_coord(float v, uint32_t n) {
+  if (!(v > 0.0f)) return 0u;
+  if (v >= (float)n) return n;
+  return (uint32_t)v;
+}
+
+// Per-instance record the blend kernels gather (48 bytes, 16-byte aligned):
+//   a = (px, py, conic_a, conic_b), b = (conic_c, opacity, depth, 0),
+//   c = (r, g, b, 0).
+struct __align__(16) Splat {
+  float4 a, b, c;
+};
+
+// ---- wave64 helpers ----
+W2W_DEV float wave_sum(float v) {
+  // butterfly over 64 lanes
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+W2W_DEV uint32_t wave_incl_scan_u32(uint32_t v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+// agent-scope (device-wide, all XCDs) relaxed atomics for look-back words
+W2W_DEV uint32_t ld_acquire_dev(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+W2W_DEV void st_release_dev(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace w2w

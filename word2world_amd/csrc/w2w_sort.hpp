@@ -1,0 +1,207 @@
+// Stable LSD radix sort of (key, u32 value) pairs, 8-bit digits, one pass per
+// digit ("onesweep"): a histogram kernel counts every digit of every pass in
+// one read of the keys; each pass kernel ranks a 4096-element tile with wave64
+// ballot multisplit, resolves its global offsets by decoupled look-back over
+// the preceding tiles, reorders the tile in LDS and writes coalesced runs.
+//
+// Replaces the 32-pass, host-synchronised 2-bit sort of prepareSort.cu:303-360
+// (block sort, D2H histogram, host prefix, H2D, merge per pass).  The result
+// is the same stable order.  Tiles are claimed through an atomic ticket, so a
+// tile only waits on tiles claimed earlier by resident workgroups (forward
+// progress without co-residency assumptions).  Every count and the element
+// total must stay below 2^30 (look-back words keep 2 flag bits).
+#pragma once
+#include "w2w_device.hpp"
+
+namespace w2w {
+
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_WAVES = SORT_THREADS / 64;
+constexpr int SORT_ITEMS = 16;
+constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;  // 4096
+constexpr int RADIX = 256;
+constexpr uint32_t LB_AGG = 1u << 30;
+constexpr uint32_t LB_INC = 2u << 30;
+constexpr uint32_t LB_VAL = (1u << 30) - 1u;
+
+W2W_DEV uint32_t sort_count(const uint32_t* d_n, uint32_t n_cap) {
+  if (!d_n) return n_cap;
+  uint32_t n = *d_n;
+  return n < n_cap ? n : n_cap;
+}
+
+// Histogram of every digit of every pass (hist[pass][256], pre-zeroed), and
+// zeroing of the look-back words status[pass][tile][256] (pass stride
+// tiles_stride tiles) and the per-pass tickets.
+template <typename KT>
+__global__ __launch_bounds__(256) void k_sort_hist(const KT* __restrict__ keys,
+                                                   const uint32_t* __restrict__ d_n, uint32_t n_cap,
+                                                   int begin_bit, int npasses, int last_bits,
+                                                   uint32_t* __restrict__ hist,
+                                                   uint32_t* __restrict__ status,
+                                                   uint32_t tiles_stride,
+                                                   uint32_t* __restrict__ tickets) {
+  __shared__ uint32_t sh[8][RADIX];
+  const uint32_t n = sort_count(d_n, n_cap);
+  for (int i = threadIdx.x; i < 8 * RADIX; i += blockDim.x) (&sh[0][0])[i] = 0;
+  __syncthreads();
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    KT k = keys[i];
+    for (int p = 0; p < npasses; p++) {
+      int bits = (p == npasses - 1) ? last_bits : 8;
+      uint32_t d = (uint32_t)(k >> (begin_bit + 8 * p)) & ((1u << bits) - 1u);
+      atomicAdd(&sh[p][d], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < npasses * RADIX; i += blockDim.x) {
+    uint32_t v = (&sh[0][0])[i];
+    if (v) atomicAdd(&hist[i], v);
+  }
+  // pass p owns status[p * tiles_stride * RADIX ...]; zero the tiles this n uses
+  const uint32_t ntiles = (n + SORT_TILE - 1) / SORT_TILE;
+  const size_t words = (size_t)ntiles * RADIX;
+  for (int p = 0; p < npasses; p++) {
+    uint32_t* sp = status + (size_t)p * tiles_stride * RADIX;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) sp[i] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < (unsigned)npasses) tickets[threadIdx.x] = 0;
+}
+
+// exclusive scan of one value per thread over a 256-thread block
+W2W_DEV uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_tmp /*[4]*/, uint32_t* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t inc = wave_incl_scan_u32(v, lane);
+  if (lane == 63) s_tmp[wave] = inc;
+  __syncthreads();
+  uint32_t wpre = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < 4; w++) {
+    uint32_t t = s_tmp[w];
+    if (w < wave) wpre += t;
+    tot += t;
+  }
+  __syncthreads();
+  if (total) *total = tot;
+  return wpre + inc - v;
+}
+
+template <typename KT, bool WRITE_KEYS>
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(
+    const KT* __restrict__ kin, const uint32_t* __restrict__ vin, KT* __restrict__ kout,
+    uint32_t* __restrict__ vout, const uint32_t* __restrict__ d_n, uint32_t n_cap, int shift,
+    int nbits, const uint32_t* __restrict__ hist, uint32_t* __restrict__ status,
+    uint32_t* __restrict__ ticket) {
+  __shared__ KT s_k[SORT_TILE];
+  __shared__ uint32_t s_v[SORT_TILE];
+  __shared__ uint32_t s_wcnt[SORT_WAVES][RADIX];
+  __shared__ uint32_t s_dstart[RADIX];
+  __shared__ uint32_t s_gbase[RADIX];
+  __shared__ uint32_t s_tmp[4];
+  __shared__ uint32_t s_tile;
+
+  const uint32_t n = sort_count(d_n, n_cap);
+  const uint32_t ntiles = (n + SORT_TILE - 1) / SORT_TILE;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t dmask = (1u << nbits) - 1u;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+  // exclusive prefix of the global digit histogram
+  const uint32_t hexcl = block_excl_scan_256(hist[tid], s_tmp, nullptr);
+
+  for (;;) {
+    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+#pragma unroll
+    for (int q = 0; q < RADIX / 64; q++) s_wcnt[wave][lane + 64 * q] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    if (tile >= ntiles) return;
+    const uint32_t base = tile * SORT_TILE;
+    const uint32_t wbase = base + wave * (SORT_ITEMS * 64);
+
+    KT keys[SORT_ITEMS];
+    uint32_t vals[SORT_ITEMS];
+    uint32_t woff[SORT_ITEMS];
+#pragma unroll
+    for (int j = 0; j < SORT_ITEMS; j++) {
+      uint32_t idx = wbase + j * 64 + lane;
+      bool valid = idx < n;
+      keys[j] = valid ? kin[idx] : (KT)0;
+      vals[j] = valid ? vin[idx] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < SORT_ITEMS; j++) {
+      uint32_t idx = wbase + j * 64 + lane;
+      bool valid = idx < n;
+      uint32_t d = (uint32_t)(keys[j] >> shift) & dmask;
+      uint64_t peers = __ballot(valid);
+      for (int b = 0; b < nbits; b++) {
+        bool bit = (d >> b) & 1u;
+        uint64_t bb = __ballot(bit);
+        peers &= bit ? bb : ~bb;
+      }
+      uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+      uint32_t cnt = (uint32_t)__popcll(peers);
+      uint32_t pre = valid ? s_wcnt[wave][d] : 0u;
+      woff[j] = pre + rank;
+      if (valid && rank == 0) s_wcnt[wave][d] = pre + cnt;
+    }
+    __syncthreads();
+    // digit totals of this tile and the per-wave exclusive offsets
+    uint32_t c0 = s_wcnt[0][tid], c1 = s_wcnt[1][tid], c2 = s_wcnt[2][tid], c3 = s_wcnt[3][tid];
+    uint32_t total = c0 + c1 + c2 + c3;
+    s_wcnt[0][tid] = 0;
+    s_wcnt[1][tid] = c0;
+    s_wcnt[2][tid] = c0 + c1;
+    s_wcnt[3][tid] = c0 + c1 + c2;
+    uint32_t dstart = block_excl_scan_256(total, s_tmp, nullptr);
+    s_dstart[tid] = dstart;
+    // decoupled look-back for digit tid
+    uint32_t* st = status + (size_t)tile * RADIX;
+    uint32_t excl = 0;
+    if (tile == 0) {
+      st_release_dev(st + tid, LB_INC | total);
+    } else {
+      st_release_dev(st + tid, LB_AGG | total);
+      int p = (int)tile - 1;
+      for (;;) {
+        uint32_t v = ld_acquire_dev(status + (size_t)p * RADIX + tid);
+        uint32_t flag = v & ~LB_VAL;
+        if (flag == 0) continue;
+        excl += v & LB_VAL;
+        if (flag == LB_INC) break;
+        p--;
+      }
+      st_release_dev(st + tid, LB_INC | (excl + total));
+    }
+    s_gbase[tid] = hexcl + excl - dstart;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SORT_ITEMS; j++) {
+      uint32_t idx = wbase + j * 64 + lane;
+      if (idx < n) {
+        uint32_t d = (uint32_t)(keys[j] >> shift) & dmask;
+        uint32_t pos = s_dstart[d] + s_wcnt[wave][d] + woff[j];
+        if (pos < (uint32_t)SORT_TILE) {
+          s_k[pos] = keys[j];
+          s_v[pos] = vals[j];
+        }
+      }
+    }
+    __syncthreads();
+    const uint32_t tile_n = (n - base) < (uint32_t)SORT_TILE ? (n - base) : (uint32_t)SORT_TILE;
+    for (uint32_t i = tid; i < tile_n; i += SORT_THREADS) {
+      KT k = s_k[i];
+      uint32_t d = (uint32_t)(k >> shift) & dmask;
+      uint32_t o = s_gbase[d] + i;
+      if (o < n) {
+        if (WRITE_KEYS) kout[o] = k;
+        vout[o] = s_v[i];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace w2w
