@@ -135,6 +135,37 @@ size_t w2w_scan_temp_bytes(int64_t n);
 int w2w_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n,
                            void* temp, size_t temp_bytes, void* stream);
 
+/* Fused Adam step over a flat parameter store split into `n_groups` (<= 8)
+ * consecutive groups (group k ends at group_end[k], host array) with
+ * learning rate group_lr[k] (host array): the torch.optim.Adam update the
+ * trainer uses (gaussians_handler.py:40, OptimizerManager.py), bias-corrected
+ * with `step` (>= 1).  params/grads/exp_avg/exp_avg_sq are device arrays of n
+ * floats. */
+int w2w_adam_step(float* params, const float* grads, float* exp_avg, float* exp_avg_sq, int64_t n,
+                  int32_t n_groups, const int64_t* group_end, const float* group_lr, int64_t step,
+                  float beta1, float beta2, float eps, void* stream);
+
+/* Optional per-kernel timing: HIP event pairs recorded on the launch stream
+ * around every launch whose kernel id bit is set in `kernel_mask`, into a
+ * pool of `capacity` records (0 disables and frees it).  Host-side state,
+ * not thread-safe; for benchmarks and profiling runs.  Kernel ids: */
+enum {
+  W2W_KID_FILL = 0, W2W_KID_PREPROCESS = 1, W2W_KID_DSORT_HIST = 2, W2W_KID_DSORT_PASS = 3,
+  W2W_KID_SCAN = 4, W2W_KID_EMIT = 5, W2W_KID_TSORT_HIST = 6, W2W_KID_TSORT_PASS = 7,
+  W2W_KID_RANGES = 8, W2W_KID_BLEND = 9, W2W_KID_BLEND_BWD = 10, W2W_KID_PREPROCESS_BWD = 11,
+  W2W_KID_ADAM = 12, W2W_KID_SORT_HIST = 13, W2W_KID_SORT_PASS = 14,
+};
+int w2w_prof_enable(int32_t capacity, uint32_t kernel_mask);
+int w2w_prof_reset(void);
+/* Waits for the recorded events; returns kernel id and milliseconds per record. */
+int w2w_prof_read(int32_t* kernel_ids, float* ms, int32_t max_records, int32_t* n_records);
+
+/* Optional work counters (NULL disables): the blend kernels add, per launch,
+ * [0] evaluated (pixel, instance) pairs, [1] contributing pairs, [2] instances
+ * loaded into LDS for the forward, and [3..5] the same for the backward, into
+ * `device_counters` (uint64 device memory).  For roofline accounting. */
+int w2w_stats_enable(uint64_t* device_counters);
+
 #ifdef __cplusplus
 }
 #endif

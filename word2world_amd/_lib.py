@@ -42,7 +42,11 @@ class W2WLayout(ctypes.Structure):
 
 EXPORTS = ["w2w_version", "w2w_workspace_layout", "w2w_forward", "w2w_backward", "w2w_sorted_keys",
            "w2w_sort_temp_bytes", "w2w_sort_pairs_u64", "w2w_scan_temp_bytes",
-           "w2w_exclusive_scan_u32"]
+           "w2w_exclusive_scan_u32", "w2w_prof_enable", "w2w_prof_reset", "w2w_prof_read", "w2w_stats_enable", "w2w_adam_step"]
+
+KERNEL_IDS = {0: "fill", 1: "preprocess", 2: "depth_sort_hist", 3: "depth_sort_pass", 4: "scan",
+              5: "emit", 6: "tile_sort_hist", 7: "tile_sort_pass", 8: "ranges", 9: "blend",
+              10: "blend_bwd", 11: "preprocess_bwd", 12: "adam", 13: "sort_hist", 14: "sort_pass"}
 
 _lib = None
 
@@ -76,6 +80,17 @@ def load(path: str = LIB_PATH):
     L.w2w_scan_temp_bytes.argtypes = [ctypes.c_int64]
     L.w2w_exclusive_scan_u32.restype = ctypes.c_int
     L.w2w_exclusive_scan_u32.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp, ctypes.c_size_t, _vp]
+    L.w2w_prof_enable.restype = ctypes.c_int
+    L.w2w_prof_enable.argtypes = [ctypes.c_int32, ctypes.c_uint32]
+    L.w2w_prof_reset.restype = ctypes.c_int
+    L.w2w_prof_reset.argtypes = []
+    L.w2w_prof_read.restype = ctypes.c_int
+    L.w2w_prof_read.argtypes = [_vp, _vp, ctypes.c_int32, P(ctypes.c_int32)]
+    L.w2w_stats_enable.restype = ctypes.c_int
+    L.w2w_stats_enable.argtypes = [_vp]
+    L.w2w_adam_step.restype = ctypes.c_int
+    L.w2w_adam_step.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp,
+                                ctypes.c_int64, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp]
     _lib = L
     return L
 
@@ -90,3 +105,14 @@ def layout(P: int, W: int, H: int, tile: int, capacity: int) -> W2WLayout:
     check(load().w2w_workspace_layout(P, W, H, tile, capacity, ctypes.byref(out)),
           "w2w_workspace_layout")
     return out
+
+
+def prof_read(max_records: int = 1 << 16):
+    """[(kernel name, ms)] of the records since the last reset (synchronises)."""
+    import numpy as np
+    ids = np.zeros(max_records, np.int32)
+    ms = np.zeros(max_records, np.float32)
+    n = ctypes.c_int32(0)
+    check(load().w2w_prof_read(ids.ctypes.data, ms.ctypes.data, max_records, ctypes.byref(n)),
+          "w2w_prof_read")
+    return [(KERNEL_IDS.get(int(i), str(int(i))), float(t)) for i, t in zip(ids[:n.value], ms[:n.value])]

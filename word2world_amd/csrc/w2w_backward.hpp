@@ -51,6 +51,7 @@ struct BlendBwdArgs {
   const float* bg;
   int W, H, tiles_x, T;
   uint32_t P;
+  unsigned long long* stats;  // optional [3]: evaluated pairs, contributing pairs, loaded instances
   const uint32_t* n_contrib;
   const float* final_T;
   const float* dL_dcolor;   // [3,H,W]
@@ -101,6 +102,7 @@ __global__ __launch_bounds__(TS* TS) void k_blend_bwd(BlendBwdArgs a) {
   const int nproc = (int)s_max;
 
   float T = Tf;
+  uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
   float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, accD = 0.0f, accA = 0.0f;
   float la = 0.0f, lc0 = 0.0f, lc1 = 0.0f, lc2 = 0.0f, ld = 0.0f;
 
@@ -108,6 +110,7 @@ __global__ __launch_bounds__(TS* TS) void k_blend_bwd(BlendBwdArgs a) {
     const int bstart = bend > B ? bend - B : 0;
     __syncthreads();
     const int k = bstart + tid;
+    n_loaded += bend - bstart;
     if (k < bend) {
       uint32_t id = a.ids[range.x + k];
       id = id < a.P ? id : 0u;
@@ -122,6 +125,7 @@ __global__ __launch_bounds__(TS* TS) void k_blend_bwd(BlendBwdArgs a) {
       float g_r = 0.0f, g_g = 0.0f, g_b = 0.0f, g_d = 0.0f;
       bool contrib = false;
       if (kk < last) {
+        n_eval++;
         const float4 sa = s_sp[j].a;
         const float4 sb = s_sp[j].b;
         const float dx = sa.x - pxf, dy = sa.y - pyf;
@@ -133,6 +137,7 @@ __global__ __launch_bounds__(TS* TS) void k_blend_bwd(BlendBwdArgs a) {
           const float al = fminf(0.99f, oG);
           if (al >= a_min) {
             contrib = true;
+            n_con++;
             T = T / (1.0f - al);
             const float aT = al * T;
             const float4 sc = s_sp[j].c;
@@ -188,6 +193,19 @@ __global__ __launch_bounds__(TS* TS) void k_blend_bwd(BlendBwdArgs a) {
           atomicAdd(dst + GA_DEPTH, g_d);
         }
       }
+    }
+  }
+  if (a.stats) {
+    unsigned long long e = n_eval, c = n_con, kf = (threadIdx.x == 0) ? n_loaded : 0u;
+    for (int off = 32; off >= 1; off >>= 1) {
+      e += __shfl_xor(e, off, 64);
+      c += __shfl_xor(c, off, 64);
+      kf += __shfl_xor(kf, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(a.stats + 0, e);
+      atomicAdd(a.stats + 1, c);
+      if (kf) atomicAdd(a.stats + 2, kf);
     }
   }
 }

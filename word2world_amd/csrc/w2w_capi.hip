@@ -6,7 +6,11 @@
 // word), so a whole forward or backward can be captured into a HIP graph.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+
 #include "../../include/w2w_raster.h"
+#include "w2w_adam.hpp"
 #include "w2w_backward.hpp"
 #include "w2w_device.hpp"
 #include "w2w_forward.hpp"
@@ -103,9 +107,41 @@ int launch_status() {
   return e == hipSuccess ? W2W_OK : W2W_ERR_HIP;
 }
 
+// ---- optional per-kernel timing with HIP events on the launch stream ----
+// Records an event pair around every launch whose kernel id is enabled in
+// the mask, into a fixed pool set up by w2w_prof_enable (host-side state,
+// not thread-safe; meant for benchmarks and profiling runs).
+struct Prof {
+  int capacity = 0;
+  int used = 0;
+  uint32_t mask = 0;
+  hipEvent_t* start = nullptr;
+  hipEvent_t* stop = nullptr;
+  int* kid = nullptr;
+} g_prof;
+
+// optional device counters for the blend kernels (w2w_stats_enable)
+unsigned long long* g_stats = nullptr;
+
+struct ProfSpan {
+  int slot = -1;
+  hipStream_t s;
+  ProfSpan(int kid, hipStream_t stream) : s(stream) {
+    if (g_prof.capacity && (g_prof.mask >> kid & 1u) && g_prof.used < g_prof.capacity) {
+      slot = g_prof.used++;
+      g_prof.kid[slot] = kid;
+      (void)hipEventRecord(g_prof.start[slot], s);
+    }
+  }
+  ~ProfSpan() {
+    if (slot >= 0) (void)hipEventRecord(g_prof.stop[slot], s);
+  }
+};
+
 void fill_u32(uint32_t* ptr, uint32_t v, size_t n, hipStream_t s) {
   if (n == 0) return;
   uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 4096);
+  ProfSpan ps(W2W_KID_FILL, s);
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, ptr, v, n);
 }
 
@@ -121,24 +157,30 @@ bool frame_ok(const W2WFrame* f) {
 template <int TS>
 void launch_blend(const BlendArgs& a, hipStream_t s) {
   const uint32_t grid = 8 * cdiv(a.T, 8);
+  ProfSpan ps(W2W_KID_BLEND, s);
   hipLaunchKernelGGL(k_blend<TS>, dim3(grid), dim3(TS * TS), 0, s, a);
 }
 
 template <int TS>
 void launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s) {
   const uint32_t grid = 8 * cdiv(a.T, 8);
+  ProfSpan ps(W2W_KID_BLEND_BWD, s);
   hipLaunchKernelGGL(k_blend_bwd<TS>, dim3(grid), dim3(TS * TS), 0, s, a);
 }
 
 template <typename KT>
 void sort_passes(const KT* kin, const uint32_t* vin, KT* kA, uint32_t* vA, KT* kB, uint32_t* vB,
                  const uint32_t* d_n, uint32_t n_cap, int npasses, int last_bits, bool write_last_keys,
-                 uint32_t* hist, uint32_t* status, uint32_t* tickets, hipStream_t s) {
+                 uint32_t* hist, uint32_t* status, uint32_t* tickets, int kid_hist, int kid_pass,
+                 hipStream_t s) {
   const uint32_t ntiles = cdiv(n_cap, SORT_TILE);
   if (ntiles == 0) return;
   const uint32_t hgrid = std::min<uint32_t>(cdiv(n_cap, 256), 2048);
+  {
+  ProfSpan ps(kid_hist, s);
   hipLaunchKernelGGL(k_sort_hist<KT>, dim3(hgrid), dim3(256), 0, s, kin, d_n, n_cap, 0, npasses,
                      last_bits, hist, status, ntiles, tickets);
+  }
   const uint32_t pgrid = std::min<uint32_t>(ntiles, 2048);
   const KT* ks = kin;
   const uint32_t* vs = vin;
@@ -148,6 +190,7 @@ void sort_passes(const KT* kin, const uint32_t* vin, KT* kA, uint32_t* vA, KT* k
     uint32_t* vd = toA ? vA : vB;
     const int nbits = (p == npasses - 1) ? last_bits : 8;
     uint32_t* st = status + (size_t)p * ntiles * RADIX;
+    ProfSpan ps(kid_pass, s);
     if (p < npasses - 1 || write_last_keys)
       hipLaunchKernelGGL((k_sort_pass<KT, true>), dim3(pgrid), dim3(SORT_THREADS), 0, s, ks, vs, kd,
                          vd, d_n, n_cap, 8 * p, nbits, hist + p * RADIX, st, tickets + p);
@@ -181,7 +224,7 @@ int w2w_forward(const W2WFrame* f, const W2WWorkspace* ws, float* out_color, flo
   if (!ws || !make_plan(f->P, f->W, f->H, f->tile, ws->capacity, p)) return W2W_ERR_INVALID;
   if (!ws_ok(ws, p)) return W2W_ERR_CAPACITY;
   hipStream_t s = (hipStream_t)stream;
-  hipGetLastError();
+  (void)hipGetLastError();
   void* G = ws->geom;
   void* B = ws->binning;
   void* I = ws->image;
@@ -208,35 +251,46 @@ int w2w_forward(const W2WFrame* f, const W2WWorkspace* ws, float* out_color, flo
     pa.counts = at<uint32_t>(G, p.L.counts);
     pa.dkey = at<uint32_t>(G, p.dkey0);
     pa.ids = at<uint32_t>(G, p.L.perm);
+    {
+    ProfSpan ps(W2W_KID_PREPROCESS, s);
     hipLaunchKernelGGL(k_preprocess, dim3(cdiv(P, 256)), dim3(256), 0, s, pa);
+    }
 
     // depth sort: (depth bits, id) over P, 4 passes, result in dkey0 / perm
     sort_passes<uint32_t>(at<uint32_t>(G, p.dkey0), at<uint32_t>(G, p.L.perm),
                           at<uint32_t>(G, p.dkey0), at<uint32_t>(G, p.L.perm),
                           at<uint32_t>(G, p.dkey1), at<uint32_t>(G, p.id1), nullptr, P, 4, 8,
                           false, at<uint32_t>(G, p.dhist), at<uint32_t>(G, p.dstatus),
-                          status + ST_DTICKET, s);
+                          status + ST_DTICKET, W2W_KID_DSORT_HIST, W2W_KID_DSORT_PASS, s);
+    {
+    ProfSpan ps(W2W_KID_SCAN, s);
 
     hipLaunchKernelGGL(k_scan<true>, dim3(p.stiles), dim3(SCAN_THREADS), 0, s,
                        at<uint32_t>(G, p.L.counts), at<uint32_t>(G, p.L.perm),
                        at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.slb) + 1,
                        at<uint32_t>(G, p.slb), status + ST_K, status + ST_OVERFLOW, P, cap);
+    }
     if (cap > 0) {
+      {
+      ProfSpan ps(W2W_KID_EMIT, s);
       hipLaunchKernelGGL(k_emit, dim3(cdiv(P, 256)), dim3(256), 0, s, at<uint32_t>(G, p.L.perm),
                          at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.L.counts),
                          at<uint2>(G, p.L.rect), P, (uint32_t)p.L.tiles_x, cap,
                          at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0));
+      }
       sort_passes<uint32_t>(at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0),
                             at<uint32_t>(B, p.L.sorted_tiles), at<uint32_t>(B, p.L.sorted_ids),
                             at<uint32_t>(B, (p.tpasses & 1) ? p.tkey0 : p.tkey1),
                             at<uint32_t>(B, (p.tpasses & 1) ? p.tid0 : p.tid1), status + ST_K, cap,
                             p.tpasses, p.tlast_bits, true, at<uint32_t>(B, p.thist),
-                            at<uint32_t>(B, p.tstatus), status + ST_TTICKET, s);
+                            at<uint32_t>(B, p.tstatus), status + ST_TTICKET, W2W_KID_TSORT_HIST,
+                            W2W_KID_TSORT_PASS, s);
       const uint32_t rgrid = std::min<uint32_t>(cdiv(cap, 256), 4096);
+      ProfSpan ps(W2W_KID_RANGES, s);
       hipLaunchKernelGGL(k_ranges, dim3(rgrid), dim3(256), 0, s, at<uint32_t>(B, p.L.sorted_tiles),
                          status, cap, p.T, at<uint2>(I, p.L.ranges));
     }
-    hipMemcpyAsync(out_radii, at<int32_t>(G, p.L.radii), 4 * (size_t)P, hipMemcpyDeviceToDevice, s);
+    (void)hipMemcpyAsync(out_radii, at<int32_t>(G, p.L.radii), 4 * (size_t)P, hipMemcpyDeviceToDevice, s);
   }
   BlendArgs ba;
   ba.ranges = at<uint2>(I, p.L.ranges);
@@ -245,6 +299,7 @@ int w2w_forward(const W2WFrame* f, const W2WWorkspace* ws, float* out_color, flo
   ba.bg = f->background;
   ba.W = f->W; ba.H = f->H; ba.tiles_x = p.L.tiles_x; ba.T = (int)p.T;
   ba.P = (uint32_t)f->P;
+  ba.stats = g_stats;
   ba.color = out_color; ba.alpha = out_alpha; ba.depth = out_depth;
   ba.n_contrib = at<uint32_t>(I, p.L.n_contrib);
   ba.final_T = at<float>(I, p.L.final_T);
@@ -269,7 +324,7 @@ int w2w_backward(const W2WFrame* f, const W2WWorkspace* ws, const float* dL_dcol
   if (!ws_ok(ws, p)) return W2W_ERR_CAPACITY;
   if (f->P == 0) return W2W_OK;
   hipStream_t s = (hipStream_t)stream;
-  hipGetLastError();
+  (void)hipGetLastError();
   void* G = ws->geom;
   void* B = ws->binning;
   void* I = ws->image;
@@ -283,6 +338,7 @@ int w2w_backward(const W2WFrame* f, const W2WWorkspace* ws, const float* dL_dcol
   ba.bg = f->background;
   ba.W = f->W; ba.H = f->H; ba.tiles_x = p.L.tiles_x; ba.T = (int)p.T;
   ba.P = (uint32_t)f->P;
+  ba.stats = g_stats ? g_stats + 3 : nullptr;
   ba.n_contrib = at<uint32_t>(I, p.L.n_contrib);
   ba.final_T = at<float>(I, p.L.final_T);
   ba.dL_dcolor = dL_dcolor; ba.dL_dalpha = dL_dalpha; ba.dL_ddepth = dL_ddepth;
@@ -302,6 +358,7 @@ int w2w_backward(const W2WFrame* f, const W2WWorkspace* ws, const float* dL_dcol
   pa.dL_dmeans2D = dL_dmeans2D; pa.dL_dcolors = dL_dcolors; pa.dL_dopacity = dL_dopacity;
   pa.dL_dmeans3D = dL_dmeans3D; pa.dL_dcov3D = dL_dcov3D; pa.dL_dsh = dL_dsh;
   pa.dL_dscales = dL_dscales; pa.dL_drots = dL_drotations;
+  ProfSpan ps(W2W_KID_PREPROCESS_BWD, s);
   hipLaunchKernelGGL(k_preprocess_bwd, dim3(cdiv(P, 256)), dim3(256), 0, s, pa);
   return launch_status();
 }
@@ -325,7 +382,7 @@ int w2w_sorted_keys(const W2WFrame* f, const W2WWorkspace* ws, uint64_t* out_key
   if (!ws_ok(ws, p)) return W2W_ERR_CAPACITY;
   if (p.cap == 0) return W2W_OK;
   hipStream_t s = (hipStream_t)stream;
-  hipGetLastError();
+  (void)hipGetLastError();
   const uint32_t grid = std::min<uint32_t>(cdiv((uint64_t)p.cap, 256), 4096);
   hipLaunchKernelGGL(k_sorted_keys, dim3(grid), dim3(256), 0, s,
                      at<uint32_t>(ws->binning, p.L.sorted_tiles),
@@ -355,7 +412,7 @@ int w2w_sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, uint64_
   if (!keys_in || !vals_in || !keys_out || !vals_out || !temp) return W2W_ERR_INVALID;
   if (temp_bytes < w2w_sort_temp_bytes(n, end_bit)) return W2W_ERR_CAPACITY;
   hipStream_t s = (hipStream_t)stream;
-  hipGetLastError();
+  (void)hipGetLastError();
   const int npasses = (end_bit + 7) / 8;
   const int last_bits = end_bit - 8 * (npasses - 1);
   const size_t tiles = cdiv((uint64_t)n, SORT_TILE);
@@ -368,7 +425,7 @@ int w2w_sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, uint64_
   uint32_t* vB = at<uint32_t>(temp, take(4 * (size_t)n));
   fill_u32(hist, 0u, (size_t)npasses * RADIX, s);
   sort_passes<uint64_t>(keys_in, vals_in, keys_out, vals_out, kB, vB, nullptr, (uint32_t)n, npasses,
-                        last_bits, true, hist, status, tickets, s);
+                        last_bits, true, hist, status, tickets, W2W_KID_SORT_HIST, W2W_KID_SORT_PASS, s);
   return launch_status();
 }
 
@@ -381,7 +438,7 @@ int w2w_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, i
                            void* temp, size_t temp_bytes, void* stream) {
   if (n < 0 || n >= (int64_t)LB_VAL || !total) return W2W_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
-  hipGetLastError();
+  (void)hipGetLastError();
   if (n == 0) {
     fill_u32(total, 0u, 1, s);
     return launch_status();
@@ -391,9 +448,97 @@ int w2w_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, i
   const uint32_t tiles = cdiv((uint64_t)n, SCAN_TILE);
   uint32_t* t = static_cast<uint32_t*>(temp);
   fill_u32(t, 0u, (size_t)tiles + 2, s);
+  ProfSpan ps(W2W_KID_SCAN, s);
   hipLaunchKernelGGL(k_scan<false>, dim3(tiles), dim3(SCAN_THREADS), 0, s, in, nullptr, out, t + 2,
                      t, total, nullptr, (uint32_t)n, 0xffffffffu);
   return launch_status();
+}
+
+int w2w_prof_enable(int32_t capacity, uint32_t kernel_mask) {
+  for (int i = 0; i < g_prof.capacity; i++) {
+    (void)hipEventDestroy(g_prof.start[i]);
+    (void)hipEventDestroy(g_prof.stop[i]);
+  }
+  delete[] g_prof.start;
+  delete[] g_prof.stop;
+  delete[] g_prof.kid;
+  g_prof = Prof();
+  if (capacity <= 0) return W2W_OK;
+  g_prof.start = new hipEvent_t[capacity];
+  g_prof.stop = new hipEvent_t[capacity];
+  g_prof.kid = new int[capacity];
+  for (int i = 0; i < capacity; i++) {
+    if (hipEventCreate(&g_prof.start[i]) != hipSuccess || hipEventCreate(&g_prof.stop[i]) != hipSuccess) {
+      g_prof.capacity = i;
+      w2w_prof_enable(0, 0);
+      return W2W_ERR_HIP;
+    }
+  }
+  g_prof.capacity = capacity;
+  g_prof.mask = kernel_mask;
+  return W2W_OK;
+}
+
+int w2w_adam_step(float* params, const float* grads, float* exp_avg, float* exp_avg_sq, int64_t n,
+                  int32_t n_groups, const int64_t* group_end, const float* group_lr, int64_t step,
+                  float beta1, float beta2, float eps, void* stream) {
+  if (n < 0 || n_groups < 1 || n_groups > ADAM_MAX_SEGMENTS || step < 1 || !group_end || !group_lr)
+    return W2W_ERR_INVALID;
+  if (n == 0) return W2W_OK;
+  if (!params || !grads || !exp_avg || !exp_avg_sq) return W2W_ERR_INVALID;
+  AdamArgs a;
+  a.p = params; a.g = grads; a.m = exp_avg; a.v = exp_avg_sq; a.n = (uint64_t)n;
+  a.nseg = n_groups;
+  const double bc1 = 1.0 - std::pow((double)beta1, (double)step);
+  const double bc2 = 1.0 - std::pow((double)beta2, (double)step);
+  int64_t prev = 0;
+  for (int k = 0; k < ADAM_MAX_SEGMENTS; k++) {
+    if (k < n_groups) {
+      if (group_end[k] < prev || group_end[k] > n) return W2W_ERR_INVALID;
+      prev = group_end[k];
+      a.seg_end[k] = (uint64_t)group_end[k];
+      a.neg_step[k] = (float)(-(double)group_lr[k] / bc1);
+    } else {
+      a.seg_end[k] = (uint64_t)n;
+      a.neg_step[k] = 0.0f;
+    }
+  }
+  if (group_end[n_groups - 1] != n) return W2W_ERR_INVALID;
+  a.inv_bc2_sqrt = (float)(1.0 / std::sqrt(bc2));
+  a.one_m_b1 = (float)(1.0 - (double)beta1);
+  a.b2 = beta2;
+  a.one_m_b2 = (float)(1.0 - (double)beta2);
+  a.eps = eps;
+  hipStream_t s = (hipStream_t)stream;
+  (void)hipGetLastError();
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(((uint64_t)n + 255) / 256, 8192);
+  ProfSpan ps(W2W_KID_ADAM, s);
+  hipLaunchKernelGGL(k_adam, dim3(grid), dim3(256), 0, s, a);
+  return launch_status();
+}
+
+int w2w_stats_enable(uint64_t* device_counters) {
+  g_stats = reinterpret_cast<unsigned long long*>(device_counters);
+  return W2W_OK;
+}
+
+int w2w_prof_reset(void) {
+  g_prof.used = 0;
+  return W2W_OK;
+}
+
+int w2w_prof_read(int32_t* kernel_ids, float* ms, int32_t max_records, int32_t* n_records) {
+  if (!n_records) return W2W_ERR_INVALID;
+  const int n = std::min<int>(g_prof.used, max_records);
+  for (int i = 0; i < n; i++) {
+    if (hipEventSynchronize(g_prof.stop[i]) != hipSuccess) return W2W_ERR_HIP;
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, g_prof.start[i], g_prof.stop[i]) != hipSuccess) return W2W_ERR_HIP;
+    if (kernel_ids) kernel_ids[i] = g_prof.kid[i];
+    if (ms) ms[i] = t;
+  }
+  *n_records = n;
+  return W2W_OK;
 }
 
 }  // extern "C"

@@ -278,6 +278,7 @@ struct BlendArgs {
   const float* bg;
   int W, H, tiles_x, T;
   uint32_t P;
+  unsigned long long* stats;  // optional [3]: evaluated pairs, contributing pairs, loaded instances
   float* color;
   float* alpha;
   float* depth;
@@ -301,9 +302,11 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs a) {
   bool done = !inside;
   float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, A = 0.0f, Dd = 0.0f;
   uint32_t last = 0;
+  uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
   const float a_min = 0x1.010102p-8f;
   for (int base = 0; base < n; base += B) {
     if (__syncthreads_count(done) == B) break;
+    n_loaded += min(B, n - base);
     const int k = base + tid;
     if (k < n) {
       uint32_t id = a.ids[range.x + k];
@@ -316,6 +319,7 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs a) {
       const float4 sb = s_sp[j].b;
       const float dx = sa.x - pxf, dy = sa.y - pyf;
       const float pw = power_of(sa.z, sa.w, sb.x, dx, dy);
+      n_eval++;
       if (pw > 0.0f) continue;
       const float G = exp_neg(pw);
       const float al = fminf(0.99f, sb.y * G);
@@ -334,6 +338,20 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs a) {
       Dd = __builtin_fmaf(sb.z, aT, Dd);
       T = nT;
       last = (uint32_t)(base + j + 1);
+      n_con++;
+    }
+  }
+  if (a.stats) {  // evaluated / contributing pairs and loaded instances
+    unsigned long long e = n_eval, c = n_con, kf = (threadIdx.x == 0) ? n_loaded : 0u;
+    for (int off = 32; off >= 1; off >>= 1) {
+      e += __shfl_xor(e, off, 64);
+      c += __shfl_xor(c, off, 64);
+      kf += __shfl_xor(kf, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(a.stats + 0, e);
+      atomicAdd(a.stats + 1, c);
+      if (kf) atomicAdd(a.stats + 2, kf);
     }
   }
   if (inside) {
