@@ -91,8 +91,23 @@ typedef struct W2WLayout {
   size_t ranges;     /* image: uint32x2[T] per-tile [start,end) */
   size_t n_contrib;  /* image: uint32[H*W] 1-based last contributor */
   size_t final_T;    /* image: float[H*W] final transmittance */
-  int32_t tiles_x, tiles_y;
+  size_t item_frame; /* geom: uint32[items] frame of each item */
+  size_t table;      /* geom: the one-frame table the single-view entry points write */
 } W2WLayout;
+
+/* Capacities of a batched workspace: every grid is sized by these, and every
+ * data-dependent count is read on the device, so one launch sequence (or one
+ * captured HIP graph) serves any batch that fits. */
+typedef struct W2WCaps {
+  int32_t scenes, frames;  /* descriptor slots in the batch table */
+  int32_t tile;            /* tile edge: 8, 16 or 32 */
+  int32_t pad;
+  int64_t items;           /* (frame, Gaussian) pairs */
+  int64_t gaussians;       /* Gaussians over all scenes */
+  int64_t tiles;           /* tiles over all frames */
+  int64_t pixels;          /* pixels over all frames */
+  int64_t capacity;        /* tile instances (K) */
+} W2WCaps;
 
 int w2w_version(void);
 
@@ -114,6 +129,41 @@ int w2w_backward(const W2WFrame* f, const W2WWorkspace* ws, const float* dL_dcol
                  const float* dL_ddepth, const float* dL_dalpha, float* dL_dmeans2D,
                  float* dL_dcolors, float* dL_dopacity, float* dL_dmeans3D, float* dL_dcov3D,
                  float* dL_dsh, float* dL_dscales, float* dL_drotations, void* stream);
+
+/* ---- batched form: many scenes and views in one launch per stage ----
+ * A batch table (device memory, w2w_batch_table_bytes bytes, filled on the
+ * host by w2w_batch_encode and copied to the device by the caller) describes
+ * scenes (Gaussian counts, views, SH stride/degree) and frames (camera, size,
+ * flags; flag bit 0 = this frame takes part in the backward).  Gaussians of
+ * all scenes are packed back to back in the parameter arrays; SH blocks too
+ * (scene k's block starts where scene k-1's ends).  Work items are (frame,
+ * Gaussian) pairs, Gaussian-major within a scene: item = item_off(scene) +
+ * g * views + v.  Per-frame outputs are packed in frame order: colour
+ * [3,H,W] blocks, and depth / alpha [H,W] blocks.
+ * totals (optional, 6 int64): Gaussians, items, frames, tiles, pixels, SH
+ * floats of the encoded batch. */
+size_t w2w_batch_table_bytes(int32_t scenes_cap, int32_t frames_cap);
+int w2w_batch_encode(void* host_table, size_t table_bytes_avail, int32_t scenes_cap,
+                     int32_t frames_cap, int32_t tile, int32_t n_scenes, const int64_t* scene_n,
+                     const int32_t* scene_views, const int32_t* scene_M, const int32_t* scene_D,
+                     const float* frame_view, const float* frame_proj, const float* frame_campos,
+                     const float* frame_bg, const float* frame_focal, const int32_t* frame_wh,
+                     const uint32_t* frame_flags, int64_t* totals);
+int w2w_batch_workspace_layout(const W2WCaps* caps, W2WLayout* out);
+/* radii per item. */
+int w2w_forward_batch(const void* table, const W2WCaps* caps, const W2WWorkspace* ws,
+                      const float* means, const float* scales, const float* rotations,
+                      const float* opacities, const float* sh, float* out_color, float* out_depth,
+                      float* out_alpha, int32_t* out_radii, void* stream);
+/* dL_dmeans2D and dL_dcolors per item [3]; the other gradients per Gaussian,
+ * summed over the views of its scene (dL_dsh packed like sh).  dL_ddepth and
+ * dL_dalpha may be NULL (zero). */
+int w2w_backward_batch(const void* table, const W2WCaps* caps, const W2WWorkspace* ws,
+                       const float* means, const float* scales, const float* rotations,
+                       const float* sh, const float* dL_dcolor, const float* dL_ddepth,
+                       const float* dL_dalpha, float* dL_dmeans2D, float* dL_dcolors,
+                       float* dL_dopacity, float* dL_dmeans3D, float* dL_dcov3D, float* dL_dsh,
+                       float* dL_dscales, float* dL_drotations, void* stream);
 
 /* 64-bit keys (tile << 32 | depth bits) of the sorted instances, i.e. the
  * reference's sorted keysArray (prepareSort.cu:431-434), for parity checks. */

@@ -1,0 +1,136 @@
+"""Batched form: one launch per stage over several scenes and views must give
+exactly the per-frame results of the single-view path (same kernels, frame
+table indirection), and a scene must render identically alone and in a mix."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import gpu_camera, scene_np
+from word2world_amd import batch, raster
+from word2world_amd.camera import orbit_camera
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(dev):
+    specs = [(800, 0, 11, [(96, 80, 10.0), ]),
+             (1500, 3, 12, [(64, 64, 40.0), (100, 120, 200.0)]),
+             (400, 1, 13, [(48, 48, 0.0), (130, 97, 95.0), (64, 64, 300.0)])]
+    scenes, arrays, cams = [], [], []
+    for n, deg, seed, views in specs:
+        arrs = scene_np(n, deg, seed, 2.0)
+        arrays.append(arrs)
+        fr, cs = [], []
+        for (w, h, az) in views:
+            cam = orbit_camera(az, 10.0, 2.5, w, h)
+            cs.append(cam)
+            fr.append(batch.frame_from_camera(cam))
+        cams.append(cs)
+        scenes.append(batch.SceneSpec(n, deg, arrs[4].shape[1], fr))
+    cat = lambda i: torch.from_numpy(np.concatenate([a[i].reshape(a[i].shape[0], -1) for a in arrays])).to(dev)
+    means, scales, rots = cat(0), cat(1), cat(2)
+    opac = torch.from_numpy(np.concatenate([a[3].reshape(-1) for a in arrays])).to(dev)
+    sh = torch.from_numpy(np.concatenate([a[4].reshape(-1) for a in arrays])).to(dev)
+    return scenes, arrays, cams, (means.contiguous(), scales.contiguous(), rots.contiguous(), opac, sh)
+
+
+def _run_batch(scenes, packed, dev, capacity=200000):
+    caps = batch.caps_for(scenes, 16, capacity)
+    ws = batch.BatchWorkspace(caps, dev)
+    ws.load_table(scenes)
+    batch.forward(ws, *packed)
+    torch.cuda.synchronize()
+    return ws
+
+
+def test_batch_forward_equals_single_view(gpu):
+    scenes, arrays, cams, packed = _setup(gpu)
+    ws = _run_batch(scenes, packed, gpu)
+    K_sum = 0
+    views = ws.frame_views(scenes)
+    fi = 0
+    radii = ws.radii.cpu().numpy()
+    item = 0
+    for s, arrs, cs in zip(scenes, arrays, cams):
+        d = [torch.from_numpy(np.ascontiguousarray(a)).to(gpu) for a in arrs]
+        per_view_radii = []
+        for cam in cs:
+            print(f"frame {fi}: {cam.width}x{cam.height}", flush=True)
+            K, color, depth, alpha, r, wsp = raster.forward_checked(*d, s.sh_degree, gpu_camera(cam, gpu), 16)
+            K_sum += K
+            bc, bd, ba = views[fi]
+            assert torch.equal(bc, color) and torch.equal(bd, depth) and torch.equal(ba, alpha)
+            per_view_radii.append(r.cpu().numpy())
+            fi += 1
+        V = len(cs)
+        got = radii[item:item + s.n * V].reshape(s.n, V)
+        for v in range(V):
+            np.testing.assert_array_equal(got[:, v], per_view_radii[v])
+        item += s.n * V
+    assert int(ws.status()[0]) == K_sum
+
+
+def test_batch_backward_equals_sum_of_views(gpu):
+    scenes, arrays, cams, packed = _setup(gpu)
+    ws = _run_batch(scenes, packed, gpu)
+    torch.manual_seed(0)
+    dLdc = torch.randn_like(ws.color)
+    sh_floats = int(ws.totals[5])
+    grads = batch.BatchGrads(ws.caps, sh_floats, gpu)
+    batch.backward(ws, grads, packed[0], packed[1], packed[2], packed[4], dLdc)
+    torch.cuda.synchronize()
+    g_off, sh_off, pix, item = 0, 0, 0, 0
+    for s, arrs, cs in zip(scenes, arrays, cams):
+        d = [torch.from_numpy(np.ascontiguousarray(a)).to(gpu) for a in arrs]
+        acc = None
+        for v, cam in enumerate(cs):
+            gcam = gpu_camera(cam, gpu)
+            K, color, depth, alpha, r, wsp = raster.forward_checked(*d, s.sh_degree, gcam, 16)
+            hw = cam.width * cam.height
+            gC = dLdc[3 * pix:3 * (pix + hw)].view(3, cam.height, cam.width).contiguous()
+            z = torch.zeros(cam.height, cam.width, device=gpu)
+            out = raster.backward(*d, s.sh_degree, gcam, 16, wsp, gC, z, z)
+            # per-item means2D of this view
+            m2 = grads.means2D[item:item + s.n * len(cs)].view(s.n, len(cs), 3)[:, v]
+            np.testing.assert_allclose(m2.cpu().numpy(), out[0].cpu().numpy(), rtol=1e-5, atol=1e-9)
+            vals = [out[2].view(-1), out[3], out[4], out[5].reshape(-1), out[6], out[7]]
+            acc = vals if acc is None else [a + b for a, b in zip(acc, vals)]
+            pix += hw
+        n, ns = s.n, s.n * s.n_coeffs * 3
+        got = [grads.opacity[g_off:g_off + n], grads.means3D[g_off:g_off + n],
+               grads.cov3D[g_off:g_off + n], grads.sh[sh_off:sh_off + ns],
+               grads.scales[g_off:g_off + n], grads.rotations[g_off:g_off + n]]
+        for name, a, b in zip(["opacity", "means3D", "cov3D", "sh", "scales", "rotations"], got, acc):
+            a, b = a.cpu().numpy(), b.cpu().numpy()
+            err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+            print(name, err, flush=True)
+            assert err < 1e-5, (name, err)
+        g_off += n
+        sh_off += ns
+        item += s.n * len(cs)
+
+
+def test_scene_alone_and_in_the_mix(gpu):
+    scenes, arrays, cams, packed = _setup(gpu)
+    ws = _run_batch(scenes, packed, gpu)
+    mix = [tuple(t.clone() for t in v) for v in ws.frame_views(scenes)]
+    # scene 1 alone
+    s1 = [scenes[1]]
+    n0 = scenes[0].n
+    n1 = scenes[1].n
+    sh0 = scenes[0].n * scenes[0].n_coeffs * 3
+    sh1 = scenes[1].n * scenes[1].n_coeffs * 3
+    p1 = (packed[0][n0:n0 + n1].contiguous(), packed[1][n0:n0 + n1].contiguous(),
+          packed[2][n0:n0 + n1].contiguous(), packed[3][n0:n0 + n1].contiguous(),
+          packed[4][sh0:sh0 + sh1].contiguous())
+    ws1 = _run_batch(s1, p1, gpu)
+    alone = ws1.frame_views(s1)
+    for (c, d, a), (mc, md, ma) in zip(alone, mix[1:3]):
+        assert torch.equal(c, mc) and torch.equal(d, md) and torch.equal(a, ma)
+
+
+def test_capacity_overflow_is_reported(gpu):
+    scenes, arrays, cams, packed = _setup(gpu)
+    ws = _run_batch(scenes, packed, gpu, capacity=1000)
+    K, ovf = (int(v) for v in ws.status().cpu())
+    assert ovf == 1 and K > 1000

@@ -36,11 +36,18 @@ class W2WWorkspace(ctypes.Structure):
 class W2WLayout(ctypes.Structure):
     _fields_ = [(n, ctypes.c_size_t) for n in (
         "geom_bytes", "binning_bytes", "image_bytes", "splats", "radii", "rect", "counts", "perm",
-        "offsets", "gacc", "sorted_ids", "sorted_tiles", "ranges", "n_contrib", "final_T")] + [
-        ("tiles_x", ctypes.c_int32), ("tiles_y", ctypes.c_int32)]
+        "offsets", "gacc", "sorted_ids", "sorted_tiles", "ranges", "n_contrib", "final_T",
+        "item_frame", "table")]
 
 
-EXPORTS = ["w2w_version", "w2w_workspace_layout", "w2w_forward", "w2w_backward", "w2w_sorted_keys",
+class W2WCaps(ctypes.Structure):
+    _fields_ = [("scenes", ctypes.c_int32), ("frames", ctypes.c_int32), ("tile", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("items", ctypes.c_int64), ("gaussians", ctypes.c_int64),
+                ("tiles", ctypes.c_int64), ("pixels", ctypes.c_int64), ("capacity", ctypes.c_int64)]
+
+
+EXPORTS = ["w2w_version", "w2w_workspace_layout", "w2w_batch_workspace_layout",
+           "w2w_batch_table_bytes", "w2w_batch_encode", "w2w_forward_batch", "w2w_backward_batch", "w2w_forward", "w2w_backward", "w2w_sorted_keys",
            "w2w_sort_temp_bytes", "w2w_sort_pairs_u64", "w2w_scan_temp_bytes",
            "w2w_exclusive_scan_u32", "w2w_prof_enable", "w2w_prof_reset", "w2w_prof_read", "w2w_stats_enable", "w2w_adam_step"]
 
@@ -65,6 +72,17 @@ def load(path: str = LIB_PATH):
     L.w2w_version.argtypes = []
     L.w2w_workspace_layout.restype = ctypes.c_int
     L.w2w_workspace_layout.argtypes = [ctypes.c_int32] * 4 + [ctypes.c_int64, P(W2WLayout)]
+    L.w2w_batch_workspace_layout.restype = ctypes.c_int
+    L.w2w_batch_workspace_layout.argtypes = [P(W2WCaps), P(W2WLayout)]
+    L.w2w_batch_table_bytes.restype = ctypes.c_size_t
+    L.w2w_batch_table_bytes.argtypes = [ctypes.c_int32, ctypes.c_int32]
+    L.w2w_batch_encode.restype = ctypes.c_int
+    L.w2w_batch_encode.argtypes = [_vp, ctypes.c_size_t, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                   ctypes.c_int32] + [_vp] * 11 + [_vp]
+    L.w2w_forward_batch.restype = ctypes.c_int
+    L.w2w_forward_batch.argtypes = [_vp, P(W2WCaps), P(W2WWorkspace)] + [_vp] * 10
+    L.w2w_backward_batch.restype = ctypes.c_int
+    L.w2w_backward_batch.argtypes = [_vp, P(W2WCaps), P(W2WWorkspace)] + [_vp] * 16
     L.w2w_forward.restype = ctypes.c_int
     L.w2w_forward.argtypes = [P(W2WFrame), P(W2WWorkspace), _vp, _vp, _vp, _vp, _vp]
     L.w2w_backward.restype = ctypes.c_int
@@ -104,6 +122,13 @@ def layout(P: int, W: int, H: int, tile: int, capacity: int) -> W2WLayout:
     out = W2WLayout()
     check(load().w2w_workspace_layout(P, W, H, tile, capacity, ctypes.byref(out)),
           "w2w_workspace_layout")
+    return out
+
+
+def batch_layout(caps: W2WCaps) -> W2WLayout:
+    out = W2WLayout()
+    check(load().w2w_batch_workspace_layout(ctypes.byref(caps), ctypes.byref(out)),
+          "w2w_batch_workspace_layout")
     return out
 
 

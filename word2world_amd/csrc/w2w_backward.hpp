@@ -45,19 +45,18 @@ W2W_DEV float lane63(float v) {
 }
 
 struct BlendBwdArgs {
+  const void* table;
+  uint32_t scenes_cap, tiles_cap, items_cap;
   const uint2* ranges;
   const uint32_t* ids;
   const Splat* splats;
-  const float* bg;
-  int W, H, tiles_x, T;
-  uint32_t P;
   unsigned long long* stats;  // optional [3]: evaluated pairs, contributing pairs, loaded instances
   const uint32_t* n_contrib;
   const float* final_T;
-  const float* dL_dcolor;   // [3,H,W]
-  const float* dL_dalpha;   // [H,W]
-  const float* dL_ddepth;   // [H,W]
-  float* gacc;              // [P][GA_STRIDE], pre-zeroed
+  const float* dL_dcolor;   // per frame [3,H,W] at 3*pix_off
+  const float* dL_dalpha;   // per frame [H,W] at pix_off; may be null (zero)
+  const float* dL_ddepth;   // per frame [H,W] at pix_off; may be null (zero)
+  float* gacc;              // [items][GA_STRIDE], pre-zeroed
 };
 
 template <int TS>
@@ -66,29 +65,38 @@ __global__ __launch_bounds__(TS* TS) void k_blend_bwd(BlendBwdArgs a) {
   __shared__ Splat s_sp[B];
   __shared__ uint32_t s_id[B];
   __shared__ uint32_t s_max;
-  const int tile = xcd_tile(blockIdx.x, a.T);
-  if (tile >= a.T) return;
+  const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
+  const uint32_t Tn = min(hdr->n_tiles, a.tiles_cap);
+  const uint32_t P = min(hdr->n_items, a.items_cap);
+  const uint32_t tile = xcd_tile(blockIdx.x, a.tiles_cap);
+  if (tile >= Tn) return;
+  const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
+  const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, tile)];
+  if (!(fd.flags & FRAME_BACKWARD)) return;
   const int tid = threadIdx.x, lane = tid & 63;
-  const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+  const int lt = (int)(tile - fd.tile_off);
+  const int tx = lt % fd.tiles_x, ty = lt / fd.tiles_x;
   const int px = tx * TS + (tid % TS), py = ty * TS + (tid / TS);
-  const bool inside = px < a.W && py < a.H;
+  const bool inside = px < fd.W && py < fd.H;
   const float pxf = (float)px, pyf = (float)py;
   const uint2 range = a.ranges[tile];
-  const size_t pid = (size_t)py * a.W + px;
-  const size_t HW = (size_t)a.W * a.H;
+  const size_t lp = (size_t)py * fd.W + px;
+  const size_t pid = fd.pix_off + lp;
+  const size_t HW = (size_t)fd.W * fd.H;
 
   const int last = inside ? (int)a.n_contrib[pid] : 0;
   const float Tf = inside ? a.final_T[pid] : 0.0f;
   float gC0 = 0.0f, gC1 = 0.0f, gC2 = 0.0f, gA = 0.0f, gD = 0.0f;
   if (inside) {
-    gC0 = a.dL_dcolor[pid];
-    gC1 = a.dL_dcolor[HW + pid];
-    gC2 = a.dL_dcolor[2 * HW + pid];
-    gA = a.dL_dalpha[pid];
-    gD = a.dL_ddepth[pid];
+    const float* gcol = a.dL_dcolor + 3 * fd.pix_off;
+    gC0 = gcol[lp];
+    gC1 = gcol[HW + lp];
+    gC2 = gcol[2 * HW + lp];
+    gA = a.dL_dalpha ? a.dL_dalpha[pid] : 0.0f;
+    gD = a.dL_ddepth ? a.dL_ddepth[pid] : 0.0f;
   }
-  const float bgdot = (a.bg[0] * gC0 + a.bg[1] * gC1) + a.bg[2] * gC2;
-  const float hw = 0.5f * (float)a.W, hh = 0.5f * (float)a.H;
+  const float bgdot = (fd.bg[0] * gC0 + fd.bg[1] * gC1) + fd.bg[2] * gC2;
+  const float hw = 0.5f * (float)fd.W, hh = 0.5f * (float)fd.H;
   const float a_min = 0x1.010102p-8f;
 
   if (tid == 0) s_max = 0;
@@ -113,7 +121,7 @@ __global__ __launch_bounds__(TS* TS) void k_blend_bwd(BlendBwdArgs a) {
     n_loaded += bend - bstart;
     if (k < bend) {
       uint32_t id = a.ids[range.x + k];
-      id = id < a.P ? id : 0u;
+      id = id < P ? id : 0u;
       s_id[tid] = id;
       s_sp[tid] = a.splats[id];
     }
@@ -211,54 +219,68 @@ __global__ __launch_bounds__(TS* TS) void k_blend_bwd(BlendBwdArgs a) {
 }
 
 struct PreBwdArgs {
-  int P, M, D, W, H;
-  float fx, fy;
-  const float* means;
-  const float* scales;
-  const float* rots;
-  const float* sh;
-  const float* view;
-  const float* proj;
-  const float* campos;
-  const int32_t* radii;
-  const float* gacc;
-  float* dL_dmeans2D;   // [P,3]
-  float* dL_dcolors;    // [P,3]
-  float* dL_dopacity;   // [P]
-  float* dL_dmeans3D;   // [P,3]
-  float* dL_dcov3D;     // [P,6]
-  float* dL_dsh;        // [P,M,3]
-  float* dL_dscales;    // [P,3]
-  float* dL_drots;      // [P,4]
+  const void* table;
+  uint32_t scenes_cap, items_cap, gaussians_cap;
+  const float* means;    // packed [G,3]
+  const float* scales;   // [G,3] activated
+  const float* rots;     // [G,4] normalised
+  const float* sh;       // packed SH blocks
+  const int32_t* radii;  // per item
+  const float* gacc;     // per item
+  float* dL_dmeans2D;    // per item [3]  (NDC units, z = 0)
+  float* dL_dcolors;     // per item [3]
+  float* dL_dopacity;    // per Gaussian
+  float* dL_dmeans3D;    // per Gaussian [3]
+  float* dL_dcov3D;      // per Gaussian [6]
+  float* dL_dsh;         // packed like sh
+  float* dL_dscales;     // per Gaussian [3]
+  float* dL_drots;       // per Gaussian [4]
 };
 
+// One thread per Gaussian: loops over the views of its scene in this batch,
+// recomputes each view's projection and sums the chain rule in registers, so
+// every per-Gaussian gradient is written once (no atomics).
 __global__ __launch_bounds__(256) void k_preprocess_bwd(PreBwdArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.P) return;
-  const float* ga = a.gacc + (size_t)i * GA_STRIDE;
-  float gm[3] = {0.0f, 0.0f, 0.0f};
-  float dcov[6] = {0, 0, 0, 0, 0, 0};
-  float dsc[3] = {0, 0, 0};
-  float dq[4] = {0, 0, 0, 0};
-  float* dsh = a.dL_dsh + (size_t)i * a.M * 3;
-  const bool vis = a.radii[i] > 0;
-  a.dL_dmeans2D[3 * i] = vis ? ga[GA_MX] : 0.0f;
-  a.dL_dmeans2D[3 * i + 1] = vis ? ga[GA_MY] : 0.0f;
-  a.dL_dmeans2D[3 * i + 2] = 0.0f;
-  a.dL_dcolors[3 * i] = vis ? ga[GA_R] : 0.0f;
-  a.dL_dcolors[3 * i + 1] = vis ? ga[GA_G] : 0.0f;
-  a.dL_dcolors[3 * i + 2] = vis ? ga[GA_B] : 0.0f;
-  a.dL_dopacity[i] = vis ? ga[GA_OP] : 0.0f;
-  for (int k = 0; k < a.M * 3; k++) dsh[k] = 0.0f;
-  if (vis) {
-    Cam c;
-    load_cam(c, a.view, a.proj, a.campos, a.W, a.H, a.fx, a.fy);
-    const float p[3] = {a.means[3 * i], a.means[3 * i + 1], a.means[3 * i + 2]};
-    const float s[3] = {a.scales[3 * i], a.scales[3 * i + 1], a.scales[3 * i + 2]};
-    const float q[4] = {a.rots[4 * i], a.rots[4 * i + 1], a.rots[4 * i + 2], a.rots[4 * i + 3]};
-    Geom g;
-    geometry(c, p, s, q, g);
-    if (g.visible) {
+  const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
+  const uint32_t G = min(hdr->n_gaussians, a.gaussians_cap);
+  const uint32_t n_scenes = min(hdr->n_scenes, a.scenes_cap);
+  const SceneDesc* scenes = table_scenes(a.table);
+  const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t gg = blockIdx.x * blockDim.x + threadIdx.x; gg < G; gg += stride) {
+    const uint32_t si = upper_index(n_scenes, gg, [&](uint32_t k) { return scenes[k].g_off; });
+    const SceneDesc& sd = scenes[si];
+    const uint32_t g = gg - sd.g_off;
+    float gm[3] = {0.0f, 0.0f, 0.0f};
+    float dcov[6] = {0, 0, 0, 0, 0, 0};
+    float dsc[3] = {0, 0, 0};
+    float dq[4] = {0, 0, 0, 0};
+    float gop = 0.0f;
+    float* dsh = a.dL_dsh + sd.sh_off + (size_t)g * sd.M * 3;
+    for (int k = 0; k < sd.M * 3; k++) dsh[k] = 0.0f;
+    const float p[3] = {a.means[3 * gg], a.means[3 * gg + 1], a.means[3 * gg + 2]};
+    const float s[3] = {a.scales[3 * gg], a.scales[3 * gg + 1], a.scales[3 * gg + 2]};
+    const float q[4] = {a.rots[4 * gg], a.rots[4 * gg + 1], a.rots[4 * gg + 2], a.rots[4 * gg + 3]};
+    const float* cf = a.sh + sd.sh_off + (size_t)g * sd.M * 3;
+    for (uint32_t v = 0; v < sd.n_views; v++) {
+      const uint32_t i = sd.item_off + g * sd.n_views + v;
+      if (i >= a.items_cap) break;
+      const FrameDesc& fd = frames[sd.frame_off + v];
+      const float* ga = a.gacc + (size_t)i * GA_STRIDE;
+      const bool vis = a.radii[i] > 0 && (fd.flags & FRAME_BACKWARD);
+      a.dL_dmeans2D[3 * (size_t)i] = vis ? ga[GA_MX] : 0.0f;
+      a.dL_dmeans2D[3 * (size_t)i + 1] = vis ? ga[GA_MY] : 0.0f;
+      a.dL_dmeans2D[3 * (size_t)i + 2] = 0.0f;
+      a.dL_dcolors[3 * (size_t)i] = vis ? ga[GA_R] : 0.0f;
+      a.dL_dcolors[3 * (size_t)i + 1] = vis ? ga[GA_G] : 0.0f;
+      a.dL_dcolors[3 * (size_t)i + 2] = vis ? ga[GA_B] : 0.0f;
+      if (!vis) continue;
+      gop += ga[GA_OP];
+      Cam c;
+      load_cam(c, fd);
+      Geom geo;
+      geometry(c, p, s, q, geo);
+      if (!geo.visible) continue;
       const float* V = c.V;
       const float* F = c.F;
       // colour: SH coefficients and view direction
@@ -267,16 +289,15 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreBwdArgs a) {
         float len = __builtin_sqrtf((dx * dx + dy * dy) + dz * dz);
         float x = dx / len, y = dy / len, z = dz / len;
         float Y[16], gx[16], gy[16], gz[16];
-        sh_basis(a.D, x, y, z, Y);
-        sh_basis_grad(a.D, x, y, z, gx, gy, gz);
-        const int nb = (a.D + 1) * (a.D + 1);
-        const float* cf = a.sh + (size_t)i * a.M * 3;
+        sh_basis(sd.D, x, y, z, Y);
+        sh_basis_grad(sd.D, x, y, z, gx, gy, gz);
+        const int nb = (sd.D + 1) * (sd.D + 1);
         const float gr0 = ga[GA_R], gr1 = ga[GA_G], gr2 = ga[GA_B];
         float gdx = 0.0f, gdy = 0.0f, gdz = 0.0f;
         for (int k = 0; k < nb; k++) {
-          dsh[3 * k] = Y[k] * gr0;
-          dsh[3 * k + 1] = Y[k] * gr1;
-          dsh[3 * k + 2] = Y[k] * gr2;
+          dsh[3 * k] += Y[k] * gr0;
+          dsh[3 * k + 1] += Y[k] * gr1;
+          dsh[3 * k + 2] += Y[k] * gr2;
           float dot = (cf[3 * k] * gr0 + cf[3 * k + 1] * gr1) + cf[3 * k + 2] * gr2;
           gdx = __builtin_fmaf(gx[k], dot, gdx);
           gdy = __builtin_fmaf(gy[k], dot, gdy);
@@ -290,80 +311,80 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreBwdArgs a) {
       // NDC mean and depth through the full projection
       {
         float gnx = ga[GA_MX], gny = ga[GA_MY], gnz = ga[GA_DEPTH];
-        float gcx = gnx * g.invW, gcy = gny * g.invW, gcz = gnz * g.invW;
-        float ginvW = (gnx * g.cx + gny * g.cy) + gnz * g.cz;
-        float gcw = -ginvW * g.invW * g.invW;
+        float gcx = gnx * geo.invW, gcy = gny * geo.invW, gcz = gnz * geo.invW;
+        float ginvW = (gnx * geo.cx + gny * geo.cy) + gnz * geo.cz;
+        float gcw = -ginvW * geo.invW * geo.invW;
 #pragma unroll
         for (int j = 0; j < 3; j++)
           gm[j] += ((F[4 * j + 0] * gcx + F[4 * j + 1] * gcy) + F[4 * j + 2] * gcz) + F[4 * j + 3] * gcw;
       }
       // conic -> cov2D
       const float gca = ga[GA_CA], gcb = ga[GA_CB], gcc = ga[GA_CC];
-      const float ca = g.ca, cb = g.cb, cc = g.cc, h = 0.5f * gcb;
+      const float ca = geo.ca, cb = geo.cb, cc = geo.cc, h = 0.5f * gcb;
       const float q00 = ca * gca + cb * h, q01 = ca * h + cb * gcc;
       const float q10 = cb * gca + cc * h, q11 = cb * h + cc * gcc;
       const float g_cxx = -(q00 * ca + q01 * cb);
       const float g_cxy = -2.0f * (q00 * cb + q01 * cc);
       const float g_cyy = -(q10 * cb + q11 * cc);
       // cov2D -> Sigma and J
-      const float* S6 = g.Sig;
-      float v[3], w[3];
-      v[0] = (S6[0] * g.jx[0] + S6[1] * g.jx[1]) + S6[2] * g.jx[2];
-      v[1] = (S6[1] * g.jx[0] + S6[3] * g.jx[1]) + S6[4] * g.jx[2];
-      v[2] = (S6[2] * g.jx[0] + S6[4] * g.jx[1]) + S6[5] * g.jx[2];
-      w[0] = (S6[0] * g.jy[0] + S6[1] * g.jy[1]) + S6[2] * g.jy[2];
-      w[1] = (S6[1] * g.jy[0] + S6[3] * g.jy[1]) + S6[4] * g.jy[2];
-      w[2] = (S6[2] * g.jy[0] + S6[4] * g.jy[1]) + S6[5] * g.jy[2];
+      const float* S6 = geo.Sig;
+      float vv[3], ww[3];
+      vv[0] = (S6[0] * geo.jx[0] + S6[1] * geo.jx[1]) + S6[2] * geo.jx[2];
+      vv[1] = (S6[1] * geo.jx[0] + S6[3] * geo.jx[1]) + S6[4] * geo.jx[2];
+      vv[2] = (S6[2] * geo.jx[0] + S6[4] * geo.jx[1]) + S6[5] * geo.jx[2];
+      ww[0] = (S6[0] * geo.jy[0] + S6[1] * geo.jy[1]) + S6[2] * geo.jy[2];
+      ww[1] = (S6[1] * geo.jy[0] + S6[3] * geo.jy[1]) + S6[4] * geo.jy[2];
+      ww[2] = (S6[2] * geo.jy[0] + S6[4] * geo.jy[1]) + S6[5] * geo.jy[2];
       const float hxy = 0.5f * g_cxy;
       float GS[9];
 #pragma unroll
       for (int k = 0; k < 3; k++)
 #pragma unroll
         for (int l = 0; l < 3; l++)
-          GS[3 * k + l] = (g_cxx * g.jx[k] * g.jx[l] + hxy * (g.jx[k] * g.jy[l] + g.jy[k] * g.jx[l])) +
-                          g_cyy * g.jy[k] * g.jy[l];
-      dcov[0] = GS[0]; dcov[1] = 2.0f * GS[1]; dcov[2] = 2.0f * GS[2];
-      dcov[3] = GS[4]; dcov[4] = 2.0f * GS[5]; dcov[5] = GS[8];
+          GS[3 * k + l] = (g_cxx * geo.jx[k] * geo.jx[l] + hxy * (geo.jx[k] * geo.jy[l] + geo.jy[k] * geo.jx[l])) +
+                          g_cyy * geo.jy[k] * geo.jy[l];
+      dcov[0] += GS[0]; dcov[1] += 2.0f * GS[1]; dcov[2] += 2.0f * GS[2];
+      dcov[3] += GS[4]; dcov[4] += 2.0f * GS[5]; dcov[5] += GS[8];
       float gjx[3], gjy[3];
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        gjx[k] = 2.0f * g_cxx * v[k] + g_cxy * w[k];
-        gjy[k] = g_cxy * v[k] + 2.0f * g_cyy * w[k];
+        gjx[k] = 2.0f * g_cxx * vv[k] + g_cxy * ww[k];
+        gjy[k] = g_cxy * vv[k] + 2.0f * g_cyy * ww[k];
       }
       // J -> view-space point
-      const float zi = 1.0f / g.vz, zi2 = zi * zi;
+      const float zi = 1.0f / geo.vz, zi2 = zi * zi;
       float g_zi = 0.0f, g_zi2 = 0.0f, g_txc = 0.0f, g_tyc = 0.0f;
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         const float right = V[4 * k + 0], up = V[4 * k + 1], fwd = V[4 * k + 2];
         g_zi += gjx[k] * c.fx * right + gjy[k] * c.fy * up;
-        g_zi2 -= gjx[k] * c.fx * g.txc * fwd + gjy[k] * c.fy * g.tyc * fwd;
+        g_zi2 -= gjx[k] * c.fx * geo.txc * fwd + gjy[k] * c.fy * geo.tyc * fwd;
         g_txc -= gjx[k] * c.fx * fwd * zi2;
         g_tyc -= gjy[k] * c.fy * fwd * zi2;
       }
       float g_vz = g_zi * (-zi * zi) + g_zi2 * (-2.0f * zi2 * zi);
-      const float g_rxc = g_txc * g.vz, g_ryc = g_tyc * g.vz;
-      g_vz += g_txc * g.rxc + g_tyc * g.ryc;
+      const float g_rxc = g_txc * geo.vz, g_ryc = g_tyc * geo.vz;
+      g_vz += g_txc * geo.rxc + g_tyc * geo.ryc;
       float g_rx, g_ry;
-      if (g.clamped) {
-        const float s3 = g.s * g.s * g.s;
+      if (geo.clamped) {
+        const float s3 = geo.s * geo.s * geo.s;
         const float lx2 = c.limx * c.limx, ly2 = c.limy * c.limy;
-        g_rx = g_rxc * (g.s - s3 * g.rx * g.rx / lx2) + g_ryc * (-s3 * g.rx * g.ry / lx2);
-        g_ry = g_rxc * (-s3 * g.rx * g.ry / ly2) + g_ryc * (g.s - s3 * g.ry * g.ry / ly2);
+        g_rx = g_rxc * (geo.s - s3 * geo.rx * geo.rx / lx2) + g_ryc * (-s3 * geo.rx * geo.ry / lx2);
+        g_ry = g_rxc * (-s3 * geo.rx * geo.ry / ly2) + g_ryc * (geo.s - s3 * geo.ry * geo.ry / ly2);
       } else {
         g_rx = g_rxc;
         g_ry = g_ryc;
       }
-      const float g_vx = g_rx / g.vz, g_vy = g_ry / g.vz;
-      g_vz -= (g_rx * g.rx + g_ry * g.ry) / g.vz;
-      const float g_tx = g_vx / g.tw, g_ty = g_vy / g.tw, g_tz = g_vz / g.tw;
-      const float g_tw = -((g_vx * g.vx + g_vy * g.vy) + g_vz * g.vz) / g.tw;
+      const float g_vx = g_rx / geo.vz, g_vy = g_ry / geo.vz;
+      g_vz -= (g_rx * geo.rx + g_ry * geo.ry) / geo.vz;
+      const float g_tx = g_vx / geo.tw, g_ty = g_vy / geo.tw, g_tz = g_vz / geo.tw;
+      const float g_tw = -((g_vx * geo.vx + g_vy * geo.vy) + g_vz * geo.vz) / geo.tw;
 #pragma unroll
       for (int j = 0; j < 3; j++)
         gm[j] += ((V[4 * j + 0] * g_tx + V[4 * j + 1] * g_ty) + V[4 * j + 2] * g_tz) + V[4 * j + 3] * g_tw;
       // Sigma -> scales and quaternion
-      const float* Mm = g.M;
-      const float* R = g.R;
+      const float* Mm = geo.M;
+      const float* R = geo.R;
       float gM[9], gR[9];
 #pragma unroll
       for (int r = 0; r < 3; r++)
@@ -376,23 +397,24 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreBwdArgs a) {
         for (int col = 0; col < 3; col++) gR[3 * r + col] = gM[3 * r + col] * s[col];
 #pragma unroll
       for (int col = 0; col < 3; col++)
-        dsc[col] = (gM[col] * R[col] + gM[3 + col] * R[3 + col]) + gM[6 + col] * R[6 + col];
+        dsc[col] += (gM[col] * R[col] + gM[3 + col] * R[3 + col]) + gM[6 + col] * R[6 + col];
       const float qw = q[0], qx = q[1], qy = q[2], qz = q[3];
-      dq[0] = 2.0f * (((-qz * gR[1] + qy * gR[2]) + (qz * gR[3] - qx * gR[5])) + (-qy * gR[6] + qx * gR[7]));
-      dq[1] = 2.0f * ((((qy * gR[1] + qz * gR[2]) + (qy * gR[3] - 2.0f * qx * gR[4])) + (-qw * gR[5] + qz * gR[6])) + (qw * gR[7] - 2.0f * qx * gR[8]));
-      dq[2] = 2.0f * ((((-2.0f * qy * gR[0] + qx * gR[1]) + (qw * gR[2] + qx * gR[3])) + (qz * gR[5] - qw * gR[6])) + (qz * gR[7] - 2.0f * qy * gR[8]));
-      dq[3] = 2.0f * ((((-2.0f * qz * gR[0] - qw * gR[1]) + (qx * gR[2] + qw * gR[3])) + (-2.0f * qz * gR[4] + qy * gR[5])) + (qx * gR[6] + qy * gR[7]));
+      dq[0] += 2.0f * (((-qz * gR[1] + qy * gR[2]) + (qz * gR[3] - qx * gR[5])) + (-qy * gR[6] + qx * gR[7]));
+      dq[1] += 2.0f * ((((qy * gR[1] + qz * gR[2]) + (qy * gR[3] - 2.0f * qx * gR[4])) + (-qw * gR[5] + qz * gR[6])) + (qw * gR[7] - 2.0f * qx * gR[8]));
+      dq[2] += 2.0f * ((((-2.0f * qy * gR[0] + qx * gR[1]) + (qw * gR[2] + qx * gR[3])) + (qz * gR[5] - qw * gR[6])) + (qz * gR[7] - 2.0f * qy * gR[8]));
+      dq[3] += 2.0f * ((((-2.0f * qz * gR[0] - qw * gR[1]) + (qx * gR[2] + qw * gR[3])) + (-2.0f * qz * gR[4] + qy * gR[5])) + (qx * gR[6] + qy * gR[7]));
     }
+    a.dL_dopacity[gg] = gop;
+    a.dL_dmeans3D[3 * (size_t)gg] = gm[0];
+    a.dL_dmeans3D[3 * (size_t)gg + 1] = gm[1];
+    a.dL_dmeans3D[3 * (size_t)gg + 2] = gm[2];
+#pragma unroll
+    for (int k = 0; k < 6; k++) a.dL_dcov3D[6 * (size_t)gg + k] = dcov[k];
+#pragma unroll
+    for (int k = 0; k < 3; k++) a.dL_dscales[3 * (size_t)gg + k] = dsc[k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) a.dL_drots[4 * (size_t)gg + k] = dq[k];
   }
-  a.dL_dmeans3D[3 * i] = gm[0];
-  a.dL_dmeans3D[3 * i + 1] = gm[1];
-  a.dL_dmeans3D[3 * i + 2] = gm[2];
-#pragma unroll
-  for (int k = 0; k < 6; k++) a.dL_dcov3D[6 * i + k] = dcov[k];
-#pragma unroll
-  for (int k = 0; k < 3; k++) a.dL_dscales[3 * i + k] = dsc[k];
-#pragma unroll
-  for (int k = 0; k < 4; k++) a.dL_drots[4 * i + k] = dq[k];
 }
 
 }  // namespace w2w

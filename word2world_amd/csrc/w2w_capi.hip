@@ -3,11 +3,14 @@
 // Host side only sequences launches on the caller's stream: it carves the
 // caller-owned workspace, never allocates, frees or synchronises, and never
 // reads K back (kernels that depend on K read it from the device status
-// word), so a whole forward or backward can be captured into a HIP graph.
+// word).  Grids are sized by the workspace capacities and every data-dependent
+// count is read on the device, so a whole forward or backward can be captured
+// into a HIP graph and replayed for any batch that fits.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "../../include/w2w_raster.h"
 #include "w2w_adam.hpp"
@@ -23,89 +26,6 @@ namespace {
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
-
-struct Plan {
-  W2WLayout L;
-  int32_t P, W, H, ts;
-  int64_t cap;
-  uint32_t T, dtiles, stiles, ctiles;
-  int tpasses, tlast_bits;
-  size_t status, dkey0, dkey1, id1, dhist, dstatus, slb;  // geom
-  size_t tkey0, tkey1, tid0, tid1, thist, tstatus;        // binning
-};
-
-int tile_bits(uint32_t T) {
-  int b = 0;
-  while ((1u << b) < T) b++;
-  return b < 1 ? 1 : b;
-}
-
-bool make_plan(int32_t P, int32_t W, int32_t H, int32_t ts, int64_t cap, Plan& p) {
-  if (P < 0 || W <= 0 || H <= 0 || cap < 0 || cap >= (int64_t)LB_VAL) return false;
-  if (ts != 8 && ts != 16 && ts != 32) return false;
-  if ((int64_t)P >= (int64_t)LB_VAL) return false;
-  p.P = P; p.W = W; p.H = H; p.ts = ts; p.cap = cap;
-  W2WLayout& L = p.L;
-  L.tiles_x = (W + ts - 1) / ts;
-  L.tiles_y = (H + ts - 1) / ts;
-  if (L.tiles_x > 65535 || L.tiles_y > 65535) return false;
-  p.T = (uint32_t)L.tiles_x * (uint32_t)L.tiles_y;
-  p.dtiles = cdiv((uint64_t)P, SORT_TILE);
-  p.stiles = cdiv((uint64_t)P, SCAN_TILE);
-  p.ctiles = cdiv((uint64_t)cap, SORT_TILE);
-  int tb = tile_bits(p.T);
-  p.tpasses = (tb + 7) / 8;
-  p.tlast_bits = tb - 8 * (p.tpasses - 1);
-  size_t o = 0;
-  auto take = [&](size_t bytes) { size_t at = o; o = align_up(o + bytes); return at; };
-  const size_t Pz = (size_t)P;
-  p.status = take(ST_WORDS * 4);
-  L.splats = take(sizeof(Splat) * Pz);
-  L.radii = take(4 * Pz);
-  L.rect = take(8 * Pz);
-  L.counts = take(4 * Pz);
-  p.dkey0 = take(4 * Pz);
-  p.dkey1 = take(4 * Pz);
-  L.perm = take(4 * Pz);
-  p.id1 = take(4 * Pz);
-  L.offsets = take(4 * Pz);
-  p.dhist = take(4 * 4 * RADIX);
-  p.dstatus = take((size_t)4 * p.dtiles * RADIX * 4);
-  p.slb = take((size_t)p.stiles * 4 + 4);
-  L.gacc = take(4 * GA_STRIDE * Pz);
-  L.geom_bytes = o;
-  o = 0;
-  const size_t C = (size_t)cap;
-  p.tkey0 = take(4 * C);
-  p.tkey1 = take(4 * C);
-  p.tid0 = take(4 * C);
-  p.tid1 = take(4 * C);
-  p.thist = take(4 * 4 * RADIX);
-  p.tstatus = take((size_t)p.tpasses * p.ctiles * RADIX * 4);
-  L.binning_bytes = o;
-  L.sorted_tiles = (p.tpasses & 1) ? p.tkey1 : p.tkey0;
-  L.sorted_ids = (p.tpasses & 1) ? p.tid1 : p.tid0;
-  o = 0;
-  L.ranges = take(8 * (size_t)p.T);
-  L.n_contrib = take(4 * (size_t)W * H);
-  L.final_T = take(4 * (size_t)W * H);
-  L.image_bytes = o;
-  return true;
-}
-
-template <typename T>
-T* at(void* base, size_t off) { return reinterpret_cast<T*>(static_cast<char*>(base) + off); }
-
-bool ws_ok(const W2WWorkspace* ws, const Plan& p) {
-  return ws && ws->geom && ws->binning && ws->image && ws->geom_bytes >= p.L.geom_bytes &&
-         ws->binning_bytes >= p.L.binning_bytes && ws->image_bytes >= p.L.image_bytes &&
-         ws->capacity >= p.cap;
-}
-
-int launch_status() {
-  hipError_t e = hipGetLastError();
-  return e == hipSuccess ? W2W_OK : W2W_ERR_HIP;
-}
 
 // ---- optional per-kernel timing with HIP events on the launch stream ----
 // Records an event pair around every launch whose kernel id is enabled in
@@ -145,6 +65,107 @@ void fill_u32(uint32_t* ptr, uint32_t v, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, ptr, v, n);
 }
 
+int launch_status() {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? W2W_OK : W2W_ERR_HIP;
+}
+
+template <typename T>
+T* at(void* base, size_t off) { return reinterpret_cast<T*>(static_cast<char*>(base) + off); }
+
+int tile_bits(uint64_t T) {
+  int b = 0;
+  while ((1ull << b) < T) b++;
+  return b < 1 ? 1 : b;
+}
+
+size_t table_bytes(uint32_t scenes_cap, uint32_t frames_cap) {
+  return TABLE_HEADER_BYTES + (size_t)scenes_cap * sizeof(SceneDesc) +
+         (size_t)frames_cap * sizeof(FrameDesc);
+}
+
+// Workspace plan: a pure function of the capacities.
+struct Plan {
+  W2WLayout L;
+  W2WCaps c;
+  uint32_t dtiles, stiles, ctiles;
+  int tpasses, tlast_bits;
+  size_t status, table, dkey0, dkey1, id1, dhist, dstatus, slb;  // geom
+  size_t tkey0, tkey1, tid0, tid1, thist, tstatus;               // binning
+};
+
+bool make_plan(const W2WCaps& c, Plan& p) {
+  if (c.scenes < 1 || c.frames < 1 || c.items < 0 || c.gaussians < 0 || c.tiles < 1 ||
+      c.pixels < 1 || c.capacity < 0)
+    return false;
+  if (c.tile != 8 && c.tile != 16 && c.tile != 32) return false;
+  if (c.items >= (int64_t)LB_VAL || c.capacity >= (int64_t)LB_VAL || c.tiles >= (int64_t)LB_VAL ||
+      c.gaussians >= (int64_t)LB_VAL)
+    return false;
+  p.c = c;
+  W2WLayout& L = p.L;
+  std::memset(&L, 0, sizeof(L));
+  p.dtiles = cdiv((uint64_t)c.items, SORT_TILE);
+  p.stiles = cdiv((uint64_t)c.items, SCAN_TILE);
+  p.ctiles = cdiv((uint64_t)c.capacity, SORT_TILE);
+  const int tb = tile_bits((uint64_t)c.tiles);
+  p.tpasses = (tb + 7) / 8;
+  p.tlast_bits = tb - 8 * (p.tpasses - 1);
+  size_t o = 0;
+  auto take = [&](size_t bytes) { size_t a = o; o = align_up(o + bytes); return a; };
+  const size_t I = (size_t)c.items;
+  p.status = take(ST_WORDS * 4);
+  p.table = take(table_bytes(1, 1));
+  L.splats = take(sizeof(Splat) * I);
+  L.radii = take(4 * I);
+  L.rect = take(8 * I);
+  L.counts = take(4 * I);
+  p.dkey0 = take(4 * I);
+  p.dkey1 = take(4 * I);
+  L.perm = take(4 * I);
+  p.id1 = take(4 * I);
+  L.offsets = take(4 * I);
+  L.item_frame = take(4 * I);
+  p.dhist = take(4 * 4 * RADIX);
+  p.dstatus = take((size_t)4 * p.dtiles * RADIX * 4);
+  p.slb = take((size_t)p.stiles * 4 + 4);
+  L.gacc = take(4 * GA_STRIDE * I);
+  L.geom_bytes = o;
+  o = 0;
+  const size_t C = (size_t)c.capacity;
+  p.tkey0 = take(4 * C);
+  p.tkey1 = take(4 * C);
+  p.tid0 = take(4 * C);
+  p.tid1 = take(4 * C);
+  p.thist = take(4 * 4 * RADIX);
+  p.tstatus = take((size_t)p.tpasses * p.ctiles * RADIX * 4);
+  L.binning_bytes = o;
+  L.sorted_tiles = (p.tpasses & 1) ? p.tkey1 : p.tkey0;
+  L.sorted_ids = (p.tpasses & 1) ? p.tid1 : p.tid0;
+  o = 0;
+  L.ranges = take(8 * (size_t)c.tiles);
+  L.n_contrib = take(4 * (size_t)c.pixels);
+  L.final_T = take(4 * (size_t)c.pixels);
+  L.image_bytes = o;
+  L.table = p.table;
+  return true;
+}
+
+bool single_caps(int32_t P, int32_t W, int32_t H, int32_t ts, int64_t cap, W2WCaps& c) {
+  if (P < 0 || W <= 0 || H <= 0 || (ts != 8 && ts != 16 && ts != 32)) return false;
+  const int64_t tx = (W + ts - 1) / ts, ty = (H + ts - 1) / ts;
+  if (tx > 65535 || ty > 65535) return false;
+  c.scenes = 1; c.frames = 1; c.tile = ts; c.pad = 0;
+  c.items = P; c.gaussians = P; c.tiles = tx * ty; c.pixels = (int64_t)W * H; c.capacity = cap;
+  return true;
+}
+
+bool ws_ok(const W2WWorkspace* ws, const Plan& p) {
+  return ws && ws->geom && ws->binning && ws->image && ws->geom_bytes >= p.L.geom_bytes &&
+         ws->binning_bytes >= p.L.binning_bytes && ws->image_bytes >= p.L.image_bytes &&
+         ws->capacity >= p.c.capacity;
+}
+
 bool frame_ok(const W2WFrame* f) {
   if (!f) return false;
   if (f->D < 0 || f->D > 3 || f->M < (f->D + 1) * (f->D + 1)) return false;
@@ -154,16 +175,46 @@ bool frame_ok(const W2WFrame* f) {
   return true;
 }
 
+// one-scene, one-frame table written on the device from the caller's
+// device-resident camera (no host read-back)
+__global__ void k_make_single(void* table, const float* __restrict__ view,
+                              const float* __restrict__ proj, const float* __restrict__ campos,
+                              const float* __restrict__ bg, uint32_t P, int32_t M, int32_t D,
+                              int32_t W, int32_t H, int32_t ts, float fx, float fy) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  BatchHeader* h = static_cast<BatchHeader*>(table);
+  SceneDesc* sd = reinterpret_cast<SceneDesc*>(static_cast<char*>(table) + TABLE_HEADER_BYTES);
+  FrameDesc* fd = reinterpret_cast<FrameDesc*>(static_cast<char*>(table) + TABLE_HEADER_BYTES +
+                                               sizeof(SceneDesc));
+  const int32_t tx = (W + ts - 1) / ts, ty = (H + ts - 1) / ts;
+  h->n_scenes = 1; h->n_frames = 1; h->n_items = P; h->n_gaussians = P;
+  h->n_tiles = (uint32_t)(tx * ty); h->pad0 = h->pad1 = h->pad2 = 0;
+  h->n_pixels = (uint64_t)W * H; h->pad3 = 0;
+  sd->g_off = 0; sd->n = P; sd->item_off = 0; sd->frame_off = 0; sd->n_views = 1;
+  sd->M = M; sd->D = D; sd->pad = 0; sd->sh_off = 0; sd->pad2 = 0;
+  for (int k = 0; k < 16; k++) {
+    fd->V[k] = view[k];
+    fd->F[k] = proj[k];
+  }
+  for (int k = 0; k < 3; k++) {
+    fd->cp[k] = campos[k];
+    fd->bg[k] = bg[k];
+  }
+  fd->fx = fx; fd->fy = fy; fd->W = W; fd->H = H; fd->tiles_x = tx; fd->tiles_y = ty;
+  fd->tile_off = 0; fd->scene = 0; fd->view = 0; fd->flags = FRAME_BACKWARD;
+  fd->pix_off = 0; fd->pad = 0;
+}
+
 template <int TS>
-void launch_blend(const BlendArgs& a, hipStream_t s) {
-  const uint32_t grid = 8 * cdiv(a.T, 8);
+void launch_blend(const BlendArgs& a, uint32_t tiles_cap, hipStream_t s) {
+  const uint32_t grid = 8 * cdiv(tiles_cap, 8);
   ProfSpan ps(W2W_KID_BLEND, s);
   hipLaunchKernelGGL(k_blend<TS>, dim3(grid), dim3(TS * TS), 0, s, a);
 }
 
 template <int TS>
-void launch_blend_bwd(const BlendBwdArgs& a, hipStream_t s) {
-  const uint32_t grid = 8 * cdiv(a.T, 8);
+void launch_blend_bwd(const BlendBwdArgs& a, uint32_t tiles_cap, hipStream_t s) {
+  const uint32_t grid = 8 * cdiv(tiles_cap, 8);
   ProfSpan ps(W2W_KID_BLEND_BWD, s);
   hipLaunchKernelGGL(k_blend_bwd<TS>, dim3(grid), dim3(TS * TS), 0, s, a);
 }
@@ -177,9 +228,9 @@ void sort_passes(const KT* kin, const uint32_t* vin, KT* kA, uint32_t* vA, KT* k
   if (ntiles == 0) return;
   const uint32_t hgrid = std::min<uint32_t>(cdiv(n_cap, 256), 2048);
   {
-  ProfSpan ps(kid_hist, s);
-  hipLaunchKernelGGL(k_sort_hist<KT>, dim3(hgrid), dim3(256), 0, s, kin, d_n, n_cap, 0, npasses,
-                     last_bits, hist, status, ntiles, tickets);
+    ProfSpan ps(kid_hist, s);
+    hipLaunchKernelGGL(k_sort_hist<KT>, dim3(hgrid), dim3(256), 0, s, kin, d_n, n_cap, 0, npasses,
+                       last_bits, hist, status, ntiles, tickets);
   }
   const uint32_t pgrid = std::min<uint32_t>(ntiles, 2048);
   const KT* ks = kin;
@@ -202,81 +253,74 @@ void sort_passes(const KT* kin, const uint32_t* vin, KT* kA, uint32_t* vA, KT* k
   }
 }
 
-}  // namespace
+struct Params {
+  const float* means;
+  const float* scales;
+  const float* rots;
+  const float* opac;
+  const float* sh;
+};
 
-extern "C" {
-
-int w2w_version(void) { return W2W_ABI_VERSION; }
-
-int w2w_workspace_layout(int32_t P, int32_t W, int32_t H, int32_t tile, int64_t capacity,
-                         W2WLayout* out) {
-  Plan p;
-  if (!out || !make_plan(P, W, H, tile, capacity, p)) return W2W_ERR_INVALID;
-  *out = p.L;
-  return W2W_OK;
-}
-
-int w2w_forward(const W2WFrame* f, const W2WWorkspace* ws, float* out_color, float* out_depth,
-                float* out_alpha, int32_t* out_radii, void* stream) {
-  if (!frame_ok(f) || !out_color || !out_depth || !out_alpha || (f->P > 0 && !out_radii))
-    return W2W_ERR_INVALID;
-  Plan p;
-  if (!ws || !make_plan(f->P, f->W, f->H, f->tile, ws->capacity, p)) return W2W_ERR_INVALID;
-  if (!ws_ok(ws, p)) return W2W_ERR_CAPACITY;
-  hipStream_t s = (hipStream_t)stream;
-  (void)hipGetLastError();
+// the forward launch sequence over a table already in device memory
+int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const Params& prm,
+                float* out_color, float* out_depth, float* out_alpha, int32_t* out_radii,
+                hipStream_t s) {
   void* G = ws->geom;
   void* B = ws->binning;
   void* I = ws->image;
   uint32_t* status = at<uint32_t>(G, p.status);
-  const uint32_t P = (uint32_t)f->P;
-  const uint32_t cap = (uint32_t)p.cap;
+  const W2WCaps& c = p.c;
+  const uint32_t items_cap = (uint32_t)c.items;
+  const uint32_t cap = (uint32_t)c.capacity;
+  const uint32_t tiles_cap = (uint32_t)c.tiles;
+  const uint32_t* d_items = &static_cast<const BatchHeader*>(table)->n_items;
 
   fill_u32(status, 0u, ST_WORDS, s);
-  fill_u32(at<uint32_t>(I, p.L.ranges), 0u, 2 * (size_t)p.T, s);
-  if (P > 0) {
+  fill_u32(at<uint32_t>(I, p.L.ranges), 0u, 2 * (size_t)c.tiles, s);
+  if (items_cap > 0) {
     fill_u32(at<uint32_t>(G, p.dhist), 0u, 4 * RADIX, s);
     fill_u32(at<uint32_t>(G, p.slb), 0u, p.stiles + 1, s);
     fill_u32(at<uint32_t>(B, p.thist), 0u, 4 * RADIX, s);
 
     PreArgs pa;
-    pa.P = f->P; pa.M = f->M; pa.D = f->D; pa.W = f->W; pa.H = f->H; pa.ts = f->tile;
-    pa.tiles_x = p.L.tiles_x; pa.tiles_y = p.L.tiles_y;
-    pa.fx = f->focal_x; pa.fy = f->focal_y;
-    pa.means = f->means; pa.scales = f->scales; pa.rots = f->rotations; pa.opac = f->opacities;
-    pa.sh = f->sh; pa.view = f->viewmatrix; pa.proj = f->projmatrix; pa.campos = f->campos;
+    pa.table = table; pa.scenes_cap = (uint32_t)c.scenes; pa.items_cap = items_cap;
+    pa.means = prm.means; pa.scales = prm.scales; pa.rots = prm.rots; pa.opac = prm.opac;
+    pa.sh = prm.sh; pa.ts = c.tile;
     pa.splats = at<Splat>(G, p.L.splats);
     pa.radii = at<int32_t>(G, p.L.radii);
     pa.rect = at<uint2>(G, p.L.rect);
     pa.counts = at<uint32_t>(G, p.L.counts);
     pa.dkey = at<uint32_t>(G, p.dkey0);
     pa.ids = at<uint32_t>(G, p.L.perm);
+    pa.item_frame = at<uint32_t>(G, p.L.item_frame);
     {
-    ProfSpan ps(W2W_KID_PREPROCESS, s);
-    hipLaunchKernelGGL(k_preprocess, dim3(cdiv(P, 256)), dim3(256), 0, s, pa);
+      ProfSpan ps(W2W_KID_PREPROCESS, s);
+      hipLaunchKernelGGL(k_preprocess, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 8192)),
+                         dim3(256), 0, s, pa);
     }
-
-    // depth sort: (depth bits, id) over P, 4 passes, result in dkey0 / perm
+    // depth sort: (depth bits, item) over the items, 4 passes, result in dkey0 / perm
     sort_passes<uint32_t>(at<uint32_t>(G, p.dkey0), at<uint32_t>(G, p.L.perm),
                           at<uint32_t>(G, p.dkey0), at<uint32_t>(G, p.L.perm),
-                          at<uint32_t>(G, p.dkey1), at<uint32_t>(G, p.id1), nullptr, P, 4, 8,
-                          false, at<uint32_t>(G, p.dhist), at<uint32_t>(G, p.dstatus),
+                          at<uint32_t>(G, p.dkey1), at<uint32_t>(G, p.id1), d_items, items_cap, 4,
+                          8, false, at<uint32_t>(G, p.dhist), at<uint32_t>(G, p.dstatus),
                           status + ST_DTICKET, W2W_KID_DSORT_HIST, W2W_KID_DSORT_PASS, s);
     {
-    ProfSpan ps(W2W_KID_SCAN, s);
-
-    hipLaunchKernelGGL(k_scan<true>, dim3(p.stiles), dim3(SCAN_THREADS), 0, s,
-                       at<uint32_t>(G, p.L.counts), at<uint32_t>(G, p.L.perm),
-                       at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.slb) + 1,
-                       at<uint32_t>(G, p.slb), status + ST_K, status + ST_OVERFLOW, P, cap);
+      ProfSpan ps(W2W_KID_SCAN, s);
+      hipLaunchKernelGGL(k_scan<true>, dim3(std::min<uint32_t>(p.stiles, 2048)), dim3(SCAN_THREADS),
+                         0, s, at<uint32_t>(G, p.L.counts), at<uint32_t>(G, p.L.perm),
+                         at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.slb) + 1,
+                         status + ST_SCAN_TICKET, status + ST_K, status + ST_OVERFLOW, d_items,
+                         items_cap, cap);
     }
     if (cap > 0) {
       {
-      ProfSpan ps(W2W_KID_EMIT, s);
-      hipLaunchKernelGGL(k_emit, dim3(cdiv(P, 256)), dim3(256), 0, s, at<uint32_t>(G, p.L.perm),
-                         at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.L.counts),
-                         at<uint2>(G, p.L.rect), P, (uint32_t)p.L.tiles_x, cap,
-                         at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0));
+        ProfSpan ps(W2W_KID_EMIT, s);
+        hipLaunchKernelGGL(k_emit, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 4096)), dim3(256),
+                           0, s, table, (uint32_t)c.scenes, at<uint32_t>(G, p.L.perm),
+                           at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.L.counts),
+                           at<uint2>(G, p.L.rect), at<uint32_t>(G, p.L.item_frame), items_cap, cap,
+                           status + ST_EMIT_TICKET, at<uint32_t>(B, p.tkey0),
+                           at<uint32_t>(B, p.tid0));
       }
       sort_passes<uint32_t>(at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0),
                             at<uint32_t>(B, p.L.sorted_tiles), at<uint32_t>(B, p.L.sorted_ids),
@@ -288,79 +332,252 @@ int w2w_forward(const W2WFrame* f, const W2WWorkspace* ws, float* out_color, flo
       const uint32_t rgrid = std::min<uint32_t>(cdiv(cap, 256), 4096);
       ProfSpan ps(W2W_KID_RANGES, s);
       hipLaunchKernelGGL(k_ranges, dim3(rgrid), dim3(256), 0, s, at<uint32_t>(B, p.L.sorted_tiles),
-                         status, cap, p.T, at<uint2>(I, p.L.ranges));
+                         status, table, cap, tiles_cap, at<uint2>(I, p.L.ranges));
     }
-    (void)hipMemcpyAsync(out_radii, at<int32_t>(G, p.L.radii), 4 * (size_t)P, hipMemcpyDeviceToDevice, s);
+    if (out_radii)
+      (void)hipMemcpyAsync(out_radii, at<int32_t>(G, p.L.radii), 4 * (size_t)items_cap,
+                           hipMemcpyDeviceToDevice, s);
   }
   BlendArgs ba;
+  ba.table = table; ba.scenes_cap = (uint32_t)c.scenes; ba.tiles_cap = tiles_cap;
+  ba.items_cap = items_cap;
   ba.ranges = at<uint2>(I, p.L.ranges);
   ba.ids = at<uint32_t>(B, p.L.sorted_ids);
   ba.splats = at<Splat>(G, p.L.splats);
-  ba.bg = f->background;
-  ba.W = f->W; ba.H = f->H; ba.tiles_x = p.L.tiles_x; ba.T = (int)p.T;
-  ba.P = (uint32_t)f->P;
   ba.stats = g_stats;
   ba.color = out_color; ba.alpha = out_alpha; ba.depth = out_depth;
   ba.n_contrib = at<uint32_t>(I, p.L.n_contrib);
   ba.final_T = at<float>(I, p.L.final_T);
-  switch (f->tile) {
-    case 8: launch_blend<8>(ba, s); break;
-    case 16: launch_blend<16>(ba, s); break;
-    default: launch_blend<32>(ba, s); break;
+  switch (c.tile) {
+    case 8: launch_blend<8>(ba, tiles_cap, s); break;
+    case 16: launch_blend<16>(ba, tiles_cap, s); break;
+    default: launch_blend<32>(ba, tiles_cap, s); break;
   }
   return launch_status();
+}
+
+int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const Params& prm,
+                 const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
+                 float* dL_dmeans2D, float* dL_dcolors, float* dL_dopacity, float* dL_dmeans3D,
+                 float* dL_dcov3D, float* dL_dsh, float* dL_dscales, float* dL_drotations,
+                 hipStream_t s) {
+  void* G = ws->geom;
+  void* B = ws->binning;
+  void* I = ws->image;
+  const W2WCaps& c = p.c;
+  if (c.items == 0) return W2W_OK;
+  float* gacc = at<float>(G, p.L.gacc);
+  fill_u32(reinterpret_cast<uint32_t*>(gacc), 0u, (size_t)GA_STRIDE * c.items, s);
+  BlendBwdArgs ba;
+  ba.table = table; ba.scenes_cap = (uint32_t)c.scenes; ba.tiles_cap = (uint32_t)c.tiles;
+  ba.items_cap = (uint32_t)c.items;
+  ba.ranges = at<uint2>(I, p.L.ranges);
+  ba.ids = at<uint32_t>(B, p.L.sorted_ids);
+  ba.splats = at<Splat>(G, p.L.splats);
+  ba.stats = g_stats ? g_stats + 3 : nullptr;
+  ba.n_contrib = at<uint32_t>(I, p.L.n_contrib);
+  ba.final_T = at<float>(I, p.L.final_T);
+  ba.dL_dcolor = dL_dcolor; ba.dL_dalpha = dL_dalpha; ba.dL_ddepth = dL_ddepth;
+  ba.gacc = gacc;
+  switch (c.tile) {
+    case 8: launch_blend_bwd<8>(ba, (uint32_t)c.tiles, s); break;
+    case 16: launch_blend_bwd<16>(ba, (uint32_t)c.tiles, s); break;
+    default: launch_blend_bwd<32>(ba, (uint32_t)c.tiles, s); break;
+  }
+  PreBwdArgs pa;
+  pa.table = table; pa.scenes_cap = (uint32_t)c.scenes; pa.items_cap = (uint32_t)c.items;
+  pa.gaussians_cap = (uint32_t)c.gaussians;
+  pa.means = prm.means; pa.scales = prm.scales; pa.rots = prm.rots; pa.sh = prm.sh;
+  pa.radii = at<int32_t>(G, p.L.radii);
+  pa.gacc = gacc;
+  pa.dL_dmeans2D = dL_dmeans2D; pa.dL_dcolors = dL_dcolors; pa.dL_dopacity = dL_dopacity;
+  pa.dL_dmeans3D = dL_dmeans3D; pa.dL_dcov3D = dL_dcov3D; pa.dL_dsh = dL_dsh;
+  pa.dL_dscales = dL_dscales; pa.dL_drots = dL_drotations;
+  if (c.gaussians > 0) {
+    ProfSpan ps(W2W_KID_PREPROCESS_BWD, s);
+    hipLaunchKernelGGL(k_preprocess_bwd, dim3(std::min<uint32_t>(cdiv((uint64_t)c.gaussians, 256), 8192)),
+                       dim3(256), 0, s, pa);
+  }
+  return launch_status();
+}
+
+int single_setup(const W2WFrame* f, const W2WWorkspace* ws, Plan& p, hipStream_t s) {
+  W2WCaps c;
+  if (!ws || !single_caps(f->P, f->W, f->H, f->tile, ws->capacity, c) || !make_plan(c, p))
+    return W2W_ERR_INVALID;
+  if (!ws_ok(ws, p)) return W2W_ERR_CAPACITY;
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_make_single, dim3(1), dim3(64), 0, s, at<void>(ws->geom, p.table),
+                     f->viewmatrix, f->projmatrix, f->campos, f->background, (uint32_t)f->P, f->M,
+                     f->D, f->W, f->H, f->tile, f->focal_x, f->focal_y);
+  return W2W_OK;
+}
+
+bool caps_ws(const W2WCaps* caps, const W2WWorkspace* ws, Plan& p, int& err) {
+  err = W2W_ERR_INVALID;
+  if (!caps || !ws || !make_plan(*caps, p)) return false;
+  if (!ws_ok(ws, p)) {
+    err = W2W_ERR_CAPACITY;
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int w2w_version(void) { return W2W_ABI_VERSION; }
+
+int w2w_workspace_layout(int32_t P, int32_t W, int32_t H, int32_t tile, int64_t capacity,
+                         W2WLayout* out) {
+  W2WCaps c;
+  Plan p;
+  if (!out || !single_caps(P, W, H, tile, capacity, c) || !make_plan(c, p)) return W2W_ERR_INVALID;
+  *out = p.L;
+  return W2W_OK;
+}
+
+int w2w_batch_workspace_layout(const W2WCaps* caps, W2WLayout* out) {
+  Plan p;
+  if (!out || !caps || !make_plan(*caps, p)) return W2W_ERR_INVALID;
+  *out = p.L;
+  return W2W_OK;
+}
+
+size_t w2w_batch_table_bytes(int32_t scenes_cap, int32_t frames_cap) {
+  if (scenes_cap < 1 || frames_cap < 1) return 0;
+  return table_bytes((uint32_t)scenes_cap, (uint32_t)frames_cap);
+}
+
+int w2w_batch_encode(void* host_table, size_t table_bytes_avail, int32_t scenes_cap,
+                     int32_t frames_cap, int32_t tile, int32_t n_scenes, const int64_t* scene_n,
+                     const int32_t* scene_views, const int32_t* scene_M, const int32_t* scene_D,
+                     const float* frame_view, const float* frame_proj, const float* frame_campos,
+                     const float* frame_bg, const float* frame_focal, const int32_t* frame_wh,
+                     const uint32_t* frame_flags, int64_t* totals) {
+  if (!host_table || scenes_cap < 1 || frames_cap < 1 || n_scenes < 0 || n_scenes > scenes_cap)
+    return W2W_ERR_INVALID;
+  if (tile != 8 && tile != 16 && tile != 32) return W2W_ERR_INVALID;
+  if (table_bytes_avail < table_bytes((uint32_t)scenes_cap, (uint32_t)frames_cap))
+    return W2W_ERR_CAPACITY;
+  std::memset(host_table, 0, table_bytes((uint32_t)scenes_cap, (uint32_t)frames_cap));
+  BatchHeader* h = static_cast<BatchHeader*>(host_table);
+  SceneDesc* sd = reinterpret_cast<SceneDesc*>(static_cast<char*>(host_table) + TABLE_HEADER_BYTES);
+  FrameDesc* fd = reinterpret_cast<FrameDesc*>(static_cast<char*>(host_table) + TABLE_HEADER_BYTES +
+                                               (size_t)scenes_cap * sizeof(SceneDesc));
+  uint64_t g = 0, items = 0, frames = 0, tiles = 0, pix = 0, shf = 0;
+  for (int32_t k = 0; k < n_scenes; k++) {
+    if (scene_n[k] < 0 || scene_views[k] < 1 || scene_D[k] < 0 || scene_D[k] > 3 ||
+        scene_M[k] < (scene_D[k] + 1) * (scene_D[k] + 1))
+      return W2W_ERR_INVALID;
+    if (frames + (uint64_t)scene_views[k] > (uint64_t)frames_cap) return W2W_ERR_CAPACITY;
+    sd[k].g_off = (uint32_t)g; sd[k].n = (uint32_t)scene_n[k]; sd[k].item_off = (uint32_t)items;
+    sd[k].frame_off = (uint32_t)frames; sd[k].n_views = (uint32_t)scene_views[k];
+    sd[k].M = scene_M[k]; sd[k].D = scene_D[k]; sd[k].sh_off = shf;
+    for (int32_t v = 0; v < scene_views[k]; v++) {
+      const uint64_t f = frames + v;
+      FrameDesc& F = fd[f];
+      std::memcpy(F.V, frame_view + 16 * f, 64);
+      std::memcpy(F.F, frame_proj + 16 * f, 64);
+      std::memcpy(F.cp, frame_campos + 3 * f, 12);
+      std::memcpy(F.bg, frame_bg + 3 * f, 12);
+      F.fx = frame_focal[2 * f]; F.fy = frame_focal[2 * f + 1];
+      F.W = frame_wh[2 * f]; F.H = frame_wh[2 * f + 1];
+      if (F.W <= 0 || F.H <= 0 || !(F.fx > 0.0f) || !(F.fy > 0.0f)) return W2W_ERR_INVALID;
+      F.tiles_x = (F.W + tile - 1) / tile; F.tiles_y = (F.H + tile - 1) / tile;
+      if (F.tiles_x > 65535 || F.tiles_y > 65535) return W2W_ERR_INVALID;
+      F.tile_off = (uint32_t)tiles; F.scene = (uint32_t)k; F.view = (uint32_t)v;
+      F.flags = frame_flags ? frame_flags[f] : FRAME_BACKWARD;
+      F.pix_off = pix;
+      tiles += (uint64_t)F.tiles_x * F.tiles_y;
+      pix += (uint64_t)F.W * F.H;
+    }
+    g += (uint64_t)scene_n[k];
+    items += (uint64_t)scene_n[k] * scene_views[k];
+    frames += (uint64_t)scene_views[k];
+    shf += (uint64_t)scene_n[k] * scene_M[k] * 3;
+  }
+  if (items >= LB_VAL || tiles >= LB_VAL || g >= LB_VAL) return W2W_ERR_CAPACITY;
+  h->n_scenes = (uint32_t)n_scenes; h->n_frames = (uint32_t)frames; h->n_items = (uint32_t)items;
+  h->n_gaussians = (uint32_t)g; h->n_tiles = (uint32_t)tiles; h->n_pixels = pix;
+  if (totals) {
+    totals[0] = (int64_t)g; totals[1] = (int64_t)items; totals[2] = (int64_t)frames;
+    totals[3] = (int64_t)tiles; totals[4] = (int64_t)pix; totals[5] = (int64_t)shf;
+  }
+  return W2W_OK;
+}
+
+int w2w_forward_batch(const void* table, const W2WCaps* caps, const W2WWorkspace* ws,
+                      const float* means, const float* scales, const float* rotations,
+                      const float* opacities, const float* sh, float* out_color, float* out_depth,
+                      float* out_alpha, int32_t* out_radii, void* stream) {
+  Plan p;
+  int err;
+  if (!table || !caps_ws(caps, ws, p, err)) return table ? err : W2W_ERR_INVALID;
+  if (!out_color || !out_depth || !out_alpha) return W2W_ERR_INVALID;
+  if (caps->items > 0 && (!means || !scales || !rotations || !opacities || !sh)) return W2W_ERR_INVALID;
+  (void)hipGetLastError();
+  Params prm{means, scales, rotations, opacities, sh};
+  return run_forward(p, table, ws, prm, out_color, out_depth, out_alpha, out_radii,
+                     (hipStream_t)stream);
+}
+
+int w2w_backward_batch(const void* table, const W2WCaps* caps, const W2WWorkspace* ws,
+                       const float* means, const float* scales, const float* rotations,
+                       const float* sh, const float* dL_dcolor, const float* dL_ddepth,
+                       const float* dL_dalpha, float* dL_dmeans2D, float* dL_dcolors,
+                       float* dL_dopacity, float* dL_dmeans3D, float* dL_dcov3D, float* dL_dsh,
+                       float* dL_dscales, float* dL_drotations, void* stream) {
+  Plan p;
+  int err;
+  if (!table || !caps_ws(caps, ws, p, err)) return table ? err : W2W_ERR_INVALID;
+  if (!dL_dcolor) return W2W_ERR_INVALID;
+  if (caps->items > 0 && (!means || !scales || !rotations || !sh || !dL_dmeans2D || !dL_dcolors ||
+                          !dL_dopacity || !dL_dmeans3D || !dL_dcov3D || !dL_dsh || !dL_dscales ||
+                          !dL_drotations))
+    return W2W_ERR_INVALID;
+  (void)hipGetLastError();
+  Params prm{means, scales, rotations, nullptr, sh};
+  return run_backward(p, table, ws, prm, dL_dcolor, dL_ddepth, dL_dalpha, dL_dmeans2D, dL_dcolors,
+                      dL_dopacity, dL_dmeans3D, dL_dcov3D, dL_dsh, dL_dscales, dL_drotations,
+                      (hipStream_t)stream);
+}
+
+int w2w_forward(const W2WFrame* f, const W2WWorkspace* ws, float* out_color, float* out_depth,
+                float* out_alpha, int32_t* out_radii, void* stream) {
+  if (!frame_ok(f) || !out_color || !out_depth || !out_alpha || (f->P > 0 && !out_radii))
+    return W2W_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  Plan p;
+  int st = single_setup(f, ws, p, s);
+  if (st != W2W_OK) return st;
+  Params prm{f->means, f->scales, f->rotations, f->opacities, f->sh};
+  return run_forward(p, at<void>(ws->geom, p.table), ws, prm, out_color, out_depth, out_alpha,
+                     out_radii, s);
 }
 
 int w2w_backward(const W2WFrame* f, const W2WWorkspace* ws, const float* dL_dcolor,
                  const float* dL_ddepth, const float* dL_dalpha, float* dL_dmeans2D,
                  float* dL_dcolors, float* dL_dopacity, float* dL_dmeans3D, float* dL_dcov3D,
                  float* dL_dsh, float* dL_dscales, float* dL_drotations, void* stream) {
-  if (!frame_ok(f) || !dL_dcolor || !dL_ddepth || !dL_dalpha) return W2W_ERR_INVALID;
+  if (!frame_ok(f) || !dL_dcolor) return W2W_ERR_INVALID;
   if (f->P > 0 && (!dL_dmeans2D || !dL_dcolors || !dL_dopacity || !dL_dmeans3D || !dL_dcov3D ||
                    !dL_dsh || !dL_dscales || !dL_drotations))
     return W2W_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
   Plan p;
-  if (!ws || !make_plan(f->P, f->W, f->H, f->tile, ws->capacity, p)) return W2W_ERR_INVALID;
+  W2WCaps c;
+  if (!ws || !single_caps(f->P, f->W, f->H, f->tile, ws->capacity, c) || !make_plan(c, p))
+    return W2W_ERR_INVALID;
   if (!ws_ok(ws, p)) return W2W_ERR_CAPACITY;
   if (f->P == 0) return W2W_OK;
-  hipStream_t s = (hipStream_t)stream;
   (void)hipGetLastError();
-  void* G = ws->geom;
-  void* B = ws->binning;
-  void* I = ws->image;
-  const uint32_t P = (uint32_t)f->P;
-  float* gacc = at<float>(G, p.L.gacc);
-  fill_u32(reinterpret_cast<uint32_t*>(gacc), 0u, (size_t)GA_STRIDE * P, s);
-  BlendBwdArgs ba;
-  ba.ranges = at<uint2>(I, p.L.ranges);
-  ba.ids = at<uint32_t>(B, p.L.sorted_ids);
-  ba.splats = at<Splat>(G, p.L.splats);
-  ba.bg = f->background;
-  ba.W = f->W; ba.H = f->H; ba.tiles_x = p.L.tiles_x; ba.T = (int)p.T;
-  ba.P = (uint32_t)f->P;
-  ba.stats = g_stats ? g_stats + 3 : nullptr;
-  ba.n_contrib = at<uint32_t>(I, p.L.n_contrib);
-  ba.final_T = at<float>(I, p.L.final_T);
-  ba.dL_dcolor = dL_dcolor; ba.dL_dalpha = dL_dalpha; ba.dL_ddepth = dL_ddepth;
-  ba.gacc = gacc;
-  switch (f->tile) {
-    case 8: launch_blend_bwd<8>(ba, s); break;
-    case 16: launch_blend_bwd<16>(ba, s); break;
-    default: launch_blend_bwd<32>(ba, s); break;
-  }
-  PreBwdArgs pa;
-  pa.P = f->P; pa.M = f->M; pa.D = f->D; pa.W = f->W; pa.H = f->H;
-  pa.fx = f->focal_x; pa.fy = f->focal_y;
-  pa.means = f->means; pa.scales = f->scales; pa.rots = f->rotations; pa.sh = f->sh;
-  pa.view = f->viewmatrix; pa.proj = f->projmatrix; pa.campos = f->campos;
-  pa.radii = at<int32_t>(G, p.L.radii);
-  pa.gacc = gacc;
-  pa.dL_dmeans2D = dL_dmeans2D; pa.dL_dcolors = dL_dcolors; pa.dL_dopacity = dL_dopacity;
-  pa.dL_dmeans3D = dL_dmeans3D; pa.dL_dcov3D = dL_dcov3D; pa.dL_dsh = dL_dsh;
-  pa.dL_dscales = dL_dscales; pa.dL_drots = dL_drotations;
-  ProfSpan ps(W2W_KID_PREPROCESS_BWD, s);
-  hipLaunchKernelGGL(k_preprocess_bwd, dim3(cdiv(P, 256)), dim3(256), 0, s, pa);
-  return launch_status();
+  // the table written by this view's forward is still in the workspace
+  Params prm{f->means, f->scales, f->rotations, nullptr, f->sh};
+  return run_backward(p, at<void>(ws->geom, p.table), ws, prm, dL_dcolor, dL_ddepth, dL_dalpha,
+                      dL_dmeans2D, dL_dcolors, dL_dopacity, dL_dmeans3D, dL_dcov3D, dL_dsh,
+                      dL_dscales, dL_drotations, s);
 }
 
 __global__ void k_sorted_keys(const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ ids,
@@ -378,16 +595,18 @@ __global__ void k_sorted_keys(const uint32_t* __restrict__ tiles, const uint32_t
 int w2w_sorted_keys(const W2WFrame* f, const W2WWorkspace* ws, uint64_t* out_keys, void* stream) {
   if (!frame_ok(f) || !out_keys) return W2W_ERR_INVALID;
   Plan p;
-  if (!ws || !make_plan(f->P, f->W, f->H, f->tile, ws->capacity, p)) return W2W_ERR_INVALID;
+  W2WCaps c;
+  if (!ws || !single_caps(f->P, f->W, f->H, f->tile, ws->capacity, c) || !make_plan(c, p))
+    return W2W_ERR_INVALID;
   if (!ws_ok(ws, p)) return W2W_ERR_CAPACITY;
-  if (p.cap == 0) return W2W_OK;
+  if (c.capacity == 0) return W2W_OK;
   hipStream_t s = (hipStream_t)stream;
   (void)hipGetLastError();
-  const uint32_t grid = std::min<uint32_t>(cdiv((uint64_t)p.cap, 256), 4096);
+  const uint32_t grid = std::min<uint32_t>(cdiv((uint64_t)c.capacity, 256), 4096);
   hipLaunchKernelGGL(k_sorted_keys, dim3(grid), dim3(256), 0, s,
                      at<uint32_t>(ws->binning, p.L.sorted_tiles),
                      at<uint32_t>(ws->binning, p.L.sorted_ids), at<Splat>(ws->geom, p.L.splats),
-                     at<uint32_t>(ws->geom, p.status), (uint32_t)p.cap, (uint32_t)f->P, out_keys);
+                     at<uint32_t>(ws->geom, p.status), (uint32_t)c.capacity, (uint32_t)f->P, out_keys);
   return launch_status();
 }
 
@@ -449,8 +668,8 @@ int w2w_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, i
   uint32_t* t = static_cast<uint32_t*>(temp);
   fill_u32(t, 0u, (size_t)tiles + 2, s);
   ProfSpan ps(W2W_KID_SCAN, s);
-  hipLaunchKernelGGL(k_scan<false>, dim3(tiles), dim3(SCAN_THREADS), 0, s, in, nullptr, out, t + 2,
-                     t, total, nullptr, (uint32_t)n, 0xffffffffu);
+  hipLaunchKernelGGL(k_scan<false>, dim3(std::min<uint32_t>(tiles, 2048)), dim3(SCAN_THREADS), 0, s,
+                     in, nullptr, out, t + 2, t, total, nullptr, nullptr, (uint32_t)n, 0xffffffffu);
   return launch_status();
 }
 

@@ -121,9 +121,61 @@ W2W_DEV void sh_basis_grad(int D, float x, float y, float z, float* gx, float* g
   gx[15] = SH_K15 * (3.0f * xx - 3.0f * yy); gy[15] = SH_K15 * (-6.0f * x * y);
 }
 
-// Camera constants read once per thread from device memory (uniform
-// addresses: scalar loads).  view/proj use the column-major storage the
-// reference's caller passes (Renderer.py:135-136).
+// ---- batch description (device memory) ----
+// A batch is a set of scenes (packed Gaussian arrays) and frames (views).
+// Work items are (frame, Gaussian) pairs; scene s owns items
+// [item_off, item_off + n * n_views) with item = item_off + g * n_views + v
+// for Gaussian g and view v (frame frame_off + v), so a Gaussian's views are
+// adjacent and, within one frame, items are in Gaussian order.  Tiles of all
+// frames share one global index space ([tile_off, tile_off + tiles) per
+// frame) and per-pixel outputs one pixel space ([pix_off, pix_off + W*H)).
+struct BatchHeader {
+  uint32_t n_scenes, n_frames, n_items, n_gaussians, n_tiles, pad0, pad1, pad2;
+  uint64_t n_pixels, pad3;
+};
+struct SceneDesc {
+  uint32_t g_off;      // first Gaussian in the packed parameter arrays
+  uint32_t n;          // Gaussians
+  uint32_t item_off;   // first item
+  uint32_t frame_off;  // first frame
+  uint32_t n_views;    // frames of this scene
+  int32_t M, D;        // SH coefficients per Gaussian, active degree
+  uint32_t pad;
+  uint64_t sh_off;     // first float of the scene's SH block [n, M, 3]
+  uint64_t pad2;
+};
+struct FrameDesc {
+  float V[16];         // world->camera, column-major (Renderer.py:135)
+  float F[16];         // full projection P*W2C, column-major
+  float cp[3], fx;     // caller's campos, focal x
+  float bg[3], fy;     // background, focal y
+  int32_t W, H, tiles_x, tiles_y;
+  uint32_t tile_off, scene, view, flags;  // flags bit 0: backward wanted
+  uint64_t pix_off, pad;
+};
+constexpr uint32_t FRAME_BACKWARD = 1u;
+constexpr size_t TABLE_HEADER_BYTES = 64;
+
+W2W_DEV const SceneDesc* table_scenes(const void* table) {
+  return reinterpret_cast<const SceneDesc*>(static_cast<const char*>(table) + TABLE_HEADER_BYTES);
+}
+W2W_DEV const FrameDesc* table_frames(const void* table, uint32_t scenes_cap) {
+  return reinterpret_cast<const FrameDesc*>(static_cast<const char*>(table) + TABLE_HEADER_BYTES +
+                                            (size_t)scenes_cap * sizeof(SceneDesc));
+}
+
+// largest index k in [0, n) with key(k) <= x (keys non-decreasing, key(0) <= x)
+template <typename F>
+W2W_DEV uint32_t upper_index(uint32_t n, uint32_t x, F key) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (key(mid) <= x) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Camera constants of one frame, read once per thread (uniform addresses).
 struct Cam {
   float V[16];
   float F[16];
@@ -132,22 +184,21 @@ struct Cam {
   int W, H;
 };
 
-W2W_DEV void load_cam(Cam& c, const float* view, const float* proj, const float* campos, int W,
-                      int H, float fx, float fy) {
+W2W_DEV void load_cam(Cam& c, const FrameDesc& fd) {
 #pragma unroll
   for (int k = 0; k < 16; k++) {
-    c.V[k] = view[k];
-    c.F[k] = proj[k];
+    c.V[k] = fd.V[k];
+    c.F[k] = fd.F[k];
   }
-  c.cp[0] = campos[0];
-  c.cp[1] = campos[1];
-  c.cp[2] = campos[2];
-  c.fx = fx;
-  c.fy = fy;
-  c.W = W;
-  c.H = H;
-  c.limx = (float)W / (2.0f * fx);
-  c.limy = (float)H / (2.0f * fy);
+  c.cp[0] = fd.cp[0];
+  c.cp[1] = fd.cp[1];
+  c.cp[2] = fd.cp[2];
+  c.fx = fd.fx;
+  c.fy = fd.fy;
+  c.W = fd.W;
+  c.H = fd.H;
+  c.limx = (float)fd.W / (2.0f * fd.fx);
+  c.limy = (float)fd.H / (2.0f * fd.fy);
 }
 
 // Per-Gaussian projection state: the EWA projection of projection.cu:7-154

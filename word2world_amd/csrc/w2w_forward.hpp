@@ -1,22 +1,27 @@
-// Forward rasterizer kernels for gfx950.
+// Forward rasterizer kernels for gfx950, over a batch of frames.
 //
 // Stage order (gaussianRasterizer.cu:118-280 restructured for HBM traffic):
 //   k_preprocess   projection.cu:7-154 + ADRculling.cu:7-109 +
-//                  gaussianForwardPass.cu:172-264 fused, one thread/Gaussian
-//   depth sort     (depth bits, id) over P with the onesweep sort
-//   k_scan_perm    tile counts gathered in depth order, single-pass
-//                  decoupled look-back exclusive scan -> offsets, K
-//   k_emit         per-tile instance keys in depth order, block-level load
-//                  balancing (prepareSort.cu:239-301 emitted one Gaussian per
-//                  thread with a serial tile loop)
-//   tile sort      stable sort of the 32-bit tile ids over ceil(log2 T) bits
+//                  gaussianForwardPass.cu:172-264 fused, one thread per item
+//                  ((frame, Gaussian) pair)
+//   depth sort     (depth bits, item) over all items with the onesweep sort
+//   k_scan         tile counts gathered in depth order, single-pass decoupled
+//                  look-back exclusive scan -> offsets, K
+//   k_emit         instance keys in depth order, block-level load balancing
+//                  (prepareSort.cu:239-301 emitted one Gaussian per thread
+//                  with a serial tile loop)
+//   tile sort      stable sort of the 32-bit global tile ids
 //   k_ranges       gaussianTiles.cu:6-33
 //   k_blend        pixelGaussiansFull.cu:3-98 / pixelGaussiansDev.cuh:20-79
 //
-// Sorting P depth keys first and then stably sorting the K tile ids gives
-// exactly the order of the reference's stable sort of the 64-bit keys
-// (tile << 32 | depth bits) emitted in Gaussian order: both order instances
-// by (tile, depth bits, Gaussian id).
+// Sorting the depth keys first and then stably sorting the tile ids gives
+// exactly the order of the reference's stable sort of 64-bit keys
+// (tile << 32 | depth bits) emitted in Gaussian order: instances end up
+// ordered by (tile, depth bits, Gaussian id), per frame.
+//
+// Every count that depends on the data (items, tiles, K) is read from device
+// memory and grids are sized by the workspace capacities, so one launch
+// sequence serves any batch that fits and can be captured in a HIP graph.
 #pragma once
 #include "w2w_device.hpp"
 #include "w2w_sort.hpp"
@@ -32,6 +37,7 @@ enum StatusWord : int {
   ST_DTICKET = 8,      // 4 depth-sort tickets
   ST_TTICKET = 16,     // up to 4 tile-sort tickets
   ST_SCAN_TICKET = 24,
+  ST_EMIT_TICKET = 25,
   ST_WORDS = 64,
 };
 
@@ -40,106 +46,135 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
+// Item -> (scene, Gaussian, view, frame).
+struct ItemRef {
+  uint32_t scene, g, view, frame, gg;
+};
+
+W2W_DEV ItemRef item_ref(const SceneDesc* scenes, uint32_t n_scenes, uint32_t i) {
+  ItemRef r;
+  r.scene = upper_index(n_scenes, i, [&](uint32_t k) { return scenes[k].item_off; });
+  const SceneDesc& sd = scenes[r.scene];
+  const uint32_t local = i - sd.item_off;
+  r.g = local / sd.n_views;
+  r.view = local - r.g * sd.n_views;
+  r.frame = sd.frame_off + r.view;
+  r.gg = sd.g_off + r.g;
+  return r;
+}
+
 struct PreArgs {
-  int P, M, D, W, H, ts, tiles_x, tiles_y;
-  float fx, fy;
-  const float* means;
-  const float* scales;
-  const float* rots;
-  const float* opac;
-  const float* sh;
-  const float* view;
-  const float* proj;
-  const float* campos;
-  Splat* splats;
+  const void* table;
+  uint32_t scenes_cap;
+  uint32_t items_cap;
+  const float* means;    // packed [G,3]
+  const float* scales;   // [G,3] activated
+  const float* rots;     // [G,4] normalised
+  const float* opac;     // [G] activated
+  const float* sh;       // packed SH blocks (SceneDesc::sh_off)
+  int ts;
+  Splat* splats;         // per item
   int32_t* radii;
-  uint2* rect;       // (x0 | y0 << 16, x1 | y1 << 16)
+  uint2* rect;           // (x0 | y0 << 16, x1 | y1 << 16)
   uint32_t* counts;
-  uint32_t* dkey;    // depth bits, CULLED_DEPTH_KEY when culled
-  uint32_t* ids;     // iota, the values of the depth sort
+  uint32_t* dkey;        // depth bits, CULLED_DEPTH_KEY when culled
+  uint32_t* ids;         // iota, the values of the depth sort
+  uint32_t* item_frame;
 };
 
 __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.P) return;
-  Cam c;
-  load_cam(c, a.view, a.proj, a.campos, a.W, a.H, a.fx, a.fy);
-  const float p[3] = {a.means[3 * i], a.means[3 * i + 1], a.means[3 * i + 2]};
-  const float s[3] = {a.scales[3 * i], a.scales[3 * i + 1], a.scales[3 * i + 2]};
-  const float q[4] = {a.rots[4 * i], a.rots[4 * i + 1], a.rots[4 * i + 2], a.rots[4 * i + 3]};
-  const float o = a.opac[i];
+  const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
+  const uint32_t n_items = min(hdr->n_items, a.items_cap);
+  const uint32_t n_scenes = min(hdr->n_scenes, a.scenes_cap);
+  const SceneDesc* scenes = table_scenes(a.table);
+  const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_items; i += stride) {
+    const ItemRef ir = item_ref(scenes, n_scenes, i);
+    const SceneDesc& sd = scenes[ir.scene];
+    const FrameDesc& fd = frames[ir.frame];
+    Cam c;
+    load_cam(c, fd);
+    const uint32_t gi = ir.gg;
+    const float p[3] = {a.means[3 * gi], a.means[3 * gi + 1], a.means[3 * gi + 2]};
+    const float s[3] = {a.scales[3 * gi], a.scales[3 * gi + 1], a.scales[3 * gi + 2]};
+    const float q[4] = {a.rots[4 * gi], a.rots[4 * gi + 1], a.rots[4 * gi + 2], a.rots[4 * gi + 3]};
+    const float o = a.opac[gi];
 
-  // SH colour, direction normalize(mean - campos), +0.5, unclamped
-  float rgb[3];
-  {
-    float dx = p[0] - c.cp[0], dy = p[1] - c.cp[1], dz = p[2] - c.cp[2];
-    float len = __builtin_sqrtf((dx * dx + dy * dy) + dz * dz);
-    float x = dx / len, y = dy / len, z = dz / len;
-    float Y[16];
-    sh_basis(a.D, x, y, z, Y);
-    const int nb = (a.D + 1) * (a.D + 1);
-    const float* c0 = a.sh + (size_t)i * a.M * 3;
-    float r0 = Y[0] * c0[0], r1 = Y[0] * c0[1], r2 = Y[0] * c0[2];
-    for (int k = 1; k < nb; k++) {
-      r0 = __builtin_fmaf(Y[k], c0[3 * k], r0);
-      r1 = __builtin_fmaf(Y[k], c0[3 * k + 1], r1);
-      r2 = __builtin_fmaf(Y[k], c0[3 * k + 2], r2);
+    // SH colour, direction normalize(mean - campos), +0.5, unclamped
+    float rgb[3];
+    {
+      float dx = p[0] - c.cp[0], dy = p[1] - c.cp[1], dz = p[2] - c.cp[2];
+      float len = __builtin_sqrtf((dx * dx + dy * dy) + dz * dz);
+      float x = dx / len, y = dy / len, z = dz / len;
+      float Y[16];
+      sh_basis(sd.D, x, y, z, Y);
+      const int nb = (sd.D + 1) * (sd.D + 1);
+      const float* c0 = a.sh + sd.sh_off + (size_t)ir.g * sd.M * 3;
+      float r0 = Y[0] * c0[0], r1 = Y[0] * c0[1], r2 = Y[0] * c0[2];
+      for (int k = 1; k < nb; k++) {
+        r0 = __builtin_fmaf(Y[k], c0[3 * k], r0);
+        r1 = __builtin_fmaf(Y[k], c0[3 * k + 1], r1);
+        r2 = __builtin_fmaf(Y[k], c0[3 * k + 2], r2);
+      }
+      rgb[0] = r0 + 0.5f;
+      rgb[1] = r1 + 0.5f;
+      rgb[2] = r2 + 0.5f;
     }
-    rgb[0] = r0 + 0.5f;
-    rgb[1] = r1 + 0.5f;
-    rgb[2] = r2 + 0.5f;
-  }
 
-  Geom g;
-  geometry(c, p, s, q, g);
-  int32_t radius = 0;
-  uint32_t cnt = 0, x0 = 0, y0 = 0, x1 = 0, y1 = 0;
-  bool keep = g.visible;
-  float r = 0.0f;
-  if (keep) {
-    float half_tr = (g.cxx + g.cyy) * 0.5f;
-    float ds = half_tr * half_tr - g.det;
-    float root = __builtin_sqrtf(fmaxf(0.1f, ds));
-    float lmax = fmaxf(half_tr + root, half_tr - root);
-    float r3 = 3.0f * __builtin_sqrtf(lmax);
-    keep = o > 0.0039f;
+    Geom g;
+    geometry(c, p, s, q, g);
+    int32_t radius = 0;
+    uint32_t cnt = 0, x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    bool keep = g.visible;
+    float r = 0.0f;
     if (keep) {
-      float lnr = log_pos(o * 255.0f);
-      float radp = __builtin_sqrtf((2.0f * lmax) * lnr);
-      r = fminf(radp, r3);
-      keep = r > 0.0f;
+      float half_tr = (g.cxx + g.cyy) * 0.5f;
+      float ds = half_tr * half_tr - g.det;
+      float root = __builtin_sqrtf(fmaxf(0.1f, ds));
+      float lmax = fmaxf(half_tr + root, half_tr - root);
+      float r3 = 3.0f * __builtin_sqrtf(lmax);
+      keep = o > 0.0039f;
+      if (keep) {
+        float lnr = log_pos(o * 255.0f);
+        float radp = __builtin_sqrtf((2.0f * lmax) * lnr);
+        r = fminf(radp, r3);
+        keep = r > 0.0f;
+      }
     }
+    if (keep) {
+      const float tsf = (float)a.ts, tsm1 = (float)(a.ts - 1);
+      x0 = tile_coord((g.px - r) / tsf, (uint32_t)fd.tiles_x);
+      y0 = tile_coord((g.py - r) / tsf, (uint32_t)fd.tiles_y);
+      x1 = tile_coord(((g.px + r) + tsm1) / tsf, (uint32_t)fd.tiles_x);
+      y1 = tile_coord(((g.py + r) + tsm1) / tsf, (uint32_t)fd.tiles_y);
+      cnt = (x1 > x0 && y1 > y0) ? (x1 - x0) * (y1 - y0) : 0u;
+      radius = (int32_t)__builtin_ceilf(r);
+    }
+    Splat sp;
+    if (keep) {
+      sp.a = make_float4(g.px, g.py, g.ca, g.cb);
+      sp.b = make_float4(g.cc, o, g.ndcz, 0.0f);
+    } else {
+      sp.a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      sp.b = make_float4(0.0f, o, 0.0f, 0.0f);
+    }
+    sp.c = make_float4(rgb[0], rgb[1], rgb[2], 0.0f);
+    a.splats[i] = sp;
+    a.radii[i] = radius;
+    a.rect[i] = make_uint2(x0 | (y0 << 16), x1 | (y1 << 16));
+    a.counts[i] = cnt;
+    a.dkey[i] = keep ? __float_as_uint(g.ndcz) : CULLED_DEPTH_KEY;
+    a.ids[i] = i;
+    a.item_frame[i] = ir.frame;
   }
-  if (keep) {
-    const float tsf = (float)a.ts, tsm1 = (float)(a.ts - 1);
-    x0 = tile_coord((g.px - r) / tsf, (uint32_t)a.tiles_x);
-    y0 = tile_coord((g.py - r) / tsf, (uint32_t)a.tiles_y);
-    x1 = tile_coord(((g.px + r) + tsm1) / tsf, (uint32_t)a.tiles_x);
-    y1 = tile_coord(((g.py + r) + tsm1) / tsf, (uint32_t)a.tiles_y);
-    cnt = (x1 > x0 && y1 > y0) ? (x1 - x0) * (y1 - y0) : 0u;
-    radius = (int32_t)__builtin_ceilf(r);
-  }
-  Splat sp;
-  if (keep) {
-    sp.a = make_float4(g.px, g.py, g.ca, g.cb);
-    sp.b = make_float4(g.cc, o, g.ndcz, 0.0f);
-  } else {
-    sp.a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    sp.b = make_float4(0.0f, o, 0.0f, 0.0f);
-  }
-  sp.c = make_float4(rgb[0], rgb[1], rgb[2], 0.0f);
-  a.splats[i] = sp;
-  a.radii[i] = radius;
-  a.rect[i] = make_uint2(x0 | (y0 << 16), x1 | (y1 << 16));
-  a.counts[i] = cnt;
-  a.dkey[i] = keep ? __float_as_uint(g.ndcz) : CULLED_DEPTH_KEY;
-  a.ids[i] = (uint32_t)i;
 }
 
 // Exclusive scan of in[perm[i]] (or in[i] without a permutation) with a
 // single-pass decoupled look-back; writes the offsets, the total and, when
-// asked, an overflow flag (total > capacity).  Look-back words lb[] and the
-// ticket must be zero on entry; grid = ceil(n / SCAN_TILE) workgroups.
+// asked, an overflow flag (total > capacity).  Tiles are claimed through a
+// ticket until none is left, so any grid size works; look-back words lb[]
+// and the ticket must be zero on entry.  n comes from *d_n when given.
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
@@ -148,113 +183,137 @@ template <bool GATHER>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan(
     const uint32_t* __restrict__ in, const uint32_t* __restrict__ perm,
     uint32_t* __restrict__ out, uint32_t* __restrict__ lb, uint32_t* __restrict__ ticket,
-    uint32_t* __restrict__ total_out, uint32_t* __restrict__ overflow_out, uint32_t n,
-    uint32_t capacity) {
+    uint32_t* __restrict__ total_out, uint32_t* __restrict__ overflow_out,
+    const uint32_t* __restrict__ d_n, uint32_t n_cap, uint32_t capacity) {
   __shared__ uint32_t s_tmp[4];
   __shared__ uint32_t s_tile, s_excl;
   const int tid = threadIdx.x;
-  if (tid == 0) s_tile = atomicAdd(ticket, 1u);
-  __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint32_t base = tile * SCAN_TILE + tid * SCAN_ITEMS;
-  uint32_t v[SCAN_ITEMS];
-  uint32_t run = 0;
+  const uint32_t n = sort_count(d_n, n_cap);
+  const uint32_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  for (;;) {
+    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    if (tile >= ntiles) return;
+    const uint32_t base = tile * SCAN_TILE + tid * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t run = 0;
 #pragma unroll
-  for (int j = 0; j < SCAN_ITEMS; j++) {
-    uint32_t idx = base + j;
-    uint32_t c = 0u;
-    if (idx < n) c = GATHER ? in[perm[idx]] : in[idx];
-    v[j] = run;
-    run += c;
-  }
-  uint32_t tot;
-  uint32_t texcl = block_excl_scan_256(run, s_tmp, &tot);
-  if (tid == 0) {
-    uint32_t excl = 0;
-    if (tile == 0) {
-      st_release_dev(lb, LB_INC | tot);
-    } else {
-      st_release_dev(lb + tile, LB_AGG | tot);
-      int p = (int)tile - 1;
-      for (;;) {
-        uint32_t w = ld_acquire_dev(lb + p);
-        uint32_t flag = w & ~LB_VAL;
-        if (flag == 0) continue;
-        excl += w & LB_VAL;
-        if (flag == LB_INC) break;
-        p--;
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+      uint32_t idx = base + j;
+      uint32_t c = 0u;
+      if (idx < n) c = GATHER ? in[perm[idx]] : in[idx];
+      v[j] = run;
+      run += c;
+    }
+    uint32_t tot;
+    uint32_t texcl = block_excl_scan_256(run, s_tmp, &tot);
+    if (tid == 0) {
+      uint32_t excl = 0;
+      if (tile == 0) {
+        st_release_dev(lb, LB_INC | tot);
+      } else {
+        st_release_dev(lb + tile, LB_AGG | tot);
+        int p = (int)tile - 1;
+        for (;;) {
+          uint32_t w = ld_acquire_dev(lb + p);
+          uint32_t flag = w & ~LB_VAL;
+          if (flag == 0) continue;
+          excl += w & LB_VAL;
+          if (flag == LB_INC) break;
+          p--;
+        }
+        st_release_dev(lb + tile, LB_INC | (excl + tot));
       }
-      st_release_dev(lb + tile, LB_INC | (excl + tot));
+      s_excl = excl;
+      if (tile == ntiles - 1) {
+        uint32_t K = excl + tot;
+        *total_out = K;
+        if (overflow_out) *overflow_out = K > capacity ? 1u : 0u;
+      }
     }
-    s_excl = excl;
-    const uint32_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
-    if (tile == ntiles - 1) {
-      uint32_t K = excl + tot;
-      *total_out = K;
-      if (overflow_out) *overflow_out = K > capacity ? 1u : 0u;
-    }
-  }
-  __syncthreads();
-  const uint32_t e0 = s_excl + texcl;
+    __syncthreads();
+    const uint32_t e0 = s_excl + texcl;
 #pragma unroll
-  for (int j = 0; j < SCAN_ITEMS; j++) {
-    uint32_t idx = base + j;
-    if (idx < n) out[idx] = e0 + v[j];
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+      uint32_t idx = base + j;
+      if (idx < n) out[idx] = e0 + v[j];
+    }
+    __syncthreads();
   }
 }
 
-// Instance emission in depth order.  Block b owns depth-sorted positions
+// Instance emission in depth order.  A workgroup owns depth-sorted positions
 // [256b, 256b+256) and the contiguous instance range they cover; each thread
-// strides over instances and finds its owner by binary search in LDS.
-__global__ __launch_bounds__(256) void k_emit(const uint32_t* __restrict__ perm,
+// strides over instances and finds its owner by binary search in LDS.  Chunks
+// are claimed through a ticket (any grid size).
+__global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, uint32_t scenes_cap,
+                                              const uint32_t* __restrict__ perm,
                                               const uint32_t* __restrict__ offs,
                                               const uint32_t* __restrict__ counts,
-                                              const uint2* __restrict__ rect, uint32_t P,
-                                              uint32_t tiles_x, uint32_t capacity,
+                                              const uint2* __restrict__ rect,
+                                              const uint32_t* __restrict__ item_frame,
+                                              uint32_t items_cap, uint32_t capacity,
+                                              uint32_t* __restrict__ ticket,
                                               uint32_t* __restrict__ tile_keys,
                                               uint32_t* __restrict__ ids) {
-  __shared__ uint32_t s_start[256], s_g[256], s_x0[256], s_y0[256], s_w[256];
-  __shared__ uint32_t s_end;
+  __shared__ uint32_t s_start[256], s_g[256], s_x0[256], s_y0[256], s_w[256], s_tx[256], s_toff[256];
+  __shared__ uint32_t s_end, s_chunk;
+  const BatchHeader* hdr = static_cast<const BatchHeader*>(table);
+  const FrameDesc* frames = table_frames(table, scenes_cap);
+  const uint32_t P = min(hdr->n_items, items_cap);
+  const uint32_t nchunks = (P + 255) / 256;
   const int tid = threadIdx.x;
-  const uint32_t i = blockIdx.x * 256 + tid;
-  const uint32_t last = min(P, blockIdx.x * 256 + 256) - 1;
-  if (i < P) {
-    uint32_t g = perm[i];
-    uint2 r = rect[g];
-    uint32_t c = counts[g];
-    s_start[tid] = offs[i];
-    s_g[tid] = g;
-    s_x0[tid] = r.x & 0xffffu;
-    s_y0[tid] = r.x >> 16;
-    s_w[tid] = (r.y & 0xffffu) - (r.x & 0xffffu);
-    if (i == last) s_end = offs[i] + c;
-  }
-  __syncthreads();
-  const int nb = (int)(last - blockIdx.x * 256 + 1);
-  const uint32_t kb = s_start[0], ke = s_end;
-  for (uint32_t k = kb + tid; k < ke; k += 256) {
-    int lo = 0, hi = nb - 1;  // largest j with s_start[j] <= k
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (s_start[mid] <= k) lo = mid; else hi = mid - 1;
+  for (;;) {
+    if (tid == 0) s_chunk = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t chunk = s_chunk;
+    if (chunk >= nchunks) return;
+    const uint32_t i = chunk * 256 + tid;
+    const uint32_t last = min(P, chunk * 256 + 256) - 1;
+    if (i < P) {
+      uint32_t g = perm[i];
+      uint2 r = rect[g];
+      uint32_t c = counts[g];
+      const FrameDesc& fd = frames[item_frame[g]];
+      s_start[tid] = offs[i];
+      s_g[tid] = g;
+      s_x0[tid] = r.x & 0xffffu;
+      s_y0[tid] = r.x >> 16;
+      s_w[tid] = (r.y & 0xffffu) - (r.x & 0xffffu);
+      s_tx[tid] = (uint32_t)fd.tiles_x;
+      s_toff[tid] = fd.tile_off;
+      if (i == last) s_end = offs[i] + c;
     }
-    uint32_t local = k - s_start[lo];
-    uint32_t w = s_w[lo];
-    uint32_t ty = s_y0[lo] + local / w;
-    uint32_t tx = s_x0[lo] + local % w;
-    if (k < capacity) {
-      tile_keys[k] = ty * tiles_x + tx;
-      ids[k] = s_g[lo];
+    __syncthreads();
+    const int nb = (int)(last - chunk * 256 + 1);
+    const uint32_t kb = s_start[0], ke = s_end;
+    for (uint32_t k = kb + tid; k < ke; k += 256) {
+      int lo = 0, hi = nb - 1;  // largest j with s_start[j] <= k
+      while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (s_start[mid] <= k) lo = mid; else hi = mid - 1;
+      }
+      uint32_t local = k - s_start[lo];
+      uint32_t w = s_w[lo];
+      uint32_t ty = s_y0[lo] + local / w;
+      uint32_t tx = s_x0[lo] + local % w;
+      if (k < capacity) {
+        tile_keys[k] = s_toff[lo] + ty * s_tx[lo] + tx;
+        ids[k] = s_g[lo];
+      }
     }
+    __syncthreads();
   }
 }
 
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ keys,
                                                 const uint32_t* __restrict__ status,
-                                                uint32_t capacity, uint32_t T,
-                                                uint2* __restrict__ ranges) {
+                                                const void* __restrict__ table, uint32_t capacity,
+                                                uint32_t tiles_cap, uint2* __restrict__ ranges) {
   uint32_t n = status[ST_K];
   n = n < capacity ? n : capacity;
+  const uint32_t T = min(static_cast<const BatchHeader*>(table)->n_tiles, tiles_cap);
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     uint32_t t = keys[i];
@@ -266,21 +325,25 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ key
 
 // tile index of a workgroup: consecutive workgroups go round-robin to the 8
 // XCDs, so give each XCD a contiguous band of tiles (shared L2 working set)
-W2W_DEV int xcd_tile(int bid, int T) {
-  const int per = (T + 7) >> 3;
-  return (bid & 7) * per + (bid >> 3);
+W2W_DEV uint32_t xcd_tile(uint32_t bid, uint32_t T) {
+  const uint32_t per = (T + 7) >> 3;
+  return (bid & 7u) * per + (bid >> 3);
+}
+
+// frame owning global tile t
+W2W_DEV uint32_t tile_frame(const FrameDesc* frames, uint32_t n_frames, uint32_t t) {
+  return upper_index(n_frames, t, [&](uint32_t k) { return frames[k].tile_off; });
 }
 
 struct BlendArgs {
+  const void* table;
+  uint32_t scenes_cap, tiles_cap, items_cap;
   const uint2* ranges;
   const uint32_t* ids;
   const Splat* splats;
-  const float* bg;
-  int W, H, tiles_x, T;
-  uint32_t P;
   unsigned long long* stats;  // optional [3]: evaluated pairs, contributing pairs, loaded instances
-  float* color;
-  float* alpha;
+  float* color;               // per frame [3,H,W] at 3*pix_off
+  float* alpha;               // per frame [H,W] at pix_off
   float* depth;
   uint32_t* n_contrib;
   float* final_T;
@@ -290,17 +353,23 @@ template <int TS>
 __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs a) {
   constexpr int B = TS * TS;
   __shared__ Splat s_sp[B];
-  const int tile = xcd_tile(blockIdx.x, a.T);
-  if (tile >= a.T) return;
+  const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
+  const uint32_t T = min(hdr->n_tiles, a.tiles_cap);
+  const uint32_t P = min(hdr->n_items, a.items_cap);
+  const uint32_t tile = xcd_tile(blockIdx.x, a.tiles_cap);
+  if (tile >= T) return;
+  const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
+  const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, tile)];
   const int tid = threadIdx.x;
-  const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+  const int lt = (int)(tile - fd.tile_off);
+  const int tx = lt % fd.tiles_x, ty = lt / fd.tiles_x;
   const int px = tx * TS + (tid % TS), py = ty * TS + (tid / TS);
-  const bool inside = px < a.W && py < a.H;
+  const bool inside = px < fd.W && py < fd.H;
   const float pxf = (float)px, pyf = (float)py;
   const uint2 range = a.ranges[tile];
   const int n = (int)(range.y - range.x);
   bool done = !inside;
-  float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, A = 0.0f, Dd = 0.0f;
+  float T_ = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, A = 0.0f, Dd = 0.0f;
   uint32_t last = 0;
   uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
   const float a_min = 0x1.010102p-8f;
@@ -310,7 +379,7 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs a) {
     const int k = base + tid;
     if (k < n) {
       uint32_t id = a.ids[range.x + k];
-      s_sp[tid] = a.splats[id < a.P ? id : 0u];
+      s_sp[tid] = a.splats[id < P ? id : 0u];
     }
     __syncthreads();
     const int cnt = min(B, n - base);
@@ -324,19 +393,19 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs a) {
       const float G = exp_neg(pw);
       const float al = fminf(0.99f, sb.y * G);
       if (al < a_min) continue;
-      const float nT = T * (1.0f - al);
+      const float nT = T_ * (1.0f - al);
       if (nT < 1e-4f) {
         done = true;
         break;
       }
-      const float aT = al * T;
+      const float aT = al * T_;
       const float4 sc = s_sp[j].c;
       C0 = __builtin_fmaf(sc.x, aT, C0);
       C1 = __builtin_fmaf(sc.y, aT, C1);
       C2 = __builtin_fmaf(sc.z, aT, C2);
       A = A + aT;
       Dd = __builtin_fmaf(sb.z, aT, Dd);
-      T = nT;
+      T_ = nT;
       last = (uint32_t)(base + j + 1);
       n_con++;
     }
@@ -355,15 +424,17 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs a) {
     }
   }
   if (inside) {
-    const size_t pid = (size_t)py * a.W + px;
-    const size_t HW = (size_t)a.W * a.H;
-    a.color[pid] = __builtin_fmaf(T, a.bg[0], C0);
-    a.color[HW + pid] = __builtin_fmaf(T, a.bg[1], C1);
-    a.color[2 * HW + pid] = __builtin_fmaf(T, a.bg[2], C2);
+    const size_t HW = (size_t)fd.W * fd.H;
+    const size_t lp = (size_t)py * fd.W + px;
+    const size_t pid = fd.pix_off + lp;
+    float* col = a.color + 3 * fd.pix_off;
+    col[lp] = __builtin_fmaf(T_, fd.bg[0], C0);
+    col[HW + lp] = __builtin_fmaf(T_, fd.bg[1], C1);
+    col[2 * HW + lp] = __builtin_fmaf(T_, fd.bg[2], C2);
     a.alpha[pid] = A;
     a.depth[pid] = Dd;
     a.n_contrib[pid] = last;
-    a.final_T[pid] = T;
+    a.final_T[pid] = T_;
   }
 }
 

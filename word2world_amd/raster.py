@@ -100,7 +100,7 @@ class Workspace:
         return self._view(self.binning, self.layout.sorted_tiles, n, torch.int32)
 
     def ranges(self):
-        T = self.layout.tiles_x * self.layout.tiles_y
+        T = ((self.W + self.tile - 1) // self.tile) * ((self.H + self.tile - 1) // self.tile)
         return self._view(self.image, self.layout.ranges, 2 * T, torch.int32, (T, 2))
 
     def n_contrib(self):
