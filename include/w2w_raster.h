@@ -140,6 +140,12 @@ int w2w_backward(const W2WFrame* f, const W2WWorkspace* ws, const float* dL_dcol
  * Gaussian) pairs, Gaussian-major within a scene: item = item_off(scene) +
  * g * views + v.  Per-frame outputs are packed in frame order: colour
  * [3,H,W] blocks, and depth / alpha [H,W] blocks.
+ * scene_base / scene_sh_off (optional): the row of each scene's first Gaussian
+ * in the parameter and per-Gaussian gradient arrays and the float offset of
+ * its SH block, for scenes that stay resident at fixed places in a larger
+ * store (NULL = packed back to back in batch order).  frame_target_off
+ * (optional): float offset of each frame's [3,H,W] loss target in the target
+ * buffer w2w_l2_loss reads (NULL = packed like the colour output).
  * totals (optional, 6 int64): Gaussians, items, frames, tiles, pixels, SH
  * floats of the encoded batch. */
 size_t w2w_batch_table_bytes(int32_t scenes_cap, int32_t frames_cap);
@@ -148,7 +154,9 @@ int w2w_batch_encode(void* host_table, size_t table_bytes_avail, int32_t scenes_
                      const int32_t* scene_views, const int32_t* scene_M, const int32_t* scene_D,
                      const float* frame_view, const float* frame_proj, const float* frame_campos,
                      const float* frame_bg, const float* frame_focal, const int32_t* frame_wh,
-                     const uint32_t* frame_flags, int64_t* totals);
+                     const uint32_t* frame_flags, const int64_t* scene_base,
+                     const int64_t* scene_sh_off, const int64_t* frame_target_off,
+                     int64_t* totals);
 int w2w_batch_workspace_layout(const W2WCaps* caps, W2WLayout* out);
 /* radii per item. */
 int w2w_forward_batch(const void* table, const W2WCaps* caps, const W2WWorkspace* ws,
@@ -194,6 +202,51 @@ int w2w_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, i
 int w2w_adam_step(float* params, const float* grads, float* exp_avg, float* exp_avg_sq, int64_t n,
                   int32_t n_groups, const int64_t* group_end, const float* group_lr, int64_t step,
                   float beta1, float beta2, float eps, void* stream);
+
+/* ---- step-level kernels of the batched trainer (no autograd, capturable) ----
+ * Activation of Renderer.py:164-167 over G Gaussians: scales = exp(svec),
+ * opacity = sigmoid(logit), rotations = q / max(|q|, 1e-12); and its
+ * backward from the gradients w.r.t. the activated values. */
+int w2w_activate(const float* svec, const float* logit, const float* quat, int64_t G, float* scales,
+                 float* opac, float* rots, void* stream);
+int w2w_activate_backward(const float* scales, const float* opac, const float* quat,
+                          const float* g_scales, const float* g_opac, const float* g_rots,
+                          int64_t G, float* g_svec, float* g_logit, float* g_quat, void* stream);
+/* Per frame: mean over 3*H*W of (clamp(color,0,1) - target)^2 into
+ * loss_per_frame[frame] and its gradient into dL_dcolor (zero for frames
+ * without the backward flag); colour/target packed as the batch outputs. */
+int w2w_l2_loss(const void* table, const W2WCaps* caps, const float* color, const float* target,
+                float* dL_dcolor, float* loss_per_frame, void* stream);
+/* Adam over a flat store with the step's scalars in device memory (so a
+ * captured graph picks them up at replay): hyper[0..7] = -lr_k/(1-b1^t) per
+ * group, hyper[8..15] = the same for a group's second rate, hyper[16] =
+ * 1/sqrt(1-b2^t), hyper[17..20] = 1-b1, b2, 1-b2, eps.  Group k may alternate
+ * rates: element e uses the second rate when (e mod group_period[k]) >=
+ * group_split[k] (period <= 0: one rate). */
+int w2w_adam_step_hyper(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
+                        int64_t n, int32_t n_groups, const int64_t* group_end,
+                        const int32_t* group_period, const int32_t* group_split,
+                        const float* hyper, void* stream);
+
+/* Range tables for stores whose scenes stay resident at fixed rows.  A row
+ * range {int64 start, off, len} covers Gaussian rows [start, start+len) and
+ * sits at [off, off+len) of the concatenated work index space (ranges sorted
+ * by off, contiguous, total = sum of len). */
+int w2w_activate_rows(const float* svec, const float* logit, const float* quat, const void* ranges,
+                      int32_t n_ranges, int64_t total_rows, float* scales, float* opac, float* rots,
+                      void* stream);
+int w2w_activate_backward_rows(const float* scales, const float* opac, const float* quat,
+                               const float* g_scales, const float* g_opac, const float* g_rots,
+                               const void* ranges, int32_t n_ranges, int64_t total_rows,
+                               float* g_svec, float* g_logit, float* g_quat, void* stream);
+/* Adam over parameter ranges {int64 start, off, len; int32 hyper; uint32
+ * period, split, pad} of a flat store: range r uses hyper[4*r.hyper + 0]
+ * (-lr/(1-b1^t)), [+1] the second rate (elements with (e mod period) >=
+ * split, period 0 = none) and [+2] 1/sqrt(1-b2^t), so jobs with different
+ * step counts and learning rates share one launch. */
+int w2w_adam_ranges(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
+                    const void* ranges, int32_t n_ranges, int64_t total, const float* hyper,
+                    float beta1, float beta2, float eps, void* stream);
 
 /* Optional per-kernel timing: HIP event pairs recorded on the launch stream
  * around every launch whose kernel id bit is set in `kernel_mask`, into a

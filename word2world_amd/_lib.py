@@ -49,7 +49,9 @@ class W2WCaps(ctypes.Structure):
 EXPORTS = ["w2w_version", "w2w_workspace_layout", "w2w_batch_workspace_layout",
            "w2w_batch_table_bytes", "w2w_batch_encode", "w2w_forward_batch", "w2w_backward_batch", "w2w_forward", "w2w_backward", "w2w_sorted_keys",
            "w2w_sort_temp_bytes", "w2w_sort_pairs_u64", "w2w_scan_temp_bytes",
-           "w2w_exclusive_scan_u32", "w2w_prof_enable", "w2w_prof_reset", "w2w_prof_read", "w2w_stats_enable", "w2w_adam_step"]
+           "w2w_exclusive_scan_u32", "w2w_prof_enable", "w2w_prof_reset", "w2w_prof_read", "w2w_stats_enable", "w2w_adam_step", "w2w_activate", "w2w_activate_backward",
+           "w2w_l2_loss", "w2w_adam_step_hyper", "w2w_activate_rows", "w2w_activate_backward_rows",
+           "w2w_adam_ranges"]
 
 KERNEL_IDS = {0: "fill", 1: "preprocess", 2: "depth_sort_hist", 3: "depth_sort_pass", 4: "scan",
               5: "emit", 6: "tile_sort_hist", 7: "tile_sort_pass", 8: "ranges", 9: "blend",
@@ -78,7 +80,7 @@ def load(path: str = LIB_PATH):
     L.w2w_batch_table_bytes.argtypes = [ctypes.c_int32, ctypes.c_int32]
     L.w2w_batch_encode.restype = ctypes.c_int
     L.w2w_batch_encode.argtypes = [_vp, ctypes.c_size_t, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
-                                   ctypes.c_int32] + [_vp] * 11 + [_vp]
+                                   ctypes.c_int32] + [_vp] * 14 + [_vp]
     L.w2w_forward_batch.restype = ctypes.c_int
     L.w2w_forward_batch.argtypes = [_vp, P(W2WCaps), P(W2WWorkspace)] + [_vp] * 10
     L.w2w_backward_batch.restype = ctypes.c_int
@@ -109,6 +111,22 @@ def load(path: str = LIB_PATH):
     L.w2w_adam_step.restype = ctypes.c_int
     L.w2w_adam_step.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp,
                                 ctypes.c_int64, ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp]
+    L.w2w_activate.restype = ctypes.c_int
+    L.w2w_activate.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp, _vp, _vp, _vp]
+    L.w2w_activate_backward.restype = ctypes.c_int
+    L.w2w_activate_backward.argtypes = [_vp] * 6 + [ctypes.c_int64] + [_vp] * 4
+    L.w2w_l2_loss.restype = ctypes.c_int
+    L.w2w_l2_loss.argtypes = [_vp, P(W2WCaps), _vp, _vp, _vp, _vp, _vp]
+    L.w2w_adam_step_hyper.restype = ctypes.c_int
+    L.w2w_adam_step_hyper.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp,
+                                      _vp, _vp, _vp]
+    L.w2w_activate_rows.restype = ctypes.c_int
+    L.w2w_activate_rows.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp, _vp]
+    L.w2w_activate_backward_rows.restype = ctypes.c_int
+    L.w2w_activate_backward_rows.argtypes = [_vp] * 7 + [ctypes.c_int32, ctypes.c_int64] + [_vp] * 4
+    L.w2w_adam_ranges.restype = ctypes.c_int
+    L.w2w_adam_ranges.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int64, _vp,
+                                  ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp]
     _lib = L
     return L
 

@@ -64,9 +64,11 @@ def caps_for(scenes: list, tile: int, capacity: int, scenes_cap: int | None = No
     return c
 
 
-def encode(scenes: list, caps: _lib.W2WCaps) -> tuple[np.ndarray, np.ndarray]:
+def encode(scenes: list, caps: _lib.W2WCaps, bases=None, sh_offs=None,
+           target_offs=None) -> tuple[np.ndarray, np.ndarray]:
     """Host bytes of the batch table and the totals (G, items, frames, tiles,
-    pixels, SH floats)."""
+    pixels, SH floats).  `bases` / `sh_offs`: per-scene row and SH float offset
+    in a resident store (None = packed in batch order)."""
     L = _lib.load()
     nbytes = L.w2w_batch_table_bytes(caps.scenes, caps.frames)
     buf = np.zeros(nbytes, np.uint8)
@@ -89,11 +91,16 @@ def encode(scenes: list, caps: _lib.W2WCaps) -> tuple[np.ndarray, np.ndarray]:
     sm = np.array([s.n_coeffs for s in scenes] or [1], np.int32)
     sd = np.array([s.sh_degree for s in scenes] or [0], np.int32)
     totals = np.zeros(6, np.int64)
+    b = None if bases is None else np.ascontiguousarray(bases, np.int64)
+    so = None if sh_offs is None else np.ascontiguousarray(sh_offs, np.int64)
+    to = None if target_offs is None else np.ascontiguousarray(target_offs, np.int64)
     st = L.w2w_batch_encode(buf.ctypes.data, nbytes, caps.scenes, caps.frames, caps.tile,
                             len(scenes), sn.ctypes.data, sv.ctypes.data, sm.ctypes.data,
                             sd.ctypes.data, view.ctypes.data, proj.ctypes.data, cp.ctypes.data,
                             bg.ctypes.data, foc.ctypes.data, wh.ctypes.data, flags.ctypes.data,
-                            totals.ctypes.data)
+                            None if b is None else b.ctypes.data,
+                            None if so is None else so.ctypes.data,
+                            None if to is None else to.ctypes.data, totals.ctypes.data)
     _lib.check(st, "w2w_batch_encode")
     return buf, totals
 
@@ -124,9 +131,9 @@ class BatchWorkspace:
         self.radii = torch.empty(max(P, 1), dtype=torch.int32, device=device)
         self.totals = np.zeros(6, np.int64)
 
-    def load_table(self, scenes: list) -> np.ndarray:
+    def load_table(self, scenes: list, bases=None, sh_offs=None, target_offs=None) -> np.ndarray:
         """Encode on the host and copy into the device table (stream-ordered)."""
-        buf, totals = encode(scenes, self.caps)
+        buf, totals = encode(scenes, self.caps, bases, sh_offs, target_offs)
         if self._copied is not None:
             self._copied.synchronize()  # the pinned staging buffer is free again
         self.table_host.numpy()[:] = buf

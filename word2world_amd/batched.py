@@ -1,0 +1,208 @@
+"""Batched trainer: many scenes optimised together, one launch per stage.
+
+All scenes' parameters live in one flat HBM store, group-major:
+``means [G,3] | sh [G,M,3] | opacity logits [G] | svec [G,3] | quaternion
+[G,4]`` with the scenes back to back inside each group, so the SH group *is*
+the packed SH array the rasterizer reads and the means group its packed
+means.  One step is a fixed sequence of native launches -- activation,
+batched forward, L2 loss, batched backward (writing the means and SH
+gradients straight into the flat gradient), activation backward, Adam -- with
+no autograd and no host synchronisation, so it can be captured once in a HIP
+graph and replayed; the per-step learning rates reach the captured Adam
+kernel through a small device buffer refreshed before each replay.
+
+Learning rates follow optimizer_config.json (means 0.01 with the
+misc_utils.py:11-27 schedule, SH DC 0.01, SH rest 0.0005, opacity 0.05, svec
+0.005, quaternion 0.005); Adam eps 1e-15 (gaussians_handler.py:40).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib, batch
+from .train import LRS, mean_lr
+
+GROUPS = ("means", "sh", "opacity", "svec", "quaternion")
+
+
+class BatchedStore:
+    def __init__(self, scenes: list, device):
+        """scenes: list of word2world_amd.scene.GaussianScene (equal SH width)."""
+        self.device = torch.device(device)
+        ms = {s.sh_ac.shape[1] + 1 for s in scenes}
+        if len(ms) != 1:
+            raise ValueError("all scenes of a batched store need the same SH coefficient count")
+        self.M = ms.pop()
+        self.ns = [s.n for s in scenes]
+        self.degrees = [s.sh_degree for s in scenes]
+        self.G = G = sum(self.ns)
+        cat = lambda f: torch.cat([f(s) for s in scenes]).reshape(-1).float()
+        parts = [cat(lambda s: s.means), cat(lambda s: torch.cat((s.sh_dc, s.sh_ac), 1)),
+                 cat(lambda s: s.opacity), cat(lambda s: s.svec), cat(lambda s: s.quaternion)]
+        self.sizes = [3 * G, 3 * self.M * G, G, 3 * G, 4 * G]
+        self.ends = np.cumsum(self.sizes).astype(np.int64)
+        self.flat = torch.cat(parts).to(self.device).contiguous()
+        self.grad = torch.zeros_like(self.flat)
+        self.exp_avg = torch.zeros_like(self.flat)
+        self.exp_avg_sq = torch.zeros_like(self.flat)
+        self.step_count = 0
+        f32 = dict(dtype=torch.float32, device=self.device)
+        self.scales = torch.empty(G, 3, **f32)
+        self.opac = torch.empty(G, **f32)
+        self.rots = torch.empty(G, 4, **f32)
+        self.g_scales = torch.empty(G, 3, **f32)
+        self.g_opac = torch.empty(G, **f32)
+        self.g_rots = torch.empty(G, 4, **f32)
+        self.g_cov = torch.empty(G, 6, **f32)
+        self.hyper = torch.zeros(32, **f32)
+        # ring of pinned staging buffers: a buffer is rewritten only after the
+        # copy that last read it has completed
+        self._ring = [torch.zeros(32, dtype=torch.float32).pin_memory() for _ in range(4)]
+        self._ring_ev = [None] * 4
+        self._ring_i = 0
+
+    def upload_hyper(self, h: np.ndarray) -> None:
+        k = self._ring_i
+        self._ring_i = (k + 1) % len(self._ring)
+        if self._ring_ev[k] is not None:
+            self._ring_ev[k].synchronize()
+        self._ring[k].numpy()[:] = h
+        self.hyper.copy_(self._ring[k], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        self._ring_ev[k] = ev
+
+    def group(self, t: torch.Tensor, name: str) -> torch.Tensor:
+        k = GROUPS.index(name)
+        a = 0 if k == 0 else int(self.ends[k - 1])
+        return t[a:int(self.ends[k])]
+
+    def view(self, name: str, t: torch.Tensor | None = None) -> torch.Tensor:
+        t = self.flat if t is None else t
+        g = self.group(t, name)
+        shapes = {"means": (self.G, 3), "sh": (self.G, self.M, 3), "opacity": (self.G,),
+                  "svec": (self.G, 3), "quaternion": (self.G, 4)}
+        return g.view(shapes[name])
+
+    def hyper_values(self, step: int, lrs: dict, b1=0.9, b2=0.999, eps=1e-15) -> np.ndarray:
+        bc1 = 1.0 - b1 ** step
+        bc2 = 1.0 - b2 ** step
+        h = np.zeros(32, np.float32)
+        h[0] = -lrs["means"] / bc1
+        h[1] = -lrs["sh_dc"] / bc1
+        h[8 + 1] = -lrs["sh_ac"] / bc1
+        h[2] = -lrs["opacity"] / bc1
+        h[3] = -lrs["svec"] / bc1
+        h[4] = -lrs["quaternion"] / bc1
+        h[16] = 1.0 / np.sqrt(bc2)
+        h[17], h[18], h[19], h[20] = 1.0 - b1, b2, 1.0 - b2, eps
+        return h
+
+
+class BatchedTrainer:
+    """fwd + bwd + Adam for every scene of the batch in one launch sequence."""
+
+    def __init__(self, scenes: list, cameras: list, targets: list | None, device, tile: int = 16,
+                 capacity: int | None = None):
+        """cameras[k]: word2world_amd.camera.Camera views of scene k;
+        targets[k][v]: [3,H,W] target image (or None for render-only)."""
+        self.store = BatchedStore(scenes, device)
+        self.device = torch.device(device)
+        self.tile = tile
+        self.specs = [batch.SceneSpec(s.n, s.sh_degree, self.store.M,
+                                      [batch.frame_from_camera(c) for c in cams])
+                      for s, cams in zip(scenes, cameras)]
+        cap = capacity if capacity is not None else max(1 << 20, 48 * sum(
+            s.n * len(c) for s, c in zip(scenes, cameras)))
+        self.caps = batch.caps_for(self.specs, tile, cap)
+        self.ws = batch.BatchWorkspace(self.caps, device)
+        self.ws.load_table(self.specs)
+        self.grads = batch.BatchGrads(self.caps, 0, device)
+        pix = int(self.caps.pixels)
+        self.target = torch.zeros(3 * pix, dtype=torch.float32, device=device)
+        if targets is not None:
+            o = 0
+            for tl in targets:
+                for t in tl:
+                    n = t.numel()
+                    self.target[o:o + n] = t.reshape(-1)
+                    o += n
+        self.dL_dcolor = torch.empty(3 * pix, dtype=torch.float32, device=device)
+        self.loss = torch.zeros(max(int(self.caps.frames), 1), dtype=torch.float32, device=device)
+        self.graph = None
+        self.it = 0
+        st = self.store
+        self._period = np.array([0, 3 * st.M, 0, 0, 0], np.int32)
+        self._split = np.array([0, 3, 0, 0, 0], np.int32)
+
+    # ---- the launch sequence ----
+    def _body(self):
+        L = _lib.load()
+        st, ws, g = self.store, self.ws, self.grads
+        s = torch.cuda.current_stream().cuda_stream
+        G = st.G
+        means = st.view("means")
+        sh = st.group(st.flat, "sh")
+        _lib.check(L.w2w_activate(st.view("svec").data_ptr(), st.view("opacity").data_ptr(),
+                                  st.view("quaternion").data_ptr(), G, st.scales.data_ptr(),
+                                  st.opac.data_ptr(), st.rots.data_ptr(), s), "w2w_activate")
+        _lib.check(L.w2w_forward_batch(ws.table.data_ptr(), ctypes.byref(self.caps),
+                                       ctypes.byref(ws.struct), means.data_ptr(), st.scales.data_ptr(),
+                                       st.rots.data_ptr(), st.opac.data_ptr(), sh.data_ptr(),
+                                       ws.color.data_ptr(), ws.depth.data_ptr(), ws.alpha.data_ptr(),
+                                       None, s), "w2w_forward_batch")
+        _lib.check(L.w2w_l2_loss(ws.table.data_ptr(), ctypes.byref(self.caps), ws.color.data_ptr(),
+                                 self.target.data_ptr(), self.dL_dcolor.data_ptr(),
+                                 self.loss.data_ptr(), s), "w2w_l2_loss")
+        _lib.check(L.w2w_backward_batch(
+            ws.table.data_ptr(), ctypes.byref(self.caps), ctypes.byref(ws.struct), means.data_ptr(),
+            st.scales.data_ptr(), st.rots.data_ptr(), sh.data_ptr(), self.dL_dcolor.data_ptr(), None,
+            None, g.means2D.data_ptr(), g.colors.data_ptr(), st.g_opac.data_ptr(),
+            st.view("means", st.grad).data_ptr(), st.g_cov.data_ptr(),
+            st.group(st.grad, "sh").data_ptr(), st.g_scales.data_ptr(), st.g_rots.data_ptr(), s),
+            "w2w_backward_batch")
+        _lib.check(L.w2w_activate_backward(
+            st.scales.data_ptr(), st.opac.data_ptr(), st.view("quaternion").data_ptr(),
+            st.g_scales.data_ptr(), st.g_opac.data_ptr(), st.g_rots.data_ptr(), G,
+            st.view("svec", st.grad).data_ptr(), st.view("opacity", st.grad).data_ptr(),
+            st.view("quaternion", st.grad).data_ptr(), s), "w2w_activate_backward")
+        _lib.check(L.w2w_adam_step_hyper(st.flat.data_ptr(), st.grad.data_ptr(),
+                                         st.exp_avg.data_ptr(), st.exp_avg_sq.data_ptr(),
+                                         int(st.ends[-1]), len(GROUPS), st.ends.ctypes.data,
+                                         self._period.ctypes.data, self._split.ctypes.data,
+                                         st.hyper.data_ptr(), s), "w2w_adam_step_hyper")
+
+    def _upload_hyper(self):
+        self.it += 1
+        st = self.store
+        h = st.hyper_values(self.it, dict(LRS, means=mean_lr(self.it), sh_dc=LRS["sh_dc"],
+                                          sh_ac=LRS["sh_ac"]))
+        st.upload_hyper(h)
+
+    def step(self):
+        """One fwd+bwd+Adam step of every scene (graph replay when captured)."""
+        self._upload_hyper()
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self._body()
+        return self.loss
+
+    def capture(self):
+        """Capture the launch sequence into a HIP graph (after one eager step
+        has run, so every buffer exists)."""
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._body()
+        torch.cuda.synchronize()
+        self.graph = g
+
+    def overflow(self) -> bool:
+        return bool(int(self.ws.status()[1]))
+
+    def frame_images(self):
+        return self.ws.frame_views(self.specs)

@@ -23,7 +23,7 @@ LIB_NAME = "libw2w_raster.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 SOURCES = ["csrc/w2w_capi.hip"]
 DEPS = ["csrc/w2w_capi.hip", "csrc/w2w_device.hpp", "csrc/w2w_sort.hpp", "csrc/w2w_forward.hpp",
-        "csrc/w2w_backward.hpp", "csrc/w2w_adam.hpp", "../include/w2w_raster.h"]
+        "csrc/w2w_backward.hpp", "csrc/w2w_adam.hpp", "csrc/w2w_step.hpp", "../include/w2w_raster.h"]
 ARCH = os.environ.get("W2W_OFFLOAD_ARCH", "gfx950")
 
 

@@ -247,10 +247,11 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreBwdArgs a) {
   const SceneDesc* scenes = table_scenes(a.table);
   const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t gg = blockIdx.x * blockDim.x + threadIdx.x; gg < G; gg += stride) {
-    const uint32_t si = upper_index(n_scenes, gg, [&](uint32_t k) { return scenes[k].g_off; });
+  for (uint32_t bg = blockIdx.x * blockDim.x + threadIdx.x; bg < G; bg += stride) {
+    const uint32_t si = upper_index(n_scenes, bg, [&](uint32_t k) { return scenes[k].g_off; });
     const SceneDesc& sd = scenes[si];
-    const uint32_t g = gg - sd.g_off;
+    const uint32_t g = bg - sd.g_off;
+    const uint32_t gg = sd.g_base + g;  // row in the parameter / gradient arrays
     float gm[3] = {0.0f, 0.0f, 0.0f};
     float dcov[6] = {0, 0, 0, 0, 0, 0};
     float dsc[3] = {0, 0, 0};

@@ -18,6 +18,7 @@
 #include "w2w_device.hpp"
 #include "w2w_forward.hpp"
 #include "w2w_sort.hpp"
+#include "w2w_step.hpp"
 
 using namespace w2w;
 
@@ -191,7 +192,7 @@ __global__ void k_make_single(void* table, const float* __restrict__ view,
   h->n_tiles = (uint32_t)(tx * ty); h->pad0 = h->pad1 = h->pad2 = 0;
   h->n_pixels = (uint64_t)W * H; h->pad3 = 0;
   sd->g_off = 0; sd->n = P; sd->item_off = 0; sd->frame_off = 0; sd->n_views = 1;
-  sd->M = M; sd->D = D; sd->pad = 0; sd->sh_off = 0; sd->pad2 = 0;
+  sd->M = M; sd->D = D; sd->g_base = 0; sd->sh_off = 0; sd->pad2 = 0;
   for (int k = 0; k < 16; k++) {
     fd->V[k] = view[k];
     fd->F[k] = proj[k];
@@ -202,7 +203,7 @@ __global__ void k_make_single(void* table, const float* __restrict__ view,
   }
   fd->fx = fx; fd->fy = fy; fd->W = W; fd->H = H; fd->tiles_x = tx; fd->tiles_y = ty;
   fd->tile_off = 0; fd->scene = 0; fd->view = 0; fd->flags = FRAME_BACKWARD;
-  fd->pix_off = 0; fd->pad = 0;
+  fd->pix_off = 0; fd->target_off = 0;
 }
 
 template <int TS>
@@ -455,7 +456,9 @@ int w2w_batch_encode(void* host_table, size_t table_bytes_avail, int32_t scenes_
                      const int32_t* scene_views, const int32_t* scene_M, const int32_t* scene_D,
                      const float* frame_view, const float* frame_proj, const float* frame_campos,
                      const float* frame_bg, const float* frame_focal, const int32_t* frame_wh,
-                     const uint32_t* frame_flags, int64_t* totals) {
+                     const uint32_t* frame_flags, const int64_t* scene_base,
+                     const int64_t* scene_sh_off, const int64_t* frame_target_off,
+                     int64_t* totals) {
   if (!host_table || scenes_cap < 1 || frames_cap < 1 || n_scenes < 0 || n_scenes > scenes_cap)
     return W2W_ERR_INVALID;
   if (tile != 8 && tile != 16 && tile != 32) return W2W_ERR_INVALID;
@@ -474,7 +477,11 @@ int w2w_batch_encode(void* host_table, size_t table_bytes_avail, int32_t scenes_
     if (frames + (uint64_t)scene_views[k] > (uint64_t)frames_cap) return W2W_ERR_CAPACITY;
     sd[k].g_off = (uint32_t)g; sd[k].n = (uint32_t)scene_n[k]; sd[k].item_off = (uint32_t)items;
     sd[k].frame_off = (uint32_t)frames; sd[k].n_views = (uint32_t)scene_views[k];
-    sd[k].M = scene_M[k]; sd[k].D = scene_D[k]; sd[k].sh_off = shf;
+    sd[k].M = scene_M[k]; sd[k].D = scene_D[k];
+    sd[k].g_base = scene_base ? (uint32_t)scene_base[k] : (uint32_t)g;
+    sd[k].sh_off = scene_sh_off ? (uint64_t)scene_sh_off[k] : shf;
+    if (scene_base && (scene_base[k] < 0 || scene_base[k] + scene_n[k] >= (int64_t)LB_VAL))
+      return W2W_ERR_INVALID;
     for (int32_t v = 0; v < scene_views[k]; v++) {
       const uint64_t f = frames + v;
       FrameDesc& F = fd[f];
@@ -490,6 +497,7 @@ int w2w_batch_encode(void* host_table, size_t table_bytes_avail, int32_t scenes_
       F.tile_off = (uint32_t)tiles; F.scene = (uint32_t)k; F.view = (uint32_t)v;
       F.flags = frame_flags ? frame_flags[f] : FRAME_BACKWARD;
       F.pix_off = pix;
+      F.target_off = frame_target_off ? (uint64_t)frame_target_off[f] : 3 * pix;
       tiles += (uint64_t)F.tiles_x * F.tiles_y;
       pix += (uint64_t)F.W * F.H;
     }
@@ -733,6 +741,132 @@ int w2w_adam_step(float* params, const float* grads, float* exp_avg, float* exp_
   const uint32_t grid = (uint32_t)std::min<uint64_t>(((uint64_t)n + 255) / 256, 8192);
   ProfSpan ps(W2W_KID_ADAM, s);
   hipLaunchKernelGGL(k_adam, dim3(grid), dim3(256), 0, s, a);
+  return launch_status();
+}
+
+int w2w_activate(const float* svec, const float* logit, const float* quat, int64_t G, float* scales,
+                 float* opac, float* rots, void* stream) {
+  if (G < 0 || G >= (int64_t)LB_VAL) return W2W_ERR_INVALID;
+  if (G == 0) return W2W_OK;
+  if (!svec || !logit || !quat || !scales || !opac || !rots) return W2W_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_activate, dim3(std::min<uint32_t>(cdiv((uint64_t)G, 256), 8192)), dim3(256), 0,
+                     s, svec, logit, quat, (uint64_t)G, (const RowRange*)nullptr, 0u, scales, opac,
+                     rots);
+  return launch_status();
+}
+
+int w2w_activate_backward(const float* scales, const float* opac, const float* quat,
+                          const float* g_scales, const float* g_opac, const float* g_rots,
+                          int64_t G, float* g_svec, float* g_logit, float* g_quat, void* stream) {
+  if (G < 0 || G >= (int64_t)LB_VAL) return W2W_ERR_INVALID;
+  if (G == 0) return W2W_OK;
+  if (!scales || !opac || !quat || !g_scales || !g_opac || !g_rots || !g_svec || !g_logit || !g_quat)
+    return W2W_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_activate_bwd, dim3(std::min<uint32_t>(cdiv((uint64_t)G, 256), 8192)), dim3(256),
+                     0, s, scales, opac, quat, g_scales, g_opac, g_rots, (uint64_t)G,
+                     (const RowRange*)nullptr, 0u, g_svec, g_logit, g_quat);
+  return launch_status();
+}
+
+int w2w_l2_loss(const void* table, const W2WCaps* caps, const float* color, const float* target,
+                float* dL_dcolor, float* loss_per_frame, void* stream) {
+  if (!table || !caps || !color || !target || !dL_dcolor || !loss_per_frame || caps->frames < 1 ||
+      caps->scenes < 1 || caps->pixels < 1)
+    return W2W_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  (void)hipGetLastError();
+  fill_u32(reinterpret_cast<uint32_t*>(loss_per_frame), 0u, (size_t)caps->frames, s);
+  hipLaunchKernelGGL(k_l2_loss, dim3(std::min<uint32_t>(cdiv((uint64_t)caps->pixels, 256), 8192)),
+                     dim3(256), 0, s, table, (uint32_t)caps->scenes, (uint64_t)caps->pixels, color,
+                     target, dL_dcolor, loss_per_frame);
+  return launch_status();
+}
+
+int w2w_adam_step_hyper(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
+                        int64_t n, int32_t n_groups, const int64_t* group_end,
+                        const int32_t* group_period, const int32_t* group_split,
+                        const float* hyper, void* stream) {
+  if (n < 0 || n_groups < 1 || n_groups > ADAM_MAX_SEGMENTS || !group_end || !hyper)
+    return W2W_ERR_INVALID;
+  if (n == 0) return W2W_OK;
+  if (!params || !grads || !exp_avg || !exp_avg_sq) return W2W_ERR_INVALID;
+  AdamGraphArgs a;
+  a.p = params; a.g = grads; a.m = exp_avg; a.v = exp_avg_sq; a.n = (uint64_t)n;
+  a.nseg = n_groups; a.hyper = hyper;
+  int64_t prev = 0;
+  for (int k = 0; k < ADAM_MAX_SEGMENTS; k++) {
+    if (k < n_groups) {
+      if (group_end[k] < prev || group_end[k] > n) return W2W_ERR_INVALID;
+      prev = group_end[k];
+      a.seg_end[k] = (uint64_t)group_end[k];
+      const int32_t per = group_period ? group_period[k] : 0;
+      const int32_t spl = group_split ? group_split[k] : 0;
+      if (per > 0 && spl >= 0 && spl < per) {
+        a.period[k] = (uint32_t)per; a.split[k] = (uint32_t)spl; a.lr2[k] = 8 + k;
+      } else {
+        a.period[k] = 1; a.split[k] = 1; a.lr2[k] = -1;
+      }
+    } else {
+      a.seg_end[k] = (uint64_t)n; a.period[k] = 1; a.split[k] = 1; a.lr2[k] = -1;
+    }
+  }
+  if (group_end[n_groups - 1] != n) return W2W_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  (void)hipGetLastError();
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(((uint64_t)n + 255) / 256, 8192);
+  ProfSpan ps(W2W_KID_ADAM, s);
+  hipLaunchKernelGGL(k_adam_hyper, dim3(grid), dim3(256), 0, s, a);
+  return launch_status();
+}
+
+int w2w_activate_rows(const float* svec, const float* logit, const float* quat, const void* ranges,
+                      int32_t n_ranges, int64_t total_rows, float* scales, float* opac, float* rots,
+                      void* stream) {
+  if (n_ranges < 1 || total_rows < 0 || !ranges) return W2W_ERR_INVALID;
+  if (total_rows == 0) return W2W_OK;
+  if (!svec || !logit || !quat || !scales || !opac || !rots) return W2W_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_activate, dim3(std::min<uint32_t>(cdiv((uint64_t)total_rows, 256), 8192)),
+                     dim3(256), 0, s, svec, logit, quat, (uint64_t)total_rows,
+                     static_cast<const RowRange*>(ranges), (uint32_t)n_ranges, scales, opac, rots);
+  return launch_status();
+}
+
+int w2w_activate_backward_rows(const float* scales, const float* opac, const float* quat,
+                               const float* g_scales, const float* g_opac, const float* g_rots,
+                               const void* ranges, int32_t n_ranges, int64_t total_rows,
+                               float* g_svec, float* g_logit, float* g_quat, void* stream) {
+  if (n_ranges < 1 || total_rows < 0 || !ranges) return W2W_ERR_INVALID;
+  if (total_rows == 0) return W2W_OK;
+  if (!scales || !opac || !quat || !g_scales || !g_opac || !g_rots || !g_svec || !g_logit || !g_quat)
+    return W2W_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(k_activate_bwd, dim3(std::min<uint32_t>(cdiv((uint64_t)total_rows, 256), 8192)),
+                     dim3(256), 0, s, scales, opac, quat, g_scales, g_opac, g_rots,
+                     (uint64_t)total_rows, static_cast<const RowRange*>(ranges), (uint32_t)n_ranges,
+                     g_svec, g_logit, g_quat);
+  return launch_status();
+}
+
+int w2w_adam_ranges(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
+                    const void* ranges, int32_t n_ranges, int64_t total, const float* hyper,
+                    float beta1, float beta2, float eps, void* stream) {
+  if (n_ranges < 1 || total < 0 || !ranges || !hyper) return W2W_ERR_INVALID;
+  if (total == 0) return W2W_OK;
+  if (!params || !grads || !exp_avg || !exp_avg_sq) return W2W_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  (void)hipGetLastError();
+  ProfSpan ps(W2W_KID_ADAM, s);
+  hipLaunchKernelGGL(k_adam_ranges, dim3(std::min<uint32_t>(cdiv((uint64_t)total, 256), 8192)),
+                     dim3(256), 0, s, params, grads, exp_avg, exp_avg_sq,
+                     static_cast<const ParamRange*>(ranges), (uint32_t)n_ranges, (uint64_t)total,
+                     hyper, (float)(1.0 - (double)beta1), beta2, (float)(1.0 - (double)beta2), eps);
   return launch_status();
 }
 

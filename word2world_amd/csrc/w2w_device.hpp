@@ -134,13 +134,14 @@ struct BatchHeader {
   uint64_t n_pixels, pad3;
 };
 struct SceneDesc {
-  uint32_t g_off;      // first Gaussian in the packed parameter arrays
+  uint32_t g_off;      // first Gaussian of the scene in the batch's Gaussian order
   uint32_t n;          // Gaussians
   uint32_t item_off;   // first item
   uint32_t frame_off;  // first frame
   uint32_t n_views;    // frames of this scene
   int32_t M, D;        // SH coefficients per Gaussian, active degree
-  uint32_t pad;
+  uint32_t g_base;     // row of the scene's first Gaussian in the parameter and
+                       // per-Gaussian gradient arrays (== g_off when packed)
   uint64_t sh_off;     // first float of the scene's SH block [n, M, 3]
   uint64_t pad2;
 };
@@ -151,7 +152,8 @@ struct FrameDesc {
   float bg[3], fy;     // background, focal y
   int32_t W, H, tiles_x, tiles_y;
   uint32_t tile_off, scene, view, flags;  // flags bit 0: backward wanted
-  uint64_t pix_off, pad;
+  uint64_t pix_off;    // first pixel in the packed per-pixel outputs
+  uint64_t target_off; // first float of this frame's [3,H,W] loss target
 };
 constexpr uint32_t FRAME_BACKWARD = 1u;
 constexpr size_t TABLE_HEADER_BYTES = 64;

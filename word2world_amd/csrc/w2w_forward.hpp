@@ -59,7 +59,7 @@ W2W_DEV ItemRef item_ref(const SceneDesc* scenes, uint32_t n_scenes, uint32_t i)
   r.g = local / sd.n_views;
   r.view = local - r.g * sd.n_views;
   r.frame = sd.frame_off + r.view;
-  r.gg = sd.g_off + r.g;
+  r.gg = sd.g_base + r.g;
   return r;
 }
 

@@ -1,0 +1,234 @@
+// Step-level kernels of the batched trainer: parameter activation and its
+// backward, the L2 image loss, and Adam with hyper-parameters in device memory
+// (so a captured HIP graph picks up the learning-rate schedule at replay).
+//
+// Activation as Renderer.py:164-167: opacity = sigmoid(logit), scale =
+// exp(svec), rotation = q / max(|q|, 1e-12) (torch.nn.functional.normalize).
+// Loss as the benchmark workloads define it: per frame, the mean over 3*H*W of
+// (clamp(colour, 0, 1) - target)^2, summed over the frames that take part in
+// the backward (GaussianTrainer.py:86 clamps the render before the loss).
+#pragma once
+#include "w2w_adam.hpp"
+#include "w2w_device.hpp"
+#include "w2w_forward.hpp"
+
+namespace w2w {
+
+// activation (and its backward) over Gaussian-row ranges of a resident store
+struct RowRange {
+  uint64_t start, off, len;
+};
+
+W2W_DEV uint32_t row_range_of(const RowRange* r, uint32_t n, uint64_t w) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (r[mid].off <= w) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Work index w -> row of the store: identity without a range table.
+W2W_DEV uint64_t row_of(const RowRange* r, uint32_t n, uint64_t w) {
+  if (!r) return w;
+  const RowRange& x = r[row_range_of(r, n, w)];
+  return x.start + (w - x.off);
+}
+
+__global__ __launch_bounds__(256) void k_activate(const float* __restrict__ svec,
+                                                  const float* __restrict__ logit,
+                                                  const float* __restrict__ quat, uint64_t G,
+                                                  const RowRange* __restrict__ ranges,
+                                                  uint32_t n_ranges, float* __restrict__ scales,
+                                                  float* __restrict__ opac,
+                                                  float* __restrict__ rots) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < G; w += stride) {
+    const uint64_t i = row_of(ranges, n_ranges, w);
+#pragma unroll
+    for (int k = 0; k < 3; k++) scales[3 * i + k] = expf(svec[3 * i + k]);
+    opac[i] = 1.0f / (1.0f + expf(-logit[i]));
+    const float q0 = quat[4 * i], q1 = quat[4 * i + 1], q2 = quat[4 * i + 2], q3 = quat[4 * i + 3];
+    const float n = fmaxf(__builtin_sqrtf(((q0 * q0 + q1 * q1) + q2 * q2) + q3 * q3), 1e-12f);
+    rots[4 * i] = q0 / n;
+    rots[4 * i + 1] = q1 / n;
+    rots[4 * i + 2] = q2 / n;
+    rots[4 * i + 3] = q3 / n;
+  }
+}
+
+// d svec = d scale * scale; d logit = d o * o (1 - o);
+// d q = (d qhat - qhat (qhat . d qhat)) / |q|  (|q| > 1e-12)
+__global__ __launch_bounds__(256) void k_activate_bwd(
+    const float* __restrict__ scales, const float* __restrict__ opac,
+    const float* __restrict__ quat, const float* __restrict__ g_scales,
+    const float* __restrict__ g_opac, const float* __restrict__ g_rots, uint64_t G,
+    const RowRange* __restrict__ ranges, uint32_t n_ranges, float* __restrict__ g_svec,
+    float* __restrict__ g_logit, float* __restrict__ g_quat) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < G; w += stride) {
+    const uint64_t i = row_of(ranges, n_ranges, w);
+#pragma unroll
+    for (int k = 0; k < 3; k++) g_svec[3 * i + k] = g_scales[3 * i + k] * scales[3 * i + k];
+    const float o = opac[i];
+    g_logit[i] = g_opac[i] * (o * (1.0f - o));
+    const float q0 = quat[4 * i], q1 = quat[4 * i + 1], q2 = quat[4 * i + 2], q3 = quat[4 * i + 3];
+    const float nr = __builtin_sqrtf(((q0 * q0 + q1 * q1) + q2 * q2) + q3 * q3);
+    const float n = fmaxf(nr, 1e-12f);
+    const float h0 = q0 / n, h1 = q1 / n, h2 = q2 / n, h3 = q3 / n;
+    const float d0 = g_rots[4 * i], d1 = g_rots[4 * i + 1], d2 = g_rots[4 * i + 2], d3 = g_rots[4 * i + 3];
+    if (nr > 1e-12f) {
+      const float dot = ((h0 * d0 + h1 * d1) + h2 * d2) + h3 * d3;
+      g_quat[4 * i] = (d0 - h0 * dot) / n;
+      g_quat[4 * i + 1] = (d1 - h1 * dot) / n;
+      g_quat[4 * i + 2] = (d2 - h2 * dot) / n;
+      g_quat[4 * i + 3] = (d3 - h3 * dot) / n;
+    } else {
+      g_quat[4 * i] = d0 / n;
+      g_quat[4 * i + 1] = d1 / n;
+      g_quat[4 * i + 2] = d2 / n;
+      g_quat[4 * i + 3] = d3 / n;
+    }
+  }
+}
+
+// L2 loss over the packed frame outputs; dL/dcolor for every pixel of every
+// frame (zero for frames without the backward flag), per-frame loss sums
+// accumulated into loss[frame] (pre-zeroed).
+__global__ __launch_bounds__(256) void k_l2_loss(const void* __restrict__ table,
+                                                 uint32_t scenes_cap, uint64_t pixels_cap,
+                                                 const float* __restrict__ color,
+                                                 const float* __restrict__ target,
+                                                 float* __restrict__ dL_dcolor,
+                                                 float* __restrict__ loss) {
+  const BatchHeader* hdr = static_cast<const BatchHeader*>(table);
+  const FrameDesc* frames = table_frames(table, scenes_cap);
+  const uint64_t n = min(hdr->n_pixels, pixels_cap);
+  const uint32_t nf = hdr->n_frames;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  float part = 0.0f;
+  uint32_t part_frame = 0xffffffffu;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+    uint32_t lo = 0, hi = nf - 1;
+    while (lo < hi) {
+      uint32_t mid = (lo + hi + 1) >> 1;
+      if (frames[mid].pix_off <= p) lo = mid; else hi = mid - 1;
+    }
+    const FrameDesc& fd = frames[lo];
+    const uint64_t HW = (uint64_t)fd.W * fd.H;
+    const uint64_t lp = p - fd.pix_off;
+    const float w = 1.0f / (3.0f * (float)HW);
+    const bool on = (fd.flags & FRAME_BACKWARD) != 0;
+    float sq = 0.0f;
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) {
+      const uint64_t e = 3 * fd.pix_off + ch * HW + lp;
+      const float c = color[e];
+      const float cc = fminf(fmaxf(c, 0.0f), 1.0f);
+      const float d = cc - target[fd.target_off + ch * HW + lp];
+      sq += d * d;
+      dL_dcolor[e] = (on && c >= 0.0f && c <= 1.0f) ? 2.0f * d * w : 0.0f;
+    }
+    if (on) {
+      if (part_frame != lo) {
+        if (part_frame != 0xffffffffu) atomicAdd(loss + part_frame, part);
+        part = 0.0f;
+        part_frame = lo;
+      }
+      part += sq * w;
+    }
+  }
+  if (part_frame != 0xffffffffu) atomicAdd(loss + part_frame, part);
+}
+
+// Adam with hyper-parameters in device memory: hyper = {neg_step[8] (-lr_k /
+// (1 - b1^t) per group), inv_bc2_sqrt, 1 - b1, b2, 1 - b2, eps}.  A group may
+// alternate two learning rates along its elements: element e of the group
+// uses neg_step[k] when (e mod period) < split and neg_step2[k] otherwise
+// (the SH block of a Gaussian: DC coefficients and the rest).
+struct AdamGraphArgs {
+  float* p;
+  const float* g;
+  float* m;
+  float* v;
+  uint64_t n;
+  int nseg;
+  uint64_t seg_end[ADAM_MAX_SEGMENTS];
+  uint32_t period[ADAM_MAX_SEGMENTS];
+  uint32_t split[ADAM_MAX_SEGMENTS];
+  int lr2[ADAM_MAX_SEGMENTS];  // index into hyper for the second rate, -1 if none
+  const float* hyper;
+};
+
+__global__ __launch_bounds__(256) void k_adam_hyper(AdamGraphArgs a) {
+  const float inv_bc2_sqrt = a.hyper[16], one_m_b1 = a.hyper[17], b2 = a.hyper[18];
+  const float one_m_b2 = a.hyper[19], eps = a.hyper[20];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < ADAM_MAX_SEGMENTS - 1; k++)
+      if (k < a.nseg - 1 && i >= a.seg_end[k]) s = k + 1;
+    float ns = a.hyper[s];
+    if (a.lr2[s] >= 0) {
+      const uint64_t start = s == 0 ? 0 : a.seg_end[s - 1];
+      if ((uint32_t)((i - start) % a.period[s]) >= a.split[s]) ns = a.hyper[a.lr2[s]];
+    }
+    const float g = a.g[i];
+    float m = a.m[i];
+    float v = a.v[i];
+    m = m + one_m_b1 * (g - m);
+    v = v * b2 + (one_m_b2 * g) * g;
+    const float denom = __builtin_sqrtf(v) * inv_bc2_sqrt + eps;
+    a.p[i] = a.p[i] + ns * (m / denom);
+    a.m[i] = m;
+    a.v[i] = v;
+  }
+}
+
+// ---- range tables for stores with resident scenes ----
+// A range is a contiguous run of the flat store; `off` is its start in the
+// concatenation of all ranges of the table (the work index space).
+struct ParamRange {
+  uint64_t start, off, len;
+  int32_t hyper;            // hyper block: {neg_step, neg_step2, inv_bc2_sqrt, 0}
+  uint32_t period, split;   // second rate when (e mod period) >= split (period 0: none)
+  uint32_t pad;
+};
+
+W2W_DEV uint32_t range_of(const ParamRange* r, uint32_t n, uint64_t w) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (r[mid].off <= w) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_adam_ranges(float* __restrict__ p, const float* __restrict__ g,
+                                                     float* __restrict__ m, float* __restrict__ v,
+                                                     const ParamRange* __restrict__ ranges,
+                                                     uint32_t n_ranges, uint64_t total,
+                                                     const float* __restrict__ hyper, float one_m_b1,
+                                                     float b2, float one_m_b2, float eps) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += stride) {
+    const ParamRange& r = ranges[range_of(ranges, n_ranges, w)];
+    const uint64_t e = w - r.off;
+    const uint64_t i = r.start + e;
+    const float* h = hyper + 4 * r.hyper;
+    float ns = h[0];
+    if (r.period && (uint32_t)(e % r.period) >= r.split) ns = h[1];
+    const float gi = g[i];
+    float mi = m[i];
+    float vi = v[i];
+    mi = mi + one_m_b1 * (gi - mi);
+    vi = vi * b2 + (one_m_b2 * gi) * gi;
+    const float denom = __builtin_sqrtf(vi) * h[2] + eps;
+    p[i] = p[i] + ns * (mi / denom);
+    m[i] = mi;
+    v[i] = vi;
+  }
+}
+
+}  // namespace w2w
