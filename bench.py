@@ -47,6 +47,12 @@ def parse():
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-gaussians", type=int, default=250_000)
+    ap.add_argument("--workload", choices=["scene1m", "batched64", "served"], default="scene1m")
+    ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--requests", type=int, default=120)
+    ap.add_argument("--rate", type=float, default=40.0)
+    ap.add_argument("--opt-steps", type=int, default=50)
+    ap.add_argument("--no-graph", action="store_true")
     return ap.parse_args()
 
 
@@ -84,6 +90,10 @@ def main():
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the rasterizer's "
               "multi-GPU sharding is a design note only", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched64":
+        return bench_batched(args)
+    if args.workload == "served":
+        return bench_served(args)
     import torch
 
     from word2world_amd import _lib, raster
@@ -244,6 +254,136 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+
+
+def _targets(jobs, dev, tile):
+    import torch
+    from word2world_amd import raster
+    from word2world_amd.scene import activate
+    out = []
+    with torch.no_grad():
+        for j in jobs:
+            m, s, r, o, sh = (t.to(dev).contiguous() for t in activate(j.target_scene.params()))
+            tl = []
+            for c in j.cameras:
+                cam = raster.camera_from_tensors(c.viewmatrix, c.projmatrix, c.campos, c.width, c.height,
+                                                 c.focal_x, c.focal_y, None, device=dev)
+                tl.append(raster.forward_checked(m, s, r, o, sh, 3, cam, tile)[1].clamp(0, 1))
+            out.append(tl)
+    return out
+
+
+def bench_batched(args):
+    """64 scenes (10k-500k Gaussians, log-uniform), 4 views each at a
+    resolution drawn from {256,512,1024}^2, fwd+bwd+Adam per step: the batched
+    trainer (one launch sequence, HIP-graph replay) against the plain path
+    (one scene and one view at a time)."""
+    import torch
+    from word2world_amd import _lib, raster
+    from word2world_amd.batched import BatchedTrainer
+    from word2world_amd.train import PlainTrainer, TrainConfig
+    from word2world_amd.workloads import batched_64
+
+    dev = torch.device("cuda:0")
+    jobs = batched_64(args.scenes)
+    G = sum(j.scene.n for j in jobs)
+    V = sum(len(j.cameras) for j in jobs)
+    log(f"batched: {len(jobs)} scenes, {G} Gaussians, {V} views")
+    targets = _targets(jobs, dev, args.tile)
+    bt = BatchedTrainer([j.scene for j in jobs], [j.cameras for j in jobs], targets, dev, args.tile)
+    for i in range(max(args.warmup, 1)):
+        bt.step()
+        torch.cuda.synchronize()
+        K, ovf = (int(v) for v in bt.ws.status().cpu())
+        log(f"warmup {i}: K={K} overflow={ovf}")
+        if ovf:
+            raise RuntimeError("batched workspace capacity too small")
+    if not args.no_graph:
+        bt.capture()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[i][0].record()
+        bt.step()
+        ev[i][1].record()
+    torch.cuda.synchronize()
+    tb = (time.perf_counter() - t0) / args.steps
+    lat_b = np.array([a.elapsed_time(b) for a, b in ev])
+    if int(bt.ws.status()[1]):
+        raise RuntimeError("overflow in the timed region")
+    log(f"batched: {tb * 1e3:.1f} ms/step, {V / tb:.1f} views/s")
+    del bt
+    torch.cuda.empty_cache()
+    # plain: one scene at a time, one view at a time
+    cams = [[raster.camera_from_tensors(c.viewmatrix, c.projmatrix, c.campos, c.width, c.height, c.focal_x,
+                                        c.focal_y, None, device=dev) for c in j.cameras] for j in jobs]
+    plains = [PlainTrainer(j.scene, cs, t, TrainConfig(tile=args.tile), dev)
+              for j, cs, t in zip(jobs, cams, targets)]
+    for p in plains:
+        p.step()
+    torch.cuda.synchronize()
+    psteps = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(psteps):
+        for p in plains:
+            p.step()
+    torch.cuda.synchronize()
+    tp = (time.perf_counter() - t0) / psteps
+    log(f"plain: {tp * 1e3:.1f} ms/step, {V / tp:.1f} views/s")
+    out = {"metric": "fwd+bwd views/sec, 64-scene mixed batch (batched vs plain)", "value": V / tb,
+           "unit": "views/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": tb * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": f"{len(jobs)} scenes, Gaussians log-uniform [10k,500k] (total {G}), SH3, "
+                                  "4 views each at {256,512,1024}^2, fwd+bwd+Adam",
+                      "graph": not args.no_graph, "tile": args.tile},
+           "plain_views_per_s": V / tp, "speedup_vs_plain": tp / tb,
+           "latency_ms": {"p50": float(np.percentile(lat_b, 50)), "p99": float(np.percentile(lat_b, 99))}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_served(args):
+    """Poisson request stream: 70 % renders (1-8 views), 30 % optimisation jobs
+    (fwd+bwd+Adam steps on a resident scene), scenes log-uniform [5k, 1M];
+    continuous batching against the plain FIFO server."""
+    import copy
+    import torch
+    from word2world_amd import serve
+    from word2world_amd.scene import make_scene
+    from word2world_amd.workloads import log_uniform_counts
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(0)
+    sizes = [int(n) for n in log_uniform_counts(16, 5_000, 1_000_000, rng)]
+    pool = serve.ScenePool([make_scene(n, 3, 500 + k) for k, n in enumerate(sizes)],
+                           [make_scene(n, 3, 600 + k) for k, n in enumerate(sizes)], dev)
+    reqs = serve.make_requests(args.requests, args.rate, sizes, seed=1, opt_steps=args.opt_steps)
+    pool.build_targets(reqs)
+    c = serve._lib.W2WCaps()
+    c.scenes, c.frames, c.tile, c.pad = 128, 512, args.tile, 0
+    c.items, c.gaussians = 48_000_000, 12_000_000
+    c.tiles, c.pixels, c.capacity = 2_000_000, 160_000_000, 400_000_000
+    srv = serve.ContinuousServer(pool, 12_000_000, c, dev, args.tile)
+    # warm up the kernels on a copy of the first requests
+    srv.serve(copy.deepcopy(reqs[:4]))
+    done_c = srv.serve(copy.deepcopy(reqs), log=log)
+    st_c = serve.latency_stats(done_c)
+    log(f"continuous: {st_c}")
+    del srv
+    torch.cuda.empty_cache()
+    done_p = serve.PlainServer(pool, dev, args.tile).serve(copy.deepcopy(reqs), log=log)
+    st_p = serve.latency_stats(done_p)
+    log(f"plain: {st_p}")
+    out = {"metric": "served fwd+bwd views/sec; p50/p99 latency, served vs plain", "value": st_c["views_per_s"],
+           "unit": "views/s", "n_gpus": 1, "steps": len(reqs), "warmup": 4, "ms_per_step": None,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+           "data": "synthetic",
+           "config": {"workload": f"{len(reqs)} requests, Poisson {args.rate}/s, 30% {args.opt_steps}-step "
+                                  "optimisation jobs (4 views), 70% renders of 1-8 views, scenes "
+                                  "log-uniform [5k,1M] SH3, views at {256,512,1024}^2"},
+           "served": st_c, "plain": st_p}
     print(json.dumps(out), flush=True)
 
 

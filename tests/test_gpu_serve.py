@@ -1,0 +1,91 @@
+"""Served form: continuous batching admits requests as slots and HBM free up,
+and each request's result is the same alone and in the mix (renders bit for
+bit; optimisation jobs within float-atomic reordering)."""
+import copy
+
+import numpy as np
+import pytest
+import torch
+
+from word2world_amd import batch, serve
+from word2world_amd.scene import make_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _pool(dev):
+    sizes = [3000, 800, 5000]
+    scenes = [make_scene(n, 3, 300 + k) for k, n in enumerate(sizes)]
+    targets = [make_scene(n, 3, 400 + k) for k, n in enumerate(sizes)]
+    return serve.ScenePool(scenes, targets, dev), sizes
+
+
+def _caps(pool):
+    c = serve._lib.W2WCaps()
+    c.scenes, c.frames, c.tile, c.pad = 16, 64, 16, 0
+    c.items, c.gaussians, c.tiles, c.pixels, c.capacity = 200_000, 40_000, 8192, 64 * 96 * 96, 4_000_000
+    return c
+
+
+def test_continuous_server_same_alone_and_in_mix(gpu):
+    pool, sizes = _pool(gpu)
+    reqs = serve.make_requests(10, 400.0, sizes, seed=3, opt_steps=4, resolutions=(64, 96))
+    pool.build_targets(reqs)
+    srv = serve.ContinuousServer(pool, 20_000, _caps(pool), gpu, keep_results=True)
+    done = srv.serve(copy.deepcopy(reqs))
+    assert len(done) == len(reqs)
+    assert all(r.done >= r.arrival for r in done)
+    mixed = {r.rid: r for r in done}
+    assert srv.ticks < sum(1 if r.kind == "render" else r.steps for r in reqs), "no batching happened"
+    # each request alone
+    for r in reqs:
+        alone = serve.ContinuousServer(pool, 20_000, _caps(pool), gpu, keep_results=True)
+        rr = copy.deepcopy(r)
+        rr.arrival = 0.0
+        out = alone.serve([rr])[0]
+        m = mixed[r.rid]
+        if r.kind == "render":
+            for a, b in zip(out.result, m.result):
+                assert torch.equal(a, b), r.rid
+        else:
+            # The backward accumulates with float atomics, so the two runs see
+            # gradients equal up to summation order; Adam (eps 1e-15) scales a
+            # gradient that is mostly cancellation to a full lr step, so a
+            # small fraction of elements drifts further than rounding.  A job
+            # mixed up with another (wrong rows, offsets or rates) moves most
+            # elements by O(lr).
+            for name in serve.GROUPS:
+                a, b = out.result[name].numpy(), m.result[name].numpy()
+                p0 = pool.params[r.scene][name].reshape(-1).cpu().numpy()
+                d = np.abs(a - b)
+                assert d.max() <= 2 * 0.05 * r.steps, (r.rid, name, d.max())
+                assert np.percentile(d, 95) <= 1e-5 * max(1.0, np.abs(b).max()), (r.rid, name)
+                upd = np.linalg.norm(b.reshape(-1) - p0)
+                assert np.linalg.norm(d) <= 0.05 * upd, (r.rid, name, np.linalg.norm(d), upd)
+
+
+def test_admission_respects_hbm_budget(gpu):
+    pool, sizes = _pool(gpu)
+    reqs = serve.make_requests(8, 1000.0, sizes, seed=5, opt_steps=2, resolutions=(64,))
+    pool.build_targets(reqs)
+    # the arena holds only as many rows as the largest scene: at no time may
+    # the resident scenes exceed it, and everything still completes
+    budget = max(sizes)
+    srv = serve.ContinuousServer(pool, budget, _caps(pool), gpu)
+    done = srv.serve(reqs)
+    assert len(done) == len(reqs)
+    events = sorted([(r.admitted, 1, sizes[r.scene]) for r in done] +
+                    [(r.done, 0, -sizes[r.scene]) for r in done])
+    resident = 0
+    for _, _, d in events:
+        resident += d
+        assert resident <= budget
+
+
+def test_plain_server_completes(gpu):
+    pool, sizes = _pool(gpu)
+    reqs = serve.make_requests(4, 1000.0, sizes, seed=7, opt_steps=2, resolutions=(64,))
+    pool.build_targets(reqs)
+    done = serve.PlainServer(pool, gpu).serve(reqs)
+    st = serve.latency_stats(done)
+    assert st["requests"] == 4 and st["p99_s"] >= st["p50_s"] > 0

@@ -47,6 +47,7 @@ W2W_DEV float lane63(float v) {
 struct BlendBwdArgs {
   const void* table;
   uint32_t scenes_cap, tiles_cap, items_cap;
+  uint32_t* ticket;         // zeroed; tiles are claimed in order through it
   const uint2* ranges;
   const uint32_t* ids;
   const Splat* splats;
@@ -64,144 +65,150 @@ __global__ __launch_bounds__(TS* TS) void k_blend_bwd(BlendBwdArgs a) {
   constexpr int B = TS * TS;
   __shared__ Splat s_sp[B];
   __shared__ uint32_t s_id[B];
-  __shared__ uint32_t s_max;
+  __shared__ uint32_t s_max, s_tile;
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
   const uint32_t Tn = min(hdr->n_tiles, a.tiles_cap);
   const uint32_t P = min(hdr->n_items, a.items_cap);
-  const uint32_t tile = xcd_tile(blockIdx.x, a.tiles_cap);
-  if (tile >= Tn) return;
   const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
-  const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, tile)];
-  if (!(fd.flags & FRAME_BACKWARD)) return;
   const int tid = threadIdx.x, lane = tid & 63;
-  const int lt = (int)(tile - fd.tile_off);
-  const int tx = lt % fd.tiles_x, ty = lt / fd.tiles_x;
-  const int px = tx * TS + (tid % TS), py = ty * TS + (tid / TS);
-  const bool inside = px < fd.W && py < fd.H;
-  const float pxf = (float)px, pyf = (float)py;
-  const uint2 range = a.ranges[tile];
-  const size_t lp = (size_t)py * fd.W + px;
-  const size_t pid = fd.pix_off + lp;
-  const size_t HW = (size_t)fd.W * fd.H;
-
-  const int last = inside ? (int)a.n_contrib[pid] : 0;
-  const float Tf = inside ? a.final_T[pid] : 0.0f;
-  float gC0 = 0.0f, gC1 = 0.0f, gC2 = 0.0f, gA = 0.0f, gD = 0.0f;
-  if (inside) {
-    const float* gcol = a.dL_dcolor + 3 * fd.pix_off;
-    gC0 = gcol[lp];
-    gC1 = gcol[HW + lp];
-    gC2 = gcol[2 * HW + lp];
-    gA = a.dL_dalpha ? a.dL_dalpha[pid] : 0.0f;
-    gD = a.dL_ddepth ? a.dL_ddepth[pid] : 0.0f;
-  }
-  const float bgdot = (fd.bg[0] * gC0 + fd.bg[1] * gC1) + fd.bg[2] * gC2;
-  const float hw = 0.5f * (float)fd.W, hh = 0.5f * (float)fd.H;
   const float a_min = 0x1.010102p-8f;
-
-  if (tid == 0) s_max = 0;
-  __syncthreads();
-  // wave-level max of last, then block max
-  int wmax = last;
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
-  if (lane == 0) atomicMax(&s_max, (uint32_t)wmax);
-  __syncthreads();
-  const int nproc = (int)s_max;
-
-  float T = Tf;
   uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
-  float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, accD = 0.0f, accA = 0.0f;
-  float la = 0.0f, lc0 = 0.0f, lc1 = 0.0f, lc2 = 0.0f, ld = 0.0f;
-
-  for (int bend = nproc; bend > 0; bend -= B) {
-    const int bstart = bend > B ? bend - B : 0;
-    __syncthreads();
-    const int k = bstart + tid;
-    n_loaded += bend - bstart;
-    if (k < bend) {
-      uint32_t id = a.ids[range.x + k];
-      id = id < P ? id : 0u;
-      s_id[tid] = id;
-      s_sp[tid] = a.splats[id];
+  for (;;) {
+    if (tid == 0) {
+      s_tile = atomicAdd(a.ticket, 1u);
+      s_max = 0;
     }
     __syncthreads();
-    const int jtop = min(bend, wmax) - 1 - bstart;  // wave-uniform
-    for (int j = jtop; j >= 0; j--) {
-      const int kk = bstart + j;
-      float g_mx = 0.0f, g_my = 0.0f, g_ca = 0.0f, g_cb = 0.0f, g_cc = 0.0f, g_op = 0.0f;
-      float g_r = 0.0f, g_g = 0.0f, g_b = 0.0f, g_d = 0.0f;
-      bool contrib = false;
-      if (kk < last) {
-        n_eval++;
-        const float4 sa = s_sp[j].a;
-        const float4 sb = s_sp[j].b;
-        const float dx = sa.x - pxf, dy = sa.y - pyf;
-        const float pw = power_of(sa.z, sa.w, sb.x, dx, dy);
-        if (pw <= 0.0f) {
-          const float G = exp_neg(pw);
-          const float o = sb.y;
-          const float oG = o * G;
-          const float al = fminf(0.99f, oG);
-          if (al >= a_min) {
-            contrib = true;
-            n_con++;
-            T = T / (1.0f - al);
-            const float aT = al * T;
-            const float4 sc = s_sp[j].c;
-            acc0 = la * lc0 + (1.0f - la) * acc0;
-            acc1 = la * lc1 + (1.0f - la) * acc1;
-            acc2 = la * lc2 + (1.0f - la) * acc2;
-            accD = la * ld + (1.0f - la) * accD;
-            accA = la + (1.0f - la) * accA;
-            float dLda = ((sc.x - acc0) * gC0 + (sc.y - acc1) * gC1) + (sc.z - acc2) * gC2;
-            dLda = dLda + (sb.z - accD) * gD;
-            dLda = dLda + (1.0f - accA) * gA;
-            dLda = dLda * T;
-            dLda = dLda - (Tf / (1.0f - al)) * bgdot;
-            lc0 = sc.x; lc1 = sc.y; lc2 = sc.z; ld = sb.z; la = al;
-            g_r = aT * gC0;
-            g_g = aT * gC1;
-            g_b = aT * gC2;
-            g_d = aT * gD;
-            if (oG <= 0.99f) {
-              g_op = G * dLda;
-              const float dLdp = (o * dLda) * G;
-              g_ca = (-0.5f * dx * dx) * dLdp;
-              g_cb = (-dx * dy) * dLdp;
-              g_cc = (-0.5f * dy * dy) * dLdp;
-              g_mx = (-(sa.z * dx + sa.w * dy) * dLdp) * hw;
-              g_my = (-(sa.w * dx + sb.x * dy) * dLdp) * hh;
+    const uint32_t tile = s_tile;
+    if (tile >= Tn) break;
+    const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, tile)];
+    const bool active_frame = (fd.flags & FRAME_BACKWARD) != 0;
+    const int lt = (int)(tile - fd.tile_off);
+    const int tx = lt % fd.tiles_x, ty = lt / fd.tiles_x;
+    const int px = tx * TS + (tid % TS), py = ty * TS + (tid / TS);
+    const bool inside = active_frame && px < fd.W && py < fd.H;
+    const float pxf = (float)px, pyf = (float)py;
+    const uint2 range = a.ranges[tile];
+    const size_t lp = (size_t)py * fd.W + px;
+    const size_t pid = fd.pix_off + lp;
+    const size_t HW = (size_t)fd.W * fd.H;
+
+    const int last = inside ? (int)a.n_contrib[pid] : 0;
+    const float Tf = inside ? a.final_T[pid] : 0.0f;
+    float gC0 = 0.0f, gC1 = 0.0f, gC2 = 0.0f, gA = 0.0f, gD = 0.0f;
+    if (inside) {
+      const float* gcol = a.dL_dcolor + 3 * fd.pix_off;
+      gC0 = gcol[lp];
+      gC1 = gcol[HW + lp];
+      gC2 = gcol[2 * HW + lp];
+      gA = a.dL_dalpha ? a.dL_dalpha[pid] : 0.0f;
+      gD = a.dL_ddepth ? a.dL_ddepth[pid] : 0.0f;
+    }
+    const float bgdot = (fd.bg[0] * gC0 + fd.bg[1] * gC1) + fd.bg[2] * gC2;
+    const float hw = 0.5f * (float)fd.W, hh = 0.5f * (float)fd.H;
+
+    // wave-level max of last, then block max
+    int wmax = last;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
+    if (lane == 0) atomicMax(&s_max, (uint32_t)wmax);
+    __syncthreads();
+    const int nproc = (int)s_max;
+
+    float T = Tf;
+    float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, accD = 0.0f, accA = 0.0f;
+    float la = 0.0f, lc0 = 0.0f, lc1 = 0.0f, lc2 = 0.0f, ld = 0.0f;
+
+    for (int bend = nproc; bend > 0; bend -= B) {
+      const int bstart = bend > B ? bend - B : 0;
+      __syncthreads();
+      const int k = bstart + tid;
+      n_loaded += bend - bstart;
+      if (k < bend) {
+        uint32_t id = a.ids[range.x + k];
+        id = id < P ? id : 0u;
+        s_id[tid] = id;
+        s_sp[tid] = a.splats[id];
+      }
+      __syncthreads();
+      const int jtop = min(bend, wmax) - 1 - bstart;  // wave-uniform
+      for (int j = jtop; j >= 0; j--) {
+        const int kk = bstart + j;
+        float g_mx = 0.0f, g_my = 0.0f, g_ca = 0.0f, g_cb = 0.0f, g_cc = 0.0f, g_op = 0.0f;
+        float g_r = 0.0f, g_g = 0.0f, g_b = 0.0f, g_d = 0.0f;
+        bool contrib = false;
+        if (kk < last) {
+          n_eval++;
+          const float4 sa = s_sp[j].a;
+          const float4 sb = s_sp[j].b;
+          const float dx = sa.x - pxf, dy = sa.y - pyf;
+          const float pw = power_of(sa.z, sa.w, sb.x, dx, dy);
+          if (pw <= 0.0f && pw >= sb.w) {
+            const float G = exp_neg(pw);
+            const float o = sb.y;
+            const float oG = o * G;
+            const float al = fminf(0.99f, oG);
+            if (al >= a_min) {
+              contrib = true;
+              n_con++;
+              T = T / (1.0f - al);
+              const float aT = al * T;
+              const float4 sc = s_sp[j].c;
+              acc0 = la * lc0 + (1.0f - la) * acc0;
+              acc1 = la * lc1 + (1.0f - la) * acc1;
+              acc2 = la * lc2 + (1.0f - la) * acc2;
+              accD = la * ld + (1.0f - la) * accD;
+              accA = la + (1.0f - la) * accA;
+              float dLda = ((sc.x - acc0) * gC0 + (sc.y - acc1) * gC1) + (sc.z - acc2) * gC2;
+              dLda = dLda + (sb.z - accD) * gD;
+              dLda = dLda + (1.0f - accA) * gA;
+              dLda = dLda * T;
+              dLda = dLda - (Tf / (1.0f - al)) * bgdot;
+              lc0 = sc.x; lc1 = sc.y; lc2 = sc.z; ld = sb.z; la = al;
+              g_r = aT * gC0;
+              g_g = aT * gC1;
+              g_b = aT * gC2;
+              g_d = aT * gD;
+              if (oG <= 0.99f) {
+                g_op = G * dLda;
+                const float dLdp = (o * dLda) * G;
+                g_ca = (-0.5f * dx * dx) * dLdp;
+                g_cb = (-dx * dy) * dLdp;
+                g_cc = (-0.5f * dy * dy) * dLdp;
+                g_mx = (-(sa.z * dx + sa.w * dy) * dLdp) * hw;
+                g_my = (-(sa.w * dx + sb.x * dy) * dLdp) * hh;
+              }
             }
           }
         }
-      }
-      if (__ballot(contrib) != 0ull) {
-        g_mx = wave_reduce_dpp(g_mx);
-        g_my = wave_reduce_dpp(g_my);
-        g_ca = wave_reduce_dpp(g_ca);
-        g_cb = wave_reduce_dpp(g_cb);
-        g_cc = wave_reduce_dpp(g_cc);
-        g_op = wave_reduce_dpp(g_op);
-        g_r = wave_reduce_dpp(g_r);
-        g_g = wave_reduce_dpp(g_g);
-        g_b = wave_reduce_dpp(g_b);
-        g_d = wave_reduce_dpp(g_d);
-        if (lane == 63) {
-          float* dst = a.gacc + (size_t)s_id[j] * GA_STRIDE;
-          atomicAdd(dst + GA_MX, g_mx);
-          atomicAdd(dst + GA_MY, g_my);
-          atomicAdd(dst + GA_CA, g_ca);
-          atomicAdd(dst + GA_CB, g_cb);
-          atomicAdd(dst + GA_CC, g_cc);
-          atomicAdd(dst + GA_OP, g_op);
-          atomicAdd(dst + GA_R, g_r);
-          atomicAdd(dst + GA_G, g_g);
-          atomicAdd(dst + GA_B, g_b);
-          atomicAdd(dst + GA_DEPTH, g_d);
+        if (__ballot(contrib) != 0ull) {
+          g_mx = wave_reduce_dpp(g_mx);
+          g_my = wave_reduce_dpp(g_my);
+          g_ca = wave_reduce_dpp(g_ca);
+          g_cb = wave_reduce_dpp(g_cb);
+          g_cc = wave_reduce_dpp(g_cc);
+          g_op = wave_reduce_dpp(g_op);
+          g_r = wave_reduce_dpp(g_r);
+          g_g = wave_reduce_dpp(g_g);
+          g_b = wave_reduce_dpp(g_b);
+          g_d = wave_reduce_dpp(g_d);
+          if (lane == 63) {
+            float* dst = a.gacc + (size_t)s_id[j] * GA_STRIDE;
+            atomicAdd(dst + GA_MX, g_mx);
+            atomicAdd(dst + GA_MY, g_my);
+            atomicAdd(dst + GA_CA, g_ca);
+            atomicAdd(dst + GA_CB, g_cb);
+            atomicAdd(dst + GA_CC, g_cc);
+            atomicAdd(dst + GA_OP, g_op);
+            atomicAdd(dst + GA_R, g_r);
+            atomicAdd(dst + GA_G, g_g);
+            atomicAdd(dst + GA_B, g_b);
+            atomicAdd(dst + GA_DEPTH, g_d);
+          }
         }
       }
     }
+    __syncthreads();  // s_sp, s_id, s_max and s_tile are reused by the next tile
   }
   if (a.stats) {
     unsigned long long e = n_eval, c = n_con, kf = (threadIdx.x == 0) ? n_loaded : 0u;

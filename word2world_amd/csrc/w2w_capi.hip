@@ -206,16 +206,20 @@ __global__ void k_make_single(void* table, const float* __restrict__ view,
   fd->pix_off = 0; fd->target_off = 0;
 }
 
+// persistent blend grids: enough workgroups to fill every CU several times
+// over; each loops claiming tiles from a ticket
+constexpr uint32_t kBlendGrid = 256 * 8;
+
 template <int TS>
 void launch_blend(const BlendArgs& a, uint32_t tiles_cap, hipStream_t s) {
-  const uint32_t grid = 8 * cdiv(tiles_cap, 8);
+  const uint32_t grid = std::min<uint32_t>(tiles_cap, kBlendGrid);
   ProfSpan ps(W2W_KID_BLEND, s);
   hipLaunchKernelGGL(k_blend<TS>, dim3(grid), dim3(TS * TS), 0, s, a);
 }
 
 template <int TS>
 void launch_blend_bwd(const BlendBwdArgs& a, uint32_t tiles_cap, hipStream_t s) {
-  const uint32_t grid = 8 * cdiv(tiles_cap, 8);
+  const uint32_t grid = std::min<uint32_t>(tiles_cap, kBlendGrid);
   ProfSpan ps(W2W_KID_BLEND_BWD, s);
   hipLaunchKernelGGL(k_blend_bwd<TS>, dim3(grid), dim3(TS * TS), 0, s, a);
 }
@@ -342,6 +346,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   BlendArgs ba;
   ba.table = table; ba.scenes_cap = (uint32_t)c.scenes; ba.tiles_cap = tiles_cap;
   ba.items_cap = items_cap;
+  ba.ticket = status + ST_BLEND_TICKET;
   ba.ranges = at<uint2>(I, p.L.ranges);
   ba.ids = at<uint32_t>(B, p.L.sorted_ids);
   ba.splats = at<Splat>(G, p.L.splats);
@@ -368,10 +373,13 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
   const W2WCaps& c = p.c;
   if (c.items == 0) return W2W_OK;
   float* gacc = at<float>(G, p.L.gacc);
+  uint32_t* status = at<uint32_t>(G, p.status);
   fill_u32(reinterpret_cast<uint32_t*>(gacc), 0u, (size_t)GA_STRIDE * c.items, s);
+  fill_u32(status + ST_BWD_TICKET, 0u, 1, s);
   BlendBwdArgs ba;
   ba.table = table; ba.scenes_cap = (uint32_t)c.scenes; ba.tiles_cap = (uint32_t)c.tiles;
   ba.items_cap = (uint32_t)c.items;
+  ba.ticket = status + ST_BWD_TICKET;
   ba.ranges = at<uint2>(I, p.L.ranges);
   ba.ids = at<uint32_t>(B, p.L.sorted_ids);
   ba.splats = at<Splat>(G, p.L.splats);

@@ -38,6 +38,8 @@ enum StatusWord : int {
   ST_TTICKET = 16,     // up to 4 tile-sort tickets
   ST_SCAN_TICKET = 24,
   ST_EMIT_TICKET = 25,
+  ST_BLEND_TICKET = 26,
+  ST_BWD_TICKET = 27,
   ST_WORDS = 64,
 };
 
@@ -128,6 +130,7 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
     uint32_t cnt = 0, x0 = 0, y0 = 0, x1 = 0, y1 = 0;
     bool keep = g.visible;
     float r = 0.0f;
+    float skip_below = 0.0f;
     if (keep) {
       float half_tr = (g.cxx + g.cyy) * 0.5f;
       float ds = half_tr * half_tr - g.det;
@@ -137,6 +140,10 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
       keep = o > 0.0039f;
       if (keep) {
         float lnr = log_pos(o * 255.0f);
+        // exponents below ln(1/(255 o)) - 1e-5 give o*exp(power) < 1/255 with a
+        // margin far above the rounding of exp and of this log, so the blend
+        // kernels may skip such pairs before evaluating the exponential
+        skip_below = -lnr - 1e-5f;
         float radp = __builtin_sqrtf((2.0f * lmax) * lnr);
         r = fminf(radp, r3);
         keep = r > 0.0f;
@@ -154,7 +161,7 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
     Splat sp;
     if (keep) {
       sp.a = make_float4(g.px, g.py, g.ca, g.cb);
-      sp.b = make_float4(g.cc, o, g.ndcz, 0.0f);
+      sp.b = make_float4(g.cc, o, g.ndcz, skip_below);
     } else {
       sp.a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       sp.b = make_float4(0.0f, o, 0.0f, 0.0f);
@@ -323,13 +330,6 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ key
   }
 }
 
-// tile index of a workgroup: consecutive workgroups go round-robin to the 8
-// XCDs, so give each XCD a contiguous band of tiles (shared L2 working set)
-W2W_DEV uint32_t xcd_tile(uint32_t bid, uint32_t T) {
-  const uint32_t per = (T + 7) >> 3;
-  return (bid & 7u) * per + (bid >> 3);
-}
-
 // frame owning global tile t
 W2W_DEV uint32_t tile_frame(const FrameDesc* frames, uint32_t n_frames, uint32_t t) {
   return upper_index(n_frames, t, [&](uint32_t k) { return frames[k].tile_off; });
@@ -338,6 +338,7 @@ W2W_DEV uint32_t tile_frame(const FrameDesc* frames, uint32_t n_frames, uint32_t
 struct BlendArgs {
   const void* table;
   uint32_t scenes_cap, tiles_cap, items_cap;
+  uint32_t* ticket;           // zeroed; tiles are claimed in order through it
   const uint2* ranges;
   const uint32_t* ids;
   const Splat* splats;
@@ -353,62 +354,83 @@ template <int TS>
 __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs a) {
   constexpr int B = TS * TS;
   __shared__ Splat s_sp[B];
+  __shared__ uint32_t s_tile;
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
   const uint32_t T = min(hdr->n_tiles, a.tiles_cap);
   const uint32_t P = min(hdr->n_items, a.items_cap);
-  const uint32_t tile = xcd_tile(blockIdx.x, a.tiles_cap);
-  if (tile >= T) return;
   const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
-  const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, tile)];
   const int tid = threadIdx.x;
-  const int lt = (int)(tile - fd.tile_off);
-  const int tx = lt % fd.tiles_x, ty = lt / fd.tiles_x;
-  const int px = tx * TS + (tid % TS), py = ty * TS + (tid / TS);
-  const bool inside = px < fd.W && py < fd.H;
-  const float pxf = (float)px, pyf = (float)py;
-  const uint2 range = a.ranges[tile];
-  const int n = (int)(range.y - range.x);
-  bool done = !inside;
-  float T_ = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, A = 0.0f, Dd = 0.0f;
-  uint32_t last = 0;
-  uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
   const float a_min = 0x1.010102p-8f;
-  for (int base = 0; base < n; base += B) {
-    if (__syncthreads_count(done) == B) break;
-    n_loaded += min(B, n - base);
-    const int k = base + tid;
-    if (k < n) {
-      uint32_t id = a.ids[range.x + k];
-      s_sp[tid] = a.splats[id < P ? id : 0u];
-    }
+  uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
+  // persistent workgroups claim tiles in order: the load balances itself
+  // across CUs and XCDs whatever the spatial distribution of the work
+  for (;;) {
+    if (tid == 0) s_tile = atomicAdd(a.ticket, 1u);
     __syncthreads();
-    const int cnt = min(B, n - base);
-    for (int j = 0; j < cnt && !done; j++) {
-      const float4 sa = s_sp[j].a;
-      const float4 sb = s_sp[j].b;
-      const float dx = sa.x - pxf, dy = sa.y - pyf;
-      const float pw = power_of(sa.z, sa.w, sb.x, dx, dy);
-      n_eval++;
-      if (pw > 0.0f) continue;
-      const float G = exp_neg(pw);
-      const float al = fminf(0.99f, sb.y * G);
-      if (al < a_min) continue;
-      const float nT = T_ * (1.0f - al);
-      if (nT < 1e-4f) {
-        done = true;
-        break;
+    const uint32_t tile = s_tile;
+    if (tile >= T) break;
+    const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, tile)];
+    const int lt = (int)(tile - fd.tile_off);
+    const int tx = lt % fd.tiles_x, ty = lt / fd.tiles_x;
+    const int px = tx * TS + (tid % TS), py = ty * TS + (tid / TS);
+    const bool inside = px < fd.W && py < fd.H;
+    const float pxf = (float)px, pyf = (float)py;
+    const uint2 range = a.ranges[tile];
+    const int n = (int)(range.y - range.x);
+    bool done = !inside;
+    float T_ = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, A = 0.0f, Dd = 0.0f;
+    uint32_t last = 0;
+    for (int base = 0; base < n; base += B) {
+      if (__syncthreads_count(done) == B) break;
+      n_loaded += min(B, n - base);
+      const int k = base + tid;
+      if (k < n) {
+        uint32_t id = a.ids[range.x + k];
+        s_sp[tid] = a.splats[id < P ? id : 0u];
       }
-      const float aT = al * T_;
-      const float4 sc = s_sp[j].c;
-      C0 = __builtin_fmaf(sc.x, aT, C0);
-      C1 = __builtin_fmaf(sc.y, aT, C1);
-      C2 = __builtin_fmaf(sc.z, aT, C2);
-      A = A + aT;
-      Dd = __builtin_fmaf(sb.z, aT, Dd);
-      T_ = nT;
-      last = (uint32_t)(base + j + 1);
-      n_con++;
+      __syncthreads();
+      const int cnt = min(B, n - base);
+      for (int j = 0; j < cnt && !done; j++) {
+        const float4 sa = s_sp[j].a;
+        const float4 sb = s_sp[j].b;
+        const float dx = sa.x - pxf, dy = sa.y - pyf;
+        const float pw = power_of(sa.z, sa.w, sb.x, dx, dy);
+        n_eval++;
+        if (pw > 0.0f || pw < sb.w) continue;
+        const float G = exp_neg(pw);
+        const float al = fminf(0.99f, sb.y * G);
+        if (al < a_min) continue;
+        const float nT = T_ * (1.0f - al);
+        if (nT < 1e-4f) {
+          done = true;
+          break;
+        }
+        const float aT = al * T_;
+        const float4 sc = s_sp[j].c;
+        C0 = __builtin_fmaf(sc.x, aT, C0);
+        C1 = __builtin_fmaf(sc.y, aT, C1);
+        C2 = __builtin_fmaf(sc.z, aT, C2);
+        A = A + aT;
+        Dd = __builtin_fmaf(sb.z, aT, Dd);
+        T_ = nT;
+        last = (uint32_t)(base + j + 1);
+        n_con++;
+      }
     }
+    if (inside) {
+      const size_t HW = (size_t)fd.W * fd.H;
+      const size_t lp = (size_t)py * fd.W + px;
+      const size_t pid = fd.pix_off + lp;
+      float* col = a.color + 3 * fd.pix_off;
+      col[lp] = __builtin_fmaf(T_, fd.bg[0], C0);
+      col[HW + lp] = __builtin_fmaf(T_, fd.bg[1], C1);
+      col[2 * HW + lp] = __builtin_fmaf(T_, fd.bg[2], C2);
+      a.alpha[pid] = A;
+      a.depth[pid] = Dd;
+      a.n_contrib[pid] = last;
+      a.final_T[pid] = T_;
+    }
+    __syncthreads();  // s_sp and s_tile are reused by the next tile
   }
   if (a.stats) {  // evaluated / contributing pairs and loaded instances
     unsigned long long e = n_eval, c = n_con, kf = (threadIdx.x == 0) ? n_loaded : 0u;
@@ -422,19 +444,6 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs a) {
       atomicAdd(a.stats + 1, c);
       if (kf) atomicAdd(a.stats + 2, kf);
     }
-  }
-  if (inside) {
-    const size_t HW = (size_t)fd.W * fd.H;
-    const size_t lp = (size_t)py * fd.W + px;
-    const size_t pid = fd.pix_off + lp;
-    float* col = a.color + 3 * fd.pix_off;
-    col[lp] = __builtin_fmaf(T_, fd.bg[0], C0);
-    col[HW + lp] = __builtin_fmaf(T_, fd.bg[1], C1);
-    col[2 * HW + lp] = __builtin_fmaf(T_, fd.bg[2], C2);
-    a.alpha[pid] = A;
-    a.depth[pid] = Dd;
-    a.n_contrib[pid] = last;
-    a.final_T[pid] = T_;
   }
 }
 
