@@ -111,6 +111,13 @@ typedef struct W2WCaps {
 
 int w2w_version(void);
 
+/* Host-side inspection (no device work; no counterpart in the reference):
+ * the cull extent the preprocess kernel packs into each splat record, for
+ * conics conic[3 i .. 3 i + 2] = (a, b, c) and skip thresholds skip[i].
+ * out[i] = two binary16 half-widths (x low, y high) of the box outside of
+ * which the blend kernels drop the pair. */
+int w2w_splat_extents(const float* conic, const float* skip, uint64_t n, uint32_t* out);
+
 int w2w_workspace_layout(int32_t P, int32_t W, int32_t H, int32_t tile, int64_t capacity,
                          W2WLayout* out);
 

@@ -60,34 +60,41 @@ struct BlendBwdArgs {
   float* gacc;              // [items][GA_STRIDE], pre-zeroed
 };
 
+// Backward blend (backward.cu:396-520 semantics, with our forward's
+// decisions): every pixel walks its list back to front from its last
+// contributor, recovering T by T <- T / (1 - alpha) from the stored final T,
+// and the wave sums each splat's 10 partials over its pixels with DPP and
+// adds them to the splat's accumulator with one atomic per field.
+//
+// Same work split as the forward: one wave per 8x8 pixel block, chunks of 64
+// gathered one per lane (walking down from the block's largest last
+// contributor), splats whose cull extent misses the block dropped by ballot
+// (every pixel of the block would skip them), the rest visited in reverse
+// order.  The next chunk's ids and records are in flight meanwhile.
 template <int TS>
-__global__ __launch_bounds__(TS* TS) void k_blend_bwd(BlendBwdArgs a) {
-  constexpr int B = TS * TS;
-  __shared__ Splat s_sp[B];
-  __shared__ uint32_t s_id[B];
-  __shared__ uint32_t s_max, s_tile;
+__global__ __launch_bounds__(256) void k_blend_bwd(BlendBwdArgs a) {
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
   const uint32_t Tn = min(hdr->n_tiles, a.tiles_cap);
   const uint32_t P = min(hdr->n_items, a.items_cap);
+  const uint32_t U = Tn * units_per_tile<TS>();
   const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
-  const int tid = threadIdx.x, lane = tid & 63;
+  const int lane = (int)(threadIdx.x & 63u);
   const float a_min = 0x1.010102p-8f;
   uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
   for (;;) {
-    if (tid == 0) {
-      s_tile = atomicAdd(a.ticket, 1u);
-      s_max = 0;
-    }
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    if (tile >= Tn) break;
-    const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, tile)];
-    const bool active_frame = (fd.flags & FRAME_BACKWARD) != 0;
-    const int lt = (int)(tile - fd.tile_off);
-    const int tx = lt % fd.tiles_x, ty = lt / fd.tiles_x;
-    const int px = tx * TS + (tid % TS), py = ty * TS + (tid / TS);
-    const bool inside = active_frame && px < fd.W && py < fd.H;
+    uint32_t unit = 0;
+    if (lane == 0) unit = atomicAdd(a.ticket, 1u);
+    unit = (uint32_t)__builtin_amdgcn_readfirstlane((int)unit);
+    if (unit >= U) break;
+    uint32_t tile;
+    int ux, uy;
+    const FrameDesc& fd = unit_block<TS>(frames, hdr->n_frames, unit, tile, ux, uy);
+    if ((fd.flags & FRAME_BACKWARD) == 0) continue;
+    const int px = ux + (lane & 7), py = uy + (lane >> 3);
+    const bool inside = px < fd.W && py < fd.H;
     const float pxf = (float)px, pyf = (float)py;
+    const float x0 = (float)ux, x1 = (float)(ux + kUnit - 1);
+    const float y0 = (float)uy, y1 = (float)(uy + kUnit - 1);
     const uint2 range = a.ranges[tile];
     const size_t lp = (size_t)py * fd.W + px;
     const size_t pid = fd.pix_off + lp;
@@ -107,45 +114,60 @@ __global__ __launch_bounds__(TS* TS) void k_blend_bwd(BlendBwdArgs a) {
     const float bgdot = (fd.bg[0] * gC0 + fd.bg[1] * gC1) + fd.bg[2] * gC2;
     const float hw = 0.5f * (float)fd.W, hh = 0.5f * (float)fd.H;
 
-    // wave-level max of last, then block max
     int wmax = last;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
-    if (lane == 0) atomicMax(&s_max, (uint32_t)wmax);
-    __syncthreads();
-    const int nproc = (int)s_max;
+    wmax = __builtin_amdgcn_readfirstlane(wmax);
+    wmax = min(wmax, (int)(range.y - range.x));
 
     float T = Tf;
     float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, accD = 0.0f, accA = 0.0f;
     float la = 0.0f, lc0 = 0.0f, lc1 = 0.0f, lc2 = 0.0f, ld = 0.0f;
 
-    for (int bend = nproc; bend > 0; bend -= B) {
-      const int bstart = bend > B ? bend - B : 0;
-      __syncthreads();
-      const int k = bstart + tid;
-      n_loaded += bend - bstart;
-      if (k < bend) {
-        uint32_t id = a.ids[range.x + k];
-        id = id < P ? id : 0u;
-        s_id[tid] = id;
-        s_sp[tid] = a.splats[id];
+    // chunk c covers [wmax - 64 (c + 1), wmax - 64 c) clipped at 0; lane l
+    // holds its instance start + l
+    // (lanes before the start gather splat 0, which exists whenever wmax > 0)
+    uint32_t id_n = 0, id_cur = 0;
+    Splat cur = {};
+    if (wmax > 0) {
+      const int k = wmax - 64 + lane;
+      id_n = k >= 0 ? a.ids[range.x + k] : 0u;
+      id_cur = id_n < P ? id_n : 0u;
+      cur = a.splats[id_cur];
+      id_n = k >= 64 ? a.ids[range.x + k - 64] : 0u;
+    }
+    for (int cend = wmax; cend > 0; cend -= 64) {
+      const int cstart = cend - 64;  // may be negative: lanes below 0 are empty
+      const Splat s = cur;
+      const uint32_t sid = id_cur;
+      if (cstart > 0) {
+        id_cur = id_n < P ? id_n : 0u;
+        cur = a.splats[id_cur];
+        const int k2 = cstart - 128 + lane;
+        id_n = k2 >= 0 ? a.ids[range.x + k2] : 0u;
       }
-      __syncthreads();
-      const int jtop = min(bend, wmax) - 1 - bstart;  // wave-uniform
-      for (int j = jtop; j >= 0; j--) {
-        const int kk = bstart + j;
+      const int kk_l = cstart + lane;
+      n_loaded += (uint32_t)min(64, cend);
+      const bool valid = kk_l >= 0;
+      const bool touch = valid && extent_touches(s.a.x, s.a.y, unpack_extent(s.c.w), x0, x1, y0, y1);
+      uint64_t m = __ballot(touch);
+      while (m) {
+        const int j = 63 - __builtin_clzll(m);
+        m &= ~(1ull << j);
+        const int kk = cstart + j;
+        const float mx = bcast(s.a.x, j), my = bcast(s.a.y, j);
+        const float ca = bcast(s.a.z, j), cb = bcast(s.a.w, j), cc = bcast(s.b.x, j);
+        const float o = bcast(s.b.y, j), dz = bcast(s.b.z, j), sk = bcast(s.b.w, j);
+        const float cr = bcast(s.c.x, j), cg = bcast(s.c.y, j), cbl = bcast(s.c.z, j);
         float g_mx = 0.0f, g_my = 0.0f, g_ca = 0.0f, g_cb = 0.0f, g_cc = 0.0f, g_op = 0.0f;
         float g_r = 0.0f, g_g = 0.0f, g_b = 0.0f, g_d = 0.0f;
         bool contrib = false;
         if (kk < last) {
           n_eval++;
-          const float4 sa = s_sp[j].a;
-          const float4 sb = s_sp[j].b;
-          const float dx = sa.x - pxf, dy = sa.y - pyf;
-          const float pw = power_of(sa.z, sa.w, sb.x, dx, dy);
-          if (pw <= 0.0f && pw >= sb.w) {
+          const float dx = mx - pxf, dy = my - pyf;
+          const float pw = power_of(ca, cb, cc, dx, dy);
+          if (pw <= 0.0f && pw >= sk) {
             const float G = exp_neg(pw);
-            const float o = sb.y;
             const float oG = o * G;
             const float al = fminf(0.99f, oG);
             if (al >= a_min) {
@@ -153,18 +175,17 @@ __global__ __launch_bounds__(TS* TS) void k_blend_bwd(BlendBwdArgs a) {
               n_con++;
               T = T / (1.0f - al);
               const float aT = al * T;
-              const float4 sc = s_sp[j].c;
               acc0 = la * lc0 + (1.0f - la) * acc0;
               acc1 = la * lc1 + (1.0f - la) * acc1;
               acc2 = la * lc2 + (1.0f - la) * acc2;
               accD = la * ld + (1.0f - la) * accD;
               accA = la + (1.0f - la) * accA;
-              float dLda = ((sc.x - acc0) * gC0 + (sc.y - acc1) * gC1) + (sc.z - acc2) * gC2;
-              dLda = dLda + (sb.z - accD) * gD;
+              float dLda = ((cr - acc0) * gC0 + (cg - acc1) * gC1) + (cbl - acc2) * gC2;
+              dLda = dLda + (dz - accD) * gD;
               dLda = dLda + (1.0f - accA) * gA;
               dLda = dLda * T;
               dLda = dLda - (Tf / (1.0f - al)) * bgdot;
-              lc0 = sc.x; lc1 = sc.y; lc2 = sc.z; ld = sb.z; la = al;
+              lc0 = cr; lc1 = cg; lc2 = cbl; ld = dz; la = al;
               g_r = aT * gC0;
               g_g = aT * gC1;
               g_b = aT * gC2;
@@ -175,8 +196,8 @@ __global__ __launch_bounds__(TS* TS) void k_blend_bwd(BlendBwdArgs a) {
                 g_ca = (-0.5f * dx * dx) * dLdp;
                 g_cb = (-dx * dy) * dLdp;
                 g_cc = (-0.5f * dy * dy) * dLdp;
-                g_mx = (-(sa.z * dx + sa.w * dy) * dLdp) * hw;
-                g_my = (-(sa.w * dx + sb.x * dy) * dLdp) * hh;
+                g_mx = (-(ca * dx + cb * dy) * dLdp) * hw;
+                g_my = (-(cb * dx + cc * dy) * dLdp) * hh;
               }
             }
           }
@@ -192,8 +213,9 @@ __global__ __launch_bounds__(TS* TS) void k_blend_bwd(BlendBwdArgs a) {
           g_g = wave_reduce_dpp(g_g);
           g_b = wave_reduce_dpp(g_b);
           g_d = wave_reduce_dpp(g_d);
+          const uint32_t gid = (uint32_t)__builtin_amdgcn_readlane((int)sid, j);
           if (lane == 63) {
-            float* dst = a.gacc + (size_t)s_id[j] * GA_STRIDE;
+            float* dst = a.gacc + (size_t)gid * GA_STRIDE;
             atomicAdd(dst + GA_MX, g_mx);
             atomicAdd(dst + GA_MY, g_my);
             atomicAdd(dst + GA_CA, g_ca);
@@ -208,16 +230,15 @@ __global__ __launch_bounds__(TS* TS) void k_blend_bwd(BlendBwdArgs a) {
         }
       }
     }
-    __syncthreads();  // s_sp, s_id, s_max and s_tile are reused by the next tile
   }
   if (a.stats) {
-    unsigned long long e = n_eval, c = n_con, kf = (threadIdx.x == 0) ? n_loaded : 0u;
+    unsigned long long e = n_eval, c = n_con, kf = (lane == 0) ? n_loaded : 0u;
     for (int off = 32; off >= 1; off >>= 1) {
       e += __shfl_xor(e, off, 64);
       c += __shfl_xor(c, off, 64);
       kf += __shfl_xor(kf, off, 64);
     }
-    if ((threadIdx.x & 63) == 0) {
+    if (lane == 0) {
       atomicAdd(a.stats + 0, e);
       atomicAdd(a.stats + 1, c);
       if (kf) atomicAdd(a.stats + 2, kf);

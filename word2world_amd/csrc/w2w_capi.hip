@@ -210,18 +210,23 @@ __global__ void k_make_single(void* table, const float* __restrict__ view,
 // over; each loops claiming tiles from a ticket
 constexpr uint32_t kBlendGrid = 256 * 8;
 
+// workgroups of four independent waves, one 8x8 pixel block per wave at a time
+template <int TS>
+uint32_t blend_grid(uint32_t tiles_cap) {
+  const uint64_t units = (uint64_t)tiles_cap * units_per_tile<TS>();
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((units + 3) / 4, kBlendGrid));
+}
+
 template <int TS>
 void launch_blend(const BlendArgs& a, uint32_t tiles_cap, hipStream_t s) {
-  const uint32_t grid = std::min<uint32_t>(tiles_cap, kBlendGrid);
   ProfSpan ps(W2W_KID_BLEND, s);
-  hipLaunchKernelGGL(k_blend<TS>, dim3(grid), dim3(TS * TS), 0, s, a);
+  hipLaunchKernelGGL(k_blend<TS>, dim3(blend_grid<TS>(tiles_cap)), dim3(256), 0, s, a);
 }
 
 template <int TS>
 void launch_blend_bwd(const BlendBwdArgs& a, uint32_t tiles_cap, hipStream_t s) {
-  const uint32_t grid = std::min<uint32_t>(tiles_cap, kBlendGrid);
   ProfSpan ps(W2W_KID_BLEND_BWD, s);
-  hipLaunchKernelGGL(k_blend_bwd<TS>, dim3(grid), dim3(TS * TS), 0, s, a);
+  hipLaunchKernelGGL(k_blend_bwd<TS>, dim3(blend_grid<TS>(tiles_cap)), dim3(256), 0, s, a);
 }
 
 template <typename KT>
@@ -437,6 +442,13 @@ bool caps_ws(const W2WCaps* caps, const W2WWorkspace* ws, Plan& p, int& err) {
 extern "C" {
 
 int w2w_version(void) { return W2W_ABI_VERSION; }
+
+int w2w_splat_extents(const float* conic, const float* skip, uint64_t n, uint32_t* out) {
+  if (n && (!conic || !skip || !out)) return W2W_ERR_INVALID;
+  for (uint64_t i = 0; i < n; i++)
+    out[i] = cull_extent(conic[3 * i], conic[3 * i + 1], conic[3 * i + 2], skip[i]);
+  return W2W_OK;
+}
 
 int w2w_workspace_layout(int32_t P, int32_t W, int32_t H, int32_t tile, int64_t capacity,
                          W2WLayout* out) {

@@ -10,6 +10,7 @@
 #include <stdint.h>
 
 #define W2W_DEV __device__ __forceinline__
+#define W2W_HD __host__ __device__ __forceinline__
 
 namespace w2w {
 
@@ -54,7 +55,7 @@ W2W_DEV float log_pos(float x) {
 }
 
 // Gaussian exponent -0.5 (a dx^2 + c dy^2) - b dx dy in a fixed order.
-W2W_DEV float power_of(float ca, float cb, float cc, float dx, float dy) {
+W2W_HD float power_of(float ca, float cb, float cc, float dx, float dy) {
   float q = __builtin_fmaf(ca * dx, dx, (cc * dy) * dy);
   return __builtin_fmaf(-0.5f, q, -((cb * dx) * dy));
 }
@@ -308,11 +309,63 @@ W2W_DEV uint32_t tile_coord(float v, uint32_t n) {
 }
 
 // Per-instance record the blend kernels gather (48 bytes, 16-byte aligned):
-//   a = (px, py, conic_a, conic_b), b = (conic_c, opacity, depth, 0),
-//   c = (r, g, b, 0).
+//   a = (px, py, conic_a, conic_b), b = (conic_c, opacity, depth, skip_below),
+//   c = (r, g, b, cull extent: two binary16 half-widths packed in 32 bits).
 struct __align__(16) Splat {
   float4 a, b, c;
 };
+
+constexpr uint32_t kExtentInf = 0x7C007C00u;  // (+inf, +inf) in binary16
+
+// Half-widths (hx, hy) of an axis-aligned box around the 2D mean outside of
+// which the exponent power_of() computes in float is below `skip` at every
+// pixel, so a blend kernel may drop the pair before evaluating it.
+//
+// With Q(d) = a dx^2 + 2 b dx dy + c dy^2 and q0 = -2 skip, the set Q <= q0
+// is an ellipse whose bounding box is |dx| <= sqrt(q0 c / det),
+// |dy| <= sqrt(q0 a / det), det = a c - b^2.  The float evaluation of
+// power_of (dx, dy rounded from the pixel offset, five rounded operations)
+// errs by at most 3 u (a dx^2 + c dy^2) <= 3 u kappa Q with
+// kappa = (a + c) / lambda_min and u = 2^-24, so a pixel that passes the
+// float test satisfies Q <= q0 (1 + 2u) / (1 - 6 u kappa); q0 is widened by
+// (1 + 4u) / (1 - 8 u kappa) and the box by a further 2^-9 relative, which
+// also covers the double-precision evaluation here, the rounding of the
+// pixel offset and the upward binary16 rounding.  Ill-conditioned conics
+// (8 u kappa > 0.5) and non-positive-definite ones get an infinite box.
+W2W_HD uint32_t cull_extent(float ca, float cb, float cc, float skip) {
+  const double a = ca, b = cb, c = cc;
+  if (!(a > 0.0) || !(c > 0.0) || !(skip < 0.0f)) return kExtentInf;
+  const double det = a * c - b * b;
+  if (!(det > 0.0)) return kExtentInf;
+  const double h = 0.5 * (a - c);
+  const double lmax = 0.5 * (a + c) + __builtin_sqrt(h * h + b * b);
+  const double kappa = (a + c) * lmax / det;
+  const double u = 0x1p-24;
+  const double den = 1.0 - 8.0 * u * kappa;
+  if (!(den > 0.5)) return kExtentInf;
+  const double q1 = (-2.0 * (double)skip) * (1.0 + 4.0 * u) / den;
+  const double grow = 1.0 + 0x1p-9;
+  const double hx = __builtin_sqrt(q1 * c / det) * grow + 1e-3;
+  const double hy = __builtin_sqrt(q1 * a / det) * grow + 1e-3;
+  // binary16 round-to-nearest errs by at most 2^-11 relative: `grow` keeps
+  // the stored value above the true half-width; beyond 60000 use +inf
+  const uint16_t bx = hx < 60000.0 ? __builtin_bit_cast(uint16_t, (_Float16)(float)hx) : 0x7C00u;
+  const uint16_t by = hy < 60000.0 ? __builtin_bit_cast(uint16_t, (_Float16)(float)hy) : 0x7C00u;
+  return (uint32_t)bx | ((uint32_t)by << 16);
+}
+
+W2W_HD float2 unpack_extent(float w) {
+  const uint32_t bits = __builtin_bit_cast(uint32_t, w);
+  const _Float16 lo = __builtin_bit_cast(_Float16, (uint16_t)(bits & 0xFFFFu));
+  const _Float16 hi = __builtin_bit_cast(_Float16, (uint16_t)(bits >> 16));
+  return make_float2((float)lo, (float)hi);
+}
+
+// Whether a splat centred at (mx, my) with extent (hx, hy) may touch pixel
+// centres in [x0, x1] x [y0, y1].  NaN positions count as touching.
+W2W_HD bool extent_touches(float mx, float my, float2 e, float x0, float x1, float y0, float y1) {
+  return !((x0 - mx) > e.x || (mx - x1) > e.x || (y0 - my) > e.y || (my - y1) > e.y);
+}
 
 // ---- wave64 helpers ----
 W2W_DEV float wave_sum(float v) {

@@ -166,7 +166,8 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
       sp.a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       sp.b = make_float4(0.0f, o, 0.0f, 0.0f);
     }
-    sp.c = make_float4(rgb[0], rgb[1], rgb[2], 0.0f);
+    const uint32_t ext = keep ? cull_extent(g.ca, g.cb, g.cc, skip_below) : kExtentInf;
+    sp.c = make_float4(rgb[0], rgb[1], rgb[2], __builtin_bit_cast(float, ext));
     a.splats[i] = sp;
     a.radii[i] = radius;
     a.rect[i] = make_uint2(x0 | (y0 << 16), x1 | (y1 << 16));
@@ -350,71 +351,133 @@ struct BlendArgs {
   float* final_T;
 };
 
+// Blend work unit: an 8x8 pixel block of a tile, one wave (lane = pixel).
+constexpr int kUnit = 8;
+
 template <int TS>
-__global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs a) {
-  constexpr int B = TS * TS;
-  __shared__ Splat s_sp[B];
-  __shared__ uint32_t s_tile;
+constexpr uint32_t units_per_tile() {
+  return (uint32_t)(TS / kUnit) * (uint32_t)(TS / kUnit);
+}
+
+// Pixel block of unit `unit`; returns the frame.
+template <int TS>
+W2W_DEV const FrameDesc& unit_block(const FrameDesc* frames, uint32_t n_frames, uint32_t unit,
+                                    uint32_t& tile, int& ux, int& uy) {
+  constexpr uint32_t UPT = units_per_tile<TS>();
+  constexpr int UX = TS / kUnit;
+  tile = unit / UPT;
+  const int q = (int)(unit % UPT);
+  const FrameDesc& fd = frames[tile_frame(frames, n_frames, tile)];
+  const int lt = (int)(tile - fd.tile_off);
+  ux = (lt % fd.tiles_x) * TS + (q % UX) * kUnit;
+  uy = (lt / fd.tiles_x) * TS + (q / UX) * kUnit;
+  return fd;
+}
+
+W2W_DEV float bcast(float v, int j) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), j));
+}
+
+W2W_DEV bool wave_all(bool p) { return __ballot(!p) == 0ull; }
+
+// Forward blend (gaussianForwardPass.cu:300-385 semantics): every pixel walks
+// its tile's depth-ordered list front to back and stops before the splat that
+// would take its transmittance below 1e-4.
+//
+// Each wave owns one 8x8 pixel block and walks the list in chunks of 64: lane
+// l gathers instance base + l, tests its cull extent against the block and a
+// ballot leaves the splats that may touch any pixel of the block; the wave
+// then visits those in order, broadcasting each from its lane.  A dropped
+// splat is one every pixel of the block would skip (its exponent is below
+// skip_below), so each pixel sees the same sequence of contributing splats,
+// in the same order and with the same arithmetic, as a walk over the whole
+// list.  The next chunk's ids and splat records are in flight while a chunk is
+// processed.  Waves claim blocks through an atomic ticket and are independent
+// (no barriers).
+template <int TS>
+__global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
   const uint32_t T = min(hdr->n_tiles, a.tiles_cap);
   const uint32_t P = min(hdr->n_items, a.items_cap);
+  const uint32_t U = T * units_per_tile<TS>();
   const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
-  const int tid = threadIdx.x;
+  const int lane = (int)(threadIdx.x & 63u);
   const float a_min = 0x1.010102p-8f;
   uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
-  // persistent workgroups claim tiles in order: the load balances itself
-  // across CUs and XCDs whatever the spatial distribution of the work
   for (;;) {
-    if (tid == 0) s_tile = atomicAdd(a.ticket, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    if (tile >= T) break;
-    const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, tile)];
-    const int lt = (int)(tile - fd.tile_off);
-    const int tx = lt % fd.tiles_x, ty = lt / fd.tiles_x;
-    const int px = tx * TS + (tid % TS), py = ty * TS + (tid / TS);
+    uint32_t unit = 0;
+    if (lane == 0) unit = atomicAdd(a.ticket, 1u);
+    unit = (uint32_t)__builtin_amdgcn_readfirstlane((int)unit);
+    if (unit >= U) break;
+    uint32_t tile;
+    int ux, uy;
+    const FrameDesc& fd = unit_block<TS>(frames, hdr->n_frames, unit, tile, ux, uy);
+    const int px = ux + (lane & 7), py = uy + (lane >> 3);
     const bool inside = px < fd.W && py < fd.H;
     const float pxf = (float)px, pyf = (float)py;
+    const float x0 = (float)ux, x1 = (float)(ux + kUnit - 1);
+    const float y0 = (float)uy, y1 = (float)(uy + kUnit - 1);
     const uint2 range = a.ranges[tile];
     const int n = (int)(range.y - range.x);
     bool done = !inside;
     float T_ = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, A = 0.0f, Dd = 0.0f;
     uint32_t last = 0;
-    for (int base = 0; base < n; base += B) {
-      if (__syncthreads_count(done) == B) break;
-      n_loaded += min(B, n - base);
-      const int k = base + tid;
-      if (k < n) {
-        uint32_t id = a.ids[range.x + k];
-        s_sp[tid] = a.splats[id < P ? id : 0u];
+
+    // software pipeline: ids two chunks ahead, splat records one chunk ahead
+    // (lanes past the end gather splat 0, which exists whenever n > 0)
+    uint32_t id_n = 0;
+    Splat cur = {};
+    if (n > 0) {
+      id_n = lane < n ? a.ids[range.x + lane] : 0u;
+      cur = a.splats[id_n < P ? id_n : 0u];
+      id_n = 64 + lane < n ? a.ids[range.x + 64 + lane] : 0u;
+    }
+    for (int base = 0; base < n; base += 64) {
+      if (wave_all(done)) break;
+      const Splat s = cur;
+      if (base + 64 < n) {
+        cur = a.splats[id_n < P ? id_n : 0u];
+        const int k2 = base + 128 + lane;
+        id_n = k2 < n ? a.ids[range.x + k2] : 0u;
       }
-      __syncthreads();
-      const int cnt = min(B, n - base);
-      for (int j = 0; j < cnt && !done; j++) {
-        const float4 sa = s_sp[j].a;
-        const float4 sb = s_sp[j].b;
-        const float dx = sa.x - pxf, dy = sa.y - pyf;
-        const float pw = power_of(sa.z, sa.w, sb.x, dx, dy);
-        n_eval++;
-        if (pw > 0.0f || pw < sb.w) continue;
-        const float G = exp_neg(pw);
-        const float al = fminf(0.99f, sb.y * G);
-        if (al < a_min) continue;
-        const float nT = T_ * (1.0f - al);
-        if (nT < 1e-4f) {
-          done = true;
-          break;
+      n_loaded += (uint32_t)min(64, n - base);
+      const bool valid = base + lane < n;
+      const bool touch = valid && extent_touches(s.a.x, s.a.y, unpack_extent(s.c.w), x0, x1, y0, y1);
+      uint64_t m = __ballot(touch);
+      while (m) {
+        const int j = __builtin_ctzll(m);
+        m &= m - 1;
+        // broadcasts with every lane active (lane j may be done)
+        const float mx = bcast(s.a.x, j), my = bcast(s.a.y, j);
+        const float ca = bcast(s.a.z, j), cb = bcast(s.a.w, j), cc = bcast(s.b.x, j);
+        const float op = bcast(s.b.y, j), dz = bcast(s.b.z, j), sk = bcast(s.b.w, j);
+        const float cr = bcast(s.c.x, j), cg = bcast(s.c.y, j), cbl = bcast(s.c.z, j);
+        if (!done) {
+          const float dx = mx - pxf, dy = my - pyf;
+          const float pw = power_of(ca, cb, cc, dx, dy);
+          n_eval++;
+          if (!(pw > 0.0f || pw < sk)) {
+            const float G = exp_neg(pw);
+            const float al = fminf(0.99f, op * G);
+            if (al >= a_min) {
+              const float nT = T_ * (1.0f - al);
+              if (nT < 1e-4f) {
+                done = true;
+              } else {
+                const float aT = al * T_;
+                C0 = __builtin_fmaf(cr, aT, C0);
+                C1 = __builtin_fmaf(cg, aT, C1);
+                C2 = __builtin_fmaf(cbl, aT, C2);
+                A = A + aT;
+                Dd = __builtin_fmaf(dz, aT, Dd);
+                T_ = nT;
+                last = (uint32_t)(base + j + 1);
+                n_con++;
+              }
+            }
+          }
         }
-        const float aT = al * T_;
-        const float4 sc = s_sp[j].c;
-        C0 = __builtin_fmaf(sc.x, aT, C0);
-        C1 = __builtin_fmaf(sc.y, aT, C1);
-        C2 = __builtin_fmaf(sc.z, aT, C2);
-        A = A + aT;
-        Dd = __builtin_fmaf(sb.z, aT, Dd);
-        T_ = nT;
-        last = (uint32_t)(base + j + 1);
-        n_con++;
+        if (wave_all(done)) break;
       }
     }
     if (inside) {
@@ -430,16 +493,15 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs a) {
       a.n_contrib[pid] = last;
       a.final_T[pid] = T_;
     }
-    __syncthreads();  // s_sp and s_tile are reused by the next tile
   }
   if (a.stats) {  // evaluated / contributing pairs and loaded instances
-    unsigned long long e = n_eval, c = n_con, kf = (threadIdx.x == 0) ? n_loaded : 0u;
+    unsigned long long e = n_eval, c = n_con, kf = (lane == 0) ? n_loaded : 0u;
     for (int off = 32; off >= 1; off >>= 1) {
       e += __shfl_xor(e, off, 64);
       c += __shfl_xor(c, off, 64);
       kf += __shfl_xor(kf, off, 64);
     }
-    if ((threadIdx.x & 63) == 0) {
+    if (lane == 0) {
       atomicAdd(a.stats + 0, e);
       atomicAdd(a.stats + 1, c);
       if (kf) atomicAdd(a.stats + 2, kf);
