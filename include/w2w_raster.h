@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define W2W_ABI_VERSION 1
+#define W2W_ABI_VERSION 2
 
 enum {
   W2W_OK = 0,
@@ -79,13 +79,13 @@ typedef struct W2WWorkspace {
  * debugging read them; the layout is a pure function of the sizes). */
 typedef struct W2WLayout {
   size_t geom_bytes, binning_bytes, image_bytes;
-  size_t splats;     /* geom: 48-byte records (px,py,a,b | c,o,depth,0 | r,g,b,0) */
+  size_t splats;     /* geom: 48-byte records (px,py,a,b | c,o,depth,skip | r,g,b,extent) */
   size_t radii;      /* geom: int32[P] */
   size_t rect;       /* geom: uint32x2[P] (x0|y0<<16, x1|y1<<16) */
   size_t counts;     /* geom: uint32[P] tiles touched */
   size_t perm;       /* geom: uint32[P] Gaussian ids in (depth, id) order */
   size_t offsets;    /* geom: uint32[P] exclusive scan of counts in depth order */
-  size_t gacc;       /* geom: float[P][12] backward accumulators */
+  size_t item_off;   /* geom: uint32[P] emission position of each item's first instance */
   size_t sorted_ids; /* binning: uint32[capacity] Gaussian id per sorted instance */
   size_t sorted_tiles; /* binning: uint32[capacity] tile id per sorted instance */
   size_t ranges;     /* image: uint32x2[T] per-tile [start,end) */
@@ -93,6 +93,8 @@ typedef struct W2WLayout {
   size_t final_T;    /* image: float[H*W] final transmittance */
   size_t item_frame; /* geom: uint32[items] frame of each item */
   size_t table;      /* geom: the one-frame table the single-view entry points write */
+  size_t grad_rec;   /* binning: float[capacity][10] per-instance backward partials */
+  size_t grad_flag;  /* binning: uint8[capacity] 1 where grad_rec holds a partial */
 } W2WLayout;
 
 /* Capacities of a batched workspace: every grid is sized by these, and every

@@ -3,29 +3,34 @@
 //   k_blend_bwd       per-pixel reverse traversal (the role of backward.cu:521-665):
 //                     starts from the stored final T and last-contributor index,
 //                     recomputes alpha with the forward's exact arithmetic and
-//                     T <- T / (1 - alpha); no per-(pixel, Gaussian) state.  The
-//                     64 lanes of a wave hold 64 pixels of one tile and visit the
-//                     same instance together, so each gradient field is reduced
-//                     across the wave with DPP and issued as one atomic per wave.
+//                     T <- T / (1 - alpha); no per-(pixel, Gaussian) state.  A
+//                     workgroup owns a tile, one wave per 8x8 block; each wave
+//                     reduces a splat's partials over its pixels with DPP, the
+//                     waves' sums are added in LDS in a fixed order and written
+//                     once per (tile, splat) instance to that instance's slot in
+//                     emission order.  No atomics: the result is deterministic.
 //   k_preprocess_bwd  per-Gaussian chain rule (the role of backward.cu:27-227 and
 //                     :229-519), differentiating the forward exactly as defined
 //                     in k_preprocess (radial clamp, degree-aware SH with the 0.3
-//                     coefficient, NDC depth, no dilation).
+//                     coefficient, NDC depth, no dilation); it first sums each
+//                     item's instance records, which lie contiguously from the
+//                     item's emission offset, in emission order.
 #pragma once
 #include "w2w_device.hpp"
 #include "w2w_forward.hpp"
 
 namespace w2w {
 
-// gradient accumulator record per Gaussian (floats)
-enum GaccField : int {
+// per-instance gradient record (floats)
+enum GradField : int {
   GA_MX = 0, GA_MY = 1,            // dL/dmean2D in NDC units
   GA_CA = 2, GA_CB = 3, GA_CC = 4, // dL/dconic (a, b, c of the power form)
   GA_OP = 5,                       // dL/dopacity
   GA_R = 6, GA_G = 7, GA_B = 8,    // dL/drgb
   GA_DEPTH = 9,                    // dL/d(NDC depth)
-  GA_STRIDE = 12,
 };
+
+constexpr int GR_FIELDS = 10;
 
 // Sum over the 64 lanes of a wave, result valid in lane 63: quad_perm
 // butterflies, row rotations, then the GFX9 row broadcasts.  All lanes must
@@ -46,52 +51,67 @@ W2W_DEV float lane63(float v) {
 
 struct BlendBwdArgs {
   const void* table;
-  uint32_t scenes_cap, tiles_cap, items_cap;
+  uint32_t scenes_cap, tiles_cap, items_cap, capacity;
   uint32_t* ticket;         // zeroed; tiles are claimed in order through it
   const uint2* ranges;
   const uint32_t* ids;
   const Splat* splats;
+  const uint2* rect;        // per item: tile rectangle (x0 | y0 << 16, x1 | y1 << 16)
+  const uint32_t* item_off; // per item: emission position of its first instance
   unsigned long long* stats;  // optional [3]: evaluated pairs, contributing pairs, loaded instances
   const uint32_t* n_contrib;
   const float* final_T;
   const float* dL_dcolor;   // per frame [3,H,W] at 3*pix_off
   const float* dL_dalpha;   // per frame [H,W] at pix_off; may be null (zero)
   const float* dL_ddepth;   // per frame [H,W] at pix_off; may be null (zero)
-  float* gacc;              // [items][GA_STRIDE], pre-zeroed
+  float* rec;               // [capacity][GR_FIELDS] per-instance partials
+  uint8_t* flag;            // [capacity], zeroed: 1 where rec was written
 };
 
 // Backward blend (backward.cu:396-520 semantics, with our forward's
 // decisions): every pixel walks its list back to front from its last
-// contributor, recovering T by T <- T / (1 - alpha) from the stored final T,
-// and the wave sums each splat's 10 partials over its pixels with DPP and
-// adds them to the splat's accumulator with one atomic per field.
+// contributor, recovering T by T <- T / (1 - alpha) from the stored final T.
 //
-// Same work split as the forward: one wave per 8x8 pixel block, chunks of 64
-// gathered one per lane (walking down from the block's largest last
-// contributor), splats whose cull extent misses the block dropped by ballot
-// (every pixel of the block would skip them), the rest visited in reverse
-// order.  The next chunk's ids and records are in flight meanwhile.
+// A workgroup claims a tile; wave w owns 8x8 block w.  The waves walk the
+// tile's list together in chunks of 64 (down from the tile's largest last
+// contributor), each gathering the chunk one instance per lane, dropping by
+// ballot the splats whose cull extent misses its block (every pixel of the
+// block would skip them) and visiting the rest in reverse order.  For a splat
+// with any contributing pixel the wave sums the 10 partials over its lanes
+// with DPP into its LDS slot; after a barrier, lane j of each wave adds the
+// waves' slots for splat j in wave order and writes the sums to the
+// instance's record (emission position = item offset + its tile's index in
+// the item's rectangle) and sets its flag.  Partial slots are double
+// buffered, so one barrier per chunk suffices.
 template <int TS>
-__global__ __launch_bounds__(256) void k_blend_bwd(BlendBwdArgs a) {
+__global__ __launch_bounds__(64 * units_per_tile<TS>()) void k_blend_bwd(BlendBwdArgs a) {
+  constexpr int UPT = (int)units_per_tile<TS>();
+  constexpr int UX = TS / kUnit;
+  __shared__ float s_part[2][UPT][GR_FIELDS][64];
+  __shared__ uint64_t s_cm[2][UPT];
+  __shared__ uint32_t s_tile, s_max;
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
   const uint32_t Tn = min(hdr->n_tiles, a.tiles_cap);
   const uint32_t P = min(hdr->n_items, a.items_cap);
-  const uint32_t U = Tn * units_per_tile<TS>();
   const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
-  const int lane = (int)(threadIdx.x & 63u);
+  const int tid = (int)threadIdx.x, lane = tid & 63, w = tid >> 6;
   const float a_min = 0x1.010102p-8f;
   uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
   for (;;) {
-    uint32_t unit = 0;
-    if (lane == 0) unit = atomicAdd(a.ticket, 1u);
-    unit = (uint32_t)__builtin_amdgcn_readfirstlane((int)unit);
-    if (unit >= U) break;
-    uint32_t tile;
-    int ux, uy;
-    const FrameDesc& fd = unit_block<TS>(frames, hdr->n_frames, unit, tile, ux, uy);
-    if ((fd.flags & FRAME_BACKWARD) == 0) continue;
+    if (tid == 0) {
+      s_tile = atomicAdd(a.ticket, 1u);
+      s_max = 0;
+    }
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    if (tile >= Tn) break;
+    const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, tile)];
+    const bool active = (fd.flags & FRAME_BACKWARD) != 0;
+    const int lt = (int)(tile - fd.tile_off);
+    const int tx = lt % fd.tiles_x, ty = lt / fd.tiles_x;
+    const int ux = tx * TS + (w % UX) * kUnit, uy = ty * TS + (w / UX) * kUnit;
     const int px = ux + (lane & 7), py = uy + (lane >> 3);
-    const bool inside = px < fd.W && py < fd.H;
+    const bool inside = active && px < fd.W && py < fd.H;
     const float pxf = (float)px, pyf = (float)py;
     const float x0 = (float)ux, x1 = (float)(ux + kUnit - 1);
     const float y0 = (float)uy, y1 = (float)(uy + kUnit - 1);
@@ -117,27 +137,30 @@ __global__ __launch_bounds__(256) void k_blend_bwd(BlendBwdArgs a) {
     int wmax = last;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
-    wmax = __builtin_amdgcn_readfirstlane(wmax);
-    wmax = min(wmax, (int)(range.y - range.x));
+    wmax = min(__builtin_amdgcn_readfirstlane(wmax), (int)(range.y - range.x));
+    if (lane == 0) atomicMax(&s_max, (uint32_t)wmax);
+    __syncthreads();
+    const int nproc = (int)s_max;
 
     float T = Tf;
     float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, accD = 0.0f, accA = 0.0f;
     float la = 0.0f, lc0 = 0.0f, lc1 = 0.0f, lc2 = 0.0f, ld = 0.0f;
 
-    // chunk c covers [wmax - 64 (c + 1), wmax - 64 c) clipped at 0; lane l
-    // holds its instance start + l
-    // (lanes before the start gather splat 0, which exists whenever wmax > 0)
+    // chunk c covers [nproc - 64 (c + 1), nproc - 64 c) clipped at 0; lane l
+    // holds instance start + l.  Ids two chunks ahead, records one chunk
+    // ahead; lanes before the start gather splat 0 (present when nproc > 0).
     uint32_t id_n = 0, id_cur = 0;
     Splat cur = {};
-    if (wmax > 0) {
-      const int k = wmax - 64 + lane;
+    if (nproc > 0) {
+      const int k = nproc - 64 + lane;
       id_n = k >= 0 ? a.ids[range.x + k] : 0u;
       id_cur = id_n < P ? id_n : 0u;
       cur = a.splats[id_cur];
       id_n = k >= 64 ? a.ids[range.x + k - 64] : 0u;
     }
-    for (int cend = wmax; cend > 0; cend -= 64) {
-      const int cstart = cend - 64;  // may be negative: lanes below 0 are empty
+    int buf = 0;
+    for (int cend = nproc; cend > 0; cend -= 64, buf ^= 1) {
+      const int cstart = cend - 64;
       const Splat s = cur;
       const uint32_t sid = id_cur;
       if (cstart > 0) {
@@ -148,9 +171,10 @@ __global__ __launch_bounds__(256) void k_blend_bwd(BlendBwdArgs a) {
       }
       const int kk_l = cstart + lane;
       n_loaded += (uint32_t)min(64, cend);
-      const bool valid = kk_l >= 0;
+      const bool valid = kk_l >= 0 && kk_l < wmax;
       const bool touch = valid && extent_touches(s.a.x, s.a.y, unpack_extent(s.c.w), x0, x1, y0, y1);
       uint64_t m = __ballot(touch);
+      uint64_t cm = 0;
       while (m) {
         const int j = 63 - __builtin_clzll(m);
         m &= ~(1ull << j);
@@ -213,26 +237,48 @@ __global__ __launch_bounds__(256) void k_blend_bwd(BlendBwdArgs a) {
           g_g = wave_reduce_dpp(g_g);
           g_b = wave_reduce_dpp(g_b);
           g_d = wave_reduce_dpp(g_d);
-          const uint32_t gid = (uint32_t)__builtin_amdgcn_readlane((int)sid, j);
           if (lane == 63) {
-            float* dst = a.gacc + (size_t)gid * GA_STRIDE;
-            atomicAdd(dst + GA_MX, g_mx);
-            atomicAdd(dst + GA_MY, g_my);
-            atomicAdd(dst + GA_CA, g_ca);
-            atomicAdd(dst + GA_CB, g_cb);
-            atomicAdd(dst + GA_CC, g_cc);
-            atomicAdd(dst + GA_OP, g_op);
-            atomicAdd(dst + GA_R, g_r);
-            atomicAdd(dst + GA_G, g_g);
-            atomicAdd(dst + GA_B, g_b);
-            atomicAdd(dst + GA_DEPTH, g_d);
+            float(*slot)[64] = s_part[buf][w];
+            slot[GA_MX][j] = g_mx;
+            slot[GA_MY][j] = g_my;
+            slot[GA_CA][j] = g_ca;
+            slot[GA_CB][j] = g_cb;
+            slot[GA_CC][j] = g_cc;
+            slot[GA_OP][j] = g_op;
+            slot[GA_R][j] = g_r;
+            slot[GA_G][j] = g_g;
+            slot[GA_B][j] = g_b;
+            slot[GA_DEPTH][j] = g_d;
           }
+          cm |= 1ull << j;
+        }
+      }
+      if (lane == 0) s_cm[buf][w] = cm;
+      __syncthreads();
+      // splat `lane` of the chunk: sum the waves' partials in wave order
+      uint64_t any = 0;
+#pragma unroll
+      for (int v = 0; v < UPT; v++) any |= s_cm[buf][v];
+      if ((any >> lane) & 1ull) {
+        const uint2 r = a.rect[sid];
+        const uint32_t rx0 = r.x & 0xFFFFu, ry0 = r.x >> 16, rx1 = r.y & 0xFFFFu;
+        const uint32_t pos = a.item_off[sid] + ((uint32_t)ty - ry0) * (rx1 - rx0) + ((uint32_t)tx - rx0);
+        if (pos < a.capacity) {
+          for (int f = w; f < GR_FIELDS; f += UPT) {
+            float sum = 0.0f;
+#pragma unroll
+            for (int v = 0; v < UPT; v++)
+              if ((s_cm[buf][v] >> lane) & 1ull) sum += s_part[buf][v][f][lane];
+            a.rec[(size_t)pos * GR_FIELDS + f] = sum;
+          }
+          if (w == 0) a.flag[pos] = 1;
         }
       }
     }
+    __syncthreads();  // s_tile, s_max and the partial slots are reused by the next tile
   }
   if (a.stats) {
-    unsigned long long e = n_eval, c = n_con, kf = (lane == 0) ? n_loaded : 0u;
+    unsigned long long e = n_eval, c = n_con, kf = (tid == 0) ? n_loaded : 0u;
     for (int off = 32; off >= 1; off >>= 1) {
       e += __shfl_xor(e, off, 64);
       c += __shfl_xor(c, off, 64);
@@ -254,7 +300,11 @@ struct PreBwdArgs {
   const float* rots;     // [G,4] normalised
   const float* sh;       // packed SH blocks
   const int32_t* radii;  // per item
-  const float* gacc;     // per item
+  const uint32_t* counts;    // per item: instances emitted
+  const uint32_t* item_off;  // per item: emission position of the first
+  const float* rec;          // [capacity][GR_FIELDS] per-instance partials
+  const uint8_t* flag;       // [capacity]
+  uint32_t capacity;
   float* dL_dmeans2D;    // per item [3]  (NDC units, z = 0)
   float* dL_dcolors;     // per item [3]
   float* dL_dopacity;    // per Gaussian
@@ -295,8 +345,19 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreBwdArgs a) {
       const uint32_t i = sd.item_off + g * sd.n_views + v;
       if (i >= a.items_cap) break;
       const FrameDesc& fd = frames[sd.frame_off + v];
-      const float* ga = a.gacc + (size_t)i * GA_STRIDE;
       const bool vis = a.radii[i] > 0 && (fd.flags & FRAME_BACKWARD);
+      // sum of the item's instance records, in emission order
+      float ga[GR_FIELDS];
+      for (int f = 0; f < GR_FIELDS; f++) ga[f] = 0.0f;
+      if (vis) {
+        const uint32_t k0 = a.item_off[i];
+        const uint32_t k1 = min(k0 + a.counts[i], a.capacity);
+        for (uint32_t k = k0; k < k1; k++) {
+          if (!a.flag[k]) continue;
+          const float* r = a.rec + (size_t)k * GR_FIELDS;
+          for (int f = 0; f < GR_FIELDS; f++) ga[f] += r[f];
+        }
+      }
       a.dL_dmeans2D[3 * (size_t)i] = vis ? ga[GA_MX] : 0.0f;
       a.dL_dmeans2D[3 * (size_t)i + 1] = vis ? ga[GA_MY] : 0.0f;
       a.dL_dmeans2D[3 * (size_t)i + 2] = 0.0f;

@@ -130,7 +130,7 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   p.dhist = take(4 * 4 * RADIX);
   p.dstatus = take((size_t)4 * p.dtiles * RADIX * 4);
   p.slb = take((size_t)p.stiles * 4 + 4);
-  L.gacc = take(4 * GA_STRIDE * I);
+  L.item_off = take(4 * I);
   L.geom_bytes = o;
   o = 0;
   const size_t C = (size_t)c.capacity;
@@ -140,6 +140,8 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   p.tid1 = take(4 * C);
   p.thist = take(4 * 4 * RADIX);
   p.tstatus = take((size_t)p.tpasses * p.ctiles * RADIX * 4);
+  L.grad_rec = take(sizeof(float) * GR_FIELDS * C);
+  L.grad_flag = take(C);  // take() rounds the end up to the 256-byte alignment
   L.binning_bytes = o;
   L.sorted_tiles = (p.tpasses & 1) ? p.tkey1 : p.tkey0;
   L.sorted_ids = (p.tpasses & 1) ? p.tid1 : p.tid0;
@@ -223,10 +225,12 @@ void launch_blend(const BlendArgs& a, uint32_t tiles_cap, hipStream_t s) {
   hipLaunchKernelGGL(k_blend<TS>, dim3(blend_grid<TS>(tiles_cap)), dim3(256), 0, s, a);
 }
 
+// one tile per workgroup at a time, one wave per 8x8 block of it
 template <int TS>
 void launch_blend_bwd(const BlendBwdArgs& a, uint32_t tiles_cap, hipStream_t s) {
   ProfSpan ps(W2W_KID_BLEND_BWD, s);
-  hipLaunchKernelGGL(k_blend_bwd<TS>, dim3(blend_grid<TS>(tiles_cap)), dim3(256), 0, s, a);
+  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(tiles_cap, kBlendGrid));
+  hipLaunchKernelGGL(k_blend_bwd<TS>, dim3(grid), dim3(64 * units_per_tile<TS>()), 0, s, a);
 }
 
 template <typename KT>
@@ -330,7 +334,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
                            at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.L.counts),
                            at<uint2>(G, p.L.rect), at<uint32_t>(G, p.L.item_frame), items_cap, cap,
                            status + ST_EMIT_TICKET, at<uint32_t>(B, p.tkey0),
-                           at<uint32_t>(B, p.tid0));
+                           at<uint32_t>(B, p.tid0), at<uint32_t>(G, p.L.item_off));
       }
       sort_passes<uint32_t>(at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0),
                             at<uint32_t>(B, p.L.sorted_tiles), at<uint32_t>(B, p.L.sorted_ids),
@@ -377,22 +381,26 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
   void* I = ws->image;
   const W2WCaps& c = p.c;
   if (c.items == 0) return W2W_OK;
-  float* gacc = at<float>(G, p.L.gacc);
   uint32_t* status = at<uint32_t>(G, p.status);
-  fill_u32(reinterpret_cast<uint32_t*>(gacc), 0u, (size_t)GA_STRIDE * c.items, s);
+  float* rec = at<float>(B, p.L.grad_rec);
+  uint8_t* flag = at<uint8_t>(B, p.L.grad_flag);
+  fill_u32(reinterpret_cast<uint32_t*>(flag), 0u, cdiv((uint64_t)c.capacity, 4), s);
   fill_u32(status + ST_BWD_TICKET, 0u, 1, s);
   BlendBwdArgs ba;
   ba.table = table; ba.scenes_cap = (uint32_t)c.scenes; ba.tiles_cap = (uint32_t)c.tiles;
-  ba.items_cap = (uint32_t)c.items;
+  ba.items_cap = (uint32_t)c.items; ba.capacity = (uint32_t)c.capacity;
   ba.ticket = status + ST_BWD_TICKET;
   ba.ranges = at<uint2>(I, p.L.ranges);
   ba.ids = at<uint32_t>(B, p.L.sorted_ids);
   ba.splats = at<Splat>(G, p.L.splats);
+  ba.rect = at<uint2>(G, p.L.rect);
+  ba.item_off = at<uint32_t>(G, p.L.item_off);
+  ba.rec = rec;
+  ba.flag = flag;
   ba.stats = g_stats ? g_stats + 3 : nullptr;
   ba.n_contrib = at<uint32_t>(I, p.L.n_contrib);
   ba.final_T = at<float>(I, p.L.final_T);
   ba.dL_dcolor = dL_dcolor; ba.dL_dalpha = dL_dalpha; ba.dL_ddepth = dL_ddepth;
-  ba.gacc = gacc;
   switch (c.tile) {
     case 8: launch_blend_bwd<8>(ba, (uint32_t)c.tiles, s); break;
     case 16: launch_blend_bwd<16>(ba, (uint32_t)c.tiles, s); break;
@@ -403,7 +411,11 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
   pa.gaussians_cap = (uint32_t)c.gaussians;
   pa.means = prm.means; pa.scales = prm.scales; pa.rots = prm.rots; pa.sh = prm.sh;
   pa.radii = at<int32_t>(G, p.L.radii);
-  pa.gacc = gacc;
+  pa.counts = at<uint32_t>(G, p.L.counts);
+  pa.item_off = at<uint32_t>(G, p.L.item_off);
+  pa.rec = rec;
+  pa.flag = flag;
+  pa.capacity = (uint32_t)c.capacity;
   pa.dL_dmeans2D = dL_dmeans2D; pa.dL_dcolors = dL_dcolors; pa.dL_dopacity = dL_dopacity;
   pa.dL_dmeans3D = dL_dmeans3D; pa.dL_dcov3D = dL_dcov3D; pa.dL_dsh = dL_dsh;
   pa.dL_dscales = dL_dscales; pa.dL_drots = dL_drotations;

@@ -264,7 +264,8 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
                                               uint32_t items_cap, uint32_t capacity,
                                               uint32_t* __restrict__ ticket,
                                               uint32_t* __restrict__ tile_keys,
-                                              uint32_t* __restrict__ ids) {
+                                              uint32_t* __restrict__ ids,
+                                              uint32_t* __restrict__ item_off) {
   __shared__ uint32_t s_start[256], s_g[256], s_x0[256], s_y0[256], s_w[256], s_tx[256], s_toff[256];
   __shared__ uint32_t s_end, s_chunk;
   const BatchHeader* hdr = static_cast<const BatchHeader*>(table);
@@ -286,6 +287,7 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
       const FrameDesc& fd = frames[item_frame[g]];
       s_start[tid] = offs[i];
       s_g[tid] = g;
+      item_off[g] = offs[i];  // the backward finds an item's instances from here
       s_x0[tid] = r.x & 0xffffu;
       s_y0[tid] = r.x >> 16;
       s_w[tid] = (r.y & 0xffffu) - (r.x & 0xffffu);
