@@ -1,13 +1,18 @@
 #!/usr/bin/env python
 """Benchmark: fwd+bwd views/sec of the HIP Gaussian rasterizer.
 
-Workload (the large-scene one-GPU configuration): 1,000,000 Gaussians, SH
-degree 3, FP32, 4 orbital views per step at 1024x1024 with elevations drawn
-uniform in [-30, 30] degrees (radius 2.5, fovy 49.1), 16x16 tiles.  One step
-= render the 4 views, L2 loss against images rendered from a second seed's
-scene, backward, Adam (eps 1e-15, the trainer's per-group learning rates).
-Inputs are synthetic, generated on the CPU from fixed seeds and resident in
-HBM before timing starts.
+Default workload (the large-scene one-GPU configuration): 1,000,000
+Gaussians, SH degree 3, FP32, 4 orbital views per step at 1024x1024 with
+elevations drawn uniform in [-30, 30] degrees (radius 2.5, fovy 49.1), 16x16
+tiles.  One step = render the 4 views, L2 loss against images rendered from a
+second seed's scene, backward, Adam (eps 1e-15, the trainer's per-group
+learning rates), all through the batched trainer (one launch per stage over
+the step's views).  Inputs are synthetic, generated on the CPU from fixed
+seeds and resident in HBM before timing starts.  The drop-in per-view path
+(autograd Function + torch Adam) is timed beside it.
+
+Other workloads: --workload batched64 (64 scenes, batched vs plain) and
+--workload served (continuous batching vs a FIFO plain server).
 
 Prints ONE JSON line.  `value` = views/s over the timed steps; `roofline` is
 the dominant kernel, timed live with HIP events on its launch stream;
@@ -53,6 +58,7 @@ def parse():
     ap.add_argument("--rate", type=float, default=40.0)
     ap.add_argument("--opt-steps", type=int, default=50)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=3)
     return ap.parse_args()
 
 
@@ -94,9 +100,52 @@ def main():
         return bench_batched(args)
     if args.workload == "served":
         return bench_served(args)
+    return bench_scene(args)
+
+
+def plain_step_fn(params, cams, targets, P, ts, dev):
+    """The drop-in path: the reference trainer's loop (one view per launch
+    sequence through the autograd Function, torch Adam)."""
+    import torch
+    from word2world_amd import raster
+    from word2world_amd.scene import activate
+    lrs = {"means": 0.01, "sh_dc": 0.01, "sh_ac": 0.0005, "opacity": 0.05, "svec": 0.005,
+           "quaternion": 0.005}  # optimizer_config.json
+    opt = torch.optim.Adam([{"params": [params[k]], "lr": lr, "name": k} for k, lr in lrs.items()],
+                           lr=0.0, eps=1e-15)
+
+    def step(i):
+        for g in opt.param_groups:
+            if g["name"] == "means":
+                g["lr"] = mean_lr(i)
+        opt.zero_grad(set_to_none=True)
+        m, s, r, o, sh = activate(params)
+        sh = sh.contiguous()
+        loss = 0.0
+        for v, cam in enumerate(cams):
+            m2d = torch.zeros(P, 3, device=dev, requires_grad=True)
+            img, radii, depth, alpha = raster.rasterize(m, m2d, sh, o, s, r, cam, 3, ts)
+            loss = loss + ((img.clamp(0, 1) - targets[v]) ** 2).mean()
+        loss.backward()
+        opt.step()
+        return loss
+    return step
+
+
+def mean_lr(step):  # misc_utils.py:11-27 with (0.01, 0.0002, 300)
+    t = min(max(step / 300, 0.0), 1.0)
+    return float(np.exp(np.log(0.01) * (1 - t) + np.log(0.0002) * t))
+
+
+def bench_scene(args):
+    """1M Gaussians, 4 views at 1024^2 per step: fwd+bwd+Adam through the
+    batched trainer (all views of the step in one launch per stage, flat
+    parameter store, fused Adam), with the drop-in per-view path timed beside
+    it."""
     import torch
 
     from word2world_amd import _lib, raster
+    from word2world_amd.batched import BatchedTrainer
     from word2world_amd.camera import orbit_camera
     from word2world_amd.scene import activate, make_scene
 
@@ -114,7 +163,6 @@ def main():
     log(f"scene: {P} Gaussians SH3, {V} views at {R}^2, tile {ts}")
     scene = make_scene(P, 3, seed=0)
     target_scene = make_scene(P, 3, seed=1)
-    params = {k: v.to(dev).requires_grad_(True) for k, v in scene.params().items()}
     M = 16
     with torch.no_grad():
         tm, ts_, tr, to, tsh = (t.to(dev).contiguous() for t in activate(target_scene.params()))
@@ -123,61 +171,26 @@ def main():
             K, color, _, _, _, _ = raster.forward_checked(tm, ts_, tr, to, tsh, 3, cam, ts)
             targets.append(color.clamp(0, 1))
         del tm, ts_, tr, to, tsh
-    lrs = {"means": 0.01, "sh_dc": 0.01, "sh_ac": 0.0005, "opacity": 0.05, "svec": 0.005,
-           "quaternion": 0.005}  # optimizer_config.json
-    opt = torch.optim.Adam([{"params": [params[k]], "lr": lr, "name": k} for k, lr in lrs.items()],
-                           lr=0.0, eps=1e-15)
+    torch.cuda.empty_cache()
 
-    def mean_lr(step):  # misc_utils.py:11-27 with (0.01, 0.0002, 300)
-        t = min(max(step / 300, 0.0), 1.0)
-        return float(np.exp(np.log(0.01) * (1 - t) + np.log(0.0002) * t))
-
-    workspaces = []
-
-    def step(i):
-        for g in opt.param_groups:
-            if g["name"] == "means":
-                g["lr"] = mean_lr(i)
-        opt.zero_grad(set_to_none=True)
-        m, s, r, o, sh = activate(params)
-        sh = sh.contiguous()
-        loss = 0.0
-        workspaces.clear()
-        for v, cam in enumerate(cams):
-            m2d = torch.zeros(P, 3, device=dev, requires_grad=True)
-            img, radii, depth, alpha = raster.rasterize(m, m2d, sh, o, s, r, cam, 3, ts,
-                                                        workspace_cb=workspaces.append)
-            loss = loss + ((img.clamp(0, 1) - targets[v]) ** 2).mean()
-        loss.backward()
-        opt.step()
-        return loss
-
-    def overflowed():
-        st = torch.stack([w.status() for w in workspaces]).cpu().numpy()
-        return st
-
-    # capacity discovery + warmup (synchronised checks; an overflowed step is redone)
-    with torch.no_grad():
-        m, s, r, o, sh = (t.contiguous() for t in activate(params))
-        for cam in cams:
-            raster.forward_checked(m, s, r, o, sh, 3, cam, ts)
+    bt = BatchedTrainer([scene], [cams_host], [targets], dev, ts)
     for i in range(args.warmup):
-        step(i)
-        st = overflowed()
-        log(f"warmup step {i}: K per view {st[:, 0].tolist()} overflow {st[:, 1].tolist()}")
-        if st[:, 1].any():
-            raster.POLICY.observe((P, R, R, ts), int(st[:, 0].max()))
-    torch.cuda.synchronize()
+        bt.step()
+        torch.cuda.synchronize()
+        K, ovf = (int(v) for v in bt.ws.status().cpu())
+        log(f"warmup step {i}: K={K} ({K / V:.0f} per view) overflow={ovf}")
+        if ovf:
+            raise RuntimeError("instance capacity overflowed")
 
     # which kernel dominates: one profiled step over every kernel id
-    L.w2w_prof_enable(4096, 0xFFFF)
+    L.w2w_prof_enable(4096, 0xFFFFFFFF)
     L.w2w_prof_reset()
-    step(args.warmup)
+    bt.step()
     torch.cuda.synchronize()
-    rec = _lib.prof_read()
     per = {}
-    for name, ms in rec:
+    for name, ms in _lib.prof_read():
         per[name] = per.get(name, 0.0) + ms
+    L.w2w_prof_enable(0, 0)
     dominant = max(per, key=per.get)
     log("per-kernel ms in one step: " + ", ".join(f"{k}={v:.3f}" for k, v in
                                                   sorted(per.items(), key=lambda kv: -kv[1])))
@@ -185,28 +198,26 @@ def main():
     # work counters for the roofline flops (one extra, untimed step)
     stats = torch.zeros(6, dtype=torch.int64, device=dev)
     L.w2w_stats_enable(stats.data_ptr())
-    step(args.warmup + 1)
+    bt.step()
     torch.cuda.synchronize()
     L.w2w_stats_enable(None)
     stats_np = stats.cpu().numpy().astype(np.float64) / V  # per view
-    Ks = overflowed()[:, 0].astype(np.float64)
+    K_total = int(bt.ws.status()[0])
 
-    # timed region: events only around the dominant kernel
+    # timed region: events only around the dominant kernel's launches
     L.w2w_prof_enable(8192, 1 << kid)
     L.w2w_prof_reset()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
           for _ in range(args.steps)]
-    ovf_acc = torch.zeros(2, dtype=torch.int32, device=dev)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         ev[i][0].record()
-        step(args.warmup + 2 + i)
+        bt.step()
         ev[i][1].record()
-        ovf_acc = torch.maximum(ovf_acc, torch.stack([w.status() for w in workspaces]).max(0).values)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    if int(ovf_acc[1].item()):
+    if bt.overflow():
         raise RuntimeError("instance capacity overflowed inside the timed region")
     step_ms = np.array([a.elapsed_time(b) for a, b in ev])
     dom = [ms for name, ms in _lib.prof_read() if name == dominant]
@@ -215,13 +226,13 @@ def main():
     views_per_s = args.steps * V / wall
     log(f"timed {args.steps} steps: {wall * 1e3 / args.steps:.2f} ms/step, {views_per_s:.1f} views/s")
 
-    # roofline of the dominant kernel
-    K_mean = float(Ks.mean())
+    # roofline of the dominant kernel (one launch covers all V views)
+    K_view = K_total / V
     T_tiles = ((R + ts - 1) // ts) ** 2
-    launches_per_view = max(1, round(len(dom) / (args.steps * V)))
-    bound, work = algorithmic_cost(dominant, P, M, K_mean, T_tiles, R * R, stats_np)
+    bound, work_view = algorithmic_cost(dominant, P, M, K_view, T_tiles, R * R, stats_np)
+    launches_per_step = max(1, round(len(dom) / args.steps))
+    work_per_launch = work_view * V / launches_per_step
     avg_ms = float(np.mean(dom)) if dom else float("nan")
-    work_per_launch = work / launches_per_view
     if bound == "valu":
         achieved = work_per_launch / (avg_ms * 1e-3) / 1e12
         roof = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -232,6 +243,27 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None}
     roof.update({"kernel": dominant, "avg_launch_ms": avg_ms, "launches": len(dom),
                  "algorithmic_per_launch": work_per_launch})
+    del bt
+    torch.cuda.empty_cache()
+
+    # the drop-in per-view path, for comparison
+    plain = None
+    if args.plain_steps > 0:
+        params = {k: v.to(dev).requires_grad_(True) for k, v in scene.params().items()}
+        step = plain_step_fn(params, cams, targets, P, ts, dev)
+        step(0)
+        torch.cuda.synchronize()
+        tp0 = time.perf_counter()
+        for i in range(args.plain_steps):
+            step(1 + i)
+        torch.cuda.synchronize()
+        tp = (time.perf_counter() - tp0) / args.plain_steps
+        plain = {"views_per_s": V / tp, "ms_per_step": tp * 1e3, "steps": args.plain_steps,
+                 "path": "drop-in rasterize_gaussians autograd Function, one view per launch "
+                         "sequence, torch.optim.Adam"}
+        log(f"plain drop-in path: {tp * 1e3:.2f} ms/step, {V / tp:.1f} views/s")
+        del params
+        torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -244,13 +276,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{P} Gaussians SH3 (seeded synthetic), {V} orbital views at {R}x{R} "
                                "(elevation U[-30,30]), fwd+bwd+Adam per step, L2 to a second "
-                               "seed's renders", "gaussians": P, "views_per_step": V,
-                   "resolution": R, "tile": ts},
+                               "seed's renders; all views of a step in one launch per stage",
+                   "gaussians": P, "views_per_step": V, "resolution": R, "tile": ts},
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
-        "per_view": {"K": K_mean, "evaluated_pairs_fwd": stats_np[0], "contributing_pairs_fwd": stats_np[1],
+        "per_view": {"K": K_view, "evaluated_pairs_fwd": stats_np[0], "contributing_pairs_fwd": stats_np[1],
                      "instances_loaded_fwd": stats_np[2], "evaluated_pairs_bwd": stats_np[3],
                      "contributing_pairs_bwd": stats_np[4], "instances_loaded_bwd": stats_np[5]},
         "kernel_ms_one_step": per,
+        "plain": plain,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

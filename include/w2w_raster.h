@@ -223,9 +223,12 @@ int w2w_activate_backward(const float* scales, const float* opac, const float* q
                           int64_t G, float* g_svec, float* g_logit, float* g_quat, void* stream);
 /* Per frame: mean over 3*H*W of (clamp(color,0,1) - target)^2 into
  * loss_per_frame[frame] and its gradient into dL_dcolor (zero for frames
- * without the backward flag); colour/target packed as the batch outputs. */
+ * without the backward flag); colour/target packed as the batch outputs.
+ * tile_scratch: caps->tiles floats (per-tile partial sums).  Every sum has a
+ * fixed order, so the loss is deterministic.  The loss replaces the SDS
+ * guidance of GaussianTrainer.py:86-114 for synthetic targets. */
 int w2w_l2_loss(const void* table, const W2WCaps* caps, const float* color, const float* target,
-                float* dL_dcolor, float* loss_per_frame, void* stream);
+                float* dL_dcolor, float* loss_per_frame, float* tile_scratch, void* stream);
 /* Adam over a flat store with the step's scalars in device memory (so a
  * captured graph picks them up at replay): hyper[0..7] = -lr_k/(1-b1^t) per
  * group, hyper[8..15] = the same for a group's second rate, hyper[16] =
@@ -265,7 +268,8 @@ enum {
   W2W_KID_FILL = 0, W2W_KID_PREPROCESS = 1, W2W_KID_DSORT_HIST = 2, W2W_KID_DSORT_PASS = 3,
   W2W_KID_SCAN = 4, W2W_KID_EMIT = 5, W2W_KID_TSORT_HIST = 6, W2W_KID_TSORT_PASS = 7,
   W2W_KID_RANGES = 8, W2W_KID_BLEND = 9, W2W_KID_BLEND_BWD = 10, W2W_KID_PREPROCESS_BWD = 11,
-  W2W_KID_ADAM = 12, W2W_KID_SORT_HIST = 13, W2W_KID_SORT_PASS = 14,
+  W2W_KID_ADAM = 12, W2W_KID_SORT_HIST = 13, W2W_KID_SORT_PASS = 14, W2W_KID_ACTIVATE = 15,
+  W2W_KID_LOSS = 16, W2W_KID_ACTIVATE_BWD = 17,
 };
 int w2w_prof_enable(int32_t capacity, uint32_t kernel_mask);
 int w2w_prof_reset(void);

@@ -56,8 +56,9 @@ def test_batched_matches_plain_trainer(gpu):
         print("step", it, per_scene, pl, flush=True)
         np.testing.assert_allclose(per_scene, pl, rtol=2e-4)
         if it == 0:
-            # same parameters in both: the gradients agree up to the order of
-            # the backward's float atomics
+            # same parameters in both: the rasterizer gradients are bit-equal
+            # per view; the activation backward (torch autograd against the
+            # fused kernel) and the sum over views differ in rounding
             g0 = 0
             for (sc, _, _), p in zip(jobs, plains):
                 ref = _plain_grads(p)
@@ -67,9 +68,10 @@ def test_batched_matches_plain_trainer(gpu):
                     scale = np.abs(r).max()
                     assert np.abs(got - r).max() <= 1e-4 * scale, (name, np.abs(got - r).max(), scale)
                 g0 += sc.n
-    # parameters after 3 steps.  Adam (eps 1e-15) turns a gradient that is
-    # atomic-order noise into a full +-lr step, so a few elements may differ by
-    # up to 2 lr per step; all others agree to float rounding.
+    # parameters after 3 steps.  The activation gradients differ in rounding
+    # (torch autograd against the fused kernels) and Adam (eps 1e-15) turns a
+    # gradient that is mostly cancellation into a full +-lr step, so a few
+    # elements may differ by up to 2 lr per step; the rest agree closely.
     g0 = 0
     for (sc, _, _), p in zip(jobs, plains):
         n = sc.n
@@ -80,7 +82,8 @@ def test_batched_matches_plain_trainer(gpu):
             ref = ref.detach().cpu().numpy()
             d = np.abs(got - ref)
             assert d.max() <= 6 * lr, (name, d.max())
-            assert np.percentile(d, 99) <= 1e-5, (name, np.percentile(d, 99))
+            # 99 % of the elements within 0.1 % of one Adam step per step
+            assert np.percentile(d, 99) <= 1e-3 * lr * 3, (name, np.percentile(d, 99))
         g0 += n
 
 
@@ -93,9 +96,9 @@ def test_graph_replay_matches_eager(gpu):
     graph.step()
     graph.capture()
     for it in range(4):
-        # start both from the same state: the forward of the replay must then be
-        # bit-identical to the eager forward, the update equal up to the order
-        # of the backward's float atomics
+        # start both from the same state: the replay must then reproduce the
+        # eager step bit for bit (no atomics anywhere in the step, every sum
+        # has a fixed order)
         for name in ("flat", "exp_avg", "exp_avg_sq"):
             getattr(graph.store, name).copy_(getattr(eager.store, name))
         graph.it = eager.it
@@ -105,11 +108,8 @@ def test_graph_replay_matches_eager(gpu):
         junk = torch.randn(1 << 16, device=gpu).sum()
         torch.cuda.synchronize()
         del junk
-        np.testing.assert_allclose(lg.cpu().numpy(), le.cpu().numpy(), rtol=1e-5, atol=1e-10)
+        assert torch.equal(lg, le), it
         for (c1, d1, a1), (c2, d2, a2) in zip(graph.frame_images(), eager.frame_images()):
             assert torch.equal(c1, c2) and torch.equal(d1, d2) and torch.equal(a1, a2)
-        a = graph.store.flat.cpu().numpy()
-        b = eager.store.flat.cpu().numpy()
-        d = np.abs(a - b)
-        assert d.max() <= 2 * 0.05, it
-        assert np.percentile(d, 99) <= 1e-5 * max(1.0, np.abs(b).max()), it
+        for name in ("grad", "flat", "exp_avg", "exp_avg_sq"):
+            assert torch.equal(getattr(graph.store, name), getattr(eager.store, name)), (it, name)

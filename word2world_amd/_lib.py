@@ -55,7 +55,8 @@ EXPORTS = ["w2w_version", "w2w_splat_extents", "w2w_workspace_layout", "w2w_batc
 
 KERNEL_IDS = {0: "fill", 1: "preprocess", 2: "depth_sort_hist", 3: "depth_sort_pass", 4: "scan",
               5: "emit", 6: "tile_sort_hist", 7: "tile_sort_pass", 8: "ranges", 9: "blend",
-              10: "blend_bwd", 11: "preprocess_bwd", 12: "adam", 13: "sort_hist", 14: "sort_pass"}
+              10: "blend_bwd", 11: "preprocess_bwd", 12: "adam", 13: "sort_hist", 14: "sort_pass",
+              15: "activate", 16: "loss", 17: "activate_bwd"}
 
 _lib = None
 
@@ -118,7 +119,7 @@ def load(path: str = LIB_PATH):
     L.w2w_activate_backward.restype = ctypes.c_int
     L.w2w_activate_backward.argtypes = [_vp] * 6 + [ctypes.c_int64] + [_vp] * 4
     L.w2w_l2_loss.restype = ctypes.c_int
-    L.w2w_l2_loss.argtypes = [_vp, P(W2WCaps), _vp, _vp, _vp, _vp, _vp]
+    L.w2w_l2_loss.argtypes = [_vp, P(W2WCaps), _vp, _vp, _vp, _vp, _vp, _vp]
     L.w2w_adam_step_hyper.restype = ctypes.c_int
     L.w2w_adam_step_hyper.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp,
                                       _vp, _vp, _vp]

@@ -132,6 +132,7 @@ class BatchedTrainer:
                     o += n
         self.dL_dcolor = torch.empty(3 * pix, dtype=torch.float32, device=device)
         self.loss = torch.zeros(max(int(self.caps.frames), 1), dtype=torch.float32, device=device)
+        self.tile_loss = torch.zeros(max(int(self.caps.tiles), 1), dtype=torch.float32, device=device)
         self.graph = None
         self.it = 0
         st = self.store
@@ -156,7 +157,7 @@ class BatchedTrainer:
                                        None, s), "w2w_forward_batch")
         _lib.check(L.w2w_l2_loss(ws.table.data_ptr(), ctypes.byref(self.caps), ws.color.data_ptr(),
                                  self.target.data_ptr(), self.dL_dcolor.data_ptr(),
-                                 self.loss.data_ptr(), s), "w2w_l2_loss")
+                                 self.loss.data_ptr(), self.tile_loss.data_ptr(), s), "w2w_l2_loss")
         _lib.check(L.w2w_backward_batch(
             ws.table.data_ptr(), ctypes.byref(self.caps), ctypes.byref(ws.struct), means.data_ptr(),
             st.scales.data_ptr(), st.rots.data_ptr(), sh.data_ptr(), self.dL_dcolor.data_ptr(), None,

@@ -45,6 +45,16 @@ W2W_DEV float wave_reduce_dpp(float v) {
   return v;
 }
 
+// Sum over each 16-lane row: quad_perm butterflies then row rotations; every
+// lane of a row ends with its row's sum.  All lanes must be active.
+W2W_DEV float row_sum16(float v) {
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x124, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x128, 0xF, 0xF, false));
+  return v;
+}
+
 W2W_DEV float lane63(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
 }
@@ -70,31 +80,39 @@ struct BlendBwdArgs {
 
 // Backward blend (backward.cu:396-520 semantics, with our forward's
 // decisions): every pixel walks its list back to front from its last
-// contributor, recovering T by T <- T / (1 - alpha) from the stored final T.
+// contributor, recovering T by T <- T / (1 - alpha) from the stored final T
+// (the reciprocal of 1 - alpha >= 0.01 is taken with the hardware rcp, 1 ulp).
 //
 // A workgroup claims a tile; wave w owns 8x8 block w.  The waves walk the
 // tile's list together in chunks of 64 (down from the tile's largest last
 // contributor), each gathering the chunk one instance per lane, dropping by
 // ballot the splats whose cull extent misses its block (every pixel of the
 // block would skip them) and visiting the rest in reverse order.  For a splat
-// with any contributing pixel the wave sums the 10 partials over its lanes
-// with DPP into its LDS slot; after a barrier, lane j of each wave adds the
-// waves' slots for splat j in wave order and writes the sums to the
+// with any contributing pixel the wave sums the 10 partials over its lanes,
+// down to one sum per 16-lane row (tile edge 8), per pair of rows (edge 16)
+// or per wave (edge 32), with DPP, and stores those sums in LDS.  After a barrier, lane j of each wave adds all
+// slots of splat j in (wave, row) order and writes the sums to the
 // instance's record (emission position = item offset + its tile's index in
-// the item's rectangle) and sets its flag.  Partial slots are double
-// buffered, so one barrier per chunk suffices.
+// the item's rectangle, prefetched with the chunk) and sets its flag; a
+// second barrier frees the slots for the next chunk.
 template <int TS>
 __global__ __launch_bounds__(64 * units_per_tile<TS>()) void k_blend_bwd(BlendBwdArgs a) {
   constexpr int UPT = (int)units_per_tile<TS>();
   constexpr int UX = TS / kUnit;
-  __shared__ float s_part[2][UPT][GR_FIELDS][64];
-  __shared__ uint64_t s_cm[2][UPT];
+  // partial sums stored per splat and wave: one per 16-lane row (edge 8),
+  // per pair of rows (edge 16) or per wave (edge 32), which keeps the slots
+  // within 40 KB
+  constexpr int ROWS = UPT == 1 ? 4 : (UPT == 4 ? 2 : 1);
+  __shared__ float s_part[UPT][ROWS][GR_FIELDS][64];
+  __shared__ uint64_t s_cm[UPT];
   __shared__ uint32_t s_tile, s_max;
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
   const uint32_t Tn = min(hdr->n_tiles, a.tiles_cap);
   const uint32_t P = min(hdr->n_items, a.items_cap);
   const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
   const int tid = (int)threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int row = ROWS == 4 ? lane >> 4 : (lane >> 4) & 1;
+  const bool row_tail = ROWS == 1 ? lane == 63 : (ROWS == 2 ? (lane == 15 || lane == 31) : (lane & 15) == 15);
   const float a_min = 0x1.010102p-8f;
   uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
   for (;;) {
@@ -147,25 +165,32 @@ __global__ __launch_bounds__(64 * units_per_tile<TS>()) void k_blend_bwd(BlendBw
     float la = 0.0f, lc0 = 0.0f, lc1 = 0.0f, lc2 = 0.0f, ld = 0.0f;
 
     // chunk c covers [nproc - 64 (c + 1), nproc - 64 c) clipped at 0; lane l
-    // holds instance start + l.  Ids two chunks ahead, records one chunk
-    // ahead; lanes before the start gather splat 0 (present when nproc > 0).
-    uint32_t id_n = 0, id_cur = 0;
+    // holds instance start + l.  Ids two chunks ahead; records and each
+    // instance's record position one chunk ahead.  Lanes before the start
+    // gather splat 0 (present when nproc > 0).
+    uint32_t id_n = 0;
     Splat cur = {};
+    uint2 rect_cur = make_uint2(0u, 0u);
+    uint32_t off_cur = 0;
     if (nproc > 0) {
       const int k = nproc - 64 + lane;
       id_n = k >= 0 ? a.ids[range.x + k] : 0u;
-      id_cur = id_n < P ? id_n : 0u;
-      cur = a.splats[id_cur];
+      const uint32_t id0 = id_n < P ? id_n : 0u;
+      cur = a.splats[id0];
+      rect_cur = a.rect[id0];
+      off_cur = a.item_off[id0];
       id_n = k >= 64 ? a.ids[range.x + k - 64] : 0u;
     }
-    int buf = 0;
-    for (int cend = nproc; cend > 0; cend -= 64, buf ^= 1) {
-      const int cstart = cend - 64;
+    for (int cend = nproc; cend > 0; cend -= 64) {
+      const int cstart = cend - 64;  // may be negative: lanes below 0 are empty
       const Splat s = cur;
-      const uint32_t sid = id_cur;
+      const uint2 srect = rect_cur;
+      const uint32_t soff = off_cur;
       if (cstart > 0) {
-        id_cur = id_n < P ? id_n : 0u;
-        cur = a.splats[id_cur];
+        const uint32_t idc = id_n < P ? id_n : 0u;
+        cur = a.splats[idc];
+        rect_cur = a.rect[idc];
+        off_cur = a.item_off[idc];
         const int k2 = cstart - 128 + lane;
         id_n = k2 >= 0 ? a.ids[range.x + k2] : 0u;
       }
@@ -197,7 +222,8 @@ __global__ __launch_bounds__(64 * units_per_tile<TS>()) void k_blend_bwd(BlendBw
             if (al >= a_min) {
               contrib = true;
               n_con++;
-              T = T / (1.0f - al);
+              const float r1 = __builtin_amdgcn_rcpf(1.0f - al);
+              T = T * r1;
               const float aT = al * T;
               acc0 = la * lc0 + (1.0f - la) * acc0;
               acc1 = la * lc1 + (1.0f - la) * acc1;
@@ -208,7 +234,7 @@ __global__ __launch_bounds__(64 * units_per_tile<TS>()) void k_blend_bwd(BlendBw
               dLda = dLda + (dz - accD) * gD;
               dLda = dLda + (1.0f - accA) * gA;
               dLda = dLda * T;
-              dLda = dLda - (Tf / (1.0f - al)) * bgdot;
+              dLda = dLda - (Tf * r1) * bgdot;
               lc0 = cr; lc1 = cg; lc2 = cbl; ld = dz; la = al;
               g_r = aT * gC0;
               g_g = aT * gC1;
@@ -227,55 +253,50 @@ __global__ __launch_bounds__(64 * units_per_tile<TS>()) void k_blend_bwd(BlendBw
           }
         }
         if (__ballot(contrib) != 0ull) {
-          g_mx = wave_reduce_dpp(g_mx);
-          g_my = wave_reduce_dpp(g_my);
-          g_ca = wave_reduce_dpp(g_ca);
-          g_cb = wave_reduce_dpp(g_cb);
-          g_cc = wave_reduce_dpp(g_cc);
-          g_op = wave_reduce_dpp(g_op);
-          g_r = wave_reduce_dpp(g_r);
-          g_g = wave_reduce_dpp(g_g);
-          g_b = wave_reduce_dpp(g_b);
-          g_d = wave_reduce_dpp(g_d);
-          if (lane == 63) {
-            float(*slot)[64] = s_part[buf][w];
-            slot[GA_MX][j] = g_mx;
-            slot[GA_MY][j] = g_my;
-            slot[GA_CA][j] = g_ca;
-            slot[GA_CB][j] = g_cb;
-            slot[GA_CC][j] = g_cc;
-            slot[GA_OP][j] = g_op;
-            slot[GA_R][j] = g_r;
-            slot[GA_G][j] = g_g;
-            slot[GA_B][j] = g_b;
-            slot[GA_DEPTH][j] = g_d;
+          float g[GR_FIELDS] = {g_mx, g_my, g_ca, g_cb, g_cc, g_op, g_r, g_g, g_b, g_d};
+#pragma unroll
+          for (int f = 0; f < GR_FIELDS; f++) {
+            if (ROWS == 1) {
+              g[f] = wave_reduce_dpp(g[f]);
+            } else {
+              g[f] = row_sum16(g[f]);
+              if (ROWS == 2) g[f] += __shfl_xor(g[f], 32, 64);  // rows r and r + 2
+            }
+          }
+          if (row_tail) {
+            const int r = ROWS == 1 ? 0 : row;
+#pragma unroll
+            for (int f = 0; f < GR_FIELDS; f++) s_part[w][r][f][j] = g[f];
           }
           cm |= 1ull << j;
         }
       }
-      if (lane == 0) s_cm[buf][w] = cm;
+      if (lane == 0) s_cm[w] = cm;
       __syncthreads();
-      // splat `lane` of the chunk: sum the waves' partials in wave order
+      // splat `lane` of the chunk: sum its slots in (wave, row) order
       uint64_t any = 0;
 #pragma unroll
-      for (int v = 0; v < UPT; v++) any |= s_cm[buf][v];
+      for (int v = 0; v < UPT; v++) any |= s_cm[v];
       if ((any >> lane) & 1ull) {
-        const uint2 r = a.rect[sid];
-        const uint32_t rx0 = r.x & 0xFFFFu, ry0 = r.x >> 16, rx1 = r.y & 0xFFFFu;
-        const uint32_t pos = a.item_off[sid] + ((uint32_t)ty - ry0) * (rx1 - rx0) + ((uint32_t)tx - rx0);
+        const uint32_t rx0 = srect.x & 0xFFFFu, ry0 = srect.x >> 16, rx1 = srect.y & 0xFFFFu;
+        const uint32_t pos = soff + ((uint32_t)ty - ry0) * (rx1 - rx0) + ((uint32_t)tx - rx0);
         if (pos < a.capacity) {
           for (int f = w; f < GR_FIELDS; f += UPT) {
             float sum = 0.0f;
 #pragma unroll
             for (int v = 0; v < UPT; v++)
-              if ((s_cm[buf][v] >> lane) & 1ull) sum += s_part[buf][v][f][lane];
+              if ((s_cm[v] >> lane) & 1ull) {
+#pragma unroll
+                for (int r = 0; r < ROWS; r++) sum += s_part[v][r][f][lane];
+              }
             a.rec[(size_t)pos * GR_FIELDS + f] = sum;
           }
           if (w == 0) a.flag[pos] = 1;
         }
       }
+      __syncthreads();  // the slots and masks are rewritten by the next chunk
     }
-    __syncthreads();  // s_tile, s_max and the partial slots are reused by the next tile
+    __syncthreads();  // s_tile and s_max are reused by the next tile
   }
   if (a.stats) {
     unsigned long long e = n_eval, c = n_con, kf = (tid == 0) ? n_loaded : 0u;
@@ -350,12 +371,19 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreBwdArgs a) {
       float ga[GR_FIELDS];
       for (int f = 0; f < GR_FIELDS; f++) ga[f] = 0.0f;
       if (vis) {
+        // flags eight at a time (the flag array is padded to 256 bytes)
         const uint32_t k0 = a.item_off[i];
         const uint32_t k1 = min(k0 + a.counts[i], a.capacity);
-        for (uint32_t k = k0; k < k1; k++) {
-          if (!a.flag[k]) continue;
-          const float* r = a.rec + (size_t)k * GR_FIELDS;
-          for (int f = 0; f < GR_FIELDS; f++) ga[f] += r[f];
+        for (uint32_t kw = k0 & ~7u; kw < k1; kw += 8) {
+          uint64_t f8 = *reinterpret_cast<const uint64_t*>(a.flag + kw);
+          if (kw < k0) f8 &= ~0ull << (8 * (k0 - kw));
+          if (k1 - kw < 8) f8 &= ~(~0ull << (8 * (k1 - kw)));
+          while (f8) {
+            const uint32_t b = (uint32_t)__builtin_ctzll(f8) >> 3;
+            f8 &= ~(0xFFull << (8 * b));
+            const float* r = a.rec + (size_t)(kw + b) * GR_FIELDS;
+            for (int f = 0; f < GR_FIELDS; f++) ga[f] += r[f];
+          }
         }
       }
       a.dL_dmeans2D[3 * (size_t)i] = vis ? ga[GA_MX] : 0.0f;

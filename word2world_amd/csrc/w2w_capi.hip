@@ -783,6 +783,7 @@ int w2w_activate(const float* svec, const float* logit, const float* quat, int64
   if (!svec || !logit || !quat || !scales || !opac || !rots) return W2W_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
   (void)hipGetLastError();
+  ProfSpan ps(W2W_KID_ACTIVATE, s);
   hipLaunchKernelGGL(k_activate, dim3(std::min<uint32_t>(cdiv((uint64_t)G, 256), 8192)), dim3(256), 0,
                      s, svec, logit, quat, (uint64_t)G, (const RowRange*)nullptr, 0u, scales, opac,
                      rots);
@@ -798,6 +799,7 @@ int w2w_activate_backward(const float* scales, const float* opac, const float* q
     return W2W_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
   (void)hipGetLastError();
+  ProfSpan ps(W2W_KID_ACTIVATE_BWD, s);
   hipLaunchKernelGGL(k_activate_bwd, dim3(std::min<uint32_t>(cdiv((uint64_t)G, 256), 8192)), dim3(256),
                      0, s, scales, opac, quat, g_scales, g_opac, g_rots, (uint64_t)G,
                      (const RowRange*)nullptr, 0u, g_svec, g_logit, g_quat);
@@ -805,16 +807,20 @@ int w2w_activate_backward(const float* scales, const float* opac, const float* q
 }
 
 int w2w_l2_loss(const void* table, const W2WCaps* caps, const float* color, const float* target,
-                float* dL_dcolor, float* loss_per_frame, void* stream) {
-  if (!table || !caps || !color || !target || !dL_dcolor || !loss_per_frame || caps->frames < 1 ||
-      caps->scenes < 1 || caps->pixels < 1)
+                float* dL_dcolor, float* loss_per_frame, float* tile_scratch, void* stream) {
+  if (!table || !caps || !color || !target || !dL_dcolor || !loss_per_frame || !tile_scratch ||
+      caps->frames < 1 || caps->scenes < 1 || caps->pixels < 1 || caps->tiles < 1 ||
+      (caps->tile != 8 && caps->tile != 16 && caps->tile != 32))
     return W2W_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
   (void)hipGetLastError();
-  fill_u32(reinterpret_cast<uint32_t*>(loss_per_frame), 0u, (size_t)caps->frames, s);
-  hipLaunchKernelGGL(k_l2_loss, dim3(std::min<uint32_t>(cdiv((uint64_t)caps->pixels, 256), 8192)),
-                     dim3(256), 0, s, table, (uint32_t)caps->scenes, (uint64_t)caps->pixels, color,
-                     target, dL_dcolor, loss_per_frame);
+  ProfSpan ps(W2W_KID_LOSS, s);
+  hipLaunchKernelGGL(k_l2_loss_tiles, dim3(std::min<uint32_t>((uint32_t)caps->tiles, 16384)), dim3(256), 0,
+                     s, table, (uint32_t)caps->scenes, (uint32_t)caps->tiles, (int)caps->tile, color,
+                     target, dL_dcolor, tile_scratch);
+  hipLaunchKernelGGL(k_loss_frames, dim3(std::min<uint32_t>(cdiv((uint64_t)caps->frames, 4), 1024)),
+                     dim3(256), 0, s, table, (uint32_t)caps->scenes, (uint32_t)caps->frames,
+                     tile_scratch, loss_per_frame);
   return launch_status();
 }
 
@@ -863,6 +869,7 @@ int w2w_activate_rows(const float* svec, const float* logit, const float* quat, 
   if (!svec || !logit || !quat || !scales || !opac || !rots) return W2W_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
   (void)hipGetLastError();
+  ProfSpan ps(W2W_KID_ACTIVATE, s);
   hipLaunchKernelGGL(k_activate, dim3(std::min<uint32_t>(cdiv((uint64_t)total_rows, 256), 8192)),
                      dim3(256), 0, s, svec, logit, quat, (uint64_t)total_rows,
                      static_cast<const RowRange*>(ranges), (uint32_t)n_ranges, scales, opac, rots);
@@ -879,6 +886,7 @@ int w2w_activate_backward_rows(const float* scales, const float* opac, const flo
     return W2W_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
   (void)hipGetLastError();
+  ProfSpan ps(W2W_KID_ACTIVATE_BWD, s);
   hipLaunchKernelGGL(k_activate_bwd, dim3(std::min<uint32_t>(cdiv((uint64_t)total_rows, 256), 8192)),
                      dim3(256), 0, s, scales, opac, quat, g_scales, g_opac, g_rots,
                      (uint64_t)total_rows, static_cast<const RowRange*>(ranges), (uint32_t)n_ranges,

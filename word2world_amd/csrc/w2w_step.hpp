@@ -9,6 +9,7 @@
 // the backward (GaussianTrainer.py:86 clamps the render before the loss).
 #pragma once
 #include "w2w_adam.hpp"
+#include "w2w_backward.hpp"
 #include "w2w_device.hpp"
 #include "w2w_forward.hpp"
 
@@ -95,50 +96,71 @@ __global__ __launch_bounds__(256) void k_activate_bwd(
 // L2 loss over the packed frame outputs; dL/dcolor for every pixel of every
 // frame (zero for frames without the backward flag), per-frame loss sums
 // accumulated into loss[frame] (pre-zeroed).
-__global__ __launch_bounds__(256) void k_l2_loss(const void* __restrict__ table,
-                                                 uint32_t scenes_cap, uint64_t pixels_cap,
-                                                 const float* __restrict__ color,
-                                                 const float* __restrict__ target,
-                                                 float* __restrict__ dL_dcolor,
-                                                 float* __restrict__ loss) {
+// L2 image loss, one workgroup per tile: dL/dcolour of the tile's pixels
+// (zero where the clamp to [0, 1] is active or the frame has no backward) and
+// the tile's share of its frame's mean squared error, summed in a fixed
+// order (lane-strided, DPP across the wave, waves in order).
+__global__ __launch_bounds__(256) void k_l2_loss_tiles(const void* __restrict__ table,
+                                                       uint32_t scenes_cap, uint32_t tiles_cap,
+                                                       int ts, const float* __restrict__ color,
+                                                       const float* __restrict__ target,
+                                                       float* __restrict__ dL_dcolor,
+                                                       float* __restrict__ tile_loss) {
+  __shared__ float s_w[4];
   const BatchHeader* hdr = static_cast<const BatchHeader*>(table);
   const FrameDesc* frames = table_frames(table, scenes_cap);
-  const uint64_t n = min(hdr->n_pixels, pixels_cap);
-  const uint32_t nf = hdr->n_frames;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  float part = 0.0f;
-  uint32_t part_frame = 0xffffffffu;
-  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
-    uint32_t lo = 0, hi = nf - 1;
-    while (lo < hi) {
-      uint32_t mid = (lo + hi + 1) >> 1;
-      if (frames[mid].pix_off <= p) lo = mid; else hi = mid - 1;
-    }
-    const FrameDesc& fd = frames[lo];
+  const uint32_t T = min(hdr->n_tiles, tiles_cap);
+  const int tid = (int)threadIdx.x;
+  for (uint32_t t = blockIdx.x; t < T; t += gridDim.x) {
+    const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, t)];
+    const int lt = (int)(t - fd.tile_off);
+    const int x0 = (lt % fd.tiles_x) * ts, y0 = (lt / fd.tiles_x) * ts;
     const uint64_t HW = (uint64_t)fd.W * fd.H;
-    const uint64_t lp = p - fd.pix_off;
     const float w = 1.0f / (3.0f * (float)HW);
     const bool on = (fd.flags & FRAME_BACKWARD) != 0;
-    float sq = 0.0f;
+    float part = 0.0f;
+    for (int k = tid; k < ts * ts; k += 256) {
+      const int px = x0 + k % ts, py = y0 + k / ts;
+      if (px >= fd.W || py >= fd.H) continue;
+      const uint64_t lp = (uint64_t)py * fd.W + px;
+      float sq = 0.0f;
 #pragma unroll
-    for (int ch = 0; ch < 3; ch++) {
-      const uint64_t e = 3 * fd.pix_off + ch * HW + lp;
-      const float c = color[e];
-      const float cc = fminf(fmaxf(c, 0.0f), 1.0f);
-      const float d = cc - target[fd.target_off + ch * HW + lp];
-      sq += d * d;
-      dL_dcolor[e] = (on && c >= 0.0f && c <= 1.0f) ? 2.0f * d * w : 0.0f;
-    }
-    if (on) {
-      if (part_frame != lo) {
-        if (part_frame != 0xffffffffu) atomicAdd(loss + part_frame, part);
-        part = 0.0f;
-        part_frame = lo;
+      for (int ch = 0; ch < 3; ch++) {
+        const uint64_t e = 3 * fd.pix_off + ch * HW + lp;
+        const float c = color[e];
+        const float cc = fminf(fmaxf(c, 0.0f), 1.0f);
+        const float d = cc - target[fd.target_off + ch * HW + lp];
+        sq += d * d;
+        dL_dcolor[e] = (on && c >= 0.0f && c <= 1.0f) ? 2.0f * d * w : 0.0f;
       }
       part += sq * w;
     }
+    part = wave_reduce_dpp(part);
+    if ((tid & 63) == 63) s_w[tid >> 6] = part;
+    __syncthreads();
+    if (tid == 0) tile_loss[t] = on ? ((s_w[0] + s_w[1]) + (s_w[2] + s_w[3])) : 0.0f;
+    __syncthreads();
   }
-  if (part_frame != 0xffffffffu) atomicAdd(loss + part_frame, part);
+}
+
+// Per-frame loss: one wave per frame sums its tiles' shares in a fixed order.
+__global__ __launch_bounds__(256) void k_loss_frames(const void* __restrict__ table,
+                                                     uint32_t scenes_cap, uint32_t frames_cap,
+                                                     const float* __restrict__ tile_loss,
+                                                     float* __restrict__ loss) {
+  const BatchHeader* hdr = static_cast<const BatchHeader*>(table);
+  const FrameDesc* frames = table_frames(table, scenes_cap);
+  const uint32_t nf = min(hdr->n_frames, frames_cap);
+  const int lane = (int)(threadIdx.x & 63u);
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t f = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); f < nf; f += waves) {
+    const FrameDesc& fd = frames[f];
+    const uint32_t n = (uint32_t)(fd.tiles_x * fd.tiles_y);
+    float part = 0.0f;
+    for (uint32_t k = (uint32_t)lane; k < n; k += 64) part += tile_loss[fd.tile_off + k];
+    part = wave_reduce_dpp(part);
+    if (lane == 63) loss[f] = part;
+  }
 }
 
 // Adam with hyper-parameters in device memory: hyper = {neg_step[8] (-lr_k /

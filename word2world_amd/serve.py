@@ -208,6 +208,7 @@ class ContinuousServer:
         pix = int(caps.pixels)
         self.dL_dcolor = torch.zeros(3 * pix, dtype=torch.float32, device=device)
         self.loss = torch.zeros(max(int(caps.frames), 1), dtype=torch.float32, device=device)
+        self.tile_loss = torch.zeros(max(int(caps.tiles), 1), dtype=torch.float32, device=device)
         self.max_jobs = max_jobs
         # per job: one row range, five parameter ranges (one per group), each
         # with a 4-float hyper block
@@ -296,7 +297,8 @@ class ContinuousServer:
         if opt:
             _lib.check(L.w2w_l2_loss(self.ws.table.data_ptr(), ctypes.byref(self.caps),
                                      self.ws.color.data_ptr(), self.pool.target_buf.data_ptr(),
-                                     self.dL_dcolor.data_ptr(), self.loss.data_ptr(), s), "w2w_l2_loss")
+                                     self.dL_dcolor.data_ptr(), self.loss.data_ptr(),
+                                     self.tile_loss.data_ptr(), s), "w2w_l2_loss")
             g = self.grads
             _lib.check(L.w2w_backward_batch(
                 self.ws.table.data_ptr(), ctypes.byref(self.caps), ctypes.byref(self.ws.struct),
