@@ -355,7 +355,13 @@ def bench_batched(args):
     plains = [PlainTrainer(j.scene, cs, t, TrainConfig(tile=args.tile), dev)
               for j, cs, t in zip(jobs, cams, targets)]
     for p in plains:
+        # a step whose workspace overflowed sized the next one; its update is
+        # kept (this is warm-up, not a timed step)
         p.step()
+        if p.observe_capacity():
+            p.step()
+            if p.observe_capacity():
+                raise RuntimeError("plain trainer workspace overflowed twice")
     torch.cuda.synchronize()
     psteps = max(1, min(args.steps, 3))
     t0 = time.perf_counter()

@@ -26,6 +26,7 @@ from . import _lib, batch
 from .train import LRS, mean_lr
 
 GROUPS = ("means", "sh", "opacity", "svec", "quaternion")
+MAX_CAPACITY = (1 << 30) - 2  # tile instances per batch (30-bit look-back values in the sort)
 
 
 class BatchedStore:
@@ -115,12 +116,19 @@ class BatchedTrainer:
         self.specs = [batch.SceneSpec(s.n, s.sh_degree, self.store.M,
                                       [batch.frame_from_camera(c) for c in cams])
                       for s, cams in zip(scenes, cameras)]
-        cap = capacity if capacity is not None else max(1 << 20, 48 * sum(
-            s.n * len(c) for s, c in zip(scenes, cameras)))
-        self.caps = batch.caps_for(self.specs, tile, cap)
-        self.ws = batch.BatchWorkspace(self.caps, device)
-        self.ws.load_table(self.specs)
-        self.grads = batch.BatchGrads(self.caps, 0, device)
+        items = sum(s.n * len(c) for s, c in zip(scenes, cameras))
+        cap = capacity if capacity is not None else min(MAX_CAPACITY, max(1 << 20, 8 * items))
+        self._alloc(cap)
+        if capacity is None:
+            # size the instance buffers from one forward pass: K is counted in
+            # full even when it overflows the probe capacity
+            self._forward()
+            K, ovf = (int(v) for v in self.ws.status().cpu())
+            if K >= MAX_CAPACITY:
+                raise RuntimeError(f"batch needs {K} tile instances; split it (limit {MAX_CAPACITY})")
+            want = min(MAX_CAPACITY, int(K * 1.15) + 4096)
+            if ovf or want > 2 * cap or want < cap // 2:
+                self._alloc(want)
         pix = int(self.caps.pixels)
         self.target = torch.zeros(3 * pix, dtype=torch.float32, device=device)
         if targets is not None:
@@ -139,7 +147,29 @@ class BatchedTrainer:
         self._period = np.array([0, 3 * st.M, 0, 0, 0], np.int32)
         self._split = np.array([0, 3, 0, 0, 0], np.int32)
 
+    def _alloc(self, capacity: int):
+        self.caps = batch.caps_for(self.specs, self.tile, capacity)
+        self.ws = None
+        self.grads = None
+        self.ws = batch.BatchWorkspace(self.caps, self.device)
+        self.ws.load_table(self.specs)
+        self.grads = batch.BatchGrads(self.caps, 0, self.device)
+
     # ---- the launch sequence ----
+    def _forward(self):
+        L = _lib.load()
+        st, ws = self.store, self.ws
+        s = torch.cuda.current_stream().cuda_stream
+        _lib.check(L.w2w_activate(st.view("svec").data_ptr(), st.view("opacity").data_ptr(),
+                                  st.view("quaternion").data_ptr(), st.G, st.scales.data_ptr(),
+                                  st.opac.data_ptr(), st.rots.data_ptr(), s), "w2w_activate")
+        _lib.check(L.w2w_forward_batch(ws.table.data_ptr(), ctypes.byref(self.caps),
+                                       ctypes.byref(ws.struct), st.view("means").data_ptr(),
+                                       st.scales.data_ptr(), st.rots.data_ptr(), st.opac.data_ptr(),
+                                       st.group(st.flat, "sh").data_ptr(), ws.color.data_ptr(),
+                                       ws.depth.data_ptr(), ws.alpha.data_ptr(), None, s),
+                   "w2w_forward_batch")
+
     def _body(self):
         L = _lib.load()
         st, ws, g = self.store, self.ws, self.grads
@@ -147,14 +177,7 @@ class BatchedTrainer:
         G = st.G
         means = st.view("means")
         sh = st.group(st.flat, "sh")
-        _lib.check(L.w2w_activate(st.view("svec").data_ptr(), st.view("opacity").data_ptr(),
-                                  st.view("quaternion").data_ptr(), G, st.scales.data_ptr(),
-                                  st.opac.data_ptr(), st.rots.data_ptr(), s), "w2w_activate")
-        _lib.check(L.w2w_forward_batch(ws.table.data_ptr(), ctypes.byref(self.caps),
-                                       ctypes.byref(ws.struct), means.data_ptr(), st.scales.data_ptr(),
-                                       st.rots.data_ptr(), st.opac.data_ptr(), sh.data_ptr(),
-                                       ws.color.data_ptr(), ws.depth.data_ptr(), ws.alpha.data_ptr(),
-                                       None, s), "w2w_forward_batch")
+        self._forward()
         _lib.check(L.w2w_l2_loss(ws.table.data_ptr(), ctypes.byref(self.caps), ws.color.data_ptr(),
                                  self.target.data_ptr(), self.dL_dcolor.data_ptr(),
                                  self.loss.data_ptr(), self.tile_loss.data_ptr(), s), "w2w_l2_loss")

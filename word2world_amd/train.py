@@ -243,3 +243,15 @@ class PlainTrainer:
     def overflow(self) -> bool:
         return any(int(w.status()[1]) for w in self.last_workspaces)
 
+    def observe_capacity(self) -> bool:
+        """Synchronise, record each view's instance count with the capacity
+        policy (so the next step's workspaces fit) and say whether any view
+        of the last step overflowed (its images and gradients are then
+        invalid and the step should be discarded)."""
+        ovf = False
+        for cam, w in zip(self.cams, self.last_workspaces):
+            K, o = (int(v) for v in w.status().cpu())
+            raster.POLICY.observe((self.store.n, int(cam.width), int(cam.height), self.cfg.tile), K)
+            ovf = ovf or bool(o)
+        return ovf
+
