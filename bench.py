@@ -49,7 +49,8 @@ def parse():
     ap.add_argument("--gaussians", type=int, default=1_000_000)
     ap.add_argument("--res", type=int, default=1024)
     ap.add_argument("--views", type=int, default=4)
-    ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--tile", type=int, default=0,
+                    help="tile edge 8, 16 or 32; 0 = pick by the autotuner (single-view sweep)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-gaussians", type=int, default=250_000)
     ap.add_argument("--workload", choices=["scene1m", "batched64", "served"], default="scene1m")
@@ -59,6 +60,9 @@ def parse():
     ap.add_argument("--opt-steps", type=int, default=50)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="after the benchmark, run a plain device copy of this many bytes three "
+                         "times (a reference kernel for hardware byte counters)")
     return ap.parse_args()
 
 
@@ -92,14 +96,16 @@ def algorithmic_cost(kernel, P, M, K, T, HW, stats):
 
 def main():
     args = parse()
+    if args.tile not in (0, 8, 16, 32):
+        print("--tile must be 0 (autotune), 8, 16 or 32", file=sys.stderr)
+        sys.exit(2)
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the rasterizer's "
               "multi-GPU sharding is a design note only", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "batched64":
-        return bench_batched(args)
-    if args.workload == "served":
-        return bench_served(args)
+    if args.workload in ("batched64", "served"):
+        args.tile = args.tile or 16  # the reference's tile edge
+        return bench_batched(args) if args.workload == "batched64" else bench_served(args)
     return bench_scene(args)
 
 
@@ -160,7 +166,7 @@ def bench_scene(args):
     cams_host = [orbit_camera(k * 360.0 / V, float(elev[k]), 2.5, R, R) for k in range(V)]
     cams = [raster.camera_from_tensors(c.viewmatrix, c.projmatrix, c.campos, R, R, c.focal_x,
                                        c.focal_y, [1.0, 1.0, 1.0], device=dev) for c in cams_host]
-    log(f"scene: {P} Gaussians SH3, {V} views at {R}^2, tile {ts}")
+    log(f"scene: {P} Gaussians SH3, {V} views at {R}^2, tile {ts or 'auto'}")
     scene = make_scene(P, 3, seed=0)
     target_scene = make_scene(P, 3, seed=1)
     M = 16
@@ -168,11 +174,21 @@ def bench_scene(args):
         tm, ts_, tr, to, tsh = (t.to(dev).contiguous() for t in activate(target_scene.params()))
         targets = []
         for cam in cams:
-            K, color, _, _, _, _ = raster.forward_checked(tm, ts_, tr, to, tsh, 3, cam, ts)
+            K, color, _, _, _, _ = raster.forward_checked(tm, ts_, tr, to, tsh, 3, cam, 16)
             targets.append(color.clamp(0, 1))
         del tm, ts_, tr, to, tsh
     torch.cuda.empty_cache()
 
+    tune = None
+    if args.tile == 0:
+        from word2world_amd import autotune
+        m, s_, r, o, sh = (t.to(dev).contiguous() for t in activate(scene.params()))
+        tune = autotune.sweep(m, s_, r, o, sh, 3, cams[0], reps=2)
+        ts = min(tune, key=tune.get)
+        log("autotune (ms per single-view fwd+bwd): " + ", ".join(f"tile {k}: {v:.2f}" for k, v in tune.items())
+            + f" -> tile {ts}")
+        del m, s_, r, o, sh
+        torch.cuda.empty_cache()
     bt = BatchedTrainer([scene], [cams_host], [targets], dev, ts)
     for i in range(args.warmup):
         bt.step()
@@ -250,7 +266,7 @@ def bench_scene(args):
     plain = None
     if args.plain_steps > 0:
         params = {k: v.to(dev).requires_grad_(True) for k, v in scene.params().items()}
-        step = plain_step_fn(params, cams, targets, P, ts, dev)
+        step = plain_step_fn(params, cams, targets, P, 16, dev)  # the reference's tile edge
         step(0)
         torch.cuda.synchronize()
         tp0 = time.perf_counter()
@@ -260,10 +276,18 @@ def bench_scene(args):
         tp = (time.perf_counter() - tp0) / args.plain_steps
         plain = {"views_per_s": V / tp, "ms_per_step": tp * 1e3, "steps": args.plain_steps,
                  "path": "drop-in rasterize_gaussians autograd Function, one view per launch "
-                         "sequence, torch.optim.Adam"}
+                         "sequence, torch.optim.Adam, tile 16"}
         log(f"plain drop-in path: {tp * 1e3:.2f} ms/step, {V / tp:.1f} views/s")
         del params
         torch.cuda.empty_cache()
+
+    if args.copy_bytes > 0:
+        src = torch.ones(args.copy_bytes // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)  # an elementwise kernel: reads and writes copy_bytes
+        torch.cuda.synchronize()
+        del src, dst
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -278,6 +302,7 @@ def bench_scene(args):
                                "(elevation U[-30,30]), fwd+bwd+Adam per step, L2 to a second "
                                "seed's renders; all views of a step in one launch per stage",
                    "gaussians": P, "views_per_step": V, "resolution": R, "tile": ts},
+        "autotune_ms": tune,
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
         "per_view": {"K": K_view, "evaluated_pairs_fwd": stats_np[0], "contributing_pairs_fwd": stats_np[1],
                      "instances_loaded_fwd": stats_np[2], "evaluated_pairs_bwd": stats_np[3],
@@ -441,7 +466,7 @@ def cpu_baseline(args, cams_host, P):
     R = cam.width
     t0 = time.perf_counter()
     f = oracle.forward(m, s, r, o, sh, 3, cam.view_flat(), cam.proj_flat(), cam.campos_np(), bg, R, R,
-                       args.tile, cam.focal_x, cam.focal_y)
+                       args.tile or 16, cam.focal_x, cam.focal_y)
     t1 = time.perf_counter()
     rng = np.random.default_rng(0)
     gC = rng.standard_normal((3, R, R)).astype(np.float32) * 1e-3

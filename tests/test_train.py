@@ -152,3 +152,26 @@ def test_plain_trainer_reduces_loss(gpu):
     assert not tr_.overflow()
     assert losses[-1] < losses[0]
     assert tr_.store.n > 0
+
+
+@pytest.mark.gpu
+def test_autotuner_picks_a_tile_and_tiles_agree(gpu):
+    """The sweep times every tile edge; the images at different edges agree
+    (the per-pixel contributor sequence only changes where the reference's
+    non-conservative rect rule drops a tile, a handful of pixels)."""
+    from word2world_amd import autotune, raster
+    from word2world_amd.camera import orbit_camera
+    from word2world_amd.scene import activate, make_scene
+    sc = make_scene(20000, 3, 5)
+    m, s, r, o, sh = (t.to(gpu).contiguous() for t in activate(sc.params()))
+    c = orbit_camera(30.0, 10.0, 2.5, 256, 192)
+    cam = raster.camera_from_tensors(c.viewmatrix, c.projmatrix, c.campos, 256, 192, c.focal_x, c.focal_y,
+                                     None, device=gpu)
+    tuner = autotune.Autotuner(reps=1)
+    best = tuner.best_tile(m, s, r, o, sh, 3, cam)
+    t = tuner.timings[tuner.key(20000, 256, 192)]
+    assert best in (8, 16, 32) and set(t) == {8, 16, 32} and all(v > 0 for v in t.values())
+    imgs = {ts: raster.forward_checked(m, s, r, o, sh, 3, cam, ts)[1] for ts in (8, 16, 32)}
+    for ts in (8, 32):
+        d = (imgs[ts] - imgs[16]).abs()
+        assert float((d > 1e-5).float().mean()) < 0.01 and float(d.max()) < 0.5
