@@ -269,7 +269,7 @@ enum {
   W2W_KID_SCAN = 4, W2W_KID_EMIT = 5, W2W_KID_TSORT_HIST = 6, W2W_KID_TSORT_PASS = 7,
   W2W_KID_RANGES = 8, W2W_KID_BLEND = 9, W2W_KID_BLEND_BWD = 10, W2W_KID_PREPROCESS_BWD = 11,
   W2W_KID_ADAM = 12, W2W_KID_SORT_HIST = 13, W2W_KID_SORT_PASS = 14, W2W_KID_ACTIVATE = 15,
-  W2W_KID_LOSS = 16, W2W_KID_ACTIVATE_BWD = 17,
+  W2W_KID_LOSS = 16, W2W_KID_ACTIVATE_BWD = 17, W2W_KID_SH_BWD = 18,
 };
 int w2w_prof_enable(int32_t capacity, uint32_t kernel_mask);
 int w2w_prof_reset(void);

@@ -56,7 +56,7 @@ EXPORTS = ["w2w_version", "w2w_splat_extents", "w2w_workspace_layout", "w2w_batc
 KERNEL_IDS = {0: "fill", 1: "preprocess", 2: "depth_sort_hist", 3: "depth_sort_pass", 4: "scan",
               5: "emit", 6: "tile_sort_hist", 7: "tile_sort_pass", 8: "ranges", 9: "blend",
               10: "blend_bwd", 11: "preprocess_bwd", 12: "adam", 13: "sort_hist", 14: "sort_pass",
-              15: "activate", 16: "loss", 17: "activate_bwd"}
+              15: "activate", 16: "loss", 17: "activate_bwd", 18: "sh_bwd"}
 
 _lib = None
 

@@ -352,16 +352,11 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreBwdArgs a) {
     const uint32_t g = bg - sd.g_off;
     const uint32_t gg = sd.g_base + g;  // row in the parameter / gradient arrays
     float gm[3] = {0.0f, 0.0f, 0.0f};
-    float dcov[6] = {0, 0, 0, 0, 0, 0};
-    float dsc[3] = {0, 0, 0};
-    float dq[4] = {0, 0, 0, 0};
+    float gs[6] = {0, 0, 0, 0, 0, 0};  // dL/dSigma summed over views (xx, xy, xz, yy, yz, zz)
     float gop = 0.0f;
-    float* dsh = a.dL_dsh + sd.sh_off + (size_t)g * sd.M * 3;
-    for (int k = 0; k < sd.M * 3; k++) dsh[k] = 0.0f;
     const float p[3] = {a.means[3 * gg], a.means[3 * gg + 1], a.means[3 * gg + 2]};
     const float s[3] = {a.scales[3 * gg], a.scales[3 * gg + 1], a.scales[3 * gg + 2]};
     const float q[4] = {a.rots[4 * gg], a.rots[4 * gg + 1], a.rots[4 * gg + 2], a.rots[4 * gg + 3]};
-    const float* cf = a.sh + sd.sh_off + (size_t)g * sd.M * 3;
     for (uint32_t v = 0; v < sd.n_views; v++) {
       const uint32_t i = sd.item_off + g * sd.n_views + v;
       if (i >= a.items_cap) break;
@@ -401,31 +396,6 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreBwdArgs a) {
       if (!geo.visible) continue;
       const float* V = c.V;
       const float* F = c.F;
-      // colour: SH coefficients and view direction
-      {
-        float dx = p[0] - c.cp[0], dy = p[1] - c.cp[1], dz = p[2] - c.cp[2];
-        float len = __builtin_sqrtf((dx * dx + dy * dy) + dz * dz);
-        float x = dx / len, y = dy / len, z = dz / len;
-        float Y[16], gx[16], gy[16], gz[16];
-        sh_basis(sd.D, x, y, z, Y);
-        sh_basis_grad(sd.D, x, y, z, gx, gy, gz);
-        const int nb = (sd.D + 1) * (sd.D + 1);
-        const float gr0 = ga[GA_R], gr1 = ga[GA_G], gr2 = ga[GA_B];
-        float gdx = 0.0f, gdy = 0.0f, gdz = 0.0f;
-        for (int k = 0; k < nb; k++) {
-          dsh[3 * k] += Y[k] * gr0;
-          dsh[3 * k + 1] += Y[k] * gr1;
-          dsh[3 * k + 2] += Y[k] * gr2;
-          float dot = (cf[3 * k] * gr0 + cf[3 * k + 1] * gr1) + cf[3 * k + 2] * gr2;
-          gdx = __builtin_fmaf(gx[k], dot, gdx);
-          gdy = __builtin_fmaf(gy[k], dot, gdy);
-          gdz = __builtin_fmaf(gz[k], dot, gdz);
-        }
-        float pd = (gdx * x + gdy * y) + gdz * z;
-        gm[0] += (gdx - pd * x) / len;
-        gm[1] += (gdy - pd * y) / len;
-        gm[2] += (gdz - pd * z) / len;
-      }
       // NDC mean and depth through the full projection
       {
         float gnx = ga[GA_MX], gny = ga[GA_MY], gnz = ga[GA_DEPTH];
@@ -461,8 +431,8 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreBwdArgs a) {
         for (int l = 0; l < 3; l++)
           GS[3 * k + l] = (g_cxx * geo.jx[k] * geo.jx[l] + hxy * (geo.jx[k] * geo.jy[l] + geo.jy[k] * geo.jx[l])) +
                           g_cyy * geo.jy[k] * geo.jy[l];
-      dcov[0] += GS[0]; dcov[1] += 2.0f * GS[1]; dcov[2] += 2.0f * GS[2];
-      dcov[3] += GS[4]; dcov[4] += 2.0f * GS[5]; dcov[5] += GS[8];
+      gs[0] += GS[0]; gs[1] += GS[1]; gs[2] += GS[2];
+      gs[3] += GS[4]; gs[4] += GS[5]; gs[5] += GS[8];
       float gjx[3], gjy[3];
 #pragma unroll
       for (int k = 0; k < 3; k++) {
@@ -500,9 +470,13 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreBwdArgs a) {
 #pragma unroll
       for (int j = 0; j < 3; j++)
         gm[j] += ((V[4 * j + 0] * g_tx + V[4 * j + 1] * g_ty) + V[4 * j + 2] * g_tz) + V[4 * j + 3] * g_tw;
-      // Sigma -> scales and quaternion
-      const float* Mm = geo.M;
-      const float* R = geo.R;
+    }
+    // Sigma = M M^T with M = R diag(s): dL/dSigma is linear, so the scale and
+    // quaternion gradients are taken once from its sum over views
+    {
+      float R[9], Mm[9];
+      rot_scale(q, s, R, Mm);
+      const float GS[9] = {gs[0], gs[1], gs[2], gs[1], gs[3], gs[4], gs[2], gs[4], gs[5]};
       float gM[9], gR[9];
 #pragma unroll
       for (int r = 0; r < 3; r++)
@@ -515,23 +489,107 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreBwdArgs a) {
         for (int col = 0; col < 3; col++) gR[3 * r + col] = gM[3 * r + col] * s[col];
 #pragma unroll
       for (int col = 0; col < 3; col++)
-        dsc[col] += (gM[col] * R[col] + gM[3 + col] * R[3 + col]) + gM[6 + col] * R[6 + col];
+        a.dL_dscales[3 * (size_t)gg + col] = (gM[col] * R[col] + gM[3 + col] * R[3 + col]) + gM[6 + col] * R[6 + col];
       const float qw = q[0], qx = q[1], qy = q[2], qz = q[3];
-      dq[0] += 2.0f * (((-qz * gR[1] + qy * gR[2]) + (qz * gR[3] - qx * gR[5])) + (-qy * gR[6] + qx * gR[7]));
-      dq[1] += 2.0f * ((((qy * gR[1] + qz * gR[2]) + (qy * gR[3] - 2.0f * qx * gR[4])) + (-qw * gR[5] + qz * gR[6])) + (qw * gR[7] - 2.0f * qx * gR[8]));
-      dq[2] += 2.0f * ((((-2.0f * qy * gR[0] + qx * gR[1]) + (qw * gR[2] + qx * gR[3])) + (qz * gR[5] - qw * gR[6])) + (qz * gR[7] - 2.0f * qy * gR[8]));
-      dq[3] += 2.0f * ((((-2.0f * qz * gR[0] - qw * gR[1]) + (qx * gR[2] + qw * gR[3])) + (-2.0f * qz * gR[4] + qy * gR[5])) + (qx * gR[6] + qy * gR[7]));
+      float* dq = a.dL_drots + 4 * (size_t)gg;
+      dq[0] = 2.0f * (((-qz * gR[1] + qy * gR[2]) + (qz * gR[3] - qx * gR[5])) + (-qy * gR[6] + qx * gR[7]));
+      dq[1] = 2.0f * ((((qy * gR[1] + qz * gR[2]) + (qy * gR[3] - 2.0f * qx * gR[4])) + (-qw * gR[5] + qz * gR[6])) + (qw * gR[7] - 2.0f * qx * gR[8]));
+      dq[2] = 2.0f * ((((-2.0f * qy * gR[0] + qx * gR[1]) + (qw * gR[2] + qx * gR[3])) + (qz * gR[5] - qw * gR[6])) + (qz * gR[7] - 2.0f * qy * gR[8]));
+      dq[3] = 2.0f * ((((-2.0f * qz * gR[0] - qw * gR[1]) + (qx * gR[2] + qw * gR[3])) + (-2.0f * qz * gR[4] + qy * gR[5])) + (qx * gR[6] + qy * gR[7]));
     }
     a.dL_dopacity[gg] = gop;
+    // the SH kernel that runs next adds the view-direction term
     a.dL_dmeans3D[3 * (size_t)gg] = gm[0];
     a.dL_dmeans3D[3 * (size_t)gg + 1] = gm[1];
     a.dL_dmeans3D[3 * (size_t)gg + 2] = gm[2];
+    float* dcov = a.dL_dcov3D + 6 * (size_t)gg;
+    dcov[0] = gs[0]; dcov[1] = 2.0f * gs[1]; dcov[2] = 2.0f * gs[2];
+    dcov[3] = gs[3]; dcov[4] = 2.0f * gs[4]; dcov[5] = gs[5];
+  }
+}
+
+struct ShBwdArgs {
+  const void* table;
+  uint32_t scenes_cap, items_cap, gaussians_cap;
+  const float* means;       // packed [G,3]
+  const float* sh;          // packed SH blocks
+  const int32_t* radii;     // per item
+  const float* dL_dcolors;  // per item [3], written by k_preprocess_bwd
+  float* dL_dsh;            // packed like sh
+  float* dL_dmeans3D;       // per Gaussian [3]: the view-direction term is added
+};
+
+// SH part of the colour backward (backward.cu:291-444 semantics, with our
+// forward's degree-aware basis): one thread per Gaussian loops over its views;
+// dL/dSH accumulates in registers (the degree is a template parameter, so the
+// 3 (D+1)^2 accumulators and the basis are straight-line code) and is written
+// once; the gradient through the view direction is added to dL/dmean3D.
+template <int D>
+W2W_DEV void sh_bwd_one(const ShBwdArgs& a, const SceneDesc& sd, const FrameDesc* frames, uint32_t g,
+                        uint32_t gg) {
+  constexpr int NB = (D + 1) * (D + 1);
+  float acc[NB][3];
 #pragma unroll
-    for (int k = 0; k < 6; k++) a.dL_dcov3D[6 * (size_t)gg + k] = dcov[k];
+  for (int k = 0; k < NB; k++) acc[k][0] = acc[k][1] = acc[k][2] = 0.0f;
+  const float p[3] = {a.means[3 * gg], a.means[3 * gg + 1], a.means[3 * gg + 2]};
+  const float* cf = a.sh + sd.sh_off + (size_t)g * sd.M * 3;
+  float gm0 = 0.0f, gm1 = 0.0f, gm2 = 0.0f;
+  for (uint32_t v = 0; v < sd.n_views; v++) {
+    const uint32_t i = sd.item_off + g * sd.n_views + v;
+    if (i >= a.items_cap) break;
+    const FrameDesc& fd = frames[sd.frame_off + v];
+    if (!(a.radii[i] > 0 && (fd.flags & FRAME_BACKWARD))) continue;
+    const float gr0 = a.dL_dcolors[3 * (size_t)i], gr1 = a.dL_dcolors[3 * (size_t)i + 1];
+    const float gr2 = a.dL_dcolors[3 * (size_t)i + 2];
+    const float dx = p[0] - fd.cp[0], dy = p[1] - fd.cp[1], dz = p[2] - fd.cp[2];
+    const float len = __builtin_sqrtf((dx * dx + dy * dy) + dz * dz);
+    const float x = dx / len, y = dy / len, z = dz / len;
+    float gdx = 0.0f, gdy = 0.0f, gdz = 0.0f;
+    sh_visit<D>(x, y, z, [&](int k, float Y, float bx, float by, float bz) {
+      acc[k][0] += Y * gr0;
+      acc[k][1] += Y * gr1;
+      acc[k][2] += Y * gr2;
+      const float dot = (cf[3 * k] * gr0 + cf[3 * k + 1] * gr1) + cf[3 * k + 2] * gr2;
+      gdx = __builtin_fmaf(bx, dot, gdx);
+      gdy = __builtin_fmaf(by, dot, gdy);
+      gdz = __builtin_fmaf(bz, dot, gdz);
+    });
+    const float pd = (gdx * x + gdy * y) + gdz * z;
+    gm0 += (gdx - pd * x) / len;
+    gm1 += (gdy - pd * y) / len;
+    gm2 += (gdz - pd * z) / len;
+  }
+  float* dsh = a.dL_dsh + sd.sh_off + (size_t)g * sd.M * 3;
 #pragma unroll
-    for (int k = 0; k < 3; k++) a.dL_dscales[3 * (size_t)gg + k] = dsc[k];
-#pragma unroll
-    for (int k = 0; k < 4; k++) a.dL_drots[4 * (size_t)gg + k] = dq[k];
+  for (int k = 0; k < NB; k++) {
+    dsh[3 * k] = acc[k][0];
+    dsh[3 * k + 1] = acc[k][1];
+    dsh[3 * k + 2] = acc[k][2];
+  }
+  for (int k = 3 * NB; k < 3 * (int)sd.M; k++) dsh[k] = 0.0f;
+  a.dL_dmeans3D[3 * (size_t)gg] += gm0;
+  a.dL_dmeans3D[3 * (size_t)gg + 1] += gm1;
+  a.dL_dmeans3D[3 * (size_t)gg + 2] += gm2;
+}
+
+__global__ __launch_bounds__(256) void k_sh_bwd(ShBwdArgs a) {
+  const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
+  const uint32_t G = min(hdr->n_gaussians, a.gaussians_cap);
+  const uint32_t n_scenes = min(hdr->n_scenes, a.scenes_cap);
+  const SceneDesc* scenes = table_scenes(a.table);
+  const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t bg = blockIdx.x * blockDim.x + threadIdx.x; bg < G; bg += stride) {
+    const uint32_t si = upper_index(n_scenes, bg, [&](uint32_t k) { return scenes[k].g_off; });
+    const SceneDesc& sd = scenes[si];
+    const uint32_t g = bg - sd.g_off;
+    const uint32_t gg = sd.g_base + g;
+    switch (sd.D) {
+      case 0: sh_bwd_one<0>(a, sd, frames, g, gg); break;
+      case 1: sh_bwd_one<1>(a, sd, frames, g, gg); break;
+      case 2: sh_bwd_one<2>(a, sd, frames, g, gg); break;
+      default: sh_bwd_one<3>(a, sd, frames, g, gg); break;
+    }
   }
 }
 

@@ -424,6 +424,16 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
     hipLaunchKernelGGL(k_preprocess_bwd, dim3(std::min<uint32_t>(cdiv((uint64_t)c.gaussians, 256), 8192)),
                        dim3(256), 0, s, pa);
   }
+  if (c.gaussians > 0) {
+    ShBwdArgs sa;
+    sa.table = table; sa.scenes_cap = (uint32_t)c.scenes; sa.items_cap = (uint32_t)c.items;
+    sa.gaussians_cap = (uint32_t)c.gaussians;
+    sa.means = prm.means; sa.sh = prm.sh; sa.radii = at<int32_t>(G, p.L.radii);
+    sa.dL_dcolors = dL_dcolors; sa.dL_dsh = dL_dsh; sa.dL_dmeans3D = dL_dmeans3D;
+    ProfSpan ps(W2W_KID_SH_BWD, s);
+    hipLaunchKernelGGL(k_sh_bwd, dim3(std::min<uint32_t>(cdiv((uint64_t)c.gaussians, 256), 8192)), dim3(256), 0,
+                       s, sa);
+  }
   return launch_status();
 }
 

@@ -221,6 +221,57 @@ struct Geom {
   float px, py;
 };
 
+// R (row-major) of the unit quaternion q = (w, x, y, z) and M = R diag(s),
+// the factors of Sigma = M M^T (the same expressions geometry() uses).
+W2W_DEV void rot_scale(const float q[4], const float s[3], float R[9], float M[9]) {
+  float w = q[0], x = q[1], y = q[2], z = q[3];
+  float xx = x * x, yy = y * y, zz = z * z;
+  float xy = x * y, xz = x * z, yz = y * z;
+  float wx = w * x, wy = w * y, wz = w * z;
+  R[0] = 1.0f - 2.0f * (yy + zz); R[1] = 2.0f * (xy - wz);        R[2] = 2.0f * (xz + wy);
+  R[3] = 2.0f * (xy + wz);        R[4] = 1.0f - 2.0f * (xx + zz); R[5] = 2.0f * (yz - wx);
+  R[6] = 2.0f * (xz - wy);        R[7] = 2.0f * (yz + wx);        R[8] = 1.0f - 2.0f * (xx + yy);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) M[3 * i + j] = R[3 * i + j] * s[j];
+}
+
+// Calls f(k, Y_k, dY_k/dx, dY_k/dy, dY_k/dz) for the real SH basis functions
+// of degree <= D at the unit direction (x, y, z), in coefficient order, with
+// the expressions of sh_basis / sh_basis_grad; everything is straight-line
+// code, so a caller's per-coefficient accumulators stay in registers.
+template <int D, typename F>
+W2W_DEV void sh_visit(float x, float y, float z, F&& f) {
+  f(0, SH_C0, 0.0f, 0.0f, 0.0f);
+  if constexpr (D >= 1) {
+    f(1, -SH_C1 * y, 0.0f, -SH_C1, 0.0f);
+    f(2, SH_C1 * z, 0.0f, 0.0f, SH_C1);
+    f(3, -SH_C1 * x, -SH_C1, 0.0f, 0.0f);
+  }
+  if constexpr (D >= 2) {
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float xy = x * y, yz = y * z, xz = x * z;
+    f(4, SH_C2A * xy, SH_C2A * y, SH_C2A * x, 0.0f);
+    f(5, -SH_C2A * yz, 0.0f, -SH_C2A * z, -SH_C2A * y);
+    f(6, SH_C26 * ((2.0f * zz - xx) - yy), SH_C26 * (-2.0f * x), SH_C26 * (-2.0f * y), SH_C26 * (4.0f * z));
+    f(7, -SH_C2A * xz, -SH_C2A * z, 0.0f, -SH_C2A * x);
+    f(8, SH_C2E * (xx - yy), SH_C2E * (2.0f * x), SH_C2E * (-2.0f * y), 0.0f);
+    if constexpr (D >= 3) {
+      f(9, (SH_K9 * y) * (3.0f * xx - yy), SH_K9 * (6.0f * x * y), SH_K9 * (3.0f * xx - 3.0f * yy), 0.0f);
+      f(10, ((SH_K10 * x) * y) * z, SH_K10 * (y * z), SH_K10 * (x * z), SH_K10 * (x * y));
+      f(11, (SH_K11 * y) * ((4.0f * zz - xx) - yy), SH_K11 * (-2.0f * x * y),
+        SH_K11 * ((4.0f * zz - xx) - 3.0f * yy), SH_K11 * (8.0f * y * z));
+      f(12, (SH_K12 * z) * ((2.0f * zz - 3.0f * xx) - 3.0f * yy), SH_K12 * (-6.0f * x * z),
+        SH_K12 * (-6.0f * y * z), SH_K12 * ((6.0f * zz - 3.0f * xx) - 3.0f * yy));
+      f(13, (SH_K13 * x) * ((4.0f * zz - xx) - yy), SH_K13 * ((4.0f * zz - 3.0f * xx) - yy),
+        SH_K13 * (-2.0f * x * y), SH_K13 * (8.0f * x * z));
+      f(14, (SH_K14 * z) * (xx - yy), SH_K14 * (2.0f * x * z), SH_K14 * (-2.0f * y * z), SH_K14 * (xx - yy));
+      f(15, (SH_K15 * x) * (xx - 3.0f * yy), SH_K15 * (3.0f * xx - 3.0f * yy), SH_K15 * (-6.0f * x * y), 0.0f);
+    }
+  }
+}
+
 W2W_DEV void geometry(const Cam& c, const float p[3], const float s[3], const float q[4], Geom& g) {
   const float* V = c.V;
   g.visible = false;
