@@ -119,6 +119,11 @@ int w2w_version(void);
  * out[i] = two binary16 half-widths (x low, y high) of the box outside of
  * which the blend kernels drop the pair. */
 int w2w_splat_extents(const float* conic, const float* skip, uint64_t n, uint32_t* out);
+/* Host-side inspection (no counterpart in the reference): whether the blend
+ * kernels' exact block test keeps splat i, splat[6 i ..] = (mx, my, conic a,
+ * b, c, skip_below), for the pixel-centre rectangle rect[4 i ..] = (x0, x1,
+ * y0, y1); out[i] = 1 when kept. */
+int w2w_block_touches(const float* splat, const float* rect, uint64_t n, uint8_t* out);
 
 int w2w_workspace_layout(int32_t P, int32_t W, int32_t H, int32_t tile, int64_t capacity,
                          W2WLayout* out);

@@ -111,3 +111,52 @@ def test_degenerate_conics_never_cull():
     skip = _f32([-2.0, -2.0, -2.0, 1.0, -2.0])
     hx, hy = _extents(conic, skip)
     assert np.all(np.isinf(hx)) and np.all(np.isinf(hy))
+
+
+def _touches(spl, rect):
+    L = _lib.load()
+    spl = np.ascontiguousarray(spl, np.float32)
+    rect = np.ascontiguousarray(rect, np.float32)
+    out = np.zeros(len(spl), np.uint8)
+    assert L.w2w_block_touches(spl.ctypes.data, rect.ctypes.data, len(spl), out.ctypes.data) == 0
+    return out.astype(bool)
+
+
+@pytest.mark.parametrize("seed,dilate,aniso", [(10, 0.3, 1e3), (11, 0.3, 1e5), (12, 0.0, 1e4)])
+def test_block_test_is_conservative_and_tighter_than_the_box(seed, dilate, aniso):
+    """A dropped (splat, 8x8 block) pair has no pixel centre of the block that
+    passes the float pre-test; brute force over all 64 centres.  Also the
+    exact test keeps no more than the box test and drops a good share of
+    what the box keeps."""
+    conic, skip, mean = _splats(3000, seed, dilate, aniso)
+    rng = np.random.default_rng(seed)
+    hx, hy = _extents(conic, skip)
+    # one 8x8 block per splat, placed near the box so both outcomes occur
+    ox = mean[:, 0] + rng.uniform(-1.2, 1.2, len(skip)) * np.minimum(hx, 400) - 4
+    oy = mean[:, 1] + rng.uniform(-1.2, 1.2, len(skip)) * np.minimum(hy, 400) - 4
+    x0, y0 = np.floor(ox).astype(np.float32), np.floor(oy).astype(np.float32)
+    rect = np.stack([x0, x0 + 7, y0, y0 + 7], 1)
+    spl = np.concatenate([mean, conic, skip[:, None]], 1)
+    keep = _touches(spl, rect)
+    box = ~(((x0 - mean[:, 0]) > hx) | ((mean[:, 0] - x0 - 7) > hx) | ((y0 - mean[:, 1]) > hy) |
+            ((mean[:, 1] - y0 - 7) > hy))
+    dropped = np.nonzero(~keep)[0]
+    assert len(dropped) > 300 and np.all(keep <= box | ~np.isfinite(hx))
+    assert (box & ~keep).sum() > 0.05 * box.sum()
+    ii, jj = np.meshgrid(np.arange(8, dtype=np.float32), np.arange(8, dtype=np.float32))
+    for i in dropped:
+        ca, cb, cc = conic[i]
+        px = x0[i] + ii.reshape(-1)
+        py = y0[i] + jj.reshape(-1)
+        pw = _power(ca, cb, cc, mean[i, 0], mean[i, 1], px, py)
+        assert not np.any(~((pw > 0) | (pw < skip[i]))), i
+
+
+def test_block_test_keeps_centres_inside_and_degenerate_cases():
+    spl = _f32([[4.5, 4.5, 1.0, 0.0, 1.0, -2.0],      # centre inside the block
+                [100.0, 100.0, 1.0, 1.0, 1.0, -2.0],  # singular conic
+                [100.0, 100.0, 1.0, 0.0, 1.0, 1.0],   # non-negative skip
+                [np.nan, 0.0, 1.0, 0.0, 1.0, -2.0],   # NaN position
+                [100.0, 100.0, 1.0, 0.0, 1.0, -2.0]]) # far away, round: dropped
+    rect = _f32([[0, 7, 0, 7]] * 5)
+    assert _touches(spl, rect).tolist() == [True, True, True, True, False]

@@ -46,7 +46,7 @@ class W2WCaps(ctypes.Structure):
                 ("tiles", ctypes.c_int64), ("pixels", ctypes.c_int64), ("capacity", ctypes.c_int64)]
 
 
-EXPORTS = ["w2w_version", "w2w_splat_extents", "w2w_workspace_layout", "w2w_batch_workspace_layout",
+EXPORTS = ["w2w_version", "w2w_splat_extents", "w2w_block_touches", "w2w_workspace_layout", "w2w_batch_workspace_layout",
            "w2w_batch_table_bytes", "w2w_batch_encode", "w2w_forward_batch", "w2w_backward_batch", "w2w_forward", "w2w_backward", "w2w_sorted_keys",
            "w2w_sort_temp_bytes", "w2w_sort_pairs_u64", "w2w_scan_temp_bytes",
            "w2w_exclusive_scan_u32", "w2w_prof_enable", "w2w_prof_reset", "w2w_prof_read", "w2w_stats_enable", "w2w_adam_step", "w2w_activate", "w2w_activate_backward",
@@ -73,6 +73,8 @@ def load(path: str = LIB_PATH):
     P = ctypes.POINTER
     L.w2w_version.restype = ctypes.c_int
     L.w2w_version.argtypes = []
+    L.w2w_block_touches.restype = ctypes.c_int
+    L.w2w_block_touches.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
     L.w2w_splat_extents.restype = ctypes.c_int
     L.w2w_splat_extents.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
     L.w2w_workspace_layout.restype = ctypes.c_int

@@ -197,7 +197,8 @@ __global__ __launch_bounds__(64 * units_per_tile<TS>()) void k_blend_bwd(BlendBw
       const int kk_l = cstart + lane;
       n_loaded += (uint32_t)min(64, cend);
       const bool valid = kk_l >= 0 && kk_l < wmax;
-      const bool touch = valid && extent_touches(s.a.x, s.a.y, unpack_extent(s.c.w), x0, x1, y0, y1);
+      const bool touch = valid && extent_touches(s.a.x, s.a.y, unpack_extent(s.c.w), x0, x1, y0, y1) &&
+                         ellipse_touches(s.a.x, s.a.y, s.a.z, s.a.w, s.b.x, s.b.w, x0, x1, y0, y1);
       uint64_t m = __ballot(touch);
       uint64_t cm = 0;
       while (m) {

@@ -465,6 +465,16 @@ extern "C" {
 
 int w2w_version(void) { return W2W_ABI_VERSION; }
 
+int w2w_block_touches(const float* splat, const float* rect, uint64_t n, uint8_t* out) {
+  if (n && (!splat || !rect || !out)) return W2W_ERR_INVALID;
+  for (uint64_t i = 0; i < n; i++) {
+    const float* f = splat + 6 * i;
+    const float* r = rect + 4 * i;
+    out[i] = ellipse_touches(f[0], f[1], f[2], f[3], f[4], f[5], r[0], r[1], r[2], r[3]) ? 1 : 0;
+  }
+  return W2W_OK;
+}
+
 int w2w_splat_extents(const float* conic, const float* skip, uint64_t n, uint32_t* out) {
   if (n && (!conic || !skip || !out)) return W2W_ERR_INVALID;
   for (uint64_t i = 0; i < n; i++)

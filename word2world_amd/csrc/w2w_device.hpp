@@ -418,6 +418,47 @@ W2W_HD bool extent_touches(float mx, float my, float2 e, float x0, float x1, flo
   return !((x0 - mx) > e.x || (mx - x1) > e.x || (y0 - my) > e.y || (my - y1) > e.y);
 }
 
+// Exact form of the same question: whether the ellipse Q(m - p) <= q1 meets
+// the rectangle [x0, x1] x [y0, y1] (pixel centres lie inside it), with q1 the
+// rounding-widened threshold of cull_extent.  In double: if the centre is
+// outside the rectangle the minimum of the convex Q over it lies on an edge,
+// and on each edge it is a clamped one-dimensional quadratic minimum.  Only a
+// minimum above q1 (1 + 1e-6) counts as a miss; every degenerate case, NaN
+// included, counts as touching.
+W2W_HD bool ellipse_touches(float mx, float my, float ca, float cb, float cc, float skip, float x0,
+                            float x1, float y0, float y1) {
+  const double a = ca, b = cb, c = cc;
+  if (!(a > 0.0) || !(c > 0.0) || !(skip < 0.0f)) return true;
+  const double det = a * c - b * b;
+  if (!(det > 0.0)) return true;
+  const double h = 0.5 * (a - c);
+  const double lmax = 0.5 * (a + c) + __builtin_sqrt(h * h + b * b);
+  const double u = 0x1p-24;
+  const double den = 1.0 - 8.0 * u * ((a + c) * lmax / det);
+  if (!(den > 0.5)) return true;
+  const double q1 = (-2.0 * (double)skip) * (1.0 + 4.0 * u) / den * (1.0 + 1e-6);
+  const double X = mx, Y = my;
+  if (!(X < x0 || X > x1 || Y < y0 || Y > y1)) return true;  // centre inside (or NaN)
+  double qmin = 1e300;
+  // vertical edges p.x = E, offset dx = X - E, dy in [Y - y1, Y - y0]
+  for (int e = 0; e < 2; e++) {
+    const double dx = X - (double)(e ? x1 : x0);
+    double dy = -b * dx / c;
+    dy = dy < Y - y1 ? Y - y1 : (dy > Y - y0 ? Y - y0 : dy);
+    const double q = (a * dx * dx + 2.0 * b * dx * dy) + c * dy * dy;
+    qmin = q < qmin ? q : qmin;
+  }
+  // horizontal edges p.y = E
+  for (int e = 0; e < 2; e++) {
+    const double dy = Y - (double)(e ? y1 : y0);
+    double dx = -b * dy / a;
+    dx = dx < X - x1 ? X - x1 : (dx > X - x0 ? X - x0 : dx);
+    const double q = (a * dx * dx + 2.0 * b * dx * dy) + c * dy * dy;
+    qmin = q < qmin ? q : qmin;
+  }
+  return !(qmin > q1);
+}
+
 // ---- wave64 helpers ----
 W2W_DEV float wave_sum(float v) {
   // butterfly over 64 lanes

@@ -444,7 +444,8 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
       }
       n_loaded += (uint32_t)min(64, n - base);
       const bool valid = base + lane < n;
-      const bool touch = valid && extent_touches(s.a.x, s.a.y, unpack_extent(s.c.w), x0, x1, y0, y1);
+      const bool touch = valid && extent_touches(s.a.x, s.a.y, unpack_extent(s.c.w), x0, x1, y0, y1) &&
+                         ellipse_touches(s.a.x, s.a.y, s.a.z, s.a.w, s.b.x, s.b.w, x0, x1, y0, y1);
       uint64_t m = __ballot(touch);
       while (m) {
         const int j = __builtin_ctzll(m);
