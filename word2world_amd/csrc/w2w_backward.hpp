@@ -55,6 +55,32 @@ W2W_DEV float row_sum16(float v) {
   return v;
 }
 
+// gfx950 half-wave and row swaps: (x, y) -> sums that pack two fields into
+// one register.  v_permlane32_swap exchanges the upper 32 lanes of x with the
+// lower 32 of y; v_permlane16_swap exchanges the odd 16-lane rows of x with
+// the even rows of y.  Written as inline asm with both operands read and
+// written: this compiler's builtins for them return the first register for
+// both results.  The s_nop pairs give the VALU -> permlane wait states.
+W2W_DEV float fold32(float x, float y) {  // lanes 0-31: x_l + x_l+32; 32-63: y_l-32 + y_l
+  asm volatile("s_nop 1\n\tv_permlane32_swap_b32 %0, %1\n\ts_nop 1" : "+v"(x), "+v"(y));
+  return x + y;
+}
+W2W_DEV float fold16(float x, float y) {  // rows (x0+x1, y0+y1, x2+x3, y2+y3)
+  asm volatile("s_nop 1\n\tv_permlane16_swap_b32 %0, %1\n\ts_nop 1" : "+v"(x), "+v"(y));
+  return x + y;
+}
+
+// Sums of four fields over the whole wave: after the folds, row r of the
+// result holds field (a, c, b, d)[r] summed over the rows; the row sum then
+// leaves the totals in every lane of that row.
+W2W_DEV float wave_sum4(float a, float b, float c, float d) {
+  return row_sum16(fold16(fold32(a, b), fold32(c, d)));
+}
+// Two fields: rows (a, 0, b, 0).
+W2W_DEV float wave_sum2(float a, float b) {
+  return row_sum16(fold16(fold32(a, b), 0.0f));
+}
+
 W2W_DEV float lane63(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
 }
@@ -88,9 +114,9 @@ struct BlendBwdArgs {
 // contributor), each gathering the chunk one instance per lane, dropping by
 // ballot the splats whose cull extent misses its block (every pixel of the
 // block would skip them) and visiting the rest in reverse order.  For a splat
-// with any contributing pixel the wave sums the 10 partials over its lanes,
-// down to one sum per 16-lane row (tile edge 8), per pair of rows (edge 16)
-// or per wave (edge 32), with DPP, and stores those sums in LDS.  After a barrier, lane j of each wave adds all
+// with any contributing pixel the wave sums the 10 partials over its lanes
+// (gfx950 half-wave and row swaps pack two and four fields per register, then
+// four DPP row steps) and stores the sums in LDS.  After a barrier, lane j of each wave adds all
 // slots of splat j in (wave, row) order and writes the sums to the
 // instance's record (emission position = item offset + its tile's index in
 // the item's rectangle, prefetched with the chunk) and sets its flag; a
@@ -99,11 +125,7 @@ template <int TS>
 __global__ __launch_bounds__(64 * units_per_tile<TS>()) void k_blend_bwd(BlendBwdArgs a) {
   constexpr int UPT = (int)units_per_tile<TS>();
   constexpr int UX = TS / kUnit;
-  // partial sums stored per splat and wave: one per 16-lane row (edge 8),
-  // per pair of rows (edge 16) or per wave (edge 32), which keeps the slots
-  // within 40 KB
-  constexpr int ROWS = UPT == 1 ? 4 : (UPT == 4 ? 2 : 1);
-  __shared__ float s_part[UPT][ROWS][GR_FIELDS][64];
+  __shared__ float s_part[UPT][GR_FIELDS][64];  // per wave, per field, per splat of the chunk
   __shared__ uint64_t s_cm[UPT];
   __shared__ uint32_t s_tile, s_max;
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
@@ -111,8 +133,10 @@ __global__ __launch_bounds__(64 * units_per_tile<TS>()) void k_blend_bwd(BlendBw
   const uint32_t P = min(hdr->n_items, a.items_cap);
   const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
   const int tid = (int)threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int row = ROWS == 4 ? lane >> 4 : (lane >> 4) & 1;
-  const bool row_tail = ROWS == 1 ? lane == 63 : (ROWS == 2 ? (lane == 15 || lane == 31) : (lane & 15) == 15);
+  const int row = lane >> 4;
+  const bool row_tail = (lane & 15) == 15;
+  // which field each row holds after wave_sum4 / wave_sum2
+  const int fa = (row & 1) ? (row >> 1) + 2 : (row >> 1);  // rows -> fields 0, 2, 1, 3
   const float a_min = 0x1.010102p-8f;
   uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
   for (;;) {
@@ -254,20 +278,13 @@ __global__ __launch_bounds__(64 * units_per_tile<TS>()) void k_blend_bwd(BlendBw
           }
         }
         if (__ballot(contrib) != 0ull) {
-          float g[GR_FIELDS] = {g_mx, g_my, g_ca, g_cb, g_cc, g_op, g_r, g_g, g_b, g_d};
-#pragma unroll
-          for (int f = 0; f < GR_FIELDS; f++) {
-            if (ROWS == 1) {
-              g[f] = wave_reduce_dpp(g[f]);
-            } else {
-              g[f] = row_sum16(g[f]);
-              if (ROWS == 2) g[f] += __shfl_xor(g[f], 32, 64);  // rows r and r + 2
-            }
-          }
+          const float s0 = wave_sum4(g_mx, g_my, g_ca, g_cb);  // rows: mx, ca, my, cb
+          const float s1 = wave_sum4(g_cc, g_op, g_r, g_g);    // rows: cc, r, op, g
+          const float s2 = wave_sum2(g_b, g_d);                // rows: b, 0, depth, 0
           if (row_tail) {
-            const int r = ROWS == 1 ? 0 : row;
-#pragma unroll
-            for (int f = 0; f < GR_FIELDS; f++) s_part[w][r][f][j] = g[f];
+            s_part[w][fa][j] = s0;
+            s_part[w][4 + fa][j] = s1;
+            if ((row & 1) == 0) s_part[w][8 + (row >> 1)][j] = s2;
           }
           cm |= 1ull << j;
         }
@@ -286,10 +303,7 @@ __global__ __launch_bounds__(64 * units_per_tile<TS>()) void k_blend_bwd(BlendBw
             float sum = 0.0f;
 #pragma unroll
             for (int v = 0; v < UPT; v++)
-              if ((s_cm[v] >> lane) & 1ull) {
-#pragma unroll
-                for (int r = 0; r < ROWS; r++) sum += s_part[v][r][f][lane];
-              }
+              if ((s_cm[v] >> lane) & 1ull) sum += s_part[v][f][lane];
             a.rec[(size_t)pos * GR_FIELDS + f] = sum;
           }
           if (w == 0) a.flag[pos] = 1;
