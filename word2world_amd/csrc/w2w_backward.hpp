@@ -233,51 +233,51 @@ __global__ __launch_bounds__(64 * units_per_tile<TS>()) void k_blend_bwd(BlendBw
         const float ca = bcast(s.a.z, j), cb = bcast(s.a.w, j), cc = bcast(s.b.x, j);
         const float o = bcast(s.b.y, j), dz = bcast(s.b.z, j), sk = bcast(s.b.w, j);
         const float cr = bcast(s.c.x, j), cg = bcast(s.c.y, j), cbl = bcast(s.c.z, j);
-        float g_mx = 0.0f, g_my = 0.0f, g_ca = 0.0f, g_cb = 0.0f, g_cc = 0.0f, g_op = 0.0f;
-        float g_r = 0.0f, g_g = 0.0f, g_b = 0.0f, g_d = 0.0f;
-        bool contrib = false;
-        if (kk < last) {
-          n_eval++;
-          const float dx = mx - pxf, dy = my - pyf;
-          const float pw = power_of(ca, cb, cc, dx, dy);
-          if (pw <= 0.0f && pw >= sk) {
-            const float G = exp_neg(pw);
-            const float oG = o * G;
-            const float al = fminf(0.99f, oG);
-            if (al >= a_min) {
-              contrib = true;
-              n_con++;
-              const float r1 = __builtin_amdgcn_rcpf(1.0f - al);
-              T = T * r1;
-              const float aT = al * T;
-              acc0 = la * lc0 + (1.0f - la) * acc0;
-              acc1 = la * lc1 + (1.0f - la) * acc1;
-              acc2 = la * lc2 + (1.0f - la) * acc2;
-              accD = la * ld + (1.0f - la) * accD;
-              accA = la + (1.0f - la) * accA;
-              float dLda = ((cr - acc0) * gC0 + (cg - acc1) * gC1) + (cbl - acc2) * gC2;
-              dLda = dLda + (dz - accD) * gD;
-              dLda = dLda + (1.0f - accA) * gA;
-              dLda = dLda * T;
-              dLda = dLda - (Tf * r1) * bgdot;
-              lc0 = cr; lc1 = cg; lc2 = cbl; ld = dz; la = al;
-              g_r = aT * gC0;
-              g_g = aT * gC1;
-              g_b = aT * gC2;
-              g_d = aT * gD;
-              if (oG <= 0.99f) {
-                g_op = G * dLda;
-                const float dLdp = (o * dLda) * G;
-                g_ca = (-0.5f * dx * dx) * dLdp;
-                g_cb = (-dx * dy) * dLdp;
-                g_cc = (-0.5f * dy * dy) * dLdp;
-                g_mx = (-(ca * dx + cb * dy) * dLdp) * hw;
-                g_my = (-(cb * dx + cc * dy) * dLdp) * hh;
-              }
-            }
-          }
+        // straight-line per lane: every lane evaluates, selects keep the
+        // state of lanes that do not contribute (no per-branch zero copies)
+        const bool act = kk < last;
+        if (__ballot(act) == 0ull) continue;
+        const float dx = mx - pxf, dy = my - pyf;
+        const float pw = power_of(ca, cb, cc, dx, dy);
+        const float G = exp_neg(pw);
+        const float oG = o * G;
+        const float al = fminf(0.99f, oG);
+        const bool contrib = act && (pw <= 0.0f && pw >= sk) && al >= a_min;
+        n_eval += act ? 1u : 0u;
+        n_con += contrib ? 1u : 0u;
+        if (__ballot(contrib) == 0ull) continue;
+        const float r1 = __builtin_amdgcn_rcpf(1.0f - al);
+        const float Tn = T * r1;
+        const float aT = al * Tn;
+        const float a0 = la * lc0 + (1.0f - la) * acc0;
+        const float a1 = la * lc1 + (1.0f - la) * acc1;
+        const float a2 = la * lc2 + (1.0f - la) * acc2;
+        const float aD = la * ld + (1.0f - la) * accD;
+        const float aA = la + (1.0f - la) * accA;
+        float dLda = ((cr - a0) * gC0 + (cg - a1) * gC1) + (cbl - a2) * gC2;
+        dLda = dLda + (dz - aD) * gD;
+        dLda = dLda + (1.0f - aA) * gA;
+        dLda = dLda * Tn;
+        dLda = dLda - (Tf * r1) * bgdot;
+        const float aTm = contrib ? aT : 0.0f;
+        const float g_r = aTm * gC0;
+        const float g_g = aTm * gC1;
+        const float g_b = aTm * gC2;
+        const float g_d = aTm * gD;
+        const bool free_alpha = contrib && oG <= 0.99f;  // a clamped alpha passes no gradient
+        const float g_op = free_alpha ? G * dLda : 0.0f;
+        const float dLdp = free_alpha ? (o * dLda) * G : 0.0f;
+        const float g_ca = (-0.5f * dx * dx) * dLdp;
+        const float g_cb = (-dx * dy) * dLdp;
+        const float g_cc = (-0.5f * dy * dy) * dLdp;
+        const float g_mx = (-(ca * dx + cb * dy) * dLdp) * hw;
+        const float g_my = (-(cb * dx + cc * dy) * dLdp) * hh;
+        if (contrib) {
+          T = Tn;
+          acc0 = a0; acc1 = a1; acc2 = a2; accD = aD; accA = aA;
+          lc0 = cr; lc1 = cg; lc2 = cbl; ld = dz; la = al;
         }
-        if (__ballot(contrib) != 0ull) {
+        {
           const float s0 = wave_sum4(g_mx, g_my, g_ca, g_cb);  // rows: mx, ca, my, cb
           const float s1 = wave_sum4(g_cc, g_op, g_r, g_g);    // rows: cc, r, op, g
           const float s2 = wave_sum2(g_b, g_d);                // rows: b, 0, depth, 0
