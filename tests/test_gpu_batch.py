@@ -134,3 +134,34 @@ def test_capacity_overflow_is_reported(gpu):
     ws = _run_batch(scenes, packed, gpu, capacity=1000)
     K, ovf = (int(v) for v in ws.status().cpu())
     assert ovf == 1 and K > 1000
+
+
+def test_tight_emission_same_images_and_gradients(gpu):
+    """Emitting only into the tiles a Gaussian's cull extent reaches drops
+    instances no pixel can use: images, alpha, depth, final T and every
+    gradient are bit-identical to the reference emission, with fewer
+    instances."""
+    import copy
+    scenes, arrays, cams, packed = _setup(gpu)
+    tight = copy.deepcopy(scenes)
+    for s in tight:
+        for f in s.frames:
+            f.tight = True
+    out = []
+    for sc in (scenes, tight):
+        ws = _run_batch(sc, packed, gpu)
+        torch.manual_seed(0)
+        dLdc = torch.randn_like(ws.color)
+        grads = batch.BatchGrads(ws.caps, int(ws.totals[5]), gpu)
+        batch.backward(ws, grads, packed[0], packed[1], packed[2], packed[4], dLdc)
+        torch.cuda.synchronize()
+        out.append((int(ws.status()[0]), [tuple(t.clone() for t in v) for v in ws.frame_views(sc)],
+                    ws.final_T().clone(), grads))
+    (K0, v0, T0, g0), (K1, v1, T1, g1) = out
+    print("K reference", K0, "tight", K1, flush=True)
+    assert K1 < K0
+    for a, b in zip(v0, v1):
+        assert all(torch.equal(x, y) for x, y in zip(a, b))
+    assert torch.equal(T0, T1)
+    for name in ("means2D", "colors", "opacity", "means3D", "cov3D", "sh", "scales", "rotations"):
+        assert torch.equal(getattr(g0, name), getattr(g1, name)), name

@@ -32,6 +32,10 @@ class FrameSpec:
     focal_y: float
     background: np.ndarray = field(default_factory=lambda: np.ones(3, np.float32))
     backward: bool = True
+    # emit into the tiles the Gaussian's cull extent meets only (fewer
+    # instances; identical images and gradients, but K, keys, ranges and
+    # n_contrib are no longer the reference's)
+    tight: bool = False
 
 
 @dataclass
@@ -42,11 +46,11 @@ class SceneSpec:
     frames: list             # FrameSpec per view
 
 
-def frame_from_camera(cam, background=None, backward=True) -> FrameSpec:
+def frame_from_camera(cam, background=None, backward=True, tight=False) -> FrameSpec:
     """From a word2world_amd.camera.Camera."""
     bg = np.ones(3, np.float32) if background is None else np.asarray(background, np.float32)
     return FrameSpec(cam.view_flat(), cam.proj_flat(), cam.campos_np(), cam.width, cam.height,
-                     cam.focal_x, cam.focal_y, bg, backward)
+                     cam.focal_x, cam.focal_y, bg, backward, tight)
 
 
 def caps_for(scenes: list, tile: int, capacity: int, scenes_cap: int | None = None,
@@ -85,7 +89,7 @@ def encode(scenes: list, caps: _lib.W2WCaps, bases=None, sh_offs=None,
         view[k], proj[k], cp[k], bg[k] = f.viewmatrix, f.projmatrix, f.campos, f.background
         foc[k] = (f.focal_x, f.focal_y)
         wh[k] = (f.width, f.height)
-        flags[k] = 1 if f.backward else 0
+        flags[k] = (1 if f.backward else 0) | (2 if f.tight else 0)
     sn = np.array([s.n for s in scenes] or [0], np.int64)
     sv = np.array([len(s.frames) for s in scenes] or [1], np.int32)
     sm = np.array([s.n_coeffs for s in scenes] or [1], np.int32)
@@ -145,6 +149,11 @@ class BatchWorkspace:
 
     def status(self) -> torch.Tensor:
         return self.geom[:8].view(torch.int32)
+
+    def final_T(self) -> torch.Tensor:
+        """Final transmittance of every pixel of the encoded batch."""
+        n, o = int(self.totals[4]), self.layout.final_T
+        return self.image[o:o + 4 * n].view(torch.float32)
 
     def frame_views(self, scenes: list):
         """(color [3,H,W], depth [H,W], alpha [H,W]) views per frame."""

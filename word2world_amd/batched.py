@@ -107,14 +107,16 @@ class BatchedTrainer:
     """fwd + bwd + Adam for every scene of the batch in one launch sequence."""
 
     def __init__(self, scenes: list, cameras: list, targets: list | None, device, tile: int = 16,
-                 capacity: int | None = None):
+                 capacity: int | None = None, tight: bool = True):
         """cameras[k]: word2world_amd.camera.Camera views of scene k;
-        targets[k][v]: [3,H,W] target image (or None for render-only)."""
+        targets[k][v]: [3,H,W] target image (or None for render-only);
+        tight: emit instances only where a Gaussian's cull extent reaches
+        (same images and gradients, fewer instances)."""
         self.store = BatchedStore(scenes, device)
         self.device = torch.device(device)
         self.tile = tile
         self.specs = [batch.SceneSpec(s.n, s.sh_degree, self.store.M,
-                                      [batch.frame_from_camera(c) for c in cams])
+                                      [batch.frame_from_camera(c, tight=tight) for c in cams])
                       for s, cams in zip(scenes, cameras)]
         items = sum(s.n * len(c) for s, c in zip(scenes, cameras))
         cap = capacity if capacity is not None else min(MAX_CAPACITY, max(1 << 20, 8 * items))

@@ -152,11 +152,17 @@ struct FrameDesc {
   float cp[3], fx;     // caller's campos, focal x
   float bg[3], fy;     // background, focal y
   int32_t W, H, tiles_x, tiles_y;
-  uint32_t tile_off, scene, view, flags;  // flags bit 0: backward wanted
+  uint32_t tile_off, scene, view, flags;  // FRAME_* bits
   uint64_t pix_off;    // first pixel in the packed per-pixel outputs
   uint64_t target_off; // first float of this frame's [3,H,W] loss target
 };
-constexpr uint32_t FRAME_BACKWARD = 1u;
+constexpr uint32_t FRAME_BACKWARD = 1u;  // gradients wanted for this frame
+// Emit a Gaussian only into the tiles of its rectangle that its cull extent
+// box meets (the rectangle of the reference rule intersected with that box):
+// fewer instances, identical images, final T and gradients; K, the keys, the
+// ranges and n_contrib (an index into the tile's list) then differ from the
+// reference's.
+constexpr uint32_t FRAME_TIGHT = 2u;
 constexpr size_t TABLE_HEADER_BYTES = 64;
 
 W2W_DEV const SceneDesc* table_scenes(const void* table) {

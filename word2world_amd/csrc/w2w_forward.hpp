@@ -149,12 +149,28 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
         keep = r > 0.0f;
       }
     }
+    const uint32_t ext = keep ? cull_extent(g.ca, g.cb, g.cc, skip_below) : kExtentInf;
     if (keep) {
       const float tsf = (float)a.ts, tsm1 = (float)(a.ts - 1);
       x0 = tile_coord((g.px - r) / tsf, (uint32_t)fd.tiles_x);
       y0 = tile_coord((g.py - r) / tsf, (uint32_t)fd.tiles_y);
       x1 = tile_coord(((g.px + r) + tsm1) / tsf, (uint32_t)fd.tiles_x);
       y1 = tile_coord(((g.py + r) + tsm1) / tsf, (uint32_t)fd.tiles_y);
+      if (fd.flags & FRAME_TIGHT) {
+        // tiles holding a pixel centre within the extent box (half a pixel of
+        // slack covers the rounding of these quotients)
+        const float2 e = unpack_extent(__builtin_bit_cast(float, ext));
+        if (e.x < 60000.0f && e.y < 60000.0f) {
+          const uint32_t bx0 = tile_coord(((g.px - e.x) - 0.5f) / tsf, (uint32_t)fd.tiles_x);
+          const uint32_t by0 = tile_coord(((g.py - e.y) - 0.5f) / tsf, (uint32_t)fd.tiles_y);
+          const uint32_t bx1 = min(tile_coord(((g.px + e.x) + 0.5f) / tsf, (uint32_t)fd.tiles_x) + 1u,
+                                   (uint32_t)fd.tiles_x);
+          const uint32_t by1 = min(tile_coord(((g.py + e.y) + 0.5f) / tsf, (uint32_t)fd.tiles_y) + 1u,
+                                   (uint32_t)fd.tiles_y);
+          x0 = max(x0, bx0); y0 = max(y0, by0);
+          x1 = min(x1, bx1); y1 = min(y1, by1);
+        }
+      }
       cnt = (x1 > x0 && y1 > y0) ? (x1 - x0) * (y1 - y0) : 0u;
       radius = (int32_t)__builtin_ceilf(r);
     }
@@ -166,7 +182,6 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
       sp.a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       sp.b = make_float4(0.0f, o, 0.0f, 0.0f);
     }
-    const uint32_t ext = keep ? cull_extent(g.ca, g.cb, g.cc, skip_below) : kExtentInf;
     sp.c = make_float4(rgb[0], rgb[1], rgb[2], __builtin_bit_cast(float, ext));
     a.splats[i] = sp;
     a.radii[i] = radius;

@@ -270,7 +270,7 @@ class ContinuousServer:
             sc = self.pool.scenes[r.scene]
             bw = r.kind == "optimize"
             specs.append(batch.SceneSpec(sc.n, sc.sh_degree, self.M,
-                                         [batch.frame_from_camera(c, backward=bw) for c in r.cameras]))
+                                         [batch.frame_from_camera(c, backward=bw, tight=True) for c in r.cameras]))
             bases.append(r.base)
             shoffs.append(A.sh_float_offset(r.base) - int(A.starts[1]))
             toffs.extend(r.target_offs if bw else [0] * len(r.cameras))
