@@ -413,12 +413,14 @@ W2W_DEV bool wave_all(bool p) { return __ballot(!p) == 0ull; }
 // (no barriers).
 template <int TS>
 __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
+  __shared__ Splat s_chunk[4][64];  // each wave's current chunk, for broadcast reads
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
   const uint32_t T = min(hdr->n_tiles, a.tiles_cap);
   const uint32_t P = min(hdr->n_items, a.items_cap);
   const uint32_t U = T * units_per_tile<TS>();
   const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
   const int lane = (int)(threadIdx.x & 63u);
+  Splat* const stage = s_chunk[threadIdx.x >> 6];
   const float a_min = 0x1.010102p-8f;
   uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
   for (;;) {
@@ -462,41 +464,58 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
       const bool touch = valid && extent_touches(s.a.x, s.a.y, unpack_extent(s.c.w), x0, x1, y0, y1) &&
                          ellipse_touches(s.a.x, s.a.y, s.a.z, s.a.w, s.b.x, s.b.w, x0, x1, y0, y1);
       uint64_t m = __ballot(touch);
-      while (m) {
-        const int j = __builtin_ctzll(m);
-        m &= m - 1;
-        // broadcasts with every lane active (lane j may be done)
-        const float mx = bcast(s.a.x, j), my = bcast(s.a.y, j);
-        const float ca = bcast(s.a.z, j), cb = bcast(s.a.w, j), cc = bcast(s.b.x, j);
-        const float op = bcast(s.b.y, j), dz = bcast(s.b.z, j), sk = bcast(s.b.w, j);
-        const float cr = bcast(s.c.x, j), cg = bcast(s.c.y, j), cbl = bcast(s.c.z, j);
-        if (!done) {
-          const float dx = mx - pxf, dy = my - pyf;
-          const float pw = power_of(ca, cb, cc, dx, dy);
-          n_eval++;
-          if (!(pw > 0.0f || pw < sk)) {
-            const float G = exp_neg(pw);
-            const float al = fminf(0.99f, op * G);
-            if (al >= a_min) {
-              const float nT = T_ * (1.0f - al);
-              if (nT < 1e-4f) {
-                done = true;
-              } else {
-                const float aT = al * T_;
-                C0 = __builtin_fmaf(cr, aT, C0);
-                C1 = __builtin_fmaf(cg, aT, C1);
-                C2 = __builtin_fmaf(cbl, aT, C2);
-                A = A + aT;
-                Dd = __builtin_fmaf(dz, aT, Dd);
-                T_ = nT;
-                last = (uint32_t)(base + j + 1);
-                n_con++;
-              }
-            }
-          }
+      if (m == 0ull) continue;
+      // the wave's chunk goes to its LDS slot; survivors are then read back
+      // with one uniform address (a broadcast), the next one while the
+      // current one is evaluated.  LDS operations of one wave complete in
+      // order, so no barrier is needed.
+      stage[lane] = s;
+      __builtin_amdgcn_wave_barrier();
+      auto eval = [&](const Splat& q, int j) {
+        if (done) return;
+        const float dx = q.a.x - pxf, dy = q.a.y - pyf;
+        const float pw = power_of(q.a.z, q.a.w, q.b.x, dx, dy);
+        n_eval++;
+        if (pw > 0.0f || pw < q.b.w) return;
+        const float G = exp_neg(pw);
+        const float al = fminf(0.99f, q.b.y * G);
+        if (al < a_min) return;
+        const float nT = T_ * (1.0f - al);
+        if (nT < 1e-4f) {
+          done = true;
+          return;
         }
-        if (wave_all(done)) break;
+        const float aT = al * T_;
+        C0 = __builtin_fmaf(q.c.x, aT, C0);
+        C1 = __builtin_fmaf(q.c.y, aT, C1);
+        C2 = __builtin_fmaf(q.c.z, aT, C2);
+        A = A + aT;
+        Dd = __builtin_fmaf(q.b.z, aT, Dd);
+        T_ = nT;
+        last = (uint32_t)(base + j + 1);
+        n_con++;
+      };
+      // two survivors in flight: the next one's read overlaps this one's work
+      int ja = __builtin_ctzll(m);
+      m &= m - 1;
+      Splat qa = stage[ja], qb;
+      for (;;) {
+        const int jb = m ? __builtin_ctzll(m) : -1;
+        if (jb >= 0) {
+          qb = stage[jb];
+          m &= m - 1;
+        }
+        eval(qa, ja);
+        if (jb < 0 || wave_all(done)) break;
+        ja = m ? __builtin_ctzll(m) : -1;
+        if (ja >= 0) {
+          qa = stage[ja];
+          m &= m - 1;
+        }
+        eval(qb, jb);
+        if (ja < 0 || wave_all(done)) break;
       }
+      __builtin_amdgcn_wave_barrier();  // reads of this chunk precede the next chunk's writes
     }
     if (inside) {
       const size_t HW = (size_t)fd.W * fd.H;
