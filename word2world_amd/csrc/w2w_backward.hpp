@@ -122,10 +122,15 @@ struct BlendBwdArgs {
 // the item's rectangle, prefetched with the chunk) and sets its flag; a
 // second barrier frees the slots for the next chunk.
 template <int TS>
-__global__ __launch_bounds__(64 * units_per_tile<TS>()) void k_blend_bwd(BlendBwdArgs a) {
+__global__ __launch_bounds__(64 * units_per_tile<TS>()) __attribute__((amdgpu_waves_per_eu(4)))
+void k_blend_bwd(BlendBwdArgs a) {
   constexpr int UPT = (int)units_per_tile<TS>();
   constexpr int UX = TS / kUnit;
   __shared__ float s_part[UPT][GR_FIELDS][64];  // per wave, per field, per splat of the chunk
+  // tile edges 8 and 16 stage each wave's chunk in LDS for broadcast reads
+  // (16 waves of edge 32 would not fit; they broadcast with readlane)
+  constexpr bool kStage = UPT <= 4;
+  __shared__ Splat s_stage[kStage ? UPT : 1][64];
   __shared__ uint64_t s_cm[UPT];
   __shared__ uint32_t s_tile, s_max;
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
@@ -225,14 +230,26 @@ __global__ __launch_bounds__(64 * units_per_tile<TS>()) void k_blend_bwd(BlendBw
                          ellipse_touches(s.a.x, s.a.y, s.a.z, s.a.w, s.b.x, s.b.w, x0, x1, y0, y1);
       uint64_t m = __ballot(touch);
       uint64_t cm = 0;
+      if constexpr (kStage) {
+        if (m) s_stage[w][lane] = s;  // read back by this wave only: LDS keeps its order
+        __builtin_amdgcn_wave_barrier();
+      }
       while (m) {
         const int j = 63 - __builtin_clzll(m);
         m &= ~(1ull << j);
         const int kk = cstart + j;
-        const float mx = bcast(s.a.x, j), my = bcast(s.a.y, j);
-        const float ca = bcast(s.a.z, j), cb = bcast(s.a.w, j), cc = bcast(s.b.x, j);
-        const float o = bcast(s.b.y, j), dz = bcast(s.b.z, j), sk = bcast(s.b.w, j);
-        const float cr = bcast(s.c.x, j), cg = bcast(s.c.y, j), cbl = bcast(s.c.z, j);
+        float mx, my, ca, cb, cc, o, dz, sk, cr, cg, cbl;
+        if constexpr (kStage) {
+          const Splat q = s_stage[w][j];
+          mx = q.a.x; my = q.a.y; ca = q.a.z; cb = q.a.w;
+          cc = q.b.x; o = q.b.y; dz = q.b.z; sk = q.b.w;
+          cr = q.c.x; cg = q.c.y; cbl = q.c.z;
+        } else {
+          mx = bcast(s.a.x, j); my = bcast(s.a.y, j);
+          ca = bcast(s.a.z, j); cb = bcast(s.a.w, j); cc = bcast(s.b.x, j);
+          o = bcast(s.b.y, j); dz = bcast(s.b.z, j); sk = bcast(s.b.w, j);
+          cr = bcast(s.c.x, j); cg = bcast(s.c.y, j); cbl = bcast(s.c.z, j);
+        }
         // straight-line per lane: every lane evaluates, selects keep the
         // state of lanes that do not contribute (no per-branch zero copies)
         const bool act = kk < last;
@@ -290,7 +307,7 @@ __global__ __launch_bounds__(64 * units_per_tile<TS>()) void k_blend_bwd(BlendBw
         }
       }
       if (lane == 0) s_cm[w] = cm;
-      __syncthreads();
+      __syncthreads();  // also orders this chunk's staged reads before the next chunk's writes
       // splat `lane` of the chunk: sum its slots in (wave, row) order
       uint64_t any = 0;
 #pragma unroll
