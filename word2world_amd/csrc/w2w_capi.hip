@@ -237,14 +237,14 @@ template <typename KT>
 void sort_passes(const KT* kin, const uint32_t* vin, KT* kA, uint32_t* vA, KT* kB, uint32_t* vB,
                  const uint32_t* d_n, uint32_t n_cap, int npasses, int last_bits, bool write_last_keys,
                  uint32_t* hist, uint32_t* status, uint32_t* tickets, int kid_hist, int kid_pass,
-                 hipStream_t s) {
+                 hipStream_t s, bool hist_ready = false) {
   const uint32_t ntiles = cdiv(n_cap, SORT_TILE);
   if (ntiles == 0) return;
   const uint32_t hgrid = std::min<uint32_t>(cdiv(n_cap, 256), 2048);
   {
     ProfSpan ps(kid_hist, s);
-    hipLaunchKernelGGL(k_sort_hist<KT>, dim3(hgrid), dim3(256), 0, s, kin, d_n, n_cap, 0, npasses,
-                       last_bits, hist, status, ntiles, tickets);
+    hipLaunchKernelGGL(k_sort_hist<KT>, dim3(hgrid), dim3(256), 0, s, hist_ready ? nullptr : kin, d_n,
+                       n_cap, 0, npasses, last_bits, hist, status, ntiles, tickets);
   }
   const uint32_t pgrid = std::min<uint32_t>(ntiles, 2048);
   const KT* ks = kin;
@@ -334,7 +334,8 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
                            at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.L.counts),
                            at<uint2>(G, p.L.rect), at<uint32_t>(G, p.L.item_frame), items_cap, cap,
                            status + ST_EMIT_TICKET, at<uint32_t>(B, p.tkey0),
-                           at<uint32_t>(B, p.tid0), at<uint32_t>(G, p.L.item_off));
+                           at<uint32_t>(B, p.tid0), at<uint32_t>(G, p.L.item_off),
+                           at<uint32_t>(B, p.thist), p.tpasses, p.tlast_bits);
       }
       sort_passes<uint32_t>(at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0),
                             at<uint32_t>(B, p.L.sorted_tiles), at<uint32_t>(B, p.L.sorted_ids),
@@ -342,7 +343,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
                             at<uint32_t>(B, (p.tpasses & 1) ? p.tid0 : p.tid1), status + ST_K, cap,
                             p.tpasses, p.tlast_bits, true, at<uint32_t>(B, p.thist),
                             at<uint32_t>(B, p.tstatus), status + ST_TTICKET, W2W_KID_TSORT_HIST,
-                            W2W_KID_TSORT_PASS, s);
+                            W2W_KID_TSORT_PASS, s, /*hist_ready=*/true);
       const uint32_t rgrid = std::min<uint32_t>(cdiv(cap, 256), 4096);
       ProfSpan ps(W2W_KID_RANGES, s);
       hipLaunchKernelGGL(k_ranges, dim3(rgrid), dim3(256), 0, s, at<uint32_t>(B, p.L.sorted_tiles),

@@ -270,6 +270,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(
 // [256b, 256b+256) and the contiguous instance range they cover; each thread
 // strides over instances and finds its owner by binary search in LDS.  Chunks
 // are claimed through a ticket (any grid size).
+// Emission also histograms the tile keys for the tile sort (every digit of
+// every pass, hist[pass][256], pre-zeroed), so the sort does not read the
+// keys an extra time.
 __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, uint32_t scenes_cap,
                                               const uint32_t* __restrict__ perm,
                                               const uint32_t* __restrict__ offs,
@@ -280,19 +283,23 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
                                               uint32_t* __restrict__ ticket,
                                               uint32_t* __restrict__ tile_keys,
                                               uint32_t* __restrict__ ids,
-                                              uint32_t* __restrict__ item_off) {
+                                              uint32_t* __restrict__ item_off,
+                                              uint32_t* __restrict__ hist, int hist_passes,
+                                              int hist_last_bits) {
   __shared__ uint32_t s_start[256], s_g[256], s_x0[256], s_y0[256], s_w[256], s_tx[256], s_toff[256];
   __shared__ uint32_t s_end, s_chunk;
+  __shared__ uint32_t s_hist[4][256];
   const BatchHeader* hdr = static_cast<const BatchHeader*>(table);
   const FrameDesc* frames = table_frames(table, scenes_cap);
   const uint32_t P = min(hdr->n_items, items_cap);
   const uint32_t nchunks = (P + 255) / 256;
   const int tid = threadIdx.x;
+  for (int i = tid; i < 4 * 256; i += 256) (&s_hist[0][0])[i] = 0u;
   for (;;) {
     if (tid == 0) s_chunk = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t chunk = s_chunk;
-    if (chunk >= nchunks) return;
+    if (chunk >= nchunks) break;
     const uint32_t i = chunk * 256 + tid;
     const uint32_t last = min(P, chunk * 256 + 256) - 1;
     if (i < P) {
@@ -324,14 +331,25 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
       uint32_t ty = s_y0[lo] + local / w;
       uint32_t tx = s_x0[lo] + local % w;
       if (k < capacity) {
-        tile_keys[k] = s_toff[lo] + ty * s_tx[lo] + tx;
+        const uint32_t key = s_toff[lo] + ty * s_tx[lo] + tx;
+        tile_keys[k] = key;
         ids[k] = s_g[lo];
+        for (int p = 0; p < hist_passes; p++) {
+          const int bits = (p == hist_passes - 1) ? hist_last_bits : 8;
+          atomicAdd(&s_hist[p][(key >> (8 * p)) & ((1u << bits) - 1u)], 1u);
+        }
       }
     }
     __syncthreads();
   }
+  for (int i = tid; i < hist_passes * 256; i += 256) {
+    const uint32_t v = (&s_hist[0][0])[i];
+    if (v) atomicAdd(hist + i, v);
+  }
 }
 
+// Per-tile [start, end) of the sorted tile keys; four keys per thread with
+// one 16-byte load (the key array is 256-byte aligned).
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ keys,
                                                 const uint32_t* __restrict__ status,
                                                 const void* __restrict__ table, uint32_t capacity,
@@ -339,12 +357,27 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ key
   uint32_t n = status[ST_K];
   n = n < capacity ? n : capacity;
   const uint32_t T = min(static_cast<const BatchHeader*>(table)->n_tiles, tiles_cap);
+  const uint32_t n4 = (n + 3) / 4;
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    uint32_t t = keys[i];
-    if (t >= T) continue;
-    if (i == 0 || keys[i - 1] != t) ranges[t].x = i;
-    if (i == n - 1 || keys[i + 1] != t) ranges[t].y = i + 1;
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += stride) {
+    const uint32_t i0 = 4 * q;
+    uint32_t k[6];  // keys i0-1 .. i0+4; out of range = all ones (no tile)
+    k[0] = i0 > 0 ? keys[i0 - 1] : 0xFFFFFFFFu;
+    if (i0 + 4 <= n) {
+      const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
+      k[1] = v.x; k[2] = v.y; k[3] = v.z; k[4] = v.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; j++) k[1 + j] = i0 + j < n ? keys[i0 + j] : 0xFFFFFFFFu;
+    }
+    k[5] = i0 + 4 < n ? keys[i0 + 4] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = 1; j <= 4; j++) {
+      const uint32_t t = k[j];
+      if (t >= T) continue;
+      if (k[j - 1] != t) ranges[t].x = i0 + j - 1;
+      if (k[j + 1] != t) ranges[t].y = i0 + j;
+    }
   }
 }
 

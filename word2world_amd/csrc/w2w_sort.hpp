@@ -46,18 +46,20 @@ __global__ __launch_bounds__(256) void k_sort_hist(const KT* __restrict__ keys,
   for (int i = threadIdx.x; i < 8 * RADIX; i += blockDim.x) (&sh[0][0])[i] = 0;
   __syncthreads();
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    KT k = keys[i];
-    for (int p = 0; p < npasses; p++) {
-      int bits = (p == npasses - 1) ? last_bits : 8;
-      uint32_t d = (uint32_t)(k >> (begin_bit + 8 * p)) & ((1u << bits) - 1u);
-      atomicAdd(&sh[p][d], 1u);
+  if (keys) {  // null: the histogram was accumulated by the producer of the keys
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+      KT k = keys[i];
+      for (int p = 0; p < npasses; p++) {
+        int bits = (p == npasses - 1) ? last_bits : 8;
+        uint32_t d = (uint32_t)(k >> (begin_bit + 8 * p)) & ((1u << bits) - 1u);
+        atomicAdd(&sh[p][d], 1u);
+      }
     }
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < npasses * RADIX; i += blockDim.x) {
-    uint32_t v = (&sh[0][0])[i];
-    if (v) atomicAdd(&hist[i], v);
+    __syncthreads();
+    for (int i = threadIdx.x; i < npasses * RADIX; i += blockDim.x) {
+      uint32_t v = (&sh[0][0])[i];
+      if (v) atomicAdd(&hist[i], v);
+    }
   }
   // pass p owns status[p * tiles_stride * RADIX ...]; zero the tiles this n uses
   const uint32_t ntiles = (n + SORT_TILE - 1) / SORT_TILE;
