@@ -259,6 +259,14 @@ def bench_scene(args):
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None}
     roof.update({"kernel": dominant, "avg_launch_ms": avg_ms, "launches": len(dom),
                  "algorithmic_per_launch": work_per_launch})
+    # HBM bytes per launch of this kernel, from the FETCH_SIZE / WRITE_SIZE
+    # counter runs of this same command (scaled by a copy kernel of known size)
+    tpath = os.path.join(ROOT, "profiles", f"{dominant}_scene1m_traffic.json")
+    if os.path.exists(tpath) and args.gaussians == 1_000_000 and args.res == 1024 and V == 4:
+        t = json.load(open(tpath))
+        if t.get("kernel") == dominant and "bytes_per_launch" in t:
+            roof["traffic"] = t["bytes_per_launch"]
+            roof["traffic_source"] = os.path.relpath(tpath, ROOT)
     del bt
     torch.cuda.empty_cache()
 
