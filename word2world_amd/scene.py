@@ -69,3 +69,27 @@ def activate(scene_params: dict):
     rots = torch.nn.functional.normalize(scene_params["quaternion"])
     sh = torch.cat((scene_params["sh_dc"], scene_params["sh_ac"]), dim=1)
     return scene_params["means"], scales, rots, opac, sh
+
+
+def save_scene(path: str, scene: GaussianScene) -> None:
+    """Writes the raw parameters as a Gaussian PLY (GaussianIO.save_as_ply)."""
+    from .ply import save_ply
+    save_ply(path, scene.means, scene.sh_dc, scene.sh_ac, scene.opacity, scene.svec, scene.quaternion)
+
+
+def load_scene(path: str, sh_degree: int | None = None) -> GaussianScene:
+    """Reads a Gaussian PLY.  The SH degree follows from the AC coefficient
+    count ((degree+1)^2 - 1 per channel) unless given."""
+    from .ply import load_ply
+    d = load_ply(path)
+    K = int(d["sh_ac"].shape[1])
+    if sh_degree is None:
+        deg = int(round((K + 1) ** 0.5)) - 1
+        if (deg + 1) ** 2 != K + 1:
+            raise ValueError(f"{K} AC coefficients per channel is not a full SH degree; pass sh_degree")
+        sh_degree = deg
+    elif (sh_degree + 1) ** 2 > K + 1:
+        raise ValueError(f"degree {sh_degree} needs {(sh_degree + 1) ** 2 - 1} AC coefficients, file has {K}")
+    t = lambda a: torch.from_numpy(a)
+    return GaussianScene(t(d["means"]), t(d["svec"]), t(d["quaternion"]), t(d["opacity"]).reshape(-1, 1),
+                         t(d["sh_dc"]), t(d["sh_ac"]), sh_degree)

@@ -26,7 +26,8 @@ import numpy as np
 import torch
 
 from . import _lib, batch, raster
-from .scene import GaussianScene, activate
+from .ply import save_ply
+from .scene import GaussianScene, activate, load_scene
 from .train import LRS, PlainTrainer, TrainConfig, mean_lr
 
 GROUPS = ("means", "sh", "opacity", "svec", "quaternion")
@@ -137,6 +138,11 @@ class ScenePool:
         self.targets = target_scenes
         self.target_buf = None
         self._target_index = {}
+
+    @classmethod
+    def from_ply(cls, paths: list, target_paths: list, device) -> "ScenePool":
+        """A pool of scenes (and target scenes) read from Gaussian PLY files."""
+        return cls([load_scene(p) for p in paths], [load_scene(p) for p in target_paths], device)
 
     def build_targets(self, requests: list, tile: int = 16):
         """Render each optimisation request's target views once, into one
@@ -434,6 +440,15 @@ class PlainServer:
             if log and len(finished) % 20 == 0:
                 log(f"plain: {len(finished)}/{len(queue)} t={r.done:.2f}s")
         return finished
+
+
+def save_result_ply(path: str, result: dict, M: int) -> None:
+    """Writes an optimisation request's final parameters (`Request.result` of
+    a ContinuousServer with keep_results) as a Gaussian PLY."""
+    n = result["means"].numel() // 3
+    sh = result["sh"].reshape(n, M, 3)
+    save_ply(path, result["means"].reshape(n, 3), sh[:, :1], sh[:, 1:], result["opacity"].reshape(n),
+             result["svec"].reshape(n, 3), result["quaternion"].reshape(n, 4))
 
 
 def latency_stats(finished: list) -> dict:

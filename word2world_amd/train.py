@@ -30,7 +30,8 @@ import numpy as np
 import torch
 
 from . import _lib, raster
-from .scene import GaussianScene
+from .ply import save_ply
+from .scene import GaussianScene, load_scene
 
 GROUPS = ("means", "sh_dc", "sh_ac", "opacity", "svec", "quaternion")
 LRS = {"means": 0.01, "sh_dc": 0.01, "sh_ac": 0.0005, "opacity": 0.05, "svec": 0.005,
@@ -62,6 +63,17 @@ class GaussianStore:
         self.step = 0
         self.grad_accum = torch.zeros(self.n, device=self.device)
         self.grad_count = torch.zeros(self.n, device=self.device)
+
+    @classmethod
+    def from_ply(cls, path: str, device, sh_degree: int | None = None) -> "GaussianStore":
+        """A store initialised from a Gaussian PLY (fresh Adam state)."""
+        sc = load_scene(path, sh_degree)
+        return cls(sc.params(), sc.sh_degree, device)
+
+    def save_ply(self, path: str) -> None:
+        """The current raw parameters in the format of GaussianIO.save_as_ply."""
+        v = {k: t.detach() for k, t in self.views().items()}
+        save_ply(path, v["means"], v["sh_dc"], v["sh_ac"], v["opacity"], v["svec"], v["quaternion"])
 
     # ---- layout ----
     def sizes(self) -> list[int]:
