@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define W2W_ABI_VERSION 2
+#define W2W_ABI_VERSION 3
 
 enum {
   W2W_OK = 0,
@@ -265,6 +265,73 @@ int w2w_adam_ranges(float* params, const float* grads, float* exp_avg, float* ex
                     const void* ranges, int32_t n_ranges, int64_t total, const float* hyper,
                     float beta1, float beta2, float eps, void* stream);
 
+/* ---- adaptive density control on the device (one adc_cycle,
+ * gaussians_handler.py:185-194) ----
+ * A store is described group by group: for each of n_groups (<= 8) groups,
+ * the floats per row and the device address of row 0 of the parameters and
+ * of both Adam moments (groups of n rows, row-major).  means / opacity /
+ * svec / quat name the groups that hold the means (3 floats), opacity
+ * logit (1), log-scales (3) and unnormalised quaternion (4); the others
+ * (SH) are copied as they are.  The trainer's store is [means | sh_dc |
+ * sh_ac | opacity | svec | quaternion], the batched store [means | sh |
+ * opacity | svec | quaternion]; one scene of a multi-scene store is the
+ * same table with each pointer advanced to the scene's first row.
+ * grad_accum / grad_count: n floats of the densification statistics
+ * (gaussians_handler.py:116-118, see w2w_densify_stats).
+ *
+ * Clone (Densification.py:33-52): |accum/(count+1e-5)| >= grad_thresh and any
+ * exp(svec) <= clone_scale_max; the copy is appended.  Split
+ * (Densification.py:56-107), on the rows after cloning: |g| >= grad_thresh
+ * and any scale > split_scale_min; split_factor children are appended block
+ * by block, mean = R(q) (scale * z) + mean, svec = log(scale /
+ * (split_factor * split_shrink)), and the split rows removed.  Prune
+ * (Densification.py:167-187): sigmoid(logit) < min_opacity or max scale >
+ * prune_scale_max.  Rows appended by clone or split get zero Adam moments,
+ * surviving rows keep theirs (OptimizerManager.py:23-45).  The trainer's
+ * values (GaussianTrainer.py:125, gaussians_handler.py:131-183) are
+ * clone 0.04, split 4, factor 2, shrink 0.8, prune scale 0.4. */
+typedef struct W2WStoreGroups {
+  int32_t n_groups;
+  int32_t width[8];
+  int32_t means, opacity, svec, quat;
+  float* param[8];
+  float* exp_avg[8];
+  float* exp_avg_sq[8];
+} W2WStoreGroups;
+typedef struct W2WDensifyConfig {
+  float grad_thresh, min_opacity, clone_scale_max, split_scale_min, prune_scale_max, split_shrink;
+  int32_t split_factor; /* 1..8 */
+  int32_t pad;
+} W2WDensifyConfig;
+/* Replaces Densification.py:33-187 with OptimizerManager.py:23-45.  Plan:
+ * per-row decisions and output positions into `temp`
+ * (w2w_densify_temp_bytes(n) bytes, device; reads the parameters only) and,
+ * into `counts` (8 int64, device): [0] clones, [1] splits, [2] rows removed
+ * by the prune phase, [3] rows out, [4] rows in.  Apply (same store, cfg and
+ * temp): writes the new store of n_out = counts[3] rows into `out` (same
+ * group table, other memory).  `normals`: standard normal samples
+ * [normal_rows][3], at least split_factor * counts[1] rows; child k of the
+ * s-th split row (rows in the order after cloning) uses row
+ * k * counts[1] + s, the order torch.normal fills the reference's
+ * [factor * S, 3] sample tensor.  Apply writes nothing when n_out differs
+ * from the plan's count.  The statistics restart at zero for the n_out rows
+ * (gaussians_handler.py:190-191). */
+size_t w2w_densify_temp_bytes(int64_t n);
+/* The statistics for a batch (gaussians_handler.py:116-118, update_filter =
+ * radii > 0 as Renderer.py:172): for every Gaussian and every frame of its
+ * scene with the backward flag, in frame order, where radii[item] > 0:
+ * count[row] += 1 and accum[row] += sqrt(gx*gx + gy*gy) of
+ * means2D_grad[item] (the w2w_backward_batch output); row = the scene's
+ * first parameter row + Gaussian.  Counts read on the device: capturable. */
+int w2w_densify_stats(const void* table, const W2WCaps* caps, const float* means2D_grad,
+                      const int32_t* radii, float* accum, float* count, void* stream);
+int w2w_densify_plan(const W2WStoreGroups* store, int64_t n, const float* grad_accum,
+                     const float* grad_count, const W2WDensifyConfig* cfg, void* temp,
+                     size_t temp_bytes, int64_t* counts, void* stream);
+int w2w_densify_apply(const W2WStoreGroups* store, int64_t n, const W2WDensifyConfig* cfg,
+                      const void* temp, const float* normals, int64_t normal_rows,
+                      const W2WStoreGroups* out, int64_t n_out, void* stream);
+
 /* Optional per-kernel timing: HIP event pairs recorded on the launch stream
  * around every launch whose kernel id bit is set in `kernel_mask`, into a
  * pool of `capacity` records (0 disables and frees it).  Host-side state,
@@ -274,7 +341,7 @@ enum {
   W2W_KID_SCAN = 4, W2W_KID_EMIT = 5, W2W_KID_TSORT_HIST = 6, W2W_KID_TSORT_PASS = 7,
   W2W_KID_RANGES = 8, W2W_KID_BLEND = 9, W2W_KID_BLEND_BWD = 10, W2W_KID_PREPROCESS_BWD = 11,
   W2W_KID_ADAM = 12, W2W_KID_SORT_HIST = 13, W2W_KID_SORT_PASS = 14, W2W_KID_ACTIVATE = 15,
-  W2W_KID_LOSS = 16, W2W_KID_ACTIVATE_BWD = 17, W2W_KID_SH_BWD = 18,
+  W2W_KID_LOSS = 16, W2W_KID_ACTIVATE_BWD = 17, W2W_KID_SH_BWD = 18, W2W_KID_DENSIFY = 19,
 };
 int w2w_prof_enable(int32_t capacity, uint32_t kernel_mask);
 int w2w_prof_reset(void);

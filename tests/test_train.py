@@ -1,6 +1,6 @@
 """Step-level harness (row A17): fused Adam against torch.optim.Adam (the
 library oracle for the trainer's optimiser), the flat parameter store, the
-mean learning-rate schedule and densification with Adam-state remap."""
+mean learning-rate schedule (densification: test_densify.py)."""
 import math
 
 import numpy as np
@@ -26,83 +26,6 @@ def test_store_views_roundtrip():
     for k in GROUPS:
         np.testing.assert_array_equal(v[k].detach().numpy(), sc.params()[k].numpy())
     assert st.group_ends()[-1] == st.flat.numel() == 50 * (3 + 3 + 45 + 1 + 3 + 4)
-
-
-def _store_with(n, svec, opacity_logit, accum, count):
-    sc = make_scene(n, 3, 5)
-    p = sc.params()
-    p["svec"] = torch.tensor(svec, dtype=torch.float32)
-    p["opacity"] = torch.tensor(opacity_logit, dtype=torch.float32).reshape(n, 1)
-    st = GaussianStore(p, 3, "cpu")
-    st.grad_accum = torch.tensor(accum, dtype=torch.float32)
-    st.grad_count = torch.tensor(count, dtype=torch.float32)
-    # distinguishable Adam moments per row
-    for t, base in ((st.exp_avg, 1.0), (st.exp_avg_sq, 2.0)):
-        vv = st.views(t)
-        for k in GROUPS:
-            vv[k].copy_((base + torch.arange(n, dtype=torch.float32)).view(-1, *[1] * (vv[k].dim() - 1))
-                        .expand_as(vv[k]))
-    return st, p
-
-
-def test_densify_clone_split_prune_and_adam_remap():
-    """Densification.py semantics: clone = grad >= thr & any scale <= 0.04;
-    split = grad >= thr & any scale > 4 (originals removed, 2 samples with
-    svec log(s/1.6)); prune = sigmoid(opacity) < thr | max scale > 0.4.
-    New rows get zero Adam moments, surviving rows keep theirs."""
-    n = 6
-    ls = math.log
-    svec = [[ls(0.01)] * 3,          # 0: small, high grad -> clone
-            [ls(5.0), ls(0.1), ls(0.1)],  # 1: large (>4), high grad -> split (then its copies prune: >0.4)
-            [ls(0.01)] * 3,          # 2: small, low grad -> kept
-            [ls(0.01)] * 3,          # 3: low opacity -> pruned
-            [ls(0.2)] * 3,           # 4: medium, high grad -> neither (0.04 < s <= 4)
-            [ls(0.5)] * 3]           # 5: max scale 0.5 > 0.4 -> pruned
-    opac = [2.0, 2.0, 2.0, -8.0, 2.0, 2.0]
-    accum = [1.0, 1.0, 1e-6, 0.0, 1.0, 0.0]
-    count = [1.0] * 6
-    st, p = _store_with(n, svec, opac, accum, count)
-    g = torch.Generator().manual_seed(0)
-    n_clone, n_split, n_prune = st.densify(2e-4, 0.005, generator=g)
-    assert (n_clone, n_split) == (1, 1)
-    # after clone: rows 0..5 + clone of 0 (row 6); split removes row 1 and
-    # appends two halves (scale 5/1.6 = 3.125 > 0.4 -> both pruned with rows 3, 5)
-    assert n_prune == 4
-    # survivors in order: originals 0, 2, 4 and the clone of row 0 (scale 0.01)
-    assert st.n == 4
-    v = st.views()
-    np.testing.assert_array_equal(v["means"].detach().numpy(), p["means"][[0, 2, 4, 0]].numpy())
-    np.testing.assert_array_equal(st.views(st.exp_avg)["means"][:, 0].numpy(), [1.0, 3.0, 5.0, 0.0])
-
-
-def test_densify_remap_keeps_moments():
-    n = 4
-    ls = math.log
-    svec = [[ls(0.01)] * 3, [ls(0.01)] * 3, [ls(0.2)] * 3, [ls(0.2)] * 3]
-    opac = [2.0, -8.0, 2.0, 2.0]          # row 1 pruned by opacity
-    accum = [1.0, 0.0, 0.0, 0.0]          # row 0 cloned
-    st, p = _store_with(n, svec, opac, accum, [1.0] * 4)
-    st.densify(2e-4, 0.005)
-    # rows: 0, 2, 3, clone(0)
-    assert st.n == 4
-    v = st.views()
-    np.testing.assert_array_equal(v["means"].detach().numpy(),
-                                  p["means"][[0, 2, 3, 0]].numpy())
-    m1 = st.views(st.exp_avg)["svec"][:, 0].numpy()
-    m2 = st.views(st.exp_avg_sq)["svec"][:, 0].numpy()
-    np.testing.assert_array_equal(m1, [1.0, 3.0, 4.0, 0.0])
-    np.testing.assert_array_equal(m2, [2.0, 4.0, 5.0, 0.0])
-    assert st.grad_accum.shape == (4,) and float(st.grad_accum.sum()) == 0.0
-
-
-def test_split_samples_statistics():
-    """Split children: svec = log(s/1.6); means = mean + R @ N(0, s)."""
-    n = 1
-    st, p = _store_with(n, [[math.log(6.0), math.log(5.0), math.log(4.5)]], [2.0], [1.0], [1.0])
-    st.densify(2e-4, 1e-9)  # children have scale > 0.4 -> pruned; check before prune instead
-    assert st.n == 0
-    # the child svec rule, checked directly
-    assert math.log(6.0 / (2 * 0.8)) == pytest.approx(math.log(3.75))
 
 
 @pytest.mark.gpu

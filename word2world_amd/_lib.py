@@ -46,17 +46,32 @@ class W2WCaps(ctypes.Structure):
                 ("tiles", ctypes.c_int64), ("pixels", ctypes.c_int64), ("capacity", ctypes.c_int64)]
 
 
+class W2WStoreGroups(ctypes.Structure):
+    _fields_ = [("n_groups", ctypes.c_int32), ("width", ctypes.c_int32 * 8), ("means", ctypes.c_int32),
+                ("opacity", ctypes.c_int32), ("svec", ctypes.c_int32), ("quat", ctypes.c_int32),
+                ("param", ctypes.c_void_p * 8), ("exp_avg", ctypes.c_void_p * 8),
+                ("exp_avg_sq", ctypes.c_void_p * 8)]
+
+
+class W2WDensifyConfig(ctypes.Structure):
+    _fields_ = [("grad_thresh", ctypes.c_float), ("min_opacity", ctypes.c_float),
+                ("clone_scale_max", ctypes.c_float), ("split_scale_min", ctypes.c_float),
+                ("prune_scale_max", ctypes.c_float), ("split_shrink", ctypes.c_float),
+                ("split_factor", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
 EXPORTS = ["w2w_version", "w2w_splat_extents", "w2w_block_touches", "w2w_workspace_layout", "w2w_batch_workspace_layout",
            "w2w_batch_table_bytes", "w2w_batch_encode", "w2w_forward_batch", "w2w_backward_batch", "w2w_forward", "w2w_backward", "w2w_sorted_keys",
            "w2w_sort_temp_bytes", "w2w_sort_pairs_u64", "w2w_scan_temp_bytes",
            "w2w_exclusive_scan_u32", "w2w_prof_enable", "w2w_prof_reset", "w2w_prof_read", "w2w_stats_enable", "w2w_adam_step", "w2w_activate", "w2w_activate_backward",
            "w2w_l2_loss", "w2w_adam_step_hyper", "w2w_activate_rows", "w2w_activate_backward_rows",
-           "w2w_adam_ranges"]
+           "w2w_adam_ranges", "w2w_densify_temp_bytes", "w2w_densify_plan", "w2w_densify_apply",
+           "w2w_densify_stats"]
 
 KERNEL_IDS = {0: "fill", 1: "preprocess", 2: "depth_sort_hist", 3: "depth_sort_pass", 4: "scan",
               5: "emit", 6: "tile_sort_hist", 7: "tile_sort_pass", 8: "ranges", 9: "blend",
               10: "blend_bwd", 11: "preprocess_bwd", 12: "adam", 13: "sort_hist", 14: "sort_pass",
-              15: "activate", 16: "loss", 17: "activate_bwd", 18: "sh_bwd"}
+              15: "activate", 16: "loss", 17: "activate_bwd", 18: "sh_bwd", 19: "densify"}
 
 _lib = None
 
@@ -132,6 +147,15 @@ def load(path: str = LIB_PATH):
     L.w2w_adam_ranges.restype = ctypes.c_int
     L.w2w_adam_ranges.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int64, _vp,
                                   ctypes.c_float, ctypes.c_float, ctypes.c_float, _vp]
+    L.w2w_densify_temp_bytes.restype = ctypes.c_size_t
+    L.w2w_densify_temp_bytes.argtypes = [ctypes.c_int64]
+    L.w2w_densify_plan.restype = ctypes.c_int
+    L.w2w_densify_plan.argtypes = [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp, ctypes.c_size_t, _vp, _vp]
+    L.w2w_densify_apply.restype = ctypes.c_int
+    L.w2w_densify_apply.argtypes = [_vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int64, _vp,
+                                    ctypes.c_int64, _vp]
+    L.w2w_densify_stats.restype = ctypes.c_int
+    L.w2w_densify_stats.argtypes = [_vp] * 6 + [_vp]
     _lib = L
     return L
 

@@ -65,6 +65,23 @@ class BatchedStore:
         self._ring_ev = [None] * 4
         self._ring_i = 0
 
+    def rebuild(self, flat: torch.Tensor, exp_avg: torch.Tensor, exp_avg_sq: torch.Tensor, ns: list):
+        """Adopt a re-laid-out store (after densification): new rows per scene."""
+        self.ns = [int(n) for n in ns]
+        self.G = G = sum(self.ns)
+        self.sizes = [3 * G, 3 * self.M * G, G, 3 * G, 4 * G]
+        self.ends = np.cumsum(self.sizes).astype(np.int64)
+        self.flat, self.exp_avg, self.exp_avg_sq = flat, exp_avg, exp_avg_sq
+        self.grad = torch.zeros_like(self.flat)
+        f32 = dict(dtype=torch.float32, device=self.device)
+        self.scales = torch.empty(G, 3, **f32)
+        self.opac = torch.empty(G, **f32)
+        self.rots = torch.empty(G, 4, **f32)
+        self.g_scales = torch.empty(G, 3, **f32)
+        self.g_opac = torch.empty(G, **f32)
+        self.g_rots = torch.empty(G, 4, **f32)
+        self.g_cov = torch.empty(G, 6, **f32)
+
     def upload_hyper(self, h: np.ndarray) -> None:
         k = self._ring_i
         self._ring_i = (k + 1) % len(self._ring)
@@ -107,30 +124,29 @@ class BatchedTrainer:
     """fwd + bwd + Adam for every scene of the batch in one launch sequence."""
 
     def __init__(self, scenes: list, cameras: list, targets: list | None, device, tile: int = 16,
-                 capacity: int | None = None, tight: bool = True):
+                 capacity: int | None = None, tight: bool = True, densify_every: int = 0,
+                 densify_grad: float = 2e-4, densify_min_opacity: float = 0.005,
+                 generator: torch.Generator | None = None):
         """cameras[k]: word2world_amd.camera.Camera views of scene k;
         targets[k][v]: [3,H,W] target image (or None for render-only);
         tight: emit instances only where a Gaussian's cull extent reaches
-        (same images and gradients, fewer instances)."""
+        (same images and gradients, fewer instances);
+        densify_every: run one densification cycle (clone / split / prune
+        with the Adam remap, gaussians_handler.py:185-194) every that many
+        steps, with the statistics of gaussians_handler.py:116-118 collected
+        on the device each step (0 = never); `generator` draws the split
+        samples."""
         self.store = BatchedStore(scenes, device)
         self.device = torch.device(device)
         self.tile = tile
         self.specs = [batch.SceneSpec(s.n, s.sh_degree, self.store.M,
                                       [batch.frame_from_camera(c, tight=tight) for c in cams])
                       for s, cams in zip(scenes, cameras)]
-        items = sum(s.n * len(c) for s, c in zip(scenes, cameras))
-        cap = capacity if capacity is not None else min(MAX_CAPACITY, max(1 << 20, 8 * items))
-        self._alloc(cap)
-        if capacity is None:
-            # size the instance buffers from one forward pass: K is counted in
-            # full even when it overflows the probe capacity
-            self._forward()
-            K, ovf = (int(v) for v in self.ws.status().cpu())
-            if K >= MAX_CAPACITY:
-                raise RuntimeError(f"batch needs {K} tile instances; split it (limit {MAX_CAPACITY})")
-            want = min(MAX_CAPACITY, int(K * 1.15) + 4096)
-            if ovf or want > 2 * cap or want < cap // 2:
-                self._alloc(want)
+        self.densify_every = int(densify_every)
+        self.densify_grad, self.densify_min_opacity = densify_grad, densify_min_opacity
+        self.generator = generator
+        self.densify_log: list = []
+        self._size_workspace(capacity)
         pix = int(self.caps.pixels)
         self.target = torch.zeros(3 * pix, dtype=torch.float32, device=device)
         if targets is not None:
@@ -144,10 +160,33 @@ class BatchedTrainer:
         self.loss = torch.zeros(max(int(self.caps.frames), 1), dtype=torch.float32, device=device)
         self.tile_loss = torch.zeros(max(int(self.caps.tiles), 1), dtype=torch.float32, device=device)
         self.graph = None
+        self._recapture = False
         self.it = 0
         st = self.store
+        self._reset_stats()
         self._period = np.array([0, 3 * st.M, 0, 0, 0], np.int32)
         self._split = np.array([0, 3, 0, 0, 0], np.int32)
+
+    def _size_workspace(self, capacity: int | None = None):
+        """Workspace for the current scenes; without a fixed capacity, size the
+        instance buffers from one forward pass (K is counted in full even when
+        it overflows the probe capacity)."""
+        items = sum(sp.n * len(sp.frames) for sp in self.specs)
+        cap = capacity if capacity is not None else min(MAX_CAPACITY, max(1 << 20, 8 * items))
+        self._alloc(cap)
+        if capacity is None:
+            self._forward()
+            K, ovf = (int(v) for v in self.ws.status().cpu())
+            if K >= MAX_CAPACITY:
+                raise RuntimeError(f"batch needs {K} tile instances; split it (limit {MAX_CAPACITY})")
+            want = min(MAX_CAPACITY, int(K * 1.15) + 4096)
+            if ovf or want > 2 * cap or want < cap // 2:
+                self._alloc(want)
+
+    def _reset_stats(self):
+        G = max(self.store.G, 1) if self.densify_every else 1
+        self.accum = torch.zeros(G, dtype=torch.float32, device=self.device)
+        self.count = torch.zeros(G, dtype=torch.float32, device=self.device)
 
     def _alloc(self, capacity: int):
         self.caps = batch.caps_for(self.specs, self.tile, capacity)
@@ -169,7 +208,8 @@ class BatchedTrainer:
                                        ctypes.byref(ws.struct), st.view("means").data_ptr(),
                                        st.scales.data_ptr(), st.rots.data_ptr(), st.opac.data_ptr(),
                                        st.group(st.flat, "sh").data_ptr(), ws.color.data_ptr(),
-                                       ws.depth.data_ptr(), ws.alpha.data_ptr(), None, s),
+                                       ws.depth.data_ptr(), ws.alpha.data_ptr(),
+                                       ws.radii.data_ptr() if self.densify_every else None, s),
                    "w2w_forward_batch")
 
     def _body(self):
@@ -190,6 +230,10 @@ class BatchedTrainer:
             st.view("means", st.grad).data_ptr(), st.g_cov.data_ptr(),
             st.group(st.grad, "sh").data_ptr(), st.g_scales.data_ptr(), st.g_rots.data_ptr(), s),
             "w2w_backward_batch")
+        if self.densify_every:
+            _lib.check(L.w2w_densify_stats(ws.table.data_ptr(), ctypes.byref(self.caps),
+                                           g.means2D.data_ptr(), ws.radii.data_ptr(), self.accum.data_ptr(),
+                                           self.count.data_ptr(), s), "w2w_densify_stats")
         _lib.check(L.w2w_activate_backward(
             st.scales.data_ptr(), st.opac.data_ptr(), st.view("quaternion").data_ptr(),
             st.g_scales.data_ptr(), st.g_opac.data_ptr(), st.g_rots.data_ptr(), G,
@@ -209,13 +253,40 @@ class BatchedTrainer:
         st.upload_hyper(h)
 
     def step(self):
-        """One fwd+bwd+Adam step of every scene (graph replay when captured)."""
+        """One fwd+bwd+Adam step of every scene (graph replay when captured),
+        then the densification cycle when one is due."""
         self._upload_hyper()
         if self.graph is not None:
             self.graph.replay()
         else:
             self._body()
+            if self._recapture:
+                self._recapture = False
+                self.capture()
+        if self.densify_every and self.it % self.densify_every == 0:
+            self.densify()
         return self.loss
+
+    def densify(self):
+        """One densification cycle of every scene on the device; the store,
+        the workspace (re-sized for the new Gaussian counts) and, when one was
+        captured, the graph (re-captured after the next eager step) follow."""
+        from . import densify as dens
+        st = self.store
+        cfg = dens.config(self.densify_grad, self.densify_min_opacity)
+        p, m, v, ns, counts = dens.densify_segments([3, 3 * st.M, 1, 3, 4], (0, 2, 3, 4), st.flat,
+                                                    st.exp_avg, st.exp_avg_sq, st.ns, self.accum,
+                                                    self.count, cfg, self.generator)
+        st.rebuild(p, m, v, ns)
+        for sp, n in zip(self.specs, ns):
+            sp.n = n
+        self._size_workspace()
+        self._reset_stats()
+        if self.graph is not None:
+            self.graph = None
+            self._recapture = True
+        self.densify_log.append((self.it, counts))
+        return counts
 
     def capture(self):
         """Capture the launch sequence into a HIP graph (after one eager step

@@ -15,6 +15,7 @@
 #include "../../include/w2w_raster.h"
 #include "w2w_adam.hpp"
 #include "w2w_backward.hpp"
+#include "w2w_densify.hpp"
 #include "w2w_device.hpp"
 #include "w2w_forward.hpp"
 #include "w2w_sort.hpp"
@@ -928,6 +929,127 @@ int w2w_adam_ranges(float* params, const float* grads, float* exp_avg, float* ex
                      dim3(256), 0, s, params, grads, exp_avg, exp_avg_sq,
                      static_cast<const ParamRange*>(ranges), (uint32_t)n_ranges, (uint64_t)total,
                      hyper, (float)(1.0 - (double)beta1), beta2, (float)(1.0 - (double)beta2), eps);
+  return launch_status();
+}
+
+size_t w2w_densify_temp_bytes(int64_t n) {
+  if (n < 0) return 0;
+  const uint64_t nb = cdiv((uint64_t)n, DENS_BLOCK);
+  return align_up((size_t)n) + align_up((size_t)nb * 8 * sizeof(uint32_t)) + kAlign;
+}
+
+namespace {
+struct DensTemp {
+  uint8_t* flags;
+  uint32_t* blockcnt;
+  uint32_t* totals;
+};
+DensTemp dens_temp(void* temp, int64_t n) {
+  uint8_t* b = static_cast<uint8_t*>(temp);
+  const uint64_t nb = cdiv((uint64_t)n, DENS_BLOCK);
+  DensTemp t;
+  t.flags = b;
+  t.blockcnt = reinterpret_cast<uint32_t*>(b + align_up((size_t)n));
+  t.totals = reinterpret_cast<uint32_t*>(b + align_up((size_t)n) + align_up((size_t)nb * 8 * 4));
+  return t;
+}
+bool dens_cfg(const W2WDensifyConfig* c, int64_t n, DensifyCfg* out) {
+  if (!c || n < 0 || c->split_factor < 1 || c->split_factor > 8) return false;
+  // output rows are indexed in 32 bits
+  if ((uint64_t)n * (uint64_t)(2 + c->split_factor) >= 0xFFFFFFFFull) return false;
+  out->grad_thresh = c->grad_thresh;
+  out->min_opacity = c->min_opacity;
+  out->clone_scale_max = c->clone_scale_max;
+  out->split_scale_min = c->split_scale_min;
+  out->prune_scale_max = c->prune_scale_max;
+  out->split_div = (float)((double)c->split_factor * (double)c->split_shrink);
+  out->factor = (uint32_t)c->split_factor;
+  return out->split_div > 0.0f;
+}
+// group table -> kernel store; moments optional (plan reads parameters only)
+bool dens_store(const W2WStoreGroups* g, bool moments, DensStore* s) {
+  if (!g || g->n_groups < 1 || g->n_groups > DENS_MAX_GROUPS) return false;
+  const int roles[4] = {g->means, g->opacity, g->svec, g->quat};
+  const int widths[4] = {3, 1, 3, 4};
+  for (int r = 0; r < 4; r++)
+    if (roles[r] < 0 || roles[r] >= g->n_groups || g->width[roles[r]] != widths[r]) return false;
+  for (int k = 0; k < DENS_MAX_GROUPS; k++) {
+    const bool on = k < g->n_groups;
+    if (on && (g->width[k] < 0 || g->width[k] > 1024)) return false;
+    if (on && g->width[k] > 0 && (!g->param[k] || (moments && (!g->exp_avg[k] || !g->exp_avg_sq[k]))))
+      return false;
+    s->p[k] = on ? g->param[k] : nullptr;
+    s->m[k] = on ? g->exp_avg[k] : nullptr;
+    s->v[k] = on ? g->exp_avg_sq[k] : nullptr;
+    s->w[k] = on ? (uint32_t)g->width[k] : 0u;
+  }
+  s->ng = g->n_groups;
+  s->means = g->means;
+  s->opacity = g->opacity;
+  s->svec = g->svec;
+  s->quat = g->quat;
+  return true;
+}
+}  // namespace
+
+int w2w_densify_plan(const W2WStoreGroups* store, int64_t n, const float* grad_accum,
+                     const float* grad_count, const W2WDensifyConfig* cfg, void* temp,
+                     size_t temp_bytes, int64_t* counts, void* stream) {
+  DensifyCfg c;
+  DensStore st;
+  if (!dens_cfg(cfg, n, &c) || !temp || !counts) return W2W_ERR_INVALID;
+  if (temp_bytes < w2w_densify_temp_bytes(n)) return W2W_ERR_CAPACITY;
+  if (n > 0 && (!dens_store(store, false, &st) || !grad_accum || !grad_count)) return W2W_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  (void)hipGetLastError();
+  const DensTemp t = dens_temp(temp, n);
+  const uint32_t nb = cdiv((uint64_t)n, DENS_BLOCK);
+  ProfSpan ps(W2W_KID_DENSIFY, s);
+  if (nb) {
+    DensFlagsArgs a{st, (uint64_t)n, grad_accum, grad_count, c, t.flags, t.blockcnt};
+    hipLaunchKernelGGL(k_dens_flags, dim3(nb), dim3(DENS_BLOCK), 0, s, a);
+  }
+  hipLaunchKernelGGL(k_dens_scan, dim3(1), dim3(512), 0, s, t.blockcnt, nb, t.totals,
+                     reinterpret_cast<long long*>(counts), (uint64_t)n, c.factor);
+  return launch_status();
+}
+
+int w2w_densify_apply(const W2WStoreGroups* store, int64_t n, const W2WDensifyConfig* cfg,
+                      const void* temp, const float* normals, int64_t normal_rows,
+                      const W2WStoreGroups* out, int64_t n_out, void* stream) {
+  DensifyCfg c;
+  DensStore si, so;
+  if (!dens_cfg(cfg, n, &c) || !temp || normal_rows < 0 || n_out < 0) return W2W_ERR_INVALID;
+  if (n == 0 || n_out == 0) return W2W_OK;
+  if (!dens_store(store, true, &si) || !dens_store(out, true, &so)) return W2W_ERR_INVALID;
+  if (si.ng != so.ng || si.means != so.means || si.opacity != so.opacity || si.svec != so.svec ||
+      si.quat != so.quat)
+    return W2W_ERR_INVALID;
+  for (int k = 0; k < si.ng; k++)
+    if (si.w[k] != so.w[k]) return W2W_ERR_INVALID;
+  if (normal_rows > 0 && !normals) return W2W_ERR_INVALID;
+  if ((uint64_t)n_out > (uint64_t)n * (uint64_t)(1 + c.factor)) return W2W_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  (void)hipGetLastError();
+  const DensTemp t = dens_temp(const_cast<void*>(temp), n);
+  DensApplyArgs a{si, (uint64_t)n, t.flags, t.blockcnt, t.totals, normals, (uint64_t)normal_rows, c,
+                  so, (uint64_t)n_out};
+  ProfSpan ps(W2W_KID_DENSIFY, s);
+  hipLaunchKernelGGL(k_dens_apply, dim3(cdiv((uint64_t)n, DENS_BLOCK)), dim3(DENS_BLOCK), 0, s, a);
+  return launch_status();
+}
+
+int w2w_densify_stats(const void* table, const W2WCaps* caps, const float* means2D_grad,
+                      const int32_t* radii, float* accum, float* count, void* stream) {
+  if (!table || !caps || caps->scenes < 1 || caps->gaussians < 0 || caps->gaussians >= (int64_t)LB_VAL)
+    return W2W_ERR_INVALID;
+  if (caps->gaussians == 0) return W2W_OK;
+  if (!means2D_grad || !radii || !accum || !count) return W2W_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  (void)hipGetLastError();
+  ProfSpan ps(W2W_KID_DENSIFY, s);
+  hipLaunchKernelGGL(k_dens_stats, dim3(std::min<uint32_t>(cdiv((uint64_t)caps->gaussians, 256), 8192)),
+                     dim3(256), 0, s, table, (uint32_t)caps->scenes, means2D_grad, radii, accum, count);
   return launch_status();
 }
 

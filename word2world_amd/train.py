@@ -16,7 +16,7 @@ Follows the reference trainer:
   > 4, two samples N(0, scale) rotated about the mean, svec = log(scale/1.6),
   originals removed; prune: opacity < thr or max scale > 0.4) and the Adam
   state remap of OptimizerManager.py:23-45 (new rows get zero moments, pruned
-  rows are dropped).
+  rows are dropped), as one on-device stream compaction (w2w_densify_*).
 
 Instead of six separate leaves the six groups are views of one flat float32
 buffer, so autograd accumulates every gradient into one flat buffer and one
@@ -30,6 +30,7 @@ import numpy as np
 import torch
 
 from . import _lib, raster
+from . import densify as dens
 from .ply import save_ply
 from .scene import GaussianScene, load_scene
 
@@ -129,79 +130,29 @@ class GaussianStore:
                                        torch.zeros((), device=self.device))
 
     @torch.no_grad()
-    def _rebuild(self, rows: dict, keep: torch.Tensor | None, new_rows: dict | None):
-        """Re-layout the store: keep[i] rows stay (in order), new rows are
-        appended with zero Adam moments (OptimizerManager.py:23-45)."""
-        old = {k: v.detach() for k, v in self.views().items()}
-        om = self.views(self.exp_avg)
-        ov = self.views(self.exp_avg_sq)
-        parts, pm, pv = {}, {}, {}
-        for k in GROUPS:
-            p, m1, m2 = old[k], om[k], ov[k]
-            if keep is not None:
-                p, m1, m2 = p[keep], m1[keep], m2[keep]
-            if new_rows is not None:
-                p = torch.cat([p, new_rows[k].to(p.dtype)], 0)
-                z = torch.zeros_like(new_rows[k], dtype=p.dtype)
-                m1, m2 = torch.cat([m1, z], 0), torch.cat([m2, z], 0)
-            parts[k], pm[k], pv[k] = p, m1, m2
-        self.n = int(parts["means"].shape[0])
-        self.flat = torch.cat([parts[k].reshape(-1) for k in GROUPS]).contiguous().requires_grad_(True)
-        self.exp_avg = torch.cat([pm[k].reshape(-1) for k in GROUPS]).contiguous()
-        self.exp_avg_sq = torch.cat([pv[k].reshape(-1) for k in GROUPS]).contiguous()
-
-    @torch.no_grad()
-    def densify(self, grad_thresh: float, min_opacity: float, generator: torch.Generator | None = None):
-        """One adc_cycle (gaussians_handler.py:185-194): clone, split, reset
-        statistics, prune.  Returns (n_clone, n_split, n_prune)."""
-        dev = self.device
-        # clone (Densification.py:33-52, clone_cycle threshold 0.04)
-        v = self.views()
-        scales = torch.exp(v["svec"])
-        g = self.grad_accum / (self.grad_count + 1e-5)
-        clone = (g.abs() >= grad_thresh) & (scales <= 0.04).any(dim=-1)
-        n_clone = int(clone.sum())
-        if n_clone:
-            self._rebuild(None, None, {k: t.detach()[clone] for k, t in v.items()})
-            self.grad_accum = torch.cat([self.grad_accum, torch.zeros(n_clone, device=dev)])
-            self.grad_count = torch.cat([self.grad_count, torch.zeros(n_clone, device=dev)])
-        # split (Densification.py:56-107, split_cycle threshold 4, factor 2, shrink 0.8)
-        v = self.views()
-        scales = torch.exp(v["svec"])
-        g = self.grad_accum / (self.grad_count + 1e-5)
-        split = (g.abs() >= grad_thresh) & (scales > 4.0).any(dim=-1)
-        n_split = int(split.sum())
-        if n_split:
-            sel = {k: t.detach()[split] for k, t in v.items()}
-            rep = lambda t: t.repeat((2,) + (1,) * (t.dim() - 1))
-            q = rep(sel["quaternion"])
-            qn = q / q.norm(dim=1, keepdim=True)
-            w, x, y, z = qn.unbind(1)
-            R = torch.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
-                             2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
-                             2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)],
-                            1).view(-1, 3, 3)
-            std = rep(scales[split])
-            sample = torch.normal(torch.zeros_like(std), std, generator=generator)
-            new = {"means": (R @ sample.unsqueeze(-1)).squeeze(-1) + rep(sel["means"]),
-                   "sh_dc": rep(sel["sh_dc"]), "sh_ac": rep(sel["sh_ac"]),
-                   "opacity": rep(sel["opacity"]), "svec": torch.log(std / (2 * 0.8)),
-                   "quaternion": q}
-            self._rebuild(None, None, new)
-            keep = torch.cat([~split, torch.ones(2 * n_split, dtype=torch.bool, device=dev)])
-            self._rebuild(None, keep, None)
-        self.grad_accum = torch.zeros(self.n, device=dev)
-        self.grad_count = torch.zeros(self.n, device=dev)
-        # prune (Densification.py:167-187; prune_cycle passes svec_thresh 4 -> 0.4)
-        v = self.views()
-        opac = torch.sigmoid(v["opacity"]).squeeze(-1)
-        scales = torch.exp(v["svec"])
-        prune = (opac < min_opacity) | (scales.max(dim=1).values > 0.1 * 4)
-        n_prune = int(prune.sum())
-        if n_prune:
-            self._rebuild(None, ~prune, None)
-            self.grad_accum = self.grad_accum[~prune]
-            self.grad_count = self.grad_count[~prune]
+    def densify(self, grad_thresh: float, min_opacity: float, generator: torch.Generator | None = None,
+                normals: torch.Tensor | None = None, clone_scale_max: float = 0.04,
+                split_scale_min: float = 4.0, prune_scale_max: float = 0.1 * 4, split_factor: int = 2,
+                split_shrink: float = 0.8):
+        """One adc_cycle (gaussians_handler.py:185-194) on the device: clone,
+        split, statistics reset, prune, with the Adam-state remap of
+        OptimizerManager.py:23-45 (w2w_densify_plan / w2w_densify_apply).
+        `normals`: standard normal samples [>= split_factor * n_split, 3] for
+        the split children (drawn with `generator` when not given).  One host
+        synchronisation reads the counts.  Returns (n_clone, n_split, n_prune)."""
+        cfg = dens.config(grad_thresh, min_opacity, clone_scale_max=clone_scale_max,
+                          split_scale_min=split_scale_min, prune_scale_max=prune_scale_max,
+                          split_factor=split_factor, split_shrink=split_shrink)
+        widths = [3, 3, 3 * self.m_ac, 1, 3, 4]
+        p, m, v, ns, counts = dens.densify_segments(widths, (0, 3, 4, 5), self.flat.detach(), self.exp_avg,
+                                                    self.exp_avg_sq, [self.n], self.grad_accum,
+                                                    self.grad_count, cfg, generator, normals)
+        n_clone, n_split, n_prune = counts[0]
+        self.n = ns[0]
+        self.flat = p.requires_grad_(True)
+        self.exp_avg, self.exp_avg_sq = m, v
+        self.grad_accum = torch.zeros(self.n, device=self.device)
+        self.grad_count = torch.zeros(self.n, device=self.device)
         return n_clone, n_split, n_prune
 
 
