@@ -44,7 +44,9 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (scene1m: up to and including the first densification "
+                         "cycle; batched64: 10)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--gaussians", type=int, default=1_000_000)
     ap.add_argument("--res", type=int, default=1024)
@@ -60,6 +62,8 @@ def parse():
     ap.add_argument("--opt-steps", type=int, default=50)
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--densify-every", type=int, default=100,
+                    help="scene1m: densification cycle every this many steps (0 = never)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="after the benchmark, run a plain device copy of this many bytes three "
                          "times (a reference kernel for hardware byte counters)")
@@ -105,6 +109,7 @@ def main():
         sys.exit(2)
     if args.workload in ("batched64", "served"):
         args.tile = args.tile or 16  # the reference's tile edge
+        args.steps = 10 if args.steps is None else args.steps
         return bench_batched(args) if args.workload == "batched64" else bench_served(args)
     return bench_scene(args)
 
@@ -189,7 +194,13 @@ def bench_scene(args):
             + f" -> tile {ts}")
         del m, s_, r, o, sh
         torch.cuda.empty_cache()
-    bt = BatchedTrainer([scene], [cams_host], [targets], dev, ts)
+    bt = BatchedTrainer([scene], [cams_host], [targets], dev, ts, densify_every=args.densify_every,
+                        densify_grad=2e-4, densify_min_opacity=0.005,
+                        generator=torch.Generator(device=dev).manual_seed(0))
+    if args.steps is None:
+        # end the timed region with the first densification cycle (warmup,
+        # the profiled step and the counter step come first)
+        args.steps = max(1, args.densify_every - args.warmup - 2) if args.densify_every else 20
     for i in range(args.warmup):
         bt.step()
         torch.cuda.synchronize()
@@ -235,6 +246,10 @@ def bench_scene(args):
     t1 = time.perf_counter()
     if bt.overflow():
         raise RuntimeError("instance capacity overflowed inside the timed region")
+    dens = {"every": args.densify_every, "cycles_timed": sum(1 for it, _ in bt.densify_log
+                                                             if it > bt.it - args.steps),
+            "counts": [(it, c[0]) for it, c in bt.densify_log], "gaussians_after": int(bt.store.G)}
+    log(f"densification: {dens}")
     step_ms = np.array([a.elapsed_time(b) for a, b in ev])
     dom = [ms for name, ms in _lib.prof_read() if name == dominant]
     L.w2w_prof_enable(0, 0)
@@ -309,7 +324,9 @@ def bench_scene(args):
         "config": {"workload": f"{P} Gaussians SH3 (seeded synthetic), {V} orbital views at {R}x{R} "
                                "(elevation U[-30,30]), fwd+bwd+Adam per step, L2 to a second "
                                "seed's renders; all views of a step in one launch per stage",
-                   "gaussians": P, "views_per_step": V, "resolution": R, "tile": ts},
+                   "gaussians": P, "views_per_step": V, "resolution": R, "tile": ts,
+                   "densify_every": args.densify_every},
+        "densify": dens,
         "autotune_ms": tune,
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99))},
         "per_view": {"K": K_view, "evaluated_pairs_fwd": stats_np[0], "contributing_pairs_fwd": stats_np[1],

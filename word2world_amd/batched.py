@@ -161,6 +161,7 @@ class BatchedTrainer:
         self.tile_loss = torch.zeros(max(int(self.caps.tiles), 1), dtype=torch.float32, device=device)
         self.graph = None
         self._recapture = False
+        self._fresh_ws, self._prev_overflow = False, False
         self.it = 0
         st = self.store
         self._reset_stats()
@@ -193,6 +194,7 @@ class BatchedTrainer:
         self.ws = None
         self.grads = None
         self.ws = batch.BatchWorkspace(self.caps, self.device)
+        self.ws.status().zero_()
         self.ws.load_table(self.specs)
         self.grads = batch.BatchGrads(self.caps, 0, self.device)
 
@@ -256,6 +258,7 @@ class BatchedTrainer:
         """One fwd+bwd+Adam step of every scene (graph replay when captured),
         then the densification cycle when one is due."""
         self._upload_hyper()
+        self._fresh_ws = False
         if self.graph is not None:
             self.graph.replay()
         else:
@@ -273,6 +276,7 @@ class BatchedTrainer:
         captured, the graph (re-captured after the next eager step) follow."""
         from . import densify as dens
         st = self.store
+        self._prev_overflow = self.overflow()
         cfg = dens.config(self.densify_grad, self.densify_min_opacity)
         p, m, v, ns, counts = dens.densify_segments([3, 3 * st.M, 1, 3, 4], (0, 2, 3, 4), st.flat,
                                                     st.exp_avg, st.exp_avg_sq, st.ns, self.accum,
@@ -281,6 +285,7 @@ class BatchedTrainer:
         for sp, n in zip(self.specs, ns):
             sp.n = n
         self._size_workspace()
+        self._fresh_ws = True
         self._reset_stats()
         if self.graph is not None:
             self.graph = None
@@ -299,6 +304,11 @@ class BatchedTrainer:
         self.graph = g
 
     def overflow(self) -> bool:
+        """Whether the last step overflowed the instance capacity (its images
+        and gradients are then invalid).  After a densification cycle the
+        workspace is new: the answer then covers the step before the cycle."""
+        if self._fresh_ws:
+            return self._prev_overflow
         return bool(int(self.ws.status()[1]))
 
     def frame_images(self):
