@@ -247,7 +247,7 @@ void sort_passes(const KT* kin, const uint32_t* vin, KT* kA, uint32_t* vA, KT* k
     hipLaunchKernelGGL(k_sort_hist<KT>, dim3(hgrid), dim3(256), 0, s, hist_ready ? nullptr : kin, d_n,
                        n_cap, 0, npasses, last_bits, hist, status, ntiles, tickets);
   }
-  const uint32_t pgrid = std::min<uint32_t>(ntiles, 2048);
+  const uint32_t pgrid = std::min<uint32_t>(ntiles, 1024);  // 2 workgroups of 512 per CU (LDS)
   const KT* ks = kin;
   const uint32_t* vs = vin;
   for (int p = 0; p < npasses; p++) {
