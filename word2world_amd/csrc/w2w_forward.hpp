@@ -319,24 +319,57 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
     }
     __syncthreads();
     const int nb = (int)(last - chunk * 256 + 1);
-    const uint32_t kb = s_start[0], ke = s_end;
-    for (uint32_t k = kb + tid; k < ke; k += 256) {
-      int lo = 0, hi = nb - 1;  // largest j with s_start[j] <= k
+    const uint32_t kb = s_start[0], ke = min(s_end, capacity);
+    // each thread emits an aligned group of four consecutive instances (one
+    // 16-byte store of keys and one of ids when the group is whole): one
+    // binary search per group, then the owner and the tile advance in place
+    for (uint32_t q = (kb & ~3u) + 4 * tid; q < ke; q += 4 * 256) {
+      const uint32_t kf = max(q, kb);
+      int lo = 0, hi = nb - 1;  // largest j with s_start[j] <= kf
       while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (s_start[mid] <= k) lo = mid; else hi = mid - 1;
+        if (s_start[mid] <= kf) lo = mid; else hi = mid - 1;
       }
-      uint32_t local = k - s_start[lo];
-      uint32_t w = s_w[lo];
-      uint32_t ty = s_y0[lo] + local / w;
-      uint32_t tx = s_x0[lo] + local % w;
-      if (k < capacity) {
+      uint32_t w = s_w[lo], x0 = s_x0[lo];
+      uint32_t local = kf - s_start[lo];
+      uint32_t ty = s_y0[lo] + local / w, tx = x0 + local % w;
+      uint32_t key4[4], id4[4];
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        const uint32_t k = q + m;
+        key4[m] = 0u;
+        id4[m] = 0u;
+        if (k < kb || k >= ke) continue;
+        if (k > kf) {  // next instance: same item's next tile, or a later item's first
+          if (lo + 1 < nb && s_start[lo + 1] <= k) {
+            do lo++; while (lo + 1 < nb && s_start[lo + 1] <= k);
+            w = s_w[lo];
+            x0 = s_x0[lo];
+            ty = s_y0[lo];
+            tx = x0;
+          } else if (++tx == x0 + w) {
+            tx = x0;
+            ty++;
+          }
+        }
         const uint32_t key = s_toff[lo] + ty * s_tx[lo] + tx;
-        tile_keys[k] = key;
-        ids[k] = s_g[lo];
+        key4[m] = key;
+        id4[m] = s_g[lo];
         for (int p = 0; p < hist_passes; p++) {
           const int bits = (p == hist_passes - 1) ? hist_last_bits : 8;
           atomicAdd(&s_hist[p][(key >> (8 * p)) & ((1u << bits) - 1u)], 1u);
+        }
+      }
+      if (q >= kb && q + 3 < ke) {
+        *reinterpret_cast<uint4*>(tile_keys + q) = make_uint4(key4[0], key4[1], key4[2], key4[3]);
+        *reinterpret_cast<uint4*>(ids + q) = make_uint4(id4[0], id4[1], id4[2], id4[3]);
+      } else {
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          if (q + m >= kb && q + m < ke) {
+            tile_keys[q + m] = key4[m];
+            ids[q + m] = id4[m];
+          }
         }
       }
     }
