@@ -98,3 +98,55 @@ def test_autotuner_picks_a_tile_and_tiles_agree(gpu):
     for ts in (8, 32):
         d = (imgs[ts] - imgs[16]).abs()
         assert float((d > 1e-5).float().mean()) < 0.01 and float(d.max()) < 0.5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("offset", [0, 1])
+def test_adam_hyper_matches_per_element_reference(gpu, offset):
+    """w2w_adam_step_hyper (device scalars, a second rate along each SH row)
+    against torch's Adam update written out per group, bit for bit: the
+    16-byte path (offset 0) and the scalar path (arrays offset by one float);
+    n not a multiple of 4, group ends not multiples of 4, an empty group."""
+    import ctypes
+    from word2world_amd import _lib
+    L = _lib.load()
+    torch.manual_seed(0)
+    G, M = 1001, 16
+    sizes = [3 * G, 3 * M * G, 0, G, 3 * G, 4 * G]
+    ends = np.cumsum(sizes).astype(np.int64)
+    n = int(ends[-1])
+    mk = lambda: torch.randn(n + 4, device=gpu)
+    p, g, m, v = mk(), mk(), mk() * 0.1, mk().abs() * 0.01
+    period = np.array([0, 3 * M, 0, 0, 0, 0], np.int32)
+    split = np.array([0, 3, 0, 0, 0, 0], np.int32)
+    step, b1, b2, eps = 7, 0.9, 0.999, 1e-15
+    lrs = [0.01, 0.01, 0.3, 0.05, 0.005, 0.005]
+    h = np.zeros(32, np.float32)
+    for k, lr in enumerate(lrs):
+        h[k] = -lr / (1 - b1 ** step)
+    h[8 + 1] = -0.0005 / (1 - b1 ** step)
+    h[16], h[17], h[18], h[19], h[20] = 1 / np.sqrt(1 - b2 ** step), 1 - b1, b2, 1 - b2, eps
+    hyper = torch.from_numpy(h).to(gpu)
+    sl = lambda t: t[offset:offset + n]
+    # reference: the same float32 expressions, per group, in torch
+    ns = torch.empty(n, device=gpu)
+    start = 0
+    for k, e in enumerate(ends):
+        ns[start:e] = float(h[k])
+        if period[k]:
+            idx = torch.arange(int(e) - start, device=gpu)
+            ns[start:e] = torch.where(idx % int(period[k]) >= int(split[k]), torch.tensor(float(h[8 + k]), device=gpu),
+                                      ns[start:e])
+        start = int(e)
+    P, Gr, Mo, Vo = (sl(t).clone() for t in (p, g, m, v))
+    Mo = Mo + torch.tensor(np.float32(1 - b1), device=gpu) * (Gr - Mo)
+    Vo = Vo * torch.tensor(np.float32(b2), device=gpu) + (torch.tensor(np.float32(1 - b2), device=gpu) * Gr) * Gr
+    den = torch.sqrt(Vo) * hyper[16] + hyper[20]
+    P = P + ns * (Mo / den)
+    st = L.w2w_adam_step_hyper(sl(p).data_ptr(), sl(g).data_ptr(), sl(m).data_ptr(), sl(v).data_ptr(), n,
+                               len(sizes), ends.ctypes.data, period.ctypes.data, split.ctypes.data,
+                               hyper.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    assert st == 0
+    torch.cuda.synchronize()
+    assert torch.equal(sl(m), Mo) and torch.equal(sl(v), Vo)
+    assert torch.equal(sl(p), P)

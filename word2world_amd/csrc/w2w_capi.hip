@@ -877,9 +877,15 @@ int w2w_adam_step_hyper(float* params, const float* grads, float* exp_avg, float
   if (group_end[n_groups - 1] != n) return W2W_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
   (void)hipGetLastError();
-  const uint32_t grid = (uint32_t)std::min<uint64_t>(((uint64_t)n + 255) / 256, 8192);
+  const bool vec = ((reinterpret_cast<uintptr_t>(params) | reinterpret_cast<uintptr_t>(grads) |
+                     reinterpret_cast<uintptr_t>(exp_avg) | reinterpret_cast<uintptr_t>(exp_avg_sq)) & 15u) == 0;
+  const uint64_t work = vec ? ((uint64_t)n + 3) / 4 : (uint64_t)n;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((work + 255) / 256, 8192);
   ProfSpan ps(W2W_KID_ADAM, s);
-  hipLaunchKernelGGL(k_adam_hyper, dim3(grid), dim3(256), 0, s, a);
+  if (vec)
+    hipLaunchKernelGGL(k_adam_hyper<true>, dim3(grid), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_adam_hyper<false>, dim3(grid), dim3(256), 0, s, a);
   return launch_status();
 }
 

@@ -182,27 +182,67 @@ struct AdamGraphArgs {
   const float* hyper;
 };
 
+W2W_DEV void adam_one(float& p, float g, float& m, float& v, float ns, float inv_bc2_sqrt, float one_m_b1,
+                      float b2, float one_m_b2, float eps) {
+  m = m + one_m_b1 * (g - m);
+  v = v * b2 + (one_m_b2 * g) * g;
+  const float denom = __builtin_sqrtf(v) * inv_bc2_sqrt + eps;
+  p = p + ns * (m / denom);
+}
+
+// Step size of element i of the store: its group's rate, or the group's
+// second rate past `split` within each period.  `r` carries (i - group
+// start) mod period from the previous element of the same thread.
+struct AdamRate {
+  int s = -1;
+  uint32_t r = 0;
+  W2W_DEV float at(const AdamGraphArgs& a, uint64_t i) {
+    if (s >= 0 && (s == a.nseg - 1 || i < a.seg_end[s])) {
+      if (a.lr2[s] >= 0 && ++r == a.period[s]) r = 0;
+    } else {
+      s = 0;
+#pragma unroll
+      for (int k = 0; k < ADAM_MAX_SEGMENTS - 1; k++)
+        if (k < a.nseg - 1 && i >= a.seg_end[k]) s = k + 1;
+      if (a.lr2[s] >= 0) {
+        const uint64_t start = s == 0 ? 0 : a.seg_end[s - 1];
+        r = (uint32_t)((i - start) % a.period[s]);
+      }
+    }
+    return (a.lr2[s] >= 0 && r >= a.split[s]) ? a.hyper[a.lr2[s]] : a.hyper[s];
+  }
+};
+
+// VEC: four consecutive elements per thread with 16-byte accesses (all four
+// arrays 16-byte aligned); the group and its period position are found once
+// per thread and then advanced.
+template <bool VEC>
 __global__ __launch_bounds__(256) void k_adam_hyper(AdamGraphArgs a) {
   const float inv_bc2_sqrt = a.hyper[16], one_m_b1 = a.hyper[17], b2 = a.hyper[18];
   const float one_m_b2 = a.hyper[19], eps = a.hyper[20];
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
-    int s = 0;
-#pragma unroll
-    for (int k = 0; k < ADAM_MAX_SEGMENTS - 1; k++)
-      if (k < a.nseg - 1 && i >= a.seg_end[k]) s = k + 1;
-    float ns = a.hyper[s];
-    if (a.lr2[s] >= 0) {
-      const uint64_t start = s == 0 ? 0 : a.seg_end[s - 1];
-      if ((uint32_t)((i - start) % a.period[s]) >= a.split[s]) ns = a.hyper[a.lr2[s]];
-    }
-    const float g = a.g[i];
-    float m = a.m[i];
-    float v = a.v[i];
-    m = m + one_m_b1 * (g - m);
-    v = v * b2 + (one_m_b2 * g) * g;
-    const float denom = __builtin_sqrtf(v) * inv_bc2_sqrt + eps;
-    a.p[i] = a.p[i] + ns * (m / denom);
+  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t n4 = VEC ? a.n / 4 : 0;
+  for (uint64_t q = t0; q < n4; q += stride) {
+    float4 P = reinterpret_cast<float4*>(a.p)[q];
+    const float4 G = reinterpret_cast<const float4*>(a.g)[q];
+    float4 M = reinterpret_cast<float4*>(a.m)[q];
+    float4 V = reinterpret_cast<float4*>(a.v)[q];
+    AdamRate rate;
+    const uint64_t i0 = 4 * q;
+    adam_one(P.x, G.x, M.x, V.x, rate.at(a, i0), inv_bc2_sqrt, one_m_b1, b2, one_m_b2, eps);
+    adam_one(P.y, G.y, M.y, V.y, rate.at(a, i0 + 1), inv_bc2_sqrt, one_m_b1, b2, one_m_b2, eps);
+    adam_one(P.z, G.z, M.z, V.z, rate.at(a, i0 + 2), inv_bc2_sqrt, one_m_b1, b2, one_m_b2, eps);
+    adam_one(P.w, G.w, M.w, V.w, rate.at(a, i0 + 3), inv_bc2_sqrt, one_m_b1, b2, one_m_b2, eps);
+    reinterpret_cast<float4*>(a.p)[q] = P;
+    reinterpret_cast<float4*>(a.m)[q] = M;
+    reinterpret_cast<float4*>(a.v)[q] = V;
+  }
+  for (uint64_t i = 4 * n4 + t0; i < a.n; i += stride) {  // the last n mod 4 elements
+    AdamRate rate;
+    float p = a.p[i], m = a.m[i], v = a.v[i];
+    adam_one(p, a.g[i], m, v, rate.at(a, i), inv_bc2_sqrt, one_m_b1, b2, one_m_b2, eps);
+    a.p[i] = p;
     a.m[i] = m;
     a.v[i] = v;
   }
