@@ -150,3 +150,52 @@ def test_adam_hyper_matches_per_element_reference(gpu, offset):
     torch.cuda.synchronize()
     assert torch.equal(sl(m), Mo) and torch.equal(sl(v), Vo)
     assert torch.equal(sl(p), P)
+
+
+@pytest.mark.gpu
+def test_adam_ranges_matches_per_element_reference(gpu):
+    """w2w_adam_ranges (per-job row ranges of a resident store, each with its
+    own hyper block and SH period) against the same update written out per
+    range, bit for bit; ranges out of store order, one empty."""
+    from word2world_amd import _lib
+    from word2world_amd.serve import PARAM_RANGE
+    L = _lib.load()
+    torch.manual_seed(1)
+    N = 50_000
+    p, g = torch.randn(N, device=gpu), torch.randn(N, device=gpu)
+    m, v = torch.randn(N, device=gpu) * 0.1, torch.randn(N, device=gpu).abs() * 0.01
+    spec = [(30_000, 4_321, 0, 48, 3), (100, 0, 1, 0, 0), (5, 17_000, 1, 0, 0), (40_000, 9_999, 2, 9, 3)]
+    rng = np.zeros(len(spec), PARAM_RANGE)
+    off = 0
+    for k, (start, ln, hy, per, spl) in enumerate(spec):
+        rng[k] = (start, off, ln, hy, per, spl, 0)
+        off += ln
+    total = off
+    h = np.zeros(12, np.float32)
+    for k in range(3):
+        h[4 * k:4 * k + 3] = (-0.01 * (k + 1), -0.0005 * (k + 1), 1.0 / np.sqrt(1 - 0.999 ** (k + 2)))
+    hyper = torch.from_numpy(h).to(gpu)
+    ranges = torch.from_numpy(rng.view(np.uint8)).to(gpu)
+    b1, b2, eps = 0.9, 0.999, 1e-15
+    # the entry point takes float beta1/beta2 and forms 1 - beta from them
+    c1 = torch.tensor(np.float32(1.0 - float(np.float32(b1))), device=gpu)
+    c2 = torch.tensor(np.float32(1.0 - float(np.float32(b2))), device=gpu)
+    P, Mo, Vo = p.clone(), m.clone(), v.clone()
+    for (start, ln, hy, per, spl) in spec:
+        if ln == 0:
+            continue
+        sl = slice(start, start + ln)
+        ns = torch.full((ln,), float(h[4 * hy]), device=gpu)
+        if per:
+            e = torch.arange(ln, device=gpu)
+            ns = torch.where(e % per >= spl, torch.tensor(float(h[4 * hy + 1]), device=gpu), ns)
+        gi = g[sl]
+        Mo[sl] = Mo[sl] + c1 * (gi - Mo[sl])
+        Vo[sl] = Vo[sl] * torch.tensor(np.float32(b2), device=gpu) + (c2 * gi) * gi
+        den = torch.sqrt(Vo[sl]) * hyper[4 * hy + 2] + torch.tensor(np.float32(eps), device=gpu)
+        P[sl] = P[sl] + ns * (Mo[sl] / den)
+    st = L.w2w_adam_ranges(p.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(), ranges.data_ptr(),
+                           len(spec), total, hyper.data_ptr(), b1, b2, eps, torch.cuda.current_stream().cuda_stream)
+    assert st == 0
+    torch.cuda.synchronize()
+    assert torch.equal(m, Mo) and torch.equal(v, Vo) and torch.equal(p, P)

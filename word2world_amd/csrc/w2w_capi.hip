@@ -931,7 +931,7 @@ int w2w_adam_ranges(float* params, const float* grads, float* exp_avg, float* ex
   hipStream_t s = (hipStream_t)stream;
   (void)hipGetLastError();
   ProfSpan ps(W2W_KID_ADAM, s);
-  hipLaunchKernelGGL(k_adam_ranges, dim3(std::min<uint32_t>(cdiv((uint64_t)total, 256), 8192)),
+  hipLaunchKernelGGL(k_adam_ranges, dim3(std::min<uint32_t>(cdiv((uint64_t)total, 1024), 8192)),
                      dim3(256), 0, s, params, grads, exp_avg, exp_avg_sq,
                      static_cast<const ParamRange*>(ranges), (uint32_t)n_ranges, (uint64_t)total,
                      hyper, (float)(1.0 - (double)beta1), beta2, (float)(1.0 - (double)beta2), eps);

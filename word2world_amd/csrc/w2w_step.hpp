@@ -267,29 +267,45 @@ W2W_DEV uint32_t range_of(const ParamRange* r, uint32_t n, uint64_t w) {
   return lo;
 }
 
+// Each thread takes four work elements 64 apart inside a 256-element block
+// (loads stay coalesced across the wave): one range search for the first,
+// then the range only advances (ranges are long runs of the store).
 __global__ __launch_bounds__(256) void k_adam_ranges(float* __restrict__ p, const float* __restrict__ g,
                                                      float* __restrict__ m, float* __restrict__ v,
                                                      const ParamRange* __restrict__ ranges,
                                                      uint32_t n_ranges, uint64_t total,
                                                      const float* __restrict__ hyper, float one_m_b1,
                                                      float b2, float one_m_b2, float eps) {
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < total; w += stride) {
-    const ParamRange& r = ranges[range_of(ranges, n_ranges, w)];
-    const uint64_t e = w - r.off;
-    const uint64_t i = r.start + e;
-    const float* h = hyper + 4 * r.hyper;
-    float ns = h[0];
-    if (r.period && (uint32_t)(e % r.period) >= r.split) ns = h[1];
-    const float gi = g[i];
-    float mi = m[i];
-    float vi = v[i];
-    mi = mi + one_m_b1 * (gi - mi);
-    vi = vi * b2 + (one_m_b2 * gi) * gi;
-    const float denom = __builtin_sqrtf(vi) * h[2] + eps;
-    p[i] = p[i] + ns * (mi / denom);
-    m[i] = mi;
-    v[i] = vi;
+  const uint64_t nblk = (total + 255) / 256;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint64_t blk = (uint64_t)blockIdx.x * 4 + wave; blk < nblk; blk += (uint64_t)gridDim.x * 4) {
+    const uint64_t w0 = blk * 256 + lane;
+    if (w0 >= total) continue;
+    uint32_t ri = range_of(ranges, n_ranges, w0);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint64_t w = w0 + 64 * j;
+      if (w >= total) break;
+      while (w >= ranges[ri].off + ranges[ri].len) ri++;
+      const ParamRange& r = ranges[ri];
+      const uint64_t e = w - r.off;
+      const uint64_t i = r.start + e;
+      const float* h = hyper + 4 * r.hyper;
+      float ns = h[0];
+      if (r.period) {
+        const uint32_t pos = (e >> 32) ? (uint32_t)(e % r.period) : (uint32_t)e % r.period;
+        if (pos >= r.split) ns = h[1];
+      }
+      const float gi = g[i];
+      float mi = m[i];
+      float vi = v[i];
+      mi = mi + one_m_b1 * (gi - mi);
+      vi = vi * b2 + (one_m_b2 * gi) * gi;
+      const float denom = __builtin_sqrtf(vi) * h[2] + eps;
+      p[i] = p[i] + ns * (mi / denom);
+      m[i] = mi;
+      v[i] = vi;
+    }
   }
 }
 
