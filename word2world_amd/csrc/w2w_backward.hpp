@@ -121,7 +121,12 @@ struct BlendBwdArgs {
 // instance's record (emission position = item offset + its tile's index in
 // the item's rectangle, prefetched with the chunk) and sets its flag; a
 // second barrier frees the slots for the next chunk.
-template <int TS>
+//
+// AUX: gradients w.r.t. the alpha and depth images are given.  Without them
+// (the training step) the alpha/depth terms and their running sums drop out;
+// the partials differ from the AUX variant's with zero inputs at most in the
+// sign of a zero.
+template <int TS, bool AUX>
 __global__ __launch_bounds__(64 * units_per_tile<TS>()) __attribute__((amdgpu_waves_per_eu(4)))
 void k_blend_bwd(BlendBwdArgs a) {
   constexpr int UPT = (int)units_per_tile<TS>();
@@ -175,8 +180,10 @@ void k_blend_bwd(BlendBwdArgs a) {
       gC0 = gcol[lp];
       gC1 = gcol[HW + lp];
       gC2 = gcol[2 * HW + lp];
-      gA = a.dL_dalpha ? a.dL_dalpha[pid] : 0.0f;
-      gD = a.dL_ddepth ? a.dL_ddepth[pid] : 0.0f;
+      if constexpr (AUX) {
+        gA = a.dL_dalpha ? a.dL_dalpha[pid] : 0.0f;
+        gD = a.dL_ddepth ? a.dL_ddepth[pid] : 0.0f;
+      }
     }
     const float bgdot = (fd.bg[0] * gC0 + fd.bg[1] * gC1) + fd.bg[2] * gC2;
     const float hw = 0.5f * (float)fd.W, hh = 0.5f * (float)fd.H;
@@ -190,8 +197,10 @@ void k_blend_bwd(BlendBwdArgs a) {
     const int nproc = (int)s_max;
 
     float T = Tf;
+    // colour (depth, alpha) accumulated behind the next contributor:
+    // acc <- a_prev c_prev + (1 - a_prev) acc, formed when a contributor is
+    // passed (the operands and order of backward.cu's update at the next one)
     float acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f, accD = 0.0f, accA = 0.0f;
-    float la = 0.0f, lc0 = 0.0f, lc1 = 0.0f, lc2 = 0.0f, ld = 0.0f;
 
     // chunk c covers [nproc - 64 (c + 1), nproc - 64 c) clipped at 0; lane l
     // holds instance start + l.  Ids two chunks ahead; records and each
@@ -266,21 +275,18 @@ void k_blend_bwd(BlendBwdArgs a) {
         const float r1 = __builtin_amdgcn_rcpf(1.0f - al);
         const float Tn = T * r1;
         const float aT = al * Tn;
-        const float a0 = la * lc0 + (1.0f - la) * acc0;
-        const float a1 = la * lc1 + (1.0f - la) * acc1;
-        const float a2 = la * lc2 + (1.0f - la) * acc2;
-        const float aD = la * ld + (1.0f - la) * accD;
-        const float aA = la + (1.0f - la) * accA;
-        float dLda = ((cr - a0) * gC0 + (cg - a1) * gC1) + (cbl - a2) * gC2;
-        dLda = dLda + (dz - aD) * gD;
-        dLda = dLda + (1.0f - aA) * gA;
+        float dLda = ((cr - acc0) * gC0 + (cg - acc1) * gC1) + (cbl - acc2) * gC2;
+        if constexpr (AUX) {
+          dLda = dLda + (dz - accD) * gD;
+          dLda = dLda + (1.0f - accA) * gA;
+        }
         dLda = dLda * Tn;
         dLda = dLda - (Tf * r1) * bgdot;
         const float aTm = contrib ? aT : 0.0f;
         const float g_r = aTm * gC0;
         const float g_g = aTm * gC1;
         const float g_b = aTm * gC2;
-        const float g_d = aTm * gD;
+        const float g_d = AUX ? aTm * gD : 0.0f;
         const bool free_alpha = contrib && oG <= 0.99f;  // a clamped alpha passes no gradient
         const float g_op = free_alpha ? G * dLda : 0.0f;
         const float dLdp = free_alpha ? (o * dLda) * G : 0.0f;
@@ -291,8 +297,14 @@ void k_blend_bwd(BlendBwdArgs a) {
         const float g_my = (-(cb * dx + cc * dy) * dLdp) * hh;
         if (contrib) {
           T = Tn;
-          acc0 = a0; acc1 = a1; acc2 = a2; accD = aD; accA = aA;
-          lc0 = cr; lc1 = cg; lc2 = cbl; ld = dz; la = al;
+          const float nl = 1.0f - al;
+          acc0 = al * cr + nl * acc0;
+          acc1 = al * cg + nl * acc1;
+          acc2 = al * cbl + nl * acc2;
+          if constexpr (AUX) {
+            accD = al * dz + nl * accD;
+            accA = al + nl * accA;
+          }
         }
         {
           const float s0 = wave_sum4(g_mx, g_my, g_ca, g_cb);  // rows: mx, ca, my, cb

@@ -231,7 +231,10 @@ template <int TS>
 void launch_blend_bwd(const BlendBwdArgs& a, uint32_t tiles_cap, hipStream_t s) {
   ProfSpan ps(W2W_KID_BLEND_BWD, s);
   const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(tiles_cap, kBlendGrid));
-  hipLaunchKernelGGL(k_blend_bwd<TS>, dim3(grid), dim3(64 * units_per_tile<TS>()), 0, s, a);
+  if (a.dL_dalpha || a.dL_ddepth)
+    hipLaunchKernelGGL((k_blend_bwd<TS, true>), dim3(grid), dim3(64 * units_per_tile<TS>()), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_blend_bwd<TS, false>), dim3(grid), dim3(64 * units_per_tile<TS>()), 0, s, a);
 }
 
 template <typename KT>
