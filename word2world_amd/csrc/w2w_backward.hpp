@@ -142,7 +142,8 @@ void k_blend_bwd(BlendBwdArgs a) {
   const uint32_t Tn = min(hdr->n_tiles, a.tiles_cap);
   const uint32_t P = min(hdr->n_items, a.items_cap);
   const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
-  const int tid = (int)threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = (int)threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: LDS addresses in scalars
   const int row = lane >> 4;
   const bool row_tail = (lane & 15) == 15;
   // which field each row holds after wave_sum4 / wave_sum2
@@ -295,15 +296,20 @@ void k_blend_bwd(BlendBwdArgs a) {
         const float g_cc = (-0.5f * dy * dy) * dLdp;
         const float g_mx = (-(ca * dx + cb * dy) * dLdp) * hw;
         const float g_my = (-(cb * dx + cc * dy) * dLdp) * hh;
-        if (contrib) {
-          T = Tn;
+        {  // selects, not branches: the state of non-contributing lanes is kept
           const float nl = 1.0f - al;
-          acc0 = al * cr + nl * acc0;
-          acc1 = al * cg + nl * acc1;
-          acc2 = al * cbl + nl * acc2;
+          const float n0 = al * cr + nl * acc0;
+          const float n1 = al * cg + nl * acc1;
+          const float n2 = al * cbl + nl * acc2;
+          T = contrib ? Tn : T;
+          acc0 = contrib ? n0 : acc0;
+          acc1 = contrib ? n1 : acc1;
+          acc2 = contrib ? n2 : acc2;
           if constexpr (AUX) {
-            accD = al * dz + nl * accD;
-            accA = al + nl * accA;
+            const float nD = al * dz + nl * accD;
+            const float nA = al + nl * accA;
+            accD = contrib ? nD : accD;
+            accA = contrib ? nA : accA;
           }
         }
         {
