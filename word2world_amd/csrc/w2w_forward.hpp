@@ -475,8 +475,8 @@ W2W_DEV bool wave_all(bool p) { return __ballot(!p) == 0ull; }
 // skip_below), so each pixel sees the same sequence of contributing splats,
 // in the same order and with the same arithmetic, as a walk over the whole
 // list.  The next chunk's ids and splat records are in flight while a chunk is
-// processed.  Waves claim blocks through an atomic ticket and are independent
-// (no barriers).
+// processed.  A workgroup claims four consecutive blocks at a time, one per
+// wave; the waves are otherwise independent.
 template <int TS>
 __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
   __shared__ Splat s_chunk[4][64];  // each wave's current chunk, for broadcast reads
@@ -489,11 +489,17 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
   Splat* const stage = s_chunk[threadIdx.x >> 6];
   const float a_min = 0x1.010102p-8f;
   uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
+  __shared__ uint32_t s_quad;
   for (;;) {
-    uint32_t unit = 0;
-    if (lane == 0) unit = atomicAdd(a.ticket, 1u);
-    unit = (uint32_t)__builtin_amdgcn_readfirstlane((int)unit);
-    if (unit >= U) break;
+    // the workgroup claims four consecutive blocks (one tile of edge 16), one
+    // per wave: the waves gather the same tile list through one CU's caches
+    if (threadIdx.x == 0) s_quad = atomicAdd(a.ticket, 1u);
+    __syncthreads();
+    const uint32_t quad = s_quad;
+    __syncthreads();
+    if (quad >= (U + 3) / 4) break;
+    const uint32_t unit = quad * 4 + (threadIdx.x >> 6);
+    if (unit >= U) continue;
     uint32_t tile;
     int ux, uy;
     const FrameDesc& fd = unit_block<TS>(frames, hdr->n_frames, unit, tile, ux, uy);
