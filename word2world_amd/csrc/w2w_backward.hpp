@@ -582,7 +582,24 @@ W2W_DEV void sh_bwd_one(const ShBwdArgs& a, const SceneDesc& sd, const FrameDesc
 #pragma unroll
   for (int k = 0; k < NB; k++) acc[k][0] = acc[k][1] = acc[k][2] = 0.0f;
   const float p[3] = {a.means[3 * gg], a.means[3 * gg + 1], a.means[3 * gg + 2]};
-  const float* cf = a.sh + sd.sh_off + (size_t)g * sd.M * 3;
+  // the coefficients used, read once (16-byte loads when the row allows)
+  constexpr int NF = 3 * NB;
+  float cf[NF];
+  {
+    const float* row = a.sh + sd.sh_off + (size_t)g * sd.M * 3;
+    if ((reinterpret_cast<uintptr_t>(row) & 15u) == 0) {
+#pragma unroll
+      for (int q = 0; q < NF / 4; q++) {
+        const float4 v = reinterpret_cast<const float4*>(row)[q];
+        cf[4 * q] = v.x; cf[4 * q + 1] = v.y; cf[4 * q + 2] = v.z; cf[4 * q + 3] = v.w;
+      }
+#pragma unroll
+      for (int k = NF / 4 * 4; k < NF; k++) cf[k] = row[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < NF; k++) cf[k] = row[k];
+    }
+  }
   float gm0 = 0.0f, gm1 = 0.0f, gm2 = 0.0f;
   for (uint32_t v = 0; v < sd.n_views; v++) {
     const uint32_t i = sd.item_off + g * sd.n_views + v;
@@ -610,13 +627,25 @@ W2W_DEV void sh_bwd_one(const ShBwdArgs& a, const SceneDesc& sd, const FrameDesc
     gm2 += (gdz - pd * z) / len;
   }
   float* dsh = a.dL_dsh + sd.sh_off + (size_t)g * sd.M * 3;
+  if ((reinterpret_cast<uintptr_t>(dsh) & 15u) == 0) {
 #pragma unroll
-  for (int k = 0; k < NB; k++) {
-    dsh[3 * k] = acc[k][0];
-    dsh[3 * k + 1] = acc[k][1];
-    dsh[3 * k + 2] = acc[k][2];
+    for (int q = 0; q < NF / 4; q++) {
+      const int f = 4 * q;
+      reinterpret_cast<float4*>(dsh)[q] = make_float4(acc[f / 3][f % 3], acc[(f + 1) / 3][(f + 1) % 3],
+                                                      acc[(f + 2) / 3][(f + 2) % 3],
+                                                      acc[(f + 3) / 3][(f + 3) % 3]);
+    }
+#pragma unroll
+    for (int f = NF / 4 * 4; f < NF; f++) dsh[f] = acc[f / 3][f % 3];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+      dsh[3 * k] = acc[k][0];
+      dsh[3 * k + 1] = acc[k][1];
+      dsh[3 * k + 2] = acc[k][2];
+    }
   }
-  for (int k = 3 * NB; k < 3 * (int)sd.M; k++) dsh[k] = 0.0f;
+  for (int k = NF; k < 3 * (int)sd.M; k++) dsh[k] = 0.0f;
   a.dL_dmeans3D[3 * (size_t)gg] += gm0;
   a.dL_dmeans3D[3 * (size_t)gg + 1] += gm1;
   a.dL_dmeans3D[3 * (size_t)gg + 2] += gm2;
