@@ -113,11 +113,33 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
       sh_basis(sd.D, x, y, z, Y);
       const int nb = (sd.D + 1) * (sd.D + 1);
       const float* c0 = a.sh + sd.sh_off + (size_t)ir.g * sd.M * 3;
-      float r0 = Y[0] * c0[0], r1 = Y[0] * c0[1], r2 = Y[0] * c0[2];
-      for (int k = 1; k < nb; k++) {
-        r0 = __builtin_fmaf(Y[k], c0[3 * k], r0);
-        r1 = __builtin_fmaf(Y[k], c0[3 * k + 1], r1);
-        r2 = __builtin_fmaf(Y[k], c0[3 * k + 2], r2);
+      // the 3 nb coefficients used, with 16-byte loads when the row allows
+      // (the loads stay inside those 3 nb floats; the array stays in registers)
+      float cf[48];
+      const int nf = 3 * nb;
+      if ((reinterpret_cast<uintptr_t>(c0) & 15u) == 0) {
+#pragma unroll
+        for (int qd = 0; qd < 12; qd++) {
+          if (4 * qd + 3 < nf) {
+            const float4 v = reinterpret_cast<const float4*>(c0)[qd];
+            cf[4 * qd] = v.x; cf[4 * qd + 1] = v.y; cf[4 * qd + 2] = v.z; cf[4 * qd + 3] = v.w;
+          } else {
+#pragma unroll
+            for (int e = 0; e < 4; e++) cf[4 * qd + e] = 4 * qd + e < nf ? c0[4 * qd + e] : 0.0f;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 48; k++) cf[k] = k < nf ? c0[k] : 0.0f;
+      }
+      float r0 = Y[0] * cf[0], r1 = Y[0] * cf[1], r2 = Y[0] * cf[2];
+#pragma unroll
+      for (int k = 1; k < 16; k++) {
+        if (k < nb) {
+          r0 = __builtin_fmaf(Y[k], cf[3 * k], r0);
+          r1 = __builtin_fmaf(Y[k], cf[3 * k + 1], r1);
+          r2 = __builtin_fmaf(Y[k], cf[3 * k + 2], r2);
+        }
       }
       rgb[0] = r0 + 0.5f;
       rgb[1] = r1 + 0.5f;
