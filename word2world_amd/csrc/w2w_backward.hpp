@@ -389,7 +389,7 @@ struct PreBwdArgs {
 // One thread per Gaussian: loops over the views of its scene in this batch,
 // recomputes each view's projection and sums the chain rule in registers, so
 // every per-Gaussian gradient is written once (no atomics).
-__global__ __launch_bounds__(256) void k_preprocess_bwd(PreBwdArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_preprocess_bwd(PreBwdArgs a) {
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
   const uint32_t G = min(hdr->n_gaussians, a.gaussians_cap);
   const uint32_t n_scenes = min(hdr->n_scenes, a.scenes_cap);
@@ -426,8 +426,13 @@ __global__ __launch_bounds__(256) void k_preprocess_bwd(PreBwdArgs a) {
           while (f8) {
             const uint32_t b = (uint32_t)__builtin_ctzll(f8) >> 3;
             f8 &= ~(0xFFull << (8 * b));
-            const float* r = a.rec + (size_t)(kw + b) * GR_FIELDS;
-            for (int f = 0; f < GR_FIELDS; f++) ga[f] += r[f];
+            const float2* r = reinterpret_cast<const float2*>(a.rec + (size_t)(kw + b) * GR_FIELDS);
+#pragma unroll
+            for (int f = 0; f < GR_FIELDS / 2; f++) {
+              const float2 rv = r[f];
+              ga[2 * f] += rv.x;
+              ga[2 * f + 1] += rv.y;
+            }
           }
         }
       }
