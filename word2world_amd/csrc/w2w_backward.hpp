@@ -326,22 +326,29 @@ void k_blend_bwd(BlendBwdArgs a) {
       }
       if (lane == 0) s_cm[w] = cm;
       __syncthreads();  // also orders this chunk's staged reads before the next chunk's writes
-      // splat `lane` of the chunk: sum its slots in (wave, row) order
+      // record of splat j: sums of its slots in (wave, row) order, written as
+      // five 8-byte pairs; thread t takes pairs t, t + 64 UPT, ... (pair p is
+      // fields 2 (p mod 5), +1 of splat p / 5).  Every wave holds the chunk's
+      // rectangles and offsets lane by lane, so the position comes by shuffle.
       uint64_t any = 0;
 #pragma unroll
       for (int v = 0; v < UPT; v++) any |= s_cm[v];
-      if ((any >> lane) & 1ull) {
-        const uint32_t rx0 = srect.x & 0xFFFFu, ry0 = srect.x >> 16, rx1 = srect.y & 0xFFFFu;
-        const uint32_t pos = soff + ((uint32_t)ty - ry0) * (rx1 - rx0) + ((uint32_t)tx - rx0);
-        if (pos < a.capacity) {
-          for (int f = w; f < GR_FIELDS; f += UPT) {
-            float sum = 0.0f;
+      const uint32_t rx0 = srect.x & 0xFFFFu, ry0 = srect.x >> 16, rx1 = srect.y & 0xFFFFu;
+      const uint32_t pos_l = soff + ((uint32_t)ty - ry0) * (rx1 - rx0) + ((uint32_t)tx - rx0);
+      for (int pr = tid; pr < 64 * (GR_FIELDS / 2); pr += 64 * UPT) {  // uniform per wave
+        const int j = pr / (GR_FIELDS / 2), fp = pr - j * (GR_FIELDS / 2);
+        const uint32_t pos = (uint32_t)__shfl((int)pos_l, j, 64);
+        if (((any >> j) & 1ull) && pos < a.capacity) {
+          float s0 = 0.0f, s1 = 0.0f;
 #pragma unroll
-            for (int v = 0; v < UPT; v++)
-              if ((s_cm[v] >> lane) & 1ull) sum += s_part[v][f][lane];
-            a.rec[(size_t)pos * GR_FIELDS + f] = sum;
+          for (int v = 0; v < UPT; v++) {
+            if ((s_cm[v] >> j) & 1ull) {
+              s0 += s_part[v][2 * fp][j];
+              s1 += s_part[v][2 * fp + 1][j];
+            }
           }
-          if (w == 0) a.flag[pos] = 1;
+          *reinterpret_cast<float2*>(a.rec + (size_t)pos * GR_FIELDS + 2 * fp) = make_float2(s0, s1);
+          if (fp == 0) a.flag[pos] = 1;
         }
       }
       __syncthreads();  // the slots and masks are rewritten by the next chunk
