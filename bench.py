@@ -55,7 +55,7 @@ def parse():
                     help="tile edge 8, 16 or 32; 0 = pick by the autotuner (single-view sweep)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-gaussians", type=int, default=250_000)
-    ap.add_argument("--workload", choices=["scene1m", "batched64", "served"], default="scene1m")
+    ap.add_argument("--workload", choices=["scene1m", "batched64", "served", "voxel"], default="scene1m")
     ap.add_argument("--scenes", type=int, default=64)
     ap.add_argument("--requests", type=int, default=120)
     ap.add_argument("--rate", type=float, default=40.0)
@@ -107,6 +107,8 @@ def main():
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the rasterizer's "
               "multi-GPU sharding is a design note only", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "voxel":
+        return bench_voxel(args)
     if args.workload in ("batched64", "served"):
         args.tile = args.tile or 16  # the reference's tile edge
         args.steps = 10 if args.steps is None else args.steps
@@ -430,6 +432,52 @@ def bench_batched(args):
                       "graph": not args.no_graph, "tile": args.tile},
            "plain_views_per_s": V / tp, "speedup_vs_plain": tp / tb,
            "latency_ms": {"p50": float(np.percentile(lat_b, 50)), "p99": float(np.percentile(lat_b, 99))}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_voxel(args):
+    """Occupancy field of the export path (extract_fields defaults,
+    utils_for_mesh.py:146: 128^3 grid, 16 blocks, relax 1.5) for the 1M-
+    Gaussian scene, against the CPU oracle (the reference's per-point loop over
+    every Gaussian) on a bounded sample."""
+    import torch
+    from word2world_amd.scene import make_scene
+    from word2world_amd.voxel import prepare_gaussians_for_extraction, voxel_field
+
+    dev = torch.device("cuda:0")
+    res, nb, relax = 128, 16, 1.5
+    sc = make_scene(args.gaussians, 3, seed=0)
+    m, covs, op, _, _ = prepare_gaussians_for_extraction(sc.means.to(dev), torch.exp(sc.svec).to(dev),
+                                                         sc.quaternion.to(dev), torch.sigmoid(sc.opacity).to(dev),
+                                                         0.005)
+    for _ in range(args.warmup):
+        voxel_field(m, covs, op, res, nb, relax)
+    torch.cuda.synchronize()
+    steps = args.steps or 5
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        occ = voxel_field(m, covs, op, res, nb, relax)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    log(f"voxel field {res}^3 over {int(m.shape[0])} Gaussians: {ms:.2f} ms")
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, ROOT)
+        from oracle import oracle
+        k = 4000  # Gaussians of the sample; 32^3 points
+        t1 = time.perf_counter()
+        oracle.voxel_field(m[:k].cpu().numpy(), covs[:k].cpu().numpy(), op[:k].cpu().numpy(), 32, nb, relax)
+        dt = time.perf_counter() - t1
+        # the oracle's work is points x Gaussians: scale to the full field
+        full_s = dt * (res ** 3 / 32 ** 3) * (int(m.shape[0]) / k)
+        cpu = {"value": 1.0 / full_s, "unit": "fields/s", "cores": 1, "kind": "port",
+               "sample": f"32^3 points x the first {k} Gaussians, {dt:.2f} s, scaled by points x Gaussians"}
+    out = {"metric": "occupancy fields/s (128^3, 16 blocks, relax 1.5)", "value": 1e3 / ms, "unit": "fields/s",
+           "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": f"occupancy field of {int(m.shape[0])} Gaussians on a {res}^3 grid",
+                      "resolution": res, "num_blocks": nb, "relax_ratio": relax},
+           "occ_max": float(occ.max()), "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 

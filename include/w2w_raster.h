@@ -332,6 +332,22 @@ int w2w_densify_apply(const W2WStoreGroups* store, int64_t n, const W2WDensifyCo
                       const void* temp, const float* normals, int64_t normal_rows,
                       const W2WStoreGroups* out, int64_t n_out, void* stream);
 
+/* ---- occupancy field for mesh export (compute_voxel_field.cu:25-79) ----
+ * occ[ix + iy*res + iz*res^2] = sum over Gaussians g whose mean lies in the
+ * relaxed box of the grid point's block of opacities[g] * exp(power), power
+ * = -0.5 d^T Sigma_g^{-1} d (0 weight when power > 0), d = point - mean;
+ * grid coordinate -1 + 2 i / (res - 1) per axis, block (int)((c + 1) /
+ * (2 / num_blocks)), relaxed box [-1 + b s - s r, -1 + (b + 1) s + s r] with
+ * s = 2 / num_blocks and r = relax_ratio; covs [n][6] = (xx, xy, xz, yy, yz,
+ * zz).  Replaces compute_voxel_field_cuda (compute_voxel_field.cu:81-124) on
+ * device pointers: res >= 2, 1 <= num_blocks <= 255, 0 <= relax_ratio <= 64;
+ * temp: w2w_voxel_temp_bytes(n, num_blocks) bytes.  Sums run in a fixed
+ * order (deterministic), not the reference's index order. */
+size_t w2w_voxel_temp_bytes(int64_t n, int32_t num_blocks);
+int w2w_voxel_field(const float* means, const float* covs, const float* opacities, int64_t n,
+                    int32_t resolution, int32_t num_blocks, float relax_ratio, float* occ, void* temp,
+                    size_t temp_bytes, void* stream);
+
 /* Optional per-kernel timing: HIP event pairs recorded on the launch stream
  * around every launch whose kernel id bit is set in `kernel_mask`, into a
  * pool of `capacity` records (0 disables and frees it).  Host-side state,
@@ -342,6 +358,7 @@ enum {
   W2W_KID_RANGES = 8, W2W_KID_BLEND = 9, W2W_KID_BLEND_BWD = 10, W2W_KID_PREPROCESS_BWD = 11,
   W2W_KID_ADAM = 12, W2W_KID_SORT_HIST = 13, W2W_KID_SORT_PASS = 14, W2W_KID_ACTIVATE = 15,
   W2W_KID_LOSS = 16, W2W_KID_ACTIVATE_BWD = 17, W2W_KID_SH_BWD = 18, W2W_KID_DENSIFY = 19,
+  W2W_KID_VOXEL = 20,
 };
 int w2w_prof_enable(int32_t capacity, uint32_t kernel_mask);
 int w2w_prof_reset(void);

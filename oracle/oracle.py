@@ -73,6 +73,8 @@ def lib():
             _c_int, _c_int, _c_int, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _c_int,
             _c_int, _c_float, _c_float, _i32p, _f32p, _f32p, _f32p, _f32p,
             _f32p, _f32p, _f32p, _f32p, _f32p]
+        L.oracle_voxel_field.restype = None
+        L.oracle_voxel_field.argtypes = [_f32p, _f32p, _f32p, _f32p, _c_int, _c_int, _c_int, _c_float]
         _lib = L
     return _lib
 
@@ -194,3 +196,16 @@ def expf(x: float) -> float:
 
 def logf(x: float) -> float:
     return float(lib().oracle_logf(float(x)))
+
+
+def voxel_field(means, covs, opacities, resolution: int, num_blocks: int, relax_ratio: float) -> np.ndarray:
+    """compute_voxel_field.cu:25-79 on the CPU: occ [res^3] indexed
+    ix + iy res + iz res^2 (reshape (res, res, res) gives [iz, iy, ix])."""
+    means, covs, opac = _f32(means).reshape(-1, 3), _f32(covs).reshape(-1, 6), _f32(opacities).reshape(-1)
+    occ = np.zeros(resolution ** 3, np.float32)
+    n = means.shape[0]
+    lib().oracle_voxel_field(occ, means if n else np.zeros((1, 3), np.float32),
+                             covs if n else np.zeros((1, 6), np.float32),
+                             opac if n else np.zeros(1, np.float32), n, resolution, num_blocks, relax_ratio)
+    return occ
+

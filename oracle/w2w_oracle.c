@@ -742,3 +742,58 @@ EXPORT int oracle_preprocess_backward(int N, int M, int D, const float* means, c
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* occupancy field for mesh export                                     */
+/* ------------------------------------------------------------------ */
+
+/* compute_3d_gaussian_coefficient_device (compute_voxel_field.cu:6-23), with
+ * libm expf standing for CUDA's expf. */
+static float vox_coef(float x, float y, float z, float a, float b, float c, float d, float e, float f) {
+    float det = a * d * f + 2 * e * c * b - e * e * a - c * c * d - b * b * f;
+    float inv_det = 1.0f / (det + 1e-24f);
+    float inv_a = (d * f - e * e) * inv_det;
+    float inv_b = (e * c - b * f) * inv_det;
+    float inv_c = (e * b - c * d) * inv_det;
+    float inv_d = (a * f - c * c) * inv_det;
+    float inv_e = (b * c - e * a) * inv_det;
+    float inv_f = (a * d - b * b) * inv_det;
+    float power = -0.5f * (x * x * inv_a + y * y * inv_d + z * z * inv_f) - x * y * inv_b - x * z * inv_c -
+                  y * z * inv_e;
+    if (power > 0) power = -1e10f;
+    return expf(power);
+}
+
+/* compute_voxel_field_kernel (compute_voxel_field.cu:25-79), one grid point
+ * at a time: every Gaussian in index order, kept when its mean lies in the
+ * relaxed box of the point's block; occ[ix + iy res + iz res^2]. */
+EXPORT void oracle_voxel_field(float* occ, const float* means, const float* covs, const float* opac, int n,
+                               int res, int num_blocks, float relax_ratio) {
+    for (int idz = 0; idz < res; idz++)
+        for (int idy = 0; idy < res; idy++)
+            for (int idx = 0; idx < res; idx++) {
+                float x = -1.0f + 2.0f * idx / (res - 1);
+                float y = -1.0f + 2.0f * idy / (res - 1);
+                float z = -1.0f + 2.0f * idz / (res - 1);
+                float block_size = 2.0f / num_blocks;
+                int block_x = (int)((x + 1.0f) / block_size);
+                int block_y = (int)((y + 1.0f) / block_size);
+                int block_z = (int)((z + 1.0f) / block_size);
+                float vmin_x = -1.0f + block_x * block_size - block_size * relax_ratio;
+                float vmax_x = -1.0f + (block_x + 1) * block_size + block_size * relax_ratio;
+                float vmin_y = -1.0f + block_y * block_size - block_size * relax_ratio;
+                float vmax_y = -1.0f + (block_y + 1) * block_size + block_size * relax_ratio;
+                float vmin_z = -1.0f + block_z * block_size - block_size * relax_ratio;
+                float vmax_z = -1.0f + (block_z + 1) * block_size + block_size * relax_ratio;
+                float val = 0.0f;
+                for (int g = 0; g < n; g++) {
+                    float mx = means[g * 3 + 0], my = means[g * 3 + 1], mz = means[g * 3 + 2];
+                    if (mx < vmin_x || mx > vmax_x || my < vmin_y || my > vmax_y || mz < vmin_z || mz > vmax_z)
+                        continue;
+                    const float* cv = covs + 6 * g;
+                    float w = vox_coef(x - mx, y - my, z - mz, cv[0], cv[1], cv[2], cv[3], cv[4], cv[5]);
+                    val += opac[g] * w;
+                }
+                occ[idx + idy * res + idz * res * res] = val;
+            }
+}

@@ -66,12 +66,13 @@ EXPORTS = ["w2w_version", "w2w_splat_extents", "w2w_block_touches", "w2w_workspa
            "w2w_exclusive_scan_u32", "w2w_prof_enable", "w2w_prof_reset", "w2w_prof_read", "w2w_stats_enable", "w2w_adam_step", "w2w_activate", "w2w_activate_backward",
            "w2w_l2_loss", "w2w_adam_step_hyper", "w2w_activate_rows", "w2w_activate_backward_rows",
            "w2w_adam_ranges", "w2w_densify_temp_bytes", "w2w_densify_plan", "w2w_densify_apply",
-           "w2w_densify_stats"]
+           "w2w_densify_stats", "w2w_voxel_temp_bytes", "w2w_voxel_field"]
 
 KERNEL_IDS = {0: "fill", 1: "preprocess", 2: "depth_sort_hist", 3: "depth_sort_pass", 4: "scan",
               5: "emit", 6: "tile_sort_hist", 7: "tile_sort_pass", 8: "ranges", 9: "blend",
               10: "blend_bwd", 11: "preprocess_bwd", 12: "adam", 13: "sort_hist", 14: "sort_pass",
-              15: "activate", 16: "loss", 17: "activate_bwd", 18: "sh_bwd", 19: "densify"}
+              15: "activate", 16: "loss", 17: "activate_bwd", 18: "sh_bwd", 19: "densify",
+              20: "voxel"}
 
 _lib = None
 
@@ -154,6 +155,11 @@ def load(path: str = LIB_PATH):
     L.w2w_densify_apply.restype = ctypes.c_int
     L.w2w_densify_apply.argtypes = [_vp, ctypes.c_int64, _vp, _vp, _vp, ctypes.c_int64, _vp,
                                     ctypes.c_int64, _vp]
+    L.w2w_voxel_temp_bytes.restype = ctypes.c_size_t
+    L.w2w_voxel_temp_bytes.argtypes = [ctypes.c_int64, ctypes.c_int32]
+    L.w2w_voxel_field.restype = ctypes.c_int
+    L.w2w_voxel_field.argtypes = [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                  _vp, _vp, ctypes.c_size_t, _vp]
     L.w2w_densify_stats.restype = ctypes.c_int
     L.w2w_densify_stats.argtypes = [_vp] * 6 + [_vp]
     _lib = L

@@ -20,6 +20,7 @@
 #include "w2w_forward.hpp"
 #include "w2w_sort.hpp"
 #include "w2w_step.hpp"
+#include "w2w_voxel.hpp"
 
 using namespace w2w;
 
@@ -1059,6 +1060,84 @@ int w2w_densify_stats(const void* table, const W2WCaps* caps, const float* means
   ProfSpan ps(W2W_KID_DENSIFY, s);
   hipLaunchKernelGGL(k_dens_stats, dim3(std::min<uint32_t>(cdiv((uint64_t)caps->gaussians, 256), 8192)),
                      dim3(256), 0, s, table, (uint32_t)caps->scenes, means2D_grad, radii, accum, count);
+  return launch_status();
+}
+
+namespace {
+struct VoxTemp {
+  uint32_t *keys0, *vals0, *kA, *vA, *kB, *vB, *hist, *status, *tickets;
+  uint2* cells;
+  int *lo, *hi;
+  int npasses, last_bits;
+  size_t bytes;
+};
+VoxTemp vox_temp(void* temp, int64_t n, int32_t nb) {
+  const uint64_t nc = (uint64_t)nb + 1, ncell = nc * nc * nc;
+  int bits = 1;
+  while ((1ull << bits) < ncell) bits++;
+  VoxTemp t;
+  t.npasses = (bits + 7) / 8;
+  t.last_bits = bits - 8 * (t.npasses - 1);
+  const size_t tiles = std::max<size_t>(cdiv((uint64_t)n, SORT_TILE), 1);
+  size_t o = 0;
+  auto take = [&](size_t b) { size_t a = o; o = align_up(o + b); return a; };
+  const size_t k0 = take(4 * (size_t)n), v0 = take(4 * (size_t)n), ka = take(4 * (size_t)n);
+  const size_t va = take(4 * (size_t)n), kb = take(4 * (size_t)n), vb = take(4 * (size_t)n);
+  const size_t hi_ = take(4 * 4 * RADIX), st = take(4 * (size_t)t.npasses * tiles * RADIX);
+  const size_t tk = take(8 * 4), ce = take(8 * ncell), lo = take(4 * nc), hh = take(4 * nc);
+  t.bytes = o;
+  if (!temp) return t;  // sizing only
+  t.keys0 = at<uint32_t>(temp, k0); t.vals0 = at<uint32_t>(temp, v0);
+  t.kA = at<uint32_t>(temp, ka); t.vA = at<uint32_t>(temp, va);
+  t.kB = at<uint32_t>(temp, kb); t.vB = at<uint32_t>(temp, vb);
+  t.hist = at<uint32_t>(temp, hi_); t.status = at<uint32_t>(temp, st); t.tickets = at<uint32_t>(temp, tk);
+  t.cells = at<uint2>(temp, ce); t.lo = at<int>(temp, lo); t.hi = at<int>(temp, hh);
+  return t;
+}
+bool vox_ok(int64_t n, int32_t res, int32_t nb, float relax) {
+  return n >= 0 && n < (int64_t)LB_VAL && res >= 2 && res <= 4096 && nb >= 1 && nb <= 255 &&
+         std::isfinite(relax) && relax >= 0.0f && relax <= 64.0f;
+}
+}  // namespace
+
+size_t w2w_voxel_temp_bytes(int64_t n, int32_t num_blocks) {
+  if (n < 0 || num_blocks < 1 || num_blocks > 255) return 0;
+  return vox_temp(nullptr, n, num_blocks).bytes;
+}
+
+int w2w_voxel_field(const float* means, const float* covs, const float* opacities, int64_t n,
+                    int32_t resolution, int32_t num_blocks, float relax_ratio, float* occ, void* temp,
+                    size_t temp_bytes, void* stream) {
+  if (!vox_ok(n, resolution, num_blocks, relax_ratio) || !occ || !temp) return W2W_ERR_INVALID;
+  if (n > 0 && (!means || !covs || !opacities)) return W2W_ERR_INVALID;
+  if (temp_bytes < w2w_voxel_temp_bytes(n, num_blocks)) return W2W_ERR_CAPACITY;
+  hipStream_t s = (hipStream_t)stream;
+  (void)hipGetLastError();
+  const VoxTemp t = vox_temp(temp, n, num_blocks);
+  VoxelGeom g;
+  g.res = resolution; g.nb = num_blocks;
+  g.block_size = 2.0f / (float)num_blocks;  // compute_voxel_field.cu:37
+  g.relax = relax_ratio;
+  const uint32_t nc = (uint32_t)num_blocks + 1, ncell = nc * nc * nc;
+  fill_u32(reinterpret_cast<uint32_t*>(t.cells), 0u, 2 * (size_t)ncell, s);
+  fill_u32(reinterpret_cast<uint32_t*>(t.lo), (uint32_t)resolution, nc, s);
+  fill_u32(reinterpret_cast<uint32_t*>(t.hi), 0u, nc, s);
+  ProfSpan ps(W2W_KID_VOXEL, s);
+  if (n > 0) {
+    const uint32_t grid = std::min<uint32_t>(cdiv((uint64_t)n, 256), 4096);
+    hipLaunchKernelGGL(k_vox_cells, dim3(grid), dim3(256), 0, s, means, (uint32_t)n, g, t.keys0, t.vals0);
+    fill_u32(t.hist, 0u, 4 * RADIX, s);
+    sort_passes<uint32_t>(t.keys0, t.vals0, t.kA, t.vA, t.kB, t.vB, nullptr, (uint32_t)n, t.npasses,
+                          t.last_bits, true, t.hist, t.status, t.tickets, W2W_KID_VOXEL, W2W_KID_VOXEL, s);
+    hipLaunchKernelGGL(k_vox_cell_ranges, dim3(grid), dim3(256), 0, s, t.kA, (uint32_t)n, t.cells);
+  }
+  hipLaunchKernelGGL(k_vox_axis, dim3(cdiv((uint64_t)resolution, 256)), dim3(256), 0, s, g, t.lo, t.hi);
+  VoxArgs va;
+  va.means = means; va.covs = covs; va.opac = opacities; va.cell_ids = t.vA; va.cell_ranges = t.cells;
+  va.axis_lo = t.lo; va.axis_hi = t.hi; va.g = g;
+  va.reach = (int)std::ceil(relax_ratio) + 2;  // every relaxed box lies in these cells (one cell of slack)
+  va.occ = occ;
+  hipLaunchKernelGGL(k_vox_field, dim3(ncell), dim3(VOX_THREADS), 0, s, va);
   return launch_status();
 }
 
