@@ -98,6 +98,56 @@ def algorithmic_cost(kernel, P, M, K, T, HW, stats):
     return "hbm", 0
 
 
+HBM_STAGES_NULL = ("fill",)  # buffer clears: no algorithmic bytes
+
+
+def stage_cost(name, P, M, V, K, T, HW, stats_view):
+    """Algorithmic bytes (or flops) of one launch of a stage over a whole
+    step: V views, K instances, T tiles and HW pixels summed over the views.
+    Stages with a per-view figure scale by V; Adam, activation, the SH
+    backward and the statistics are per Gaussian or per item."""
+    p_in = 44 + 12 * M
+    if name == "activate":
+        return "hbm", 64 * P            # svec, quaternion, opacity in; activated out
+    if name == "activate_bwd":
+        return "hbm", 96 * P            # three gradients and the parameters in, gradients out
+    if name == "depth_sort_hist":
+        return "hbm", 4 * V * P
+    if name == "loss":
+        return "hbm", 36 * HW           # render and target in, dL/dcolor out
+    if name == "sh_bwd":
+        return "hbm", P * (24 * M + 24) + 12 * V * P
+    if name == "densify":
+        return "hbm", 16 * V * P + 16 * P
+    if name == "adam":
+        return "hbm", 28 * (11 + 3 * M) * P
+    if name == "tile_sort_hist":
+        return "hbm", None              # clears look-back words; emission built the histogram
+    if name in HBM_STAGES_NULL:
+        return "hbm", None
+    bound, per_view = algorithmic_cost(name, P, M, K / V, T / V, HW / V, stats_view)
+    return bound, per_view * V if per_view else None
+
+
+def stage_table(per, counts, P, M, V, K, T, HW, stats_view):
+    """Per-stage roofline of one step and the path ratio sum(ideal) /
+    sum(measured), ideal = bytes / HBM peak or flops / FP32 peak."""
+    stages, ideal, measured = {}, 0.0, 0.0
+    for name, ms in sorted(per.items(), key=lambda kv: -kv[1]):
+        bound, work = stage_cost(name, P, M, V, K, T, HW, stats_view)
+        measured += ms
+        row = {"ms": ms, "launches": counts[name], "bound": bound}
+        if work:
+            work *= counts[name]
+            peak, unit, scale = ((FP32_PEAK_TFLOPS, "TFLOP/s", 1e12) if bound == "valu"
+                                 else (HBM_PEAK_GBS, "GB/s", 1e9))
+            achieved = work / (ms * 1e-3) / scale
+            ideal += work / (peak * scale) * 1e3
+            row.update({"achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak})
+        stages[name] = row
+    return stages, {"ideal_ms": ideal, "measured_ms": measured, "frac": ideal / measured if measured else None}
+
+
 def main():
     args = parse()
     if args.tile not in (0, 8, 16, 32):
@@ -200,9 +250,9 @@ def bench_scene(args):
                         densify_grad=2e-4, densify_min_opacity=0.005,
                         generator=torch.Generator(device=dev).manual_seed(0))
     if args.steps is None:
-        # end the timed region with the first densification cycle (warmup,
-        # the profiled step and the counter step come first)
-        args.steps = max(1, args.densify_every - args.warmup - 2) if args.densify_every else 20
+        # end the timed region with the first densification cycle (warmup
+        # and the profiled step come first)
+        args.steps = max(1, args.densify_every - args.warmup - 1) if args.densify_every else 20
     for i in range(args.warmup):
         bt.step()
         torch.cuda.synchronize()
@@ -224,14 +274,6 @@ def bench_scene(args):
     log("per-kernel ms in one step: " + ", ".join(f"{k}={v:.3f}" for k, v in
                                                   sorted(per.items(), key=lambda kv: -kv[1])))
     kid = {v: k for k, v in _lib.KERNEL_IDS.items()}[dominant]
-    # work counters for the roofline flops (one extra, untimed step)
-    stats = torch.zeros(6, dtype=torch.int64, device=dev)
-    L.w2w_stats_enable(stats.data_ptr())
-    bt.step()
-    torch.cuda.synchronize()
-    L.w2w_stats_enable(None)
-    stats_np = stats.cpu().numpy().astype(np.float64) / V  # per view
-    K_total = int(bt.ws.status()[0])
 
     # timed region: events only around the dominant kernel's launches
     L.w2w_prof_enable(8192, 1 << kid)
@@ -258,6 +300,32 @@ def bench_scene(args):
     wall = t1 - t0
     views_per_s = args.steps * V / wall
     log(f"timed {args.steps} steps: {wall * 1e3 / args.steps:.2f} ms/step, {views_per_s:.1f} views/s")
+
+    # work counters for the roofline flops: one extra, untimed step right
+    # after the timed region (the scene as the last timed steps saw it)
+    stats = torch.zeros(6, dtype=torch.int64, device=dev)
+    L.w2w_stats_enable(stats.data_ptr())
+    bt.step()
+    torch.cuda.synchronize()
+    L.w2w_stats_enable(None)
+    stats_np = stats.cpu().numpy().astype(np.float64) / V  # per view
+    K_total = int(bt.ws.status()[0])
+
+    # every stage of one more (untimed) step after the timed region, where
+    # the scene has trained as far as the timed steps
+    L.w2w_prof_enable(4096, 0xFFFFFFFF)
+    L.w2w_prof_reset()
+    bt.step()
+    torch.cuda.synchronize()
+    per_end, cnt_end = {}, {}
+    for name, ms in _lib.prof_read():
+        per_end[name] = per_end.get(name, 0.0) + ms
+        cnt_end[name] = cnt_end.get(name, 0) + 1
+    L.w2w_prof_enable(0, 0)
+    K_end = int(bt.ws.status()[0])
+    T_step = V * ((R + ts - 1) // ts) ** 2
+    stages, path = stage_table(per_end, cnt_end, int(bt.store.G), M, V, K_end, T_step, V * R * R, stats_np)
+    log(f"path: ideal {path['ideal_ms']:.3f} ms of measured {path['measured_ms']:.3f} ms per step")
 
     # roofline of the dominant kernel (one launch covers all V views)
     K_view = K_total / V
@@ -335,6 +403,8 @@ def bench_scene(args):
                      "instances_loaded_fwd": stats_np[2], "evaluated_pairs_bwd": stats_np[3],
                      "contributing_pairs_bwd": stats_np[4], "instances_loaded_bwd": stats_np[5]},
         "kernel_ms_one_step": per,
+        "stages": stages,
+        "path_roofline": path,
         "plain": plain,
         "roofline": roof,
         "cpu_baseline": cpu,

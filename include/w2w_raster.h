@@ -202,7 +202,9 @@ int w2w_sort_pairs_u64(const uint64_t* keys_in, const uint32_t* vals_in, uint64_
 
 /* Exclusive scan of uint32 values (the reference's kogge-stone scan,
  * gaussianRasterizer.cu:77-117, is inclusive: inclusive[i] = out[i] + in[i]).
- * Writes the total to *total (device).  temp: w2w_scan_temp_bytes(n). */
+ * Writes the total to *total (device).  temp: w2w_scan_temp_bytes(n).
+ * Sums are exact below 2^30; a total of 2^30 or more is reported as
+ * 2^30 - 1 (and the offsets from that point on are not meaningful). */
 size_t w2w_scan_temp_bytes(int64_t n);
 int w2w_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n,
                            void* temp, size_t temp_bytes, void* stream);

@@ -223,7 +223,7 @@ def test_sort_pairs_six_distributions(gpu, n):
         np.testing.assert_array_equal(vout.cpu().numpy().view(np.uint32), vals[order])
 
 
-@pytest.mark.parametrize("n", [1, 1000, 4096, 4097, 15000, 1_000_001])
+@pytest.mark.parametrize("n", [1, 1000, 4096, 4097, 15000, 16384, 16385, 1_000_001, 1_150_001])
 def test_exclusive_scan(gpu, n):
     from word2world_amd import _lib
     L = _lib.load()
@@ -242,3 +242,26 @@ def test_exclusive_scan(gpu, n):
     excl = np.concatenate([[0], c[:-1]]).astype(np.uint32)
     np.testing.assert_array_equal(out.cpu().numpy().view(np.uint32), excl)
     assert int(tot.cpu().numpy().view(np.uint32)[0]) == int(c[-1])
+
+
+def test_exclusive_scan_saturates(gpu):
+    """A total of 2^30 or more (the look-back words hold 30 bits) reads as
+    2^30 - 1: larger than every workspace capacity, so the instance count
+    of an oversized batch is flagged as an overflow instead of wrapping."""
+    from word2world_amd import _lib
+    L = _lib.load()
+    n = 300_000
+    x = np.full(n, 4000, dtype=np.uint32)  # total 1.2e9 > 2^30
+    xin = torch.from_numpy(x.view(np.int32)).to(gpu)
+    out = torch.empty_like(xin)
+    tot = torch.zeros(1, dtype=torch.int32, device=gpu)
+    tb = L.w2w_scan_temp_bytes(n)
+    temp = torch.empty(tb, dtype=torch.uint8, device=gpu)
+    assert L.w2w_exclusive_scan_u32(xin.data_ptr(), out.data_ptr(), tot.data_ptr(), n, temp.data_ptr(), tb,
+                                    torch.cuda.current_stream().cuda_stream) == 0
+    torch.cuda.synchronize()
+    assert int(tot.cpu().numpy().view(np.uint32)[0]) == (1 << 30) - 1
+    # below the limit the offsets are exact
+    o = out.cpu().numpy().view(np.uint32)
+    k = (1 << 30) // 4000 - 16384
+    np.testing.assert_array_equal(o[:k], 4000 * np.arange(k, dtype=np.uint64))

@@ -220,9 +220,46 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
 // asked, an overflow flag (total > capacity).  Tiles are claimed through a
 // ticket until none is left, so any grid size works; look-back words lb[]
 // and the ticket must be zero on entry.  n comes from *d_n when given.
-constexpr int SCAN_THREADS = 256;
+// Tiles are large (16384 values: few look-back links), loads and stores are
+// striped over the workgroup (coalesced) and transposed through LDS so that
+// each thread scans 16 consecutive values, and one wave looks back over 64
+// predecessors per step.
+constexpr int SCAN_THREADS = 1024;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+// Look-back by a whole wave: lane l reads the word of tile p - l (a tile
+// before the first counts as inclusive 0) and the wave adds the words up to
+// the nearest inclusive one once all of them are published.  Sums saturate
+// at LB_VAL (above every capacity), so a total too large for the 30-bit
+// field still reads as an overflow.
+W2W_DEV uint32_t scan_lookback_wave(uint32_t* lb, uint32_t tile, uint32_t tot, int lane) {
+  if (tile == 0) {
+    if (lane == 0) st_release_dev(lb, LB_INC | tot);
+    return 0u;
+  }
+  if (lane == 0) st_release_dev(lb + tile, LB_AGG | tot);
+  uint32_t excl = 0;
+  int p = (int)tile - 1;
+  for (;;) {
+    const int q = p - lane;
+    const uint32_t w = q >= 0 ? ld_acquire_dev(lb + q) : LB_INC;
+    const uint32_t flag = w & ~LB_VAL;
+    const uint64_t inc = __ballot(flag == LB_INC);
+    const uint64_t unready = __ballot(flag == 0u);
+    const int first = inc ? __builtin_ctzll(inc) : 64;
+    const uint64_t need = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+    if (unready & need) continue;  // a word up to the nearest inclusive one is not published yet
+    uint64_t v = lane <= first ? (uint64_t)(w & LB_VAL) : 0ull;  // 64 words of up to 30 bits
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    excl = (uint32_t)min<uint64_t>((uint64_t)excl + v, LB_VAL);
+    if (first < 64) break;
+    p -= 64;
+  }
+  if (lane == 0) st_release_dev(lb + tile, LB_INC | min(excl + tot, LB_VAL));
+  return excl;
+}
 
 template <bool GATHER>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan(
@@ -230,61 +267,70 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(
     uint32_t* __restrict__ out, uint32_t* __restrict__ lb, uint32_t* __restrict__ ticket,
     uint32_t* __restrict__ total_out, uint32_t* __restrict__ overflow_out,
     const uint32_t* __restrict__ d_n, uint32_t n_cap, uint32_t capacity) {
-  __shared__ uint32_t s_tmp[4];
-  __shared__ uint32_t s_tile, s_excl;
+  __shared__ uint32_t s_v[SCAN_TILE + SCAN_TILE / 32];  // one pad word per 32: conflict-free transposes
+  __shared__ uint32_t s_tmp[SCAN_THREADS / 64];
+  __shared__ uint32_t s_tile, s_excl, s_tot;
   const int tid = threadIdx.x;
   const uint32_t n = sort_count(d_n, n_cap);
   const uint32_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  auto pad = [](int i) { return i + (i >> 5); };
   for (;;) {
     if (tid == 0) s_tile = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
     if (tile >= ntiles) return;
-    const uint32_t base = tile * SCAN_TILE + tid * SCAN_ITEMS;
+    const uint32_t tb = tile * SCAN_TILE;
+    if constexpr (GATHER) {
+      uint32_t pr[SCAN_ITEMS];
+#pragma unroll
+      for (int j = 0; j < SCAN_ITEMS; j++) {
+        const uint32_t idx = tb + j * SCAN_THREADS + tid;
+        pr[j] = idx < n ? perm[idx] : 0xFFFFFFFFu;
+      }
+#pragma unroll
+      for (int j = 0; j < SCAN_ITEMS; j++) s_v[pad(j * SCAN_THREADS + tid)] = pr[j] != 0xFFFFFFFFu ? in[pr[j]] : 0u;
+    } else {
+#pragma unroll
+      for (int j = 0; j < SCAN_ITEMS; j++) {
+        const uint32_t idx = tb + j * SCAN_THREADS + tid;
+        s_v[pad(j * SCAN_THREADS + tid)] = idx < n ? in[idx] : 0u;
+      }
+    }
+    __syncthreads();
     uint32_t v[SCAN_ITEMS];
     uint32_t run = 0;
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; j++) {
-      uint32_t idx = base + j;
-      uint32_t c = 0u;
-      if (idx < n) c = GATHER ? in[perm[idx]] : in[idx];
+      const uint32_t c = s_v[pad(tid * SCAN_ITEMS + j)];
       v[j] = run;
       run += c;
     }
-    uint32_t tot;
-    uint32_t texcl = block_excl_scan_256(run, s_tmp, &tot);
-    if (tid == 0) {
-      uint32_t excl = 0;
-      if (tile == 0) {
-        st_release_dev(lb, LB_INC | tot);
-      } else {
-        st_release_dev(lb + tile, LB_AGG | tot);
-        int p = (int)tile - 1;
-        for (;;) {
-          uint32_t w = ld_acquire_dev(lb + p);
-          uint32_t flag = w & ~LB_VAL;
-          if (flag == 0) continue;
-          excl += w & LB_VAL;
-          if (flag == LB_INC) break;
-          p--;
+    const uint32_t texcl = block_excl_scan<SCAN_THREADS>(run, s_tmp);
+    // each thread rewrites only the slots it read
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) s_v[pad(tid * SCAN_ITEMS + j)] = texcl + v[j];
+    if (tid == SCAN_THREADS - 1) s_tot = texcl + run;
+    __syncthreads();
+    if (tid < 64) {
+      const uint32_t tot = min(s_tot, LB_VAL);
+      const uint32_t excl = scan_lookback_wave(lb, tile, tot, tid);
+      if (tid == 0) {
+        s_excl = excl;
+        if (tile == ntiles - 1) {
+          const uint32_t K = min(excl + tot, LB_VAL);
+          *total_out = K;
+          if (overflow_out) *overflow_out = K > capacity ? 1u : 0u;
         }
-        st_release_dev(lb + tile, LB_INC | (excl + tot));
-      }
-      s_excl = excl;
-      if (tile == ntiles - 1) {
-        uint32_t K = excl + tot;
-        *total_out = K;
-        if (overflow_out) *overflow_out = K > capacity ? 1u : 0u;
       }
     }
     __syncthreads();
-    const uint32_t e0 = s_excl + texcl;
+    const uint32_t e0 = s_excl;
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; j++) {
-      uint32_t idx = base + j;
-      if (idx < n) out[idx] = e0 + v[j];
+      const uint32_t idx = tb + j * SCAN_THREADS + tid;
+      if (idx < n) out[idx] = e0 + s_v[pad(j * SCAN_THREADS + tid)];
     }
-    __syncthreads();
+    __syncthreads();  // s_v and s_tile are rewritten by the next tile
   }
 }
 
