@@ -64,81 +64,71 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--densify-every", type=int, default=100,
                     help="scene1m: densification cycle every this many steps (0 = never)")
+    ap.add_argument("--depth-split", type=float, default=-1.0,
+                    help="depth-sliced binning: fraction of the items (nearest first) binned and "
+                         "blended first, the rest only on tiles with unsaturated pixels (same "
+                         "outputs); 0 = one binning pass, negative = pick by timing a few steps")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="after the benchmark, run a plain device copy of this many bytes three "
                          "times (a reference kernel for hardware byte counters)")
     return ap.parse_args()
 
 
-# per-kernel algorithmic bytes / flops (fp32; P_in = 44 + 12 M bytes per Gaussian)
-def algorithmic_cost(kernel, P, M, K, T, HW, stats):
-    p_in = 44 + 12 * M
-    if kernel == "preprocess":
-        return "hbm", P * (p_in + 49)
-    if kernel == "depth_sort_pass":
-        return "hbm", 16 * P
-    if kernel == "scan":
-        return "hbm", 8 * P + 4 * P
-    if kernel == "emit":
-        return "hbm", 20 * P + 8 * K
-    if kernel == "tile_sort_hist":
-        return "hbm", 4 * K
-    if kernel == "tile_sort_pass":
-        return "hbm", 16 * K
-    if kernel == "ranges":
-        return "hbm", 4 * K + 8 * T
-    if kernel == "blend":
-        E, C = stats[0], stats[1]
-        return "valu", 18 * E + 12 * C
-    if kernel == "blend_bwd":
-        E, C = stats[3], stats[4]
-        return "valu", 18 * E + 62 * C
-    if kernel == "preprocess_bwd":
-        return "hbm", P * (2 * p_in + 36)
-    return "hbm", 0
-
-
 HBM_STAGES_NULL = ("fill",)  # buffer clears: no algorithmic bytes
+SPLIT_CANDIDATES = (0.0, 0.1, 0.2, 0.3, 0.5)
 
 
-def stage_cost(name, P, M, V, K, T, HW, stats_view):
-    """Algorithmic bytes (or flops) of one launch of a stage over a whole
-    step: V views, K instances, T tiles and HW pixels summed over the views.
-    Stages with a per-view figure scale by V; Adam, activation, the SH
-    backward and the statistics are per Gaussian or per item."""
+def stage_cost(name, P, M, V, K, T, HW, stats_view, tpasses, dpasses=4):
+    """Algorithmic bytes (or flops) of a stage over a whole step (all its
+    launches): V views, K tile instances sorted (over both slices when the
+    binning is depth-sliced), T tiles and HW pixels summed over the views,
+    tile sorts of `tpasses` passes.  Adam, activation, the SH backward and
+    the statistics are per Gaussian or per item."""
     p_in = 44 + 12 * M
-    if name == "activate":
-        return "hbm", 64 * P            # svec, quaternion, opacity in; activated out
-    if name == "activate_bwd":
-        return "hbm", 96 * P            # three gradients and the parameters in, gradients out
-    if name == "depth_sort_hist":
-        return "hbm", 4 * V * P
-    if name == "loss":
-        return "hbm", 36 * HW           # render and target in, dL/dcolor out
-    if name == "sh_bwd":
-        return "hbm", P * (24 * M + 24) + 12 * V * P
-    if name == "densify":
-        return "hbm", 16 * V * P + 16 * P
-    if name == "adam":
-        return "hbm", 28 * (11 + 3 * M) * P
-    if name == "tile_sort_hist":
-        return "hbm", None              # clears look-back words; emission built the histogram
-    if name in HBM_STAGES_NULL:
-        return "hbm", None
-    bound, per_view = algorithmic_cost(name, P, M, K / V, T / V, HW / V, stats_view)
-    return bound, per_view * V if per_view else None
+    I = V * P  # items
+    table = {
+        "activate": ("hbm", 64 * P),            # svec, quaternion, opacity in; activated out
+        "activate_bwd": ("hbm", 96 * P),        # three gradients and the parameters in, gradients out
+        "preprocess": ("hbm", I * (p_in + 49)),
+        "depth_sort_hist": ("hbm", 4 * I),
+        "depth_sort_pass": ("hbm", 16 * I * dpasses),
+        "scan": ("hbm", 12 * I),
+        "emit": ("hbm", 20 * I + 8 * K),
+        "tile_sort_hist": ("hbm", None),        # clears look-back words; emission built the histogram
+        "tile_sort_pass": ("hbm", 16 * K * tpasses),
+        "ranges": ("hbm", 4 * K + 8 * T),
+        "loss": ("hbm", 36 * HW),               # render and target in, dL/dcolor out
+        "preprocess_bwd": ("hbm", I * (2 * p_in + 36)),
+        "sh_bwd": ("hbm", P * (24 * M + 24) + 12 * I),
+        "densify": ("hbm", 16 * I + 16 * P),
+        "adam": ("hbm", 28 * (11 + 3 * M) * P),
+    }
+    if name == "blend":
+        return "valu", V * (18 * stats_view[0] + 12 * stats_view[1])
+    if name == "blend_bwd":
+        return "valu", V * (18 * stats_view[3] + 62 * stats_view[4])
+    return table.get(name, ("hbm", None))
 
 
-def stage_table(per, counts, P, M, V, K, T, HW, stats_view):
+# kernels of the far slice (depth-sliced binning) belong to these stages
+STAGE_OF = {"blend_far": "blend", "emit_far": "emit", "count_open": "emit"}
+
+
+def stage_table(per, counts, P, M, V, K, T, HW, stats_view, tpasses):
     """Per-stage roofline of one step and the path ratio sum(ideal) /
     sum(measured), ideal = bytes / HBM peak or flops / FP32 peak."""
+    folded, fcounts = {}, {}
+    for name, ms in per.items():
+        st = STAGE_OF.get(name, name)
+        folded[st] = folded.get(st, 0.0) + ms
+        fcounts[st] = fcounts.get(st, 0) + counts[name]
+    per, counts = folded, fcounts
     stages, ideal, measured = {}, 0.0, 0.0
     for name, ms in sorted(per.items(), key=lambda kv: -kv[1]):
-        bound, work = stage_cost(name, P, M, V, K, T, HW, stats_view)
+        bound, work = stage_cost(name, P, M, V, K, T, HW, stats_view, tpasses)
         measured += ms
         row = {"ms": ms, "launches": counts[name], "bound": bound}
         if work:
-            work *= counts[name]
             peak, unit, scale = ((FP32_PEAK_TFLOPS, "TFLOP/s", 1e12) if bound == "valu"
                                  else (HBM_PEAK_GBS, "GB/s", 1e9))
             achieved = work / (ms * 1e-3) / scale
@@ -246,9 +236,29 @@ def bench_scene(args):
             + f" -> tile {ts}")
         del m, s_, r, o, sh
         torch.cuda.empty_cache()
+    split_ms = None
+    if args.depth_split < 0:
+        # depth split: time a few trainer steps per candidate (the outputs are
+        # the same for every choice)
+        split_ms = {}
+        for frac in SPLIT_CANDIDATES:
+            t = BatchedTrainer([scene], [cams_host], [targets], dev, ts, depth_split=frac)
+            for _ in range(30):  # past the first steps, where the scene changes fastest
+                t.step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(10):
+                t.step()
+            torch.cuda.synchronize()
+            split_ms[frac] = (time.perf_counter() - t0) / 10 * 1e3
+            del t
+            torch.cuda.empty_cache()
+        args.depth_split = min(split_ms, key=split_ms.get)
+        log("depth split (ms per step): " + ", ".join(f"{k}: {v:.3f}" for k, v in split_ms.items())
+            + f" -> {args.depth_split}")
     bt = BatchedTrainer([scene], [cams_host], [targets], dev, ts, densify_every=args.densify_every,
                         densify_grad=2e-4, densify_min_opacity=0.005,
-                        generator=torch.Generator(device=dev).manual_seed(0))
+                        generator=torch.Generator(device=dev).manual_seed(0), depth_split=args.depth_split)
     if args.steps is None:
         # end the timed region with the first densification cycle (warmup
         # and the profiled step come first)
@@ -323,16 +333,19 @@ def bench_scene(args):
         cnt_end[name] = cnt_end.get(name, 0) + 1
     L.w2w_prof_enable(0, 0)
     K_end = int(bt.ws.status()[0])
+    _, KA, KB, _, KH, KN = (int(v) for v in bt.ws.sliced_counts().cpu())
+    K_sorted = KN + KB if bt.caps.depth_split else K_end
     T_step = V * ((R + ts - 1) // ts) ** 2
-    stages, path = stage_table(per_end, cnt_end, int(bt.store.G), M, V, K_end, T_step, V * R * R, stats_np)
+    tpasses = (max(1, int(np.ceil(np.log2(T_step)))) + 7) // 8
+    stages, path = stage_table(per_end, cnt_end, int(bt.store.G), M, V, K_sorted, T_step, V * R * R, stats_np,
+                               tpasses)
     log(f"path: ideal {path['ideal_ms']:.3f} ms of measured {path['measured_ms']:.3f} ms per step")
 
     # roofline of the dominant kernel (one launch covers all V views)
     K_view = K_total / V
-    T_tiles = ((R + ts - 1) // ts) ** 2
-    bound, work_view = algorithmic_cost(dominant, P, M, K_view, T_tiles, R * R, stats_np)
+    bound, work_step = stage_cost(dominant, int(bt.store.G), M, V, K_sorted, T_step, V * R * R, stats_np, tpasses)
     launches_per_step = max(1, round(len(dom) / args.steps))
-    work_per_launch = work_view * V / launches_per_step
+    work_per_launch = (work_step or 0.0) / launches_per_step
     avg_ms = float(np.mean(dom)) if dom else float("nan")
     if bound == "valu":
         achieved = work_per_launch / (avg_ms * 1e-3) / 1e12
@@ -402,7 +415,11 @@ def bench_scene(args):
         "per_view": {"K": K_view, "evaluated_pairs_fwd": stats_np[0], "contributing_pairs_fwd": stats_np[1],
                      "instances_loaded_fwd": stats_np[2], "evaluated_pairs_bwd": stats_np[3],
                      "contributing_pairs_bwd": stats_np[4], "instances_loaded_bwd": stats_np[5]},
+        "binning": {"depth_split": args.depth_split, "instances_sorted_per_view": K_sorted / V,
+                    "near_per_view": KA / V, "hinted_far_per_view": KH / V, "far_per_view": KB / V,
+                    "split_ms_per_step": split_ms},
         "kernel_ms_one_step": per,
+        "kernel_ms_last_step": per_end,
         "stages": stages,
         "path_roofline": path,
         "plain": plain,
@@ -569,7 +586,7 @@ def bench_served(args):
     reqs = serve.make_requests(args.requests, args.rate, sizes, seed=1, opt_steps=args.opt_steps)
     pool.build_targets(reqs)
     c = serve._lib.W2WCaps()
-    c.scenes, c.frames, c.tile, c.pad = 128, 512, args.tile, 0
+    c.scenes, c.frames, c.tile, c.depth_split = 128, 512, args.tile, 0
     c.items, c.gaussians = 48_000_000, 12_000_000
     c.tiles, c.pixels, c.capacity = 2_000_000, 160_000_000, 400_000_000
     srv = serve.ContinuousServer(pool, 12_000_000, c, dev, args.tile)

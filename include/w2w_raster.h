@@ -103,7 +103,13 @@ typedef struct W2WLayout {
 typedef struct W2WCaps {
   int32_t scenes, frames;  /* descriptor slots in the batch table */
   int32_t tile;            /* tile edge: 8, 16 or 32 */
-  int32_t pad;
+  int32_t depth_split;     /* 0: one binning pass.  1..999: depth-sliced
+                            * binning (no reference counterpart; same
+                            * outputs): the nearest depth_split/1000 of the
+                            * items (in depth order) are binned and blended
+                            * first, the rest only on tiles that still have
+                            * unsaturated pixels.  Needs a larger workspace
+                            * (w2w_batch_workspace_layout). */
   int64_t items;           /* (frame, Gaussian) pairs */
   int64_t gaussians;       /* Gaussians over all scenes */
   int64_t tiles;           /* tiles over all frames */
@@ -360,7 +366,7 @@ enum {
   W2W_KID_RANGES = 8, W2W_KID_BLEND = 9, W2W_KID_BLEND_BWD = 10, W2W_KID_PREPROCESS_BWD = 11,
   W2W_KID_ADAM = 12, W2W_KID_SORT_HIST = 13, W2W_KID_SORT_PASS = 14, W2W_KID_ACTIVATE = 15,
   W2W_KID_LOSS = 16, W2W_KID_ACTIVATE_BWD = 17, W2W_KID_SH_BWD = 18, W2W_KID_DENSIFY = 19,
-  W2W_KID_VOXEL = 20,
+  W2W_KID_VOXEL = 20, W2W_KID_COUNT_OPEN = 21, W2W_KID_EMIT_FAR = 22, W2W_KID_BLEND_FAR = 23,
 };
 int w2w_prof_enable(int32_t capacity, uint32_t kernel_mask);
 int w2w_prof_reset(void);

@@ -165,3 +165,67 @@ def test_tight_emission_same_images_and_gradients(gpu):
     assert torch.equal(T0, T1)
     for name in ("means2D", "colors", "opacity", "means3D", "cov3D", "sh", "scales", "rotations"):
         assert torch.equal(getattr(g0, name), getattr(g1, name)), name
+
+
+def _dense_setup(dev, n=60000, res=96):
+    """A scene dense enough that most tiles saturate early (the case the
+    depth-sliced binning is for): the benchmark scene's distribution."""
+    arrs = scene_np(n, 1, 5, 1.5)
+    fr = [batch.frame_from_camera(orbit_camera(az, el, 2.5, res, res + 8), tight=True)
+          for az, el in ((0.0, 0.0), (70.0, 20.0), (200.0, -25.0))]
+    scenes = [batch.SceneSpec(n, 1, arrs[4].shape[1], fr)]
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    packed = (t(arrs[0]), t(arrs[1]), t(arrs[2]), t(arrs[3].reshape(-1)), t(arrs[4].reshape(-1)))
+    return scenes, packed
+
+
+def _render_and_grads(scenes, packed, dev, tile, split, capacity=3_000_000, forwards=1):
+    caps = batch.caps_for(scenes, tile, capacity, depth_split=split)
+    ws = batch.BatchWorkspace(caps, dev)
+    ws.load_table(scenes)
+    for _ in range(forwards):  # later forwards use the earlier ones' tile hint
+        batch.forward(ws, *packed)
+    g = torch.Generator(device=dev).manual_seed(0)
+    dLdc = torch.randn(ws.color.shape, generator=g, device=dev)
+    dLdd = torch.randn(ws.depth.shape, generator=g, device=dev)
+    dLda = torch.randn(ws.alpha.shape, generator=g, device=dev)
+    grads = batch.BatchGrads(ws.caps, int(ws.totals[5]), dev)
+    batch.backward(ws, grads, packed[0], packed[1], packed[2], packed[4], dLdc, dLdd, dLda)
+    torch.cuda.synchronize()
+    assert int(ws.status()[1]) == 0
+    counts = [int(v) for v in ws.sliced_counts().cpu()]
+    return ws, grads, counts
+
+
+@pytest.mark.parametrize("dense", [False, True])
+@pytest.mark.parametrize("tile,split,forwards", [(16, 0.3, 1), (16, 0.3, 2), (8, 0.3, 2), (16, 0.05, 1),
+                                                 (8, 0.7, 2), (32, 0.5, 2), (16, 0.001, 2), (8, 0.999, 1)])
+def test_depth_sliced_binning_is_exact(gpu, dense, tile, split, forwards):
+    """Near lists first (the near slice, plus the far items on the tiles the
+    previous forward left unfinished), far lists only on the other tiles
+    with unsaturated pixels: every image, alpha, depth, final T, n_contrib
+    and gradient (with depth and alpha gradients given) is bit-identical to
+    one binning pass, with or without a hint.  Tile edges 8 and 16 here need
+    two and one tile-sort passes (both buffer parities of the far sort)."""
+    print(f"dense={dense} tile={tile} split={split} forwards={forwards}", flush=True)
+    scenes, packed = _dense_setup(gpu) if dense else _setup(gpu)[0::3]
+    ref_ws, ref_g, _ = _render_and_grads(scenes, packed, gpu, tile, 0.0)
+    ws, g, (S, KA, KB, n_open, KH, KN) = _render_and_grads(scenes, packed, gpu, tile, split, forwards=forwards)
+    K = int(ref_ws.status()[0])
+    assert int(ws.status()[0]) == K
+    P = int(ws.totals[1])
+    assert S == min(P, -(-P * int(round(1000 * split)) // 1000))
+    assert KN == KA + KH and 0 <= KA <= K and KN + KB <= K
+    if forwards == 1:
+        assert KH == 0  # no hint before the first forward
+    print(f"K={K} near={KA} hinted={KH} far={KB} open blocks={n_open} (near items {S} of {P})", flush=True)
+    for a, b in zip(ref_ws.frame_views(scenes), ws.frame_views(scenes)):
+        assert all(torch.equal(x, y) for x, y in zip(a, b))
+    assert torch.equal(ref_ws.final_T(), ws.final_T())
+    n = int(ws.totals[4])
+    o = ws.layout.n_contrib
+    assert torch.equal(ref_ws.image[o:o + 4 * n], ws.image[o:o + 4 * n])
+    for name in ("means2D", "colors", "opacity", "means3D", "cov3D", "sh", "scales", "rotations"):
+        assert torch.equal(getattr(ref_g, name), getattr(g, name)), name
+    if dense and split == 0.3:
+        assert KN + KB < K  # tiles saturated by the near lists drop their far instances

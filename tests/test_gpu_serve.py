@@ -22,7 +22,7 @@ def _pool(dev):
 
 def _caps(pool):
     c = serve._lib.W2WCaps()
-    c.scenes, c.frames, c.tile, c.pad = 16, 64, 16, 0
+    c.scenes, c.frames, c.tile, c.depth_split = 16, 64, 16, 0
     c.items, c.gaussians, c.tiles, c.pixels, c.capacity = 200_000, 40_000, 8192, 64 * 96 * 96, 4_000_000
     return c
 

@@ -42,7 +42,7 @@ class W2WLayout(ctypes.Structure):
 
 class W2WCaps(ctypes.Structure):
     _fields_ = [("scenes", ctypes.c_int32), ("frames", ctypes.c_int32), ("tile", ctypes.c_int32),
-                ("pad", ctypes.c_int32), ("items", ctypes.c_int64), ("gaussians", ctypes.c_int64),
+                ("depth_split", ctypes.c_int32), ("items", ctypes.c_int64), ("gaussians", ctypes.c_int64),
                 ("tiles", ctypes.c_int64), ("pixels", ctypes.c_int64), ("capacity", ctypes.c_int64)]
 
 
@@ -72,7 +72,7 @@ KERNEL_IDS = {0: "fill", 1: "preprocess", 2: "depth_sort_hist", 3: "depth_sort_p
               5: "emit", 6: "tile_sort_hist", 7: "tile_sort_pass", 8: "ranges", 9: "blend",
               10: "blend_bwd", 11: "preprocess_bwd", 12: "adam", 13: "sort_hist", 14: "sort_pass",
               15: "activate", 16: "loss", 17: "activate_bwd", 18: "sh_bwd", 19: "densify",
-              20: "voxel"}
+              20: "voxel", 21: "count_open", 22: "emit_far", 23: "blend_far"}
 
 _lib = None
 

@@ -54,7 +54,9 @@ def frame_from_camera(cam, background=None, backward=True, tight=False) -> Frame
 
 
 def caps_for(scenes: list, tile: int, capacity: int, scenes_cap: int | None = None,
-             frames_cap: int | None = None) -> _lib.W2WCaps:
+             frames_cap: int | None = None, depth_split: float = 0.0) -> _lib.W2WCaps:
+    """Capacities of a batch.  depth_split in (0, 1): depth-sliced binning
+    with that fraction of the items (nearest first) in the near slice."""
     g = sum(s.n for s in scenes)
     items = sum(s.n * len(s.frames) for s in scenes)
     frames = [f for s in scenes for f in s.frames]
@@ -63,7 +65,10 @@ def caps_for(scenes: list, tile: int, capacity: int, scenes_cap: int | None = No
     c = _lib.W2WCaps()
     c.scenes = scenes_cap or max(len(scenes), 1)
     c.frames = frames_cap or max(len(frames), 1)
-    c.tile, c.pad = tile, 0
+    c.tile = tile
+    c.depth_split = int(round(1000 * depth_split)) if 0.0 < depth_split < 1.0 else 0
+    if 0.0 < depth_split < 1.0:
+        c.depth_split = min(max(c.depth_split, 1), 999)
     c.items, c.gaussians, c.tiles, c.pixels, c.capacity = items, g, max(tiles, 1), max(pix, 1), capacity
     return c
 
@@ -117,7 +122,9 @@ class BatchWorkspace:
         self.device = torch.device(device)
         self.layout = _lib.batch_layout(caps)
         L = self.layout
-        self.geom = torch.empty(max(L.geom_bytes, 1), dtype=torch.uint8, device=device)
+        # zeroed once: depth-sliced binning keeps a tile hint here between
+        # forwards (any content is correct, an empty one is the neutral start)
+        self.geom = torch.zeros(max(L.geom_bytes, 1), dtype=torch.uint8, device=device)
         self.binning = torch.empty(max(L.binning_bytes, 1), dtype=torch.uint8, device=device)
         self.image = torch.empty(max(L.image_bytes, 1), dtype=torch.uint8, device=device)
         nbytes = _lib.load().w2w_batch_table_bytes(caps.scenes, caps.frames)
@@ -148,7 +155,14 @@ class BatchWorkspace:
         return totals
 
     def status(self) -> torch.Tensor:
+        """[K (all instances of the batch), overflow flag]."""
         return self.geom[:8].view(torch.int32)
+
+    def sliced_counts(self) -> torch.Tensor:
+        """Depth-sliced binning, last forward: [near depth positions, their
+        instances, far instances on open unhinted tiles, unfinished blocks of
+        unhinted tiles, far instances on hinted tiles, near-list instances]."""
+        return self.geom[8:32].view(torch.int32)
 
     def final_T(self) -> torch.Tensor:
         """Final transmittance of every pixel of the encoded batch."""

@@ -126,7 +126,7 @@ class BatchedTrainer:
     def __init__(self, scenes: list, cameras: list, targets: list | None, device, tile: int = 16,
                  capacity: int | None = None, tight: bool = True, densify_every: int = 0,
                  densify_grad: float = 2e-4, densify_min_opacity: float = 0.005,
-                 generator: torch.Generator | None = None):
+                 generator: torch.Generator | None = None, depth_split: float = 0.0):
         """cameras[k]: word2world_amd.camera.Camera views of scene k;
         targets[k][v]: [3,H,W] target image (or None for render-only);
         tight: emit instances only where a Gaussian's cull extent reaches
@@ -135,8 +135,11 @@ class BatchedTrainer:
         with the Adam remap, gaussians_handler.py:185-194) every that many
         steps, with the statistics of gaussians_handler.py:116-118 collected
         on the device each step (0 = never); `generator` draws the split
-        samples."""
+        samples; depth_split in (0, 1): depth-sliced binning (the nearest
+        fraction of the items binned and blended first, the rest only on
+        tiles with unsaturated pixels; same images and gradients)."""
         self.store = BatchedStore(scenes, device)
+        self.depth_split = float(depth_split)
         self.device = torch.device(device)
         self.tile = tile
         self.specs = [batch.SceneSpec(s.n, s.sh_degree, self.store.M,
@@ -190,7 +193,7 @@ class BatchedTrainer:
         self.count = torch.zeros(G, dtype=torch.float32, device=self.device)
 
     def _alloc(self, capacity: int):
-        self.caps = batch.caps_for(self.specs, self.tile, capacity)
+        self.caps = batch.caps_for(self.specs, self.tile, capacity, depth_split=self.depth_split)
         self.ws = None
         self.grads = None
         self.ws = batch.BatchWorkspace(self.caps, self.device)

@@ -102,6 +102,10 @@ struct BlendBwdArgs {
   const float* dL_ddepth;   // per frame [H,W] at pix_off; may be null (zero)
   float* rec;               // [capacity][GR_FIELDS] per-instance partials
   uint8_t* flag;            // [capacity], zeroed: 1 where rec was written
+  // depth-sliced binning: a tile's list is its near list followed by its far
+  // list (null: one list)
+  const uint2* ranges_far;
+  const uint32_t* ids_far;
 };
 
 // Backward blend (backward.cu:396-520 semantics, with our forward's
@@ -169,6 +173,13 @@ void k_blend_bwd(BlendBwdArgs a) {
     const float x0 = (float)ux, x1 = (float)(ux + kUnit - 1);
     const float y0 = (float)uy, y1 = (float)(uy + kUnit - 1);
     const uint2 range = a.ranges[tile];
+    const uint2 range_far = a.ranges_far ? a.ranges_far[tile] : make_uint2(0u, 0u);
+    const int n_near = (int)(range.y - range.x);
+    const int n_list = n_near + (int)(range_far.y - range_far.x);
+    // instance k of the tile's list (near, then far)
+    auto id_at = [&](int k) -> uint32_t {
+      return k < n_near ? a.ids[range.x + k] : a.ids_far[range_far.x + (uint32_t)(k - n_near)];
+    };
     const size_t lp = (size_t)py * fd.W + px;
     const size_t pid = fd.pix_off + lp;
     const size_t HW = (size_t)fd.W * fd.H;
@@ -192,7 +203,7 @@ void k_blend_bwd(BlendBwdArgs a) {
     int wmax = last;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
-    wmax = min(__builtin_amdgcn_readfirstlane(wmax), (int)(range.y - range.x));
+    wmax = min(__builtin_amdgcn_readfirstlane(wmax), n_list);
     if (lane == 0) atomicMax(&s_max, (uint32_t)wmax);
     __syncthreads();
     const int nproc = (int)s_max;
@@ -213,12 +224,12 @@ void k_blend_bwd(BlendBwdArgs a) {
     uint32_t off_cur = 0;
     if (nproc > 0) {
       const int k = nproc - 64 + lane;
-      id_n = k >= 0 ? a.ids[range.x + k] : 0u;
+      id_n = k >= 0 ? id_at(k) : 0u;
       const uint32_t id0 = id_n < P ? id_n : 0u;
       cur = a.splats[id0];
       rect_cur = a.rect[id0];
       off_cur = a.item_off[id0];
-      id_n = k >= 64 ? a.ids[range.x + k - 64] : 0u;
+      id_n = k >= 64 ? id_at(k - 64) : 0u;
     }
     for (int cend = nproc; cend > 0; cend -= 64) {
       const int cstart = cend - 64;  // may be negative: lanes below 0 are empty
@@ -231,7 +242,7 @@ void k_blend_bwd(BlendBwdArgs a) {
         rect_cur = a.rect[idc];
         off_cur = a.item_off[idc];
         const int k2 = cstart - 128 + lane;
-        id_n = k2 >= 0 ? a.ids[range.x + k2] : 0u;
+        id_n = k2 >= 0 ? id_at(k2) : 0u;
       }
       const int kk_l = cstart + lane;
       n_loaded += (uint32_t)min(64, cend);

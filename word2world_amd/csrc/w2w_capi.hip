@@ -95,6 +95,23 @@ struct Plan {
   int tpasses, tlast_bits;
   size_t status, table, dkey0, dkey1, id1, dhist, dstatus, slb;  // geom
   size_t tkey0, tkey1, tid0, tid1, thist, tstatus;               // binning
+  // depth-sliced binning (c.depth_split in 1..999)
+  bool sliced;
+  size_t ocount, boff, slb_far, open_bm, hint_bm, open_units, hint_sat;  // geom
+  size_t tid_far;                         // binning: the far sort's second id buffer
+  size_t ranges_far;                      // image
+  // the near tile sort ends in (L.sorted_tiles, L.sorted_ids); its other
+  // buffer pair is free afterwards and takes the far slice's emission
+  size_t scratch_keys() const { return (tpasses & 1) ? tkey0 : tkey1; }
+  size_t scratch_ids() const { return (tpasses & 1) ? tid0 : tid1; }
+  // the far sort runs from the scratch pair; it ends there after an even
+  // number of passes, else in (the near sorted keys, tid_far).  The near
+  // sorted keys are free once the near ranges are built.
+  size_t far_keys() const { return (tpasses & 1) ? L.sorted_tiles : scratch_keys(); }
+  size_t far_ids() const { return (tpasses & 1) ? tid_far : scratch_ids(); }
+  size_t far_scratch_keys() const { return (tpasses & 1) ? scratch_keys() : L.sorted_tiles; }
+  size_t far_scratch_ids() const { return (tpasses & 1) ? scratch_ids() : tid_far; }
+  uint64_t units_per_tile() const { return (uint64_t)(c.tile / kUnit) * (uint64_t)(c.tile / kUnit); }
 };
 
 bool make_plan(const W2WCaps& c, Plan& p) {
@@ -102,6 +119,7 @@ bool make_plan(const W2WCaps& c, Plan& p) {
       c.pixels < 1 || c.capacity < 0)
     return false;
   if (c.tile != 8 && c.tile != 16 && c.tile != 32) return false;
+  if (c.depth_split < 0) return false;
   if (c.items >= (int64_t)LB_VAL || c.capacity >= (int64_t)LB_VAL || c.tiles >= (int64_t)LB_VAL ||
       c.gaussians >= (int64_t)LB_VAL)
     return false;
@@ -133,6 +151,18 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   p.dstatus = take((size_t)4 * p.dtiles * RADIX * 4);
   p.slb = take((size_t)p.stiles * 4 + 4);
   L.item_off = take(4 * I);
+  p.sliced = c.depth_split > 0 && c.depth_split < 1000;
+  p.ocount = p.boff = p.slb_far = p.open_bm = p.hint_bm = p.open_units = p.hint_sat = 0;
+  p.tid_far = p.ranges_far = 0;
+  if (p.sliced) {
+    p.ocount = take(4 * I);
+    p.boff = take(4 * I);
+    p.slb_far = take((size_t)p.stiles * 4 + 4);
+    p.open_bm = take(4 * (size_t)cdiv((uint64_t)c.tiles, 32));
+    p.hint_bm = take(4 * (size_t)cdiv((uint64_t)c.tiles, 32));
+    p.hint_sat = take(4 * (size_t)c.tiles);
+    p.open_units = take(4 * (size_t)c.tiles * p.units_per_tile());
+  }
   L.geom_bytes = o;
   o = 0;
   const size_t C = (size_t)c.capacity;
@@ -144,6 +174,7 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   p.tstatus = take((size_t)p.tpasses * p.ctiles * RADIX * 4);
   L.grad_rec = take(sizeof(float) * GR_FIELDS * C);
   L.grad_flag = take(C);  // take() rounds the end up to the 256-byte alignment
+  if (p.sliced) p.tid_far = take(4 * C);
   L.binning_bytes = o;
   L.sorted_tiles = (p.tpasses & 1) ? p.tkey1 : p.tkey0;
   L.sorted_ids = (p.tpasses & 1) ? p.tid1 : p.tid0;
@@ -151,6 +182,7 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   L.ranges = take(8 * (size_t)c.tiles);
   L.n_contrib = take(4 * (size_t)c.pixels);
   L.final_T = take(4 * (size_t)c.pixels);
+  if (p.sliced) p.ranges_far = take(8 * (size_t)c.tiles);
   L.image_bytes = o;
   L.table = p.table;
   return true;
@@ -160,7 +192,7 @@ bool single_caps(int32_t P, int32_t W, int32_t H, int32_t ts, int64_t cap, W2WCa
   if (P < 0 || W <= 0 || H <= 0 || (ts != 8 && ts != 16 && ts != 32)) return false;
   const int64_t tx = (W + ts - 1) / ts, ty = (H + ts - 1) / ts;
   if (tx > 65535 || ty > 65535) return false;
-  c.scenes = 1; c.frames = 1; c.tile = ts; c.pad = 0;
+  c.scenes = 1; c.frames = 1; c.tile = ts; c.depth_split = 0;
   c.items = P; c.gaussians = P; c.tiles = tx * ty; c.pixels = (int64_t)W * H; c.capacity = cap;
   return true;
 }
@@ -222,9 +254,23 @@ uint32_t blend_grid(uint32_t tiles_cap) {
 }
 
 template <int TS>
-void launch_blend(const BlendArgs& a, uint32_t tiles_cap, hipStream_t s) {
-  ProfSpan ps(W2W_KID_BLEND, s);
-  hipLaunchKernelGGL(k_blend<TS>, dim3(blend_grid<TS>(tiles_cap)), dim3(256), 0, s, a);
+void launch_blend(const BlendArgs& a, int mode, uint32_t tiles_cap, hipStream_t s) {
+  ProfSpan ps(mode == BLEND_FAR ? W2W_KID_BLEND_FAR : W2W_KID_BLEND, s);
+  const dim3 grid(blend_grid<TS>(tiles_cap));
+  if (mode == BLEND_NEAR)
+    hipLaunchKernelGGL((k_blend<TS, BLEND_NEAR>), grid, dim3(256), 0, s, a);
+  else if (mode == BLEND_FAR)
+    hipLaunchKernelGGL((k_blend<TS, BLEND_FAR>), grid, dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_blend<TS, BLEND_FULL>), grid, dim3(256), 0, s, a);
+}
+
+void launch_blend_tile(const BlendArgs& a, int mode, int tile, uint32_t tiles_cap, hipStream_t s) {
+  switch (tile) {
+    case 8: launch_blend<8>(a, mode, tiles_cap, s); break;
+    case 16: launch_blend<16>(a, mode, tiles_cap, s); break;
+    default: launch_blend<32>(a, mode, tiles_cap, s); break;
+  }
 }
 
 // one tile per workgroup at a time, one wave per 8x8 block of it
@@ -332,27 +378,73 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
                          items_cap, cap);
     }
     if (cap > 0) {
+      // depth-sliced binning: the near slice is binned here, the far slice
+      // after the near blend (only on the tiles it leaves open)
+      const uint32_t* d_near_end = nullptr;
+      const uint32_t* d_kn = status + ST_K;
+      if (p.sliced) {
+        hipLaunchKernelGGL(k_split, dim3(1), dim3(1), 0, s, table, items_cap, (uint32_t)c.depth_split,
+                           at<uint32_t>(G, p.L.offsets), status, cap);
+        // hinted tiles (those the last forward's near lists left unfinished:
+        // a guess that only steers work) get their far items in their near
+        // lists too, behind the near items (depth order is kept)
+        const size_t words = cdiv((uint64_t)c.tiles, 32);
+        hipLaunchKernelGGL(k_move_u32, dim3((uint32_t)std::min<size_t>(cdiv(words, 256), 1024)), dim3(256), 0, s,
+                           at<uint32_t>(G, p.hint_bm), at<uint32_t>(G, p.open_bm), words);
+        hipLaunchKernelGGL(k_tile_sat, dim3((uint32_t)std::min<int32_t>(c.frames, 4096)), dim3(256), 0, s, table,
+                           (uint32_t)c.scenes, (uint32_t)c.frames, at<uint32_t>(G, p.hint_bm),
+                           at<uint32_t>(G, p.hint_sat));
+        {
+          ProfSpan ps(W2W_KID_COUNT_OPEN, s);
+          hipLaunchKernelGGL(k_count_tiles, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 8192)), dim3(256), 0,
+                             s, table, (uint32_t)c.scenes, items_cap, status, at<uint32_t>(G, p.L.perm),
+                             at<uint32_t>(G, p.L.offsets), at<uint2>(G, p.L.rect),
+                             at<uint32_t>(G, p.L.item_frame), at<uint32_t>(G, p.hint_bm), nullptr,
+                             at<uint32_t>(G, p.ocount), at<uint32_t>(G, p.L.item_off), nullptr,
+                             at<uint32_t>(G, p.hint_sat));
+        }
+        fill_u32(at<uint32_t>(G, p.slb_far), 0u, p.stiles + 1, s);
+        {
+          ProfSpan ps(W2W_KID_SCAN, s);
+          hipLaunchKernelGGL(k_scan<false>, dim3(std::min<uint32_t>(p.stiles, 2048)), dim3(SCAN_THREADS), 0, s,
+                             at<uint32_t>(G, p.ocount), nullptr, at<uint32_t>(G, p.boff),
+                             at<uint32_t>(G, p.slb_far) + 1, status + ST_SCANH_TICKET, status + ST_KH, nullptr,
+                             d_items, items_cap, cap);
+        }
+        hipLaunchKernelGGL(k_near_total, dim3(1), dim3(1), 0, s, status, cap);
+        d_near_end = status + ST_SPLIT;
+        d_kn = status + ST_KNEAR;
+      }
       {
         ProfSpan ps(W2W_KID_EMIT, s);
-        hipLaunchKernelGGL(k_emit, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 4096)), dim3(256),
-                           0, s, table, (uint32_t)c.scenes, at<uint32_t>(G, p.L.perm),
+        hipLaunchKernelGGL(k_emit, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 4096)),
+                           dim3(256), 0, s, table, (uint32_t)c.scenes, at<uint32_t>(G, p.L.perm),
                            at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.L.counts),
                            at<uint2>(G, p.L.rect), at<uint32_t>(G, p.L.item_frame), items_cap, cap,
-                           status + ST_EMIT_TICKET, at<uint32_t>(B, p.tkey0),
-                           at<uint32_t>(B, p.tid0), at<uint32_t>(G, p.L.item_off),
-                           at<uint32_t>(B, p.thist), p.tpasses, p.tlast_bits);
+                           d_near_end, status + ST_EMIT_TICKET,
+                           at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0),
+                           at<uint32_t>(G, p.L.item_off), at<uint32_t>(B, p.thist), p.tpasses,
+                           p.tlast_bits);
+      }
+      if (p.sliced) {
+        ProfSpan ps(W2W_KID_EMIT_FAR, s);
+        hipLaunchKernelGGL(k_emit_far, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 8192)), dim3(256), 0, s,
+                           table, (uint32_t)c.scenes, items_cap, cap, status, at<uint32_t>(G, p.L.perm),
+                           at<uint32_t>(G, p.boff), at<uint32_t>(G, p.ocount), at<uint2>(G, p.L.rect),
+                           at<uint32_t>(G, p.L.item_frame), at<uint32_t>(G, p.hint_bm), nullptr, status + ST_KA,
+                           at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0), at<uint32_t>(B, p.thist),
+                           p.tpasses, p.tlast_bits);
       }
       sort_passes<uint32_t>(at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0),
                             at<uint32_t>(B, p.L.sorted_tiles), at<uint32_t>(B, p.L.sorted_ids),
-                            at<uint32_t>(B, (p.tpasses & 1) ? p.tkey0 : p.tkey1),
-                            at<uint32_t>(B, (p.tpasses & 1) ? p.tid0 : p.tid1), status + ST_K, cap,
-                            p.tpasses, p.tlast_bits, true, at<uint32_t>(B, p.thist),
+                            at<uint32_t>(B, p.scratch_keys()), at<uint32_t>(B, p.scratch_ids()), d_kn,
+                            cap, p.tpasses, p.tlast_bits, true, at<uint32_t>(B, p.thist),
                             at<uint32_t>(B, p.tstatus), status + ST_TTICKET, W2W_KID_TSORT_HIST,
                             W2W_KID_TSORT_PASS, s, /*hist_ready=*/true);
       const uint32_t rgrid = std::min<uint32_t>(cdiv(cap, 256), 4096);
       ProfSpan ps(W2W_KID_RANGES, s);
       hipLaunchKernelGGL(k_ranges, dim3(rgrid), dim3(256), 0, s, at<uint32_t>(B, p.L.sorted_tiles),
-                         status, table, cap, tiles_cap, at<uint2>(I, p.L.ranges));
+                         d_kn, table, cap, tiles_cap, at<uint2>(I, p.L.ranges));
     }
     if (out_radii)
       (void)hipMemcpyAsync(out_radii, at<int32_t>(G, p.L.radii), 4 * (size_t)items_cap,
@@ -369,11 +461,69 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   ba.color = out_color; ba.alpha = out_alpha; ba.depth = out_depth;
   ba.n_contrib = at<uint32_t>(I, p.L.n_contrib);
   ba.final_T = at<float>(I, p.L.final_T);
-  switch (c.tile) {
-    case 8: launch_blend<8>(ba, tiles_cap, s); break;
-    case 16: launch_blend<16>(ba, tiles_cap, s); break;
-    default: launch_blend<32>(ba, tiles_cap, s); break;
+  ba.ranges_far = nullptr;
+  ba.ids_far = nullptr;
+  ba.open_bm = nullptr;
+  ba.hint_bm = nullptr;
+  ba.open_units = nullptr;
+  ba.n_open_units = nullptr;
+  if (!p.sliced || items_cap == 0 || cap == 0) {
+    launch_blend_tile(ba, BLEND_FULL, c.tile, tiles_cap, s);
+    return launch_status();
   }
+  // near lists: final outputs of finished blocks and hinted tiles, unfinished
+  // pixel state otherwise; the open tiles (zeroed by the hint move)
+  uint32_t* open_bm = at<uint32_t>(G, p.open_bm);
+  const uint32_t* hint_bm = at<uint32_t>(G, p.hint_bm);
+  ba.open_bm = open_bm;
+  ba.hint_bm = hint_bm;
+  ba.open_units = at<uint32_t>(G, p.open_units);
+  ba.n_open_units = status + ST_OPEN_UNITS;  // zeroed with the status block
+  launch_blend_tile(ba, BLEND_NEAR, c.tile, tiles_cap, s);
+  // far lists on the open tiles that were not hinted: counts, compact
+  // offsets, emission, sort, ranges
+  {
+    ProfSpan ps(W2W_KID_COUNT_OPEN, s);
+    hipLaunchKernelGGL(k_count_tiles, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 8192)), dim3(256), 0, s,
+                       table, (uint32_t)c.scenes, items_cap, status, at<uint32_t>(G, p.L.perm),
+                       at<uint32_t>(G, p.L.offsets), at<uint2>(G, p.L.rect), at<uint32_t>(G, p.L.item_frame),
+                       open_bm, hint_bm, at<uint32_t>(G, p.ocount), nullptr, status + ST_OPEN_UNITS, nullptr);
+  }
+  fill_u32(at<uint32_t>(G, p.slb_far), 0u, p.stiles + 1, s);
+  {
+    ProfSpan ps(W2W_KID_SCAN, s);
+    hipLaunchKernelGGL(k_scan<false>, dim3(std::min<uint32_t>(p.stiles, 2048)), dim3(SCAN_THREADS), 0, s,
+                       at<uint32_t>(G, p.ocount), nullptr, at<uint32_t>(G, p.boff),
+                       at<uint32_t>(G, p.slb_far) + 1, status + ST_SCANB_TICKET, status + ST_KB, nullptr,
+                       d_items, items_cap, cap);
+  }
+  fill_u32(at<uint32_t>(B, p.thist), 0u, 4 * RADIX, s);
+  {
+    ProfSpan ps(W2W_KID_EMIT_FAR, s);
+    hipLaunchKernelGGL(k_emit_far, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 8192)), dim3(256), 0, s,
+                       table, (uint32_t)c.scenes, items_cap, cap, status, at<uint32_t>(G, p.L.perm),
+                       at<uint32_t>(G, p.boff), at<uint32_t>(G, p.ocount), at<uint2>(G, p.L.rect),
+                       at<uint32_t>(G, p.L.item_frame), open_bm, hint_bm, nullptr,
+                       at<uint32_t>(B, p.scratch_keys()), at<uint32_t>(B, p.scratch_ids()), at<uint32_t>(B, p.thist),
+                       p.tpasses, p.tlast_bits);
+  }
+  sort_passes<uint32_t>(at<uint32_t>(B, p.scratch_keys()), at<uint32_t>(B, p.scratch_ids()),
+                        at<uint32_t>(B, p.far_keys()), at<uint32_t>(B, p.far_ids()),
+                        at<uint32_t>(B, p.far_scratch_keys()), at<uint32_t>(B, p.far_scratch_ids()),
+                        status + ST_KB, cap, p.tpasses, p.tlast_bits, true, at<uint32_t>(B, p.thist),
+                        at<uint32_t>(B, p.tstatus), status + ST_TTICKET, W2W_KID_TSORT_HIST,
+                        W2W_KID_TSORT_PASS, s, /*hist_ready=*/true);
+  uint2* ranges_far = at<uint2>(I, p.ranges_far);
+  fill_u32(reinterpret_cast<uint32_t*>(ranges_far), 0u, 2 * (size_t)c.tiles, s);
+  {
+    ProfSpan ps(W2W_KID_RANGES, s);
+    hipLaunchKernelGGL(k_ranges, dim3(std::min<uint32_t>(cdiv(cap, 256), 4096)), dim3(256), 0, s,
+                       at<uint32_t>(B, p.far_keys()), status + ST_KB, table, cap, tiles_cap, ranges_far);
+  }
+  ba.ticket = status + ST_BLENDB_TICKET;
+  ba.ranges_far = ranges_far;
+  ba.ids_far = at<uint32_t>(B, p.far_ids());
+  launch_blend_tile(ba, BLEND_FAR, c.tile, tiles_cap, s);
   return launch_status();
 }
 
@@ -407,6 +557,8 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
   ba.n_contrib = at<uint32_t>(I, p.L.n_contrib);
   ba.final_T = at<float>(I, p.L.final_T);
   ba.dL_dcolor = dL_dcolor; ba.dL_dalpha = dL_dalpha; ba.dL_ddepth = dL_ddepth;
+  ba.ranges_far = p.sliced ? at<uint2>(I, p.ranges_far) : nullptr;
+  ba.ids_far = p.sliced ? at<uint32_t>(B, p.far_ids()) : nullptr;
   switch (c.tile) {
     case 8: launch_blend_bwd<8>(ba, (uint32_t)c.tiles, s); break;
     case 16: launch_blend_bwd<16>(ba, (uint32_t)c.tiles, s); break;

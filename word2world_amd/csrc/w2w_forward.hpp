@@ -34,18 +34,36 @@ constexpr uint32_t CULLED_DEPTH_KEY = 0xffffffffu;
 enum StatusWord : int {
   ST_K = 0,            // instances emitted (the reference's num_rendered)
   ST_OVERFLOW = 1,     // nonzero when K exceeded the binning capacity
+  ST_SPLIT = 2,        // depth-sliced binning: depth positions in the near slice
+  ST_KA = 3,           //   instances of the near slice
+  ST_KB = 4,           //   instances of the far slice on open tiles
+  ST_OPEN_UNITS = 5,   //   8x8 blocks left unfinished by the near slice
+  ST_KH = 6,           //   far-slice instances on hinted tiles (binned with the near slice)
+  ST_KNEAR = 7,        //   the near lists' instances (ST_KA + ST_KH)
   ST_DTICKET = 8,      // 4 depth-sort tickets
   ST_TTICKET = 16,     // up to 4 tile-sort tickets
   ST_SCAN_TICKET = 24,
   ST_EMIT_TICKET = 25,
   ST_BLEND_TICKET = 26,
   ST_BWD_TICKET = 27,
+  ST_SCANB_TICKET = 28,
+  ST_SCANH_TICKET = 29,
+  ST_BLENDB_TICKET = 30,
   ST_WORDS = 64,
 };
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// dst <- src, then src <- 0
+__global__ void k_move_u32(uint32_t* __restrict__ dst, uint32_t* __restrict__ src, size_t n) {
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    dst[i] = src[i];
+    src[i] = 0u;
+  }
 }
 
 // Item -> (scene, Gaussian, view, frame).
@@ -334,6 +352,202 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(
   }
 }
 
+// Tile sets of depth-sliced binning, as bitmaps over the global tile ids
+// (bit t of bm[t >> 5]): the tiles a set `a` holds and a set `b` (null: the
+// empty set) does not.
+W2W_DEV uint32_t tileset_word(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b, uint32_t w) {
+  return b ? (a[w] & ~b[w]) : a[w];
+}
+
+// members among tiles [b0, b0 + n)
+W2W_DEV uint32_t tileset_count(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b, uint32_t b0,
+                               uint32_t n) {
+  uint32_t c = 0;
+  while (n > 0) {
+    const uint32_t o = b0 & 31u, k = min(32u - o, n);
+    uint32_t m = tileset_word(a, b, b0 >> 5) >> o;
+    if (k < 32u) m &= (1u << k) - 1u;
+    c += (uint32_t)__builtin_popcount(m);
+    b0 += k;
+    n -= k;
+  }
+  return c;
+}
+
+// Depth-sliced binning: the near slice is the first S = ceil(P * permille /
+// 1000) depth positions; K_A (its instances) is the scan's offset at S.
+__global__ void k_split(const void* __restrict__ table, uint32_t items_cap, uint32_t permille,
+                        const uint32_t* __restrict__ offs, uint32_t* __restrict__ status, uint32_t capacity) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t P = min(static_cast<const BatchHeader*>(table)->n_items, items_cap);
+  const uint32_t S = (uint32_t)min<uint64_t>(((uint64_t)P * permille + 999u) / 1000u, P);
+  const uint32_t KA = S < P ? offs[S] : status[ST_K];
+  status[ST_SPLIT] = S;
+  status[ST_KA] = min(KA, capacity);
+}
+
+// Inclusive 2-D prefix counts of a tile set per frame, indexed like the
+// tiles: sat[tile_off + y tiles_x + x] = members in columns 0..x of rows
+// 0..y.  One workgroup per frame: row prefixes, then column prefixes.
+__global__ __launch_bounds__(256) void k_tile_sat(const void* __restrict__ table, uint32_t scenes_cap,
+                                                  uint32_t frames_cap, const uint32_t* __restrict__ set,
+                                                  uint32_t* __restrict__ sat) {
+  const BatchHeader* hdr = static_cast<const BatchHeader*>(table);
+  const FrameDesc* frames = table_frames(table, scenes_cap);
+  const uint32_t nf = min(hdr->n_frames, frames_cap);
+  for (uint32_t f = blockIdx.x; f < nf; f += gridDim.x) {
+    const FrameDesc& fd = frames[f];
+    const uint32_t tx = (uint32_t)fd.tiles_x, ty = (uint32_t)fd.tiles_y, t0 = fd.tile_off;
+    for (uint32_t y = threadIdx.x; y < ty; y += blockDim.x) {
+      uint32_t run = 0;
+      for (uint32_t x = 0; x < tx; x++) {
+        const uint32_t t = t0 + y * tx + x;
+        run += (set[t >> 5] >> (t & 31u)) & 1u;
+        sat[t] = run;
+      }
+    }
+    __syncthreads();
+    for (uint32_t x = threadIdx.x; x < tx; x += blockDim.x) {
+      uint32_t run = 0;
+      for (uint32_t y = 0; y < ty; y++) {
+        const uint32_t t = t0 + y * tx + x;
+        run += sat[t];
+        sat[t] = run;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// members of the tile rectangle [x0, x1) x [y0, y1) (non-empty) from the
+// prefix counts
+W2W_DEV uint32_t sat_rect(const uint32_t* __restrict__ sat, uint32_t t0, uint32_t tx, uint32_t x0, uint32_t x1,
+                          uint32_t y0, uint32_t y1) {
+  const uint32_t* r1 = sat + t0 + (y1 - 1) * tx;
+  uint32_t c = r1[x1 - 1];
+  if (x0 > 0) c -= r1[x0 - 1];
+  if (y0 > 0) {
+    const uint32_t* r0 = sat + t0 + (y0 - 1) * tx;
+    c -= r0[x1 - 1];
+    if (x0 > 0) c += r0[x0 - 1];
+  }
+  return c;
+}
+
+// The near lists' total: the near slice's instances and the far items'
+// instances on hinted tiles.
+__global__ void k_near_total(uint32_t* __restrict__ status, uint32_t capacity) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) status[ST_KNEAR] = min(status[ST_KA] + status[ST_KH], capacity);
+}
+
+// Per depth position of the far slice (0 in the near slice): how many tiles
+// of its item's rectangle are in the tile set (a minus b).  With item_off
+// given, far items also get their full-layout offset there (the backward's
+// record slots), whether or not a tile is counted.  With *gate == 0 the set
+// is known to be empty: every count is 0.  With sat given (the set's prefix
+// counts, b null) a rectangle costs four reads.
+__global__ __launch_bounds__(256) void k_count_tiles(const void* __restrict__ table, uint32_t scenes_cap,
+                                                     uint32_t items_cap, const uint32_t* __restrict__ status,
+                                                     const uint32_t* __restrict__ perm,
+                                                     const uint32_t* __restrict__ offs,
+                                                     const uint2* __restrict__ rect,
+                                                     const uint32_t* __restrict__ item_frame,
+                                                     const uint32_t* __restrict__ set_a,
+                                                     const uint32_t* __restrict__ set_b,
+                                                     uint32_t* __restrict__ count,
+                                                     uint32_t* __restrict__ item_off,
+                                                     const uint32_t* __restrict__ gate,
+                                                     const uint32_t* __restrict__ sat) {
+  const FrameDesc* frames = table_frames(table, scenes_cap);
+  const uint32_t P = min(static_cast<const BatchHeader*>(table)->n_items, items_cap);
+  const uint32_t S = status[ST_SPLIT];
+  const bool empty = gate && *gate == 0u;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
+    uint32_t c = 0;
+    if (i >= S && !(empty && !item_off)) {
+      const uint32_t g = perm[i];
+      if (item_off) item_off[g] = offs[i];
+      const uint2 r = rect[g];
+      const uint32_t x0 = r.x & 0xffffu, y0 = r.x >> 16, x1 = r.y & 0xffffu, y1 = r.y >> 16;
+      if (!empty && x1 > x0 && y1 > y0) {
+        const FrameDesc& fd = frames[item_frame[g]];
+        if (sat) {
+          c = sat_rect(sat, fd.tile_off, (uint32_t)fd.tiles_x, x0, x1, y0, y1);
+        } else {
+          for (uint32_t ty = y0; ty < y1; ty++)
+            c += tileset_count(set_a, set_b, fd.tile_off + ty * (uint32_t)fd.tiles_x + x0, x1 - x0);
+        }
+      }
+    }
+    count[i] = c;
+  }
+}
+
+// Far items' instances on the tiles of a set (a minus b): one thread per far
+// depth position (most have none and stop at once), the item's member tiles
+// in row-major order from *d_base (null: 0) + its compact offset.  An
+// instance's key is its tile's bit index.  Adds the keys' digits to the tile
+// sort's histogram (hist[pass][256]) like k_emit.
+__global__ __launch_bounds__(256) void k_emit_far(const void* __restrict__ table, uint32_t scenes_cap,
+                                                  uint32_t items_cap, uint32_t capacity,
+                                                  const uint32_t* __restrict__ status,
+                                                  const uint32_t* __restrict__ perm,
+                                                  const uint32_t* __restrict__ coff,
+                                                  const uint32_t* __restrict__ count,
+                                                  const uint2* __restrict__ rect,
+                                                  const uint32_t* __restrict__ item_frame,
+                                                  const uint32_t* __restrict__ set_a,
+                                                  const uint32_t* __restrict__ set_b,
+                                                  const uint32_t* __restrict__ d_base,
+                                                  uint32_t* __restrict__ tile_keys,
+                                                  uint32_t* __restrict__ ids,
+                                                  uint32_t* __restrict__ hist, int hist_passes,
+                                                  int hist_last_bits) {
+  __shared__ uint32_t s_hist[4][256];
+  for (int k = threadIdx.x; k < 4 * 256; k += blockDim.x) (&s_hist[0][0])[k] = 0u;
+  __syncthreads();
+  const FrameDesc* frames = table_frames(table, scenes_cap);
+  const uint32_t P = min(static_cast<const BatchHeader*>(table)->n_items, items_cap);
+  const uint32_t S = status[ST_SPLIT];
+  const uint32_t base = d_base ? *d_base : 0u;
+  for (uint32_t i = S + blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
+    const uint32_t c = count[i];
+    if (c == 0) continue;
+    const uint32_t g = perm[i];
+    const uint2 r = rect[g];
+    const uint32_t x0 = r.x & 0xffffu, y0 = r.x >> 16, x1 = r.y & 0xffffu, y1 = r.y >> 16;
+    const FrameDesc& fd = frames[item_frame[g]];
+    uint32_t o = min(base + coff[i], capacity);
+    const uint32_t oe = min(o + c, capacity);
+    for (uint32_t ty = y0; ty < y1 && o < oe; ty++) {
+      uint32_t pos = fd.tile_off + ty * (uint32_t)fd.tiles_x + x0, rem = x1 - x0;
+      while (rem > 0 && o < oe) {
+        const uint32_t sh = pos & 31u, k = min(32u - sh, rem);
+        uint32_t m = tileset_word(set_a, set_b, pos >> 5) >> sh;
+        if (k < 32u) m &= (1u << k) - 1u;
+        while (m && o < oe) {
+          const uint32_t key = pos + (uint32_t)__builtin_ctz(m);
+          tile_keys[o] = key;
+          ids[o] = g;
+          o++;
+          m &= m - 1u;
+          for (int p = 0; p < hist_passes; p++) {
+            const int bits = (p == hist_passes - 1) ? hist_last_bits : 8;
+            atomicAdd(&s_hist[p][(key >> (8 * p)) & ((1u << bits) - 1u)], 1u);
+          }
+        }
+        pos += k;
+        rem -= k;
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < hist_passes * 256; k += blockDim.x) {
+    const uint32_t v = (&s_hist[0][0])[k];
+    if (v) atomicAdd(hist + k, v);
+  }
+}
+
 // Instance emission in depth order.  A workgroup owns depth-sorted positions
 // [256b, 256b+256) and the contiguous instance range they cover; each thread
 // strides over instances and finds its owner by binary search in LDS.  Chunks
@@ -341,6 +555,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(
 // Emission also histograms the tile keys for the tile sort (every digit of
 // every pass, hist[pass][256], pre-zeroed), so the sort does not read the
 // keys an extra time.
+// Depth positions [0, *d_end) (null: all; the near slice of depth-sliced
+// binning otherwise).
 __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, uint32_t scenes_cap,
                                               const uint32_t* __restrict__ perm,
                                               const uint32_t* __restrict__ offs,
@@ -348,6 +564,7 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
                                               const uint2* __restrict__ rect,
                                               const uint32_t* __restrict__ item_frame,
                                               uint32_t items_cap, uint32_t capacity,
+                                              const uint32_t* __restrict__ d_end,
                                               uint32_t* __restrict__ ticket,
                                               uint32_t* __restrict__ tile_keys,
                                               uint32_t* __restrict__ ids,
@@ -360,7 +577,8 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
   const BatchHeader* hdr = static_cast<const BatchHeader*>(table);
   const FrameDesc* frames = table_frames(table, scenes_cap);
   const uint32_t P = min(hdr->n_items, items_cap);
-  const uint32_t nchunks = (P + 255) / 256;
+  const uint32_t end = d_end ? min(*d_end, P) : P;
+  const uint32_t nchunks = (end + 255) / 256;
   const int tid = threadIdx.x;
   for (int i = tid; i < 4 * 256; i += 256) (&s_hist[0][0])[i] = 0u;
   for (;;) {
@@ -368,16 +586,17 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
     __syncthreads();
     const uint32_t chunk = s_chunk;
     if (chunk >= nchunks) break;
-    const uint32_t i = chunk * 256 + tid;
-    const uint32_t last = min(P, chunk * 256 + 256) - 1;
-    if (i < P) {
+    const uint32_t cbase = chunk * 256;
+    const uint32_t i = cbase + tid;
+    const uint32_t last = min(end, cbase + 256) - 1;
+    if (i <= last) {
       uint32_t g = perm[i];
       uint2 r = rect[g];
       uint32_t c = counts[g];
       const FrameDesc& fd = frames[item_frame[g]];
       s_start[tid] = offs[i];
       s_g[tid] = g;
-      item_off[g] = offs[i];  // the backward finds an item's instances from here
+      item_off[g] = offs[i];  // the backward finds an item's records from here
       s_x0[tid] = r.x & 0xffffu;
       s_y0[tid] = r.x >> 16;
       s_w[tid] = (r.y & 0xffffu) - (r.x & 0xffffu);
@@ -386,7 +605,7 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
       if (i == last) s_end = offs[i] + c;
     }
     __syncthreads();
-    const int nb = (int)(last - chunk * 256 + 1);
+    const int nb = (int)(last - cbase + 1);
     const uint32_t kb = s_start[0], ke = min(s_end, capacity);
     // each thread emits an aligned group of four consecutive instances (one
     // 16-byte store of keys and one of ids when the group is whole): one
@@ -452,10 +671,10 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
 // Per-tile [start, end) of the sorted tile keys; four keys per thread with
 // one 16-byte load (the key array is 256-byte aligned).
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ keys,
-                                                const uint32_t* __restrict__ status,
+                                                const uint32_t* __restrict__ d_n,
                                                 const void* __restrict__ table, uint32_t capacity,
                                                 uint32_t tiles_cap, uint2* __restrict__ ranges) {
-  uint32_t n = status[ST_K];
+  uint32_t n = *d_n;
   n = n < capacity ? n : capacity;
   const uint32_t T = min(static_cast<const BatchHeader*>(table)->n_tiles, tiles_cap);
   const uint32_t n4 = (n + 3) / 4;
@@ -500,7 +719,28 @@ struct BlendArgs {
   float* depth;
   uint32_t* n_contrib;
   float* final_T;
+  // depth-sliced binning (modes BLEND_NEAR / BLEND_FAR): the far slice's
+  // ranges and sorted ids, and the open-tile bitmap
+  const uint2* ranges_far;
+  const uint32_t* ids_far;
+  uint32_t* open_bm;        // tiles with unfinished pixels after their near list
+  const uint32_t* hint_bm;  // tiles whose near list is their whole list
+  uint32_t* open_units;     // list of the unfinished 8x8 blocks (units) of other tiles, any order
+  uint32_t* n_open_units;
 };
+
+// BLEND_FULL: one list per tile, final outputs.  BLEND_NEAR: the near lists;
+// a block with unfinished pixels marks its tile open, and unless the tile is
+// hinted (its near list is its whole list, so its outputs are final) it
+// stores every pixel's state unfinished (colour without background,
+// n_contrib with bit 31 set once the pixel is done) and appends itself to
+// the open-block list; other blocks write final outputs.  BLEND_FAR: the
+// waves take the open blocks of that list in turn (each on its own: their
+// work is uneven, so no workgroup barrier, and the list is short, so no
+// ticket), resume the state with the far list (instance indices continue
+// after the near list's) and write the final outputs.
+enum BlendMode : int { BLEND_FULL = 0, BLEND_NEAR = 1, BLEND_FAR = 2 };
+constexpr uint32_t kDoneBit = 1u << 31;
 
 // Blend work unit: an 8x8 pixel block of a tile, one wave (lane = pixel).
 constexpr int kUnit = 8;
@@ -545,7 +785,7 @@ W2W_DEV bool wave_all(bool p) { return __ballot(!p) == 0ull; }
 // list.  The next chunk's ids and splat records are in flight while a chunk is
 // processed.  A workgroup claims four consecutive blocks at a time, one per
 // wave; the waves are otherwise independent.
-template <int TS>
+template <int TS, int MODE>
 __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
   __shared__ Splat s_chunk[4][64];  // each wave's current chunk, for broadcast reads
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
@@ -558,16 +798,27 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
   const float a_min = 0x1.010102p-8f;
   uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
   __shared__ uint32_t s_quad;
+  const uint32_t n_open = MODE == BLEND_FAR ? min(*a.n_open_units, U) : 0u;
+  uint32_t far_next = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);  // this wave's next list entry
   for (;;) {
-    // the workgroup claims four consecutive blocks (one tile of edge 16), one
-    // per wave: the waves gather the same tile list through one CU's caches
-    if (threadIdx.x == 0) s_quad = atomicAdd(a.ticket, 1u);
-    __syncthreads();
-    const uint32_t quad = s_quad;
-    __syncthreads();
-    if (quad >= (U + 3) / 4) break;
-    const uint32_t unit = quad * 4 + (threadIdx.x >> 6);
-    if (unit >= U) continue;
+    uint32_t unit;
+    if constexpr (MODE == BLEND_FAR) {
+      const uint32_t w = far_next;
+      far_next += gridDim.x * (blockDim.x >> 6);
+      if (w >= n_open) break;
+      unit = a.open_units[w];
+    } else {
+      // the workgroup claims four consecutive blocks (one tile of edge 16),
+      // one per wave: the waves gather the same tile list through one CU's
+      // caches
+      if (threadIdx.x == 0) s_quad = atomicAdd(a.ticket, 1u);
+      __syncthreads();
+      const uint32_t quad = s_quad;
+      __syncthreads();
+      if (quad >= (U + 3) / 4) break;
+      unit = quad * 4 + (threadIdx.x >> 6);
+      if (unit >= U) continue;
+    }
     uint32_t tile;
     int ux, uy;
     const FrameDesc& fd = unit_block<TS>(frames, hdr->n_frames, unit, tile, ux, uy);
@@ -576,20 +827,38 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
     const float pxf = (float)px, pyf = (float)py;
     const float x0 = (float)ux, x1 = (float)(ux + kUnit - 1);
     const float y0 = (float)uy, y1 = (float)(uy + kUnit - 1);
-    const uint2 range = a.ranges[tile];
+    const uint2 range_near = a.ranges[tile];
+    const uint2 range = MODE == BLEND_FAR ? a.ranges_far[tile] : range_near;
+    const uint32_t* const ids = MODE == BLEND_FAR ? a.ids_far : a.ids;
+    const uint32_t lbase = MODE == BLEND_FAR ? range_near.y - range_near.x : 0u;
     const int n = (int)(range.y - range.x);
     bool done = !inside;
     float T_ = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, A = 0.0f, Dd = 0.0f;
     uint32_t last = 0;
+    const size_t HW = (size_t)fd.W * fd.H;
+    const size_t lp = (size_t)py * fd.W + px;
+    const size_t pid = fd.pix_off + lp;
+    float* col = a.color + 3 * fd.pix_off;
+    if (MODE == BLEND_FAR && inside) {
+      C0 = col[lp];
+      C1 = col[HW + lp];
+      C2 = col[2 * HW + lp];
+      A = a.alpha[pid];
+      Dd = a.depth[pid];
+      T_ = a.final_T[pid];
+      const uint32_t v = a.n_contrib[pid];
+      last = v & ~kDoneBit;
+      done = (v & kDoneBit) != 0u;
+    }
 
     // software pipeline: ids two chunks ahead, splat records one chunk ahead
     // (lanes past the end gather splat 0, which exists whenever n > 0)
     uint32_t id_n = 0;
     Splat cur = {};
-    if (n > 0) {
-      id_n = lane < n ? a.ids[range.x + lane] : 0u;
+    if (n > 0 && !wave_all(done)) {
+      id_n = lane < n ? ids[range.x + lane] : 0u;
       cur = a.splats[id_n < P ? id_n : 0u];
-      id_n = 64 + lane < n ? a.ids[range.x + 64 + lane] : 0u;
+      id_n = 64 + lane < n ? ids[range.x + 64 + lane] : 0u;
     }
     for (int base = 0; base < n; base += 64) {
       if (wave_all(done)) break;
@@ -597,7 +866,7 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
       if (base + 64 < n) {
         cur = a.splats[id_n < P ? id_n : 0u];
         const int k2 = base + 128 + lane;
-        id_n = k2 < n ? a.ids[range.x + k2] : 0u;
+        id_n = k2 < n ? ids[range.x + k2] : 0u;
       }
       n_loaded += (uint32_t)min(64, n - base);
       const bool valid = base + lane < n;
@@ -632,7 +901,7 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
         A = A + aT;
         Dd = __builtin_fmaf(q.b.z, aT, Dd);
         T_ = nT;
-        last = (uint32_t)(base + j + 1);
+        last = lbase + (uint32_t)(base + j + 1);
         n_con++;
       };
       // two survivors in flight: the next one's read overlaps this one's work
@@ -657,11 +926,21 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
       }
       __builtin_amdgcn_wave_barrier();  // reads of this chunk precede the next chunk's writes
     }
-    if (inside) {
-      const size_t HW = (size_t)fd.W * fd.H;
-      const size_t lp = (size_t)py * fd.W + px;
-      const size_t pid = fd.pix_off + lp;
-      float* col = a.color + 3 * fd.pix_off;
+    const bool open = MODE == BLEND_NEAR && !wave_all(done);
+    const bool unfinished = open && !((a.hint_bm[tile >> 5] >> (tile & 31u)) & 1u);
+    if (open && lane == 0) atomicOr(a.open_bm + (tile >> 5), 1u << (tile & 31u));
+    if (unfinished) {
+      if (inside) {
+        col[lp] = C0;
+        col[HW + lp] = C1;
+        col[2 * HW + lp] = C2;
+        a.alpha[pid] = A;
+        a.depth[pid] = Dd;
+        a.n_contrib[pid] = last | (done ? kDoneBit : 0u);
+        a.final_T[pid] = T_;
+      }
+      if (lane == 0) a.open_units[atomicAdd(a.n_open_units, 1u)] = unit;
+    } else if (inside) {
       col[lp] = __builtin_fmaf(T_, fd.bg[0], C0);
       col[HW + lp] = __builtin_fmaf(T_, fd.bg[1], C1);
       col[2 * HW + lp] = __builtin_fmaf(T_, fd.bg[2], C2);
