@@ -97,7 +97,7 @@ struct Plan {
   size_t tkey0, tkey1, tid0, tid1, thist, tstatus;               // binning
   // depth-sliced binning (c.depth_split in 1..999)
   bool sliced;
-  size_t ocount, boff, slb_far, open_bm, hint_bm, open_units, hint_sat;  // geom
+  size_t ocount, boff, slb_far, open_bm, hint_bm, open_units, hint_sat, big;  // geom
   size_t tid_far;                         // binning: the far sort's second id buffer
   size_t ranges_far;                      // image
   // the near tile sort ends in (L.sorted_tiles, L.sorted_ids); its other
@@ -152,7 +152,7 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   p.slb = take((size_t)p.stiles * 4 + 4);
   L.item_off = take(4 * I);
   p.sliced = c.depth_split > 0 && c.depth_split < 1000;
-  p.ocount = p.boff = p.slb_far = p.open_bm = p.hint_bm = p.open_units = p.hint_sat = 0;
+  p.ocount = p.boff = p.slb_far = p.open_bm = p.hint_bm = p.open_units = p.hint_sat = p.big = 0;
   p.tid_far = p.ranges_far = 0;
   if (p.sliced) {
     p.ocount = take(4 * I);
@@ -161,6 +161,7 @@ bool make_plan(const W2WCaps& c, Plan& p) {
     p.open_bm = take(4 * (size_t)cdiv((uint64_t)c.tiles, 32));
     p.hint_bm = take(4 * (size_t)cdiv((uint64_t)c.tiles, 32));
     p.hint_sat = take(4 * (size_t)c.tiles);
+    p.big = take(4 * I);
     p.open_units = take(4 * (size_t)c.tiles * p.units_per_tile());
   }
   L.geom_bytes = o;
@@ -358,6 +359,22 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
     pa.dkey = at<uint32_t>(G, p.dkey0);
     pa.ids = at<uint32_t>(G, p.L.perm);
     pa.item_frame = at<uint32_t>(G, p.L.item_frame);
+    pa.hint_sat = nullptr;
+    pa.hcount = nullptr;
+    if (p.sliced && cap > 0) {
+      // depth-sliced binning: the hint is the last forward's set of tiles its
+      // near lists left unfinished (a guess that only steers work; any
+      // content gives the same outputs); hinted tiles get their far items in
+      // their near lists too, behind the near items, so depth order is kept
+      const size_t words = cdiv((uint64_t)c.tiles, 32);
+      hipLaunchKernelGGL(k_move_u32, dim3((uint32_t)std::min<size_t>(cdiv(words, 256), 1024)), dim3(256), 0, s,
+                         at<uint32_t>(G, p.hint_bm), at<uint32_t>(G, p.open_bm), words);
+      hipLaunchKernelGGL(k_tile_sat, dim3((uint32_t)std::min<int32_t>(c.frames, 4096)), dim3(256), 0, s, table,
+                         (uint32_t)c.scenes, (uint32_t)c.frames, at<uint32_t>(G, p.hint_bm),
+                         at<uint32_t>(G, p.hint_sat));
+      pa.hint_sat = at<uint32_t>(G, p.hint_sat);
+      pa.hcount = at<uint32_t>(G, p.ocount);
+    }
     {
       ProfSpan ps(W2W_KID_PREPROCESS, s);
       hipLaunchKernelGGL(k_preprocess, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 8192)),
@@ -375,7 +392,8 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
                          0, s, at<uint32_t>(G, p.L.counts), at<uint32_t>(G, p.L.perm),
                          at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.slb) + 1,
                          status + ST_SCAN_TICKET, status + ST_K, status + ST_OVERFLOW, d_items,
-                         items_cap, cap);
+                         items_cap, cap, nullptr,
+                         at<uint32_t>(G, p.L.item_off));  // the backward finds an item's records from here
     }
     if (cap > 0) {
       // depth-sliced binning: the near slice is binned here, the far slice
@@ -385,31 +403,14 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
       if (p.sliced) {
         hipLaunchKernelGGL(k_split, dim3(1), dim3(1), 0, s, table, items_cap, (uint32_t)c.depth_split,
                            at<uint32_t>(G, p.L.offsets), status, cap);
-        // hinted tiles (those the last forward's near lists left unfinished:
-        // a guess that only steers work) get their far items in their near
-        // lists too, behind the near items (depth order is kept)
-        const size_t words = cdiv((uint64_t)c.tiles, 32);
-        hipLaunchKernelGGL(k_move_u32, dim3((uint32_t)std::min<size_t>(cdiv(words, 256), 1024)), dim3(256), 0, s,
-                           at<uint32_t>(G, p.hint_bm), at<uint32_t>(G, p.open_bm), words);
-        hipLaunchKernelGGL(k_tile_sat, dim3((uint32_t)std::min<int32_t>(c.frames, 4096)), dim3(256), 0, s, table,
-                           (uint32_t)c.scenes, (uint32_t)c.frames, at<uint32_t>(G, p.hint_bm),
-                           at<uint32_t>(G, p.hint_sat));
-        {
-          ProfSpan ps(W2W_KID_COUNT_OPEN, s);
-          hipLaunchKernelGGL(k_count_tiles, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 8192)), dim3(256), 0,
-                             s, table, (uint32_t)c.scenes, items_cap, status, at<uint32_t>(G, p.L.perm),
-                             at<uint32_t>(G, p.L.offsets), at<uint2>(G, p.L.rect),
-                             at<uint32_t>(G, p.L.item_frame), at<uint32_t>(G, p.hint_bm), nullptr,
-                             at<uint32_t>(G, p.ocount), at<uint32_t>(G, p.L.item_off), nullptr,
-                             at<uint32_t>(G, p.hint_sat));
-        }
+        // far items' hinted tiles (counted by the preprocess) in depth order
         fill_u32(at<uint32_t>(G, p.slb_far), 0u, p.stiles + 1, s);
         {
           ProfSpan ps(W2W_KID_SCAN, s);
-          hipLaunchKernelGGL(k_scan<false>, dim3(std::min<uint32_t>(p.stiles, 2048)), dim3(SCAN_THREADS), 0, s,
-                             at<uint32_t>(G, p.ocount), nullptr, at<uint32_t>(G, p.boff),
+          hipLaunchKernelGGL(k_scan<true>, dim3(std::min<uint32_t>(p.stiles, 2048)), dim3(SCAN_THREADS), 0, s,
+                             at<uint32_t>(G, p.ocount), at<uint32_t>(G, p.L.perm), at<uint32_t>(G, p.boff),
                              at<uint32_t>(G, p.slb_far) + 1, status + ST_SCANH_TICKET, status + ST_KH, nullptr,
-                             d_items, items_cap, cap);
+                             d_items, items_cap, cap, status + ST_SPLIT, nullptr, nullptr);
         }
         hipLaunchKernelGGL(k_near_total, dim3(1), dim3(1), 0, s, status, cap);
         d_near_end = status + ST_SPLIT;
@@ -422,18 +423,22 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
                            at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.L.counts),
                            at<uint2>(G, p.L.rect), at<uint32_t>(G, p.L.item_frame), items_cap, cap,
                            d_near_end, status + ST_EMIT_TICKET,
-                           at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0),
-                           at<uint32_t>(G, p.L.item_off), at<uint32_t>(B, p.thist), p.tpasses,
-                           p.tlast_bits);
+                           at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0), at<uint32_t>(B, p.thist),
+                           p.tpasses, p.tlast_bits);
       }
       if (p.sliced) {
         ProfSpan ps(W2W_KID_EMIT_FAR, s);
         hipLaunchKernelGGL(k_emit_far, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 8192)), dim3(256), 0, s,
                            table, (uint32_t)c.scenes, items_cap, cap, status, at<uint32_t>(G, p.L.perm),
-                           at<uint32_t>(G, p.boff), at<uint32_t>(G, p.ocount), at<uint2>(G, p.L.rect),
+                           at<uint32_t>(G, p.boff), status + ST_KH, at<uint2>(G, p.L.rect),
                            at<uint32_t>(G, p.L.item_frame), at<uint32_t>(G, p.hint_bm), nullptr, status + ST_KA,
                            at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0), at<uint32_t>(B, p.thist),
-                           p.tpasses, p.tlast_bits);
+                           p.tpasses, p.tlast_bits, nullptr, at<uint32_t>(G, p.big), status + ST_BIG_H);
+        hipLaunchKernelGGL(k_emit_far_big, dim3(1024), dim3(256), 0, s, table, (uint32_t)c.scenes, cap,
+                           at<uint32_t>(G, p.L.perm), at<uint32_t>(G, p.boff), at<uint2>(G, p.L.rect),
+                           at<uint32_t>(G, p.L.item_frame), at<uint32_t>(G, p.hint_bm), nullptr, status + ST_KA,
+                           at<uint32_t>(G, p.big), status + ST_BIG_H, items_cap, at<uint32_t>(B, p.tkey0),
+                           at<uint32_t>(B, p.tid0), at<uint32_t>(B, p.thist), p.tpasses, p.tlast_bits, nullptr);
       }
       sort_passes<uint32_t>(at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0),
                             at<uint32_t>(B, p.L.sorted_tiles), at<uint32_t>(B, p.L.sorted_ids),
@@ -486,8 +491,8 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
     ProfSpan ps(W2W_KID_COUNT_OPEN, s);
     hipLaunchKernelGGL(k_count_tiles, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 8192)), dim3(256), 0, s,
                        table, (uint32_t)c.scenes, items_cap, status, at<uint32_t>(G, p.L.perm),
-                       at<uint32_t>(G, p.L.offsets), at<uint2>(G, p.L.rect), at<uint32_t>(G, p.L.item_frame),
-                       open_bm, hint_bm, at<uint32_t>(G, p.ocount), nullptr, status + ST_OPEN_UNITS, nullptr);
+                       at<uint2>(G, p.L.rect), at<uint32_t>(G, p.L.item_frame), open_bm, hint_bm,
+                       at<uint32_t>(G, p.ocount), status + ST_OPEN_UNITS);
   }
   fill_u32(at<uint32_t>(G, p.slb_far), 0u, p.stiles + 1, s);
   {
@@ -495,17 +500,23 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
     hipLaunchKernelGGL(k_scan<false>, dim3(std::min<uint32_t>(p.stiles, 2048)), dim3(SCAN_THREADS), 0, s,
                        at<uint32_t>(G, p.ocount), nullptr, at<uint32_t>(G, p.boff),
                        at<uint32_t>(G, p.slb_far) + 1, status + ST_SCANB_TICKET, status + ST_KB, nullptr,
-                       d_items, items_cap, cap);
+                       d_items, items_cap, cap, nullptr, nullptr, status + ST_OPEN_UNITS);
   }
   fill_u32(at<uint32_t>(B, p.thist), 0u, 4 * RADIX, s);
   {
     ProfSpan ps(W2W_KID_EMIT_FAR, s);
     hipLaunchKernelGGL(k_emit_far, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 8192)), dim3(256), 0, s,
                        table, (uint32_t)c.scenes, items_cap, cap, status, at<uint32_t>(G, p.L.perm),
-                       at<uint32_t>(G, p.boff), at<uint32_t>(G, p.ocount), at<uint2>(G, p.L.rect),
+                       at<uint32_t>(G, p.boff), status + ST_KB, at<uint2>(G, p.L.rect),
                        at<uint32_t>(G, p.L.item_frame), open_bm, hint_bm, nullptr,
                        at<uint32_t>(B, p.scratch_keys()), at<uint32_t>(B, p.scratch_ids()), at<uint32_t>(B, p.thist),
-                       p.tpasses, p.tlast_bits);
+                       p.tpasses, p.tlast_bits, status + ST_OPEN_UNITS, at<uint32_t>(G, p.big), status + ST_BIG_F);
+    hipLaunchKernelGGL(k_emit_far_big, dim3(1024), dim3(256), 0, s, table, (uint32_t)c.scenes, cap,
+                       at<uint32_t>(G, p.L.perm), at<uint32_t>(G, p.boff), at<uint2>(G, p.L.rect),
+                       at<uint32_t>(G, p.L.item_frame), open_bm, hint_bm, nullptr, at<uint32_t>(G, p.big),
+                       status + ST_BIG_F, items_cap, at<uint32_t>(B, p.scratch_keys()),
+                       at<uint32_t>(B, p.scratch_ids()), at<uint32_t>(B, p.thist), p.tpasses, p.tlast_bits,
+                       status + ST_OPEN_UNITS);
   }
   sort_passes<uint32_t>(at<uint32_t>(B, p.scratch_keys()), at<uint32_t>(B, p.scratch_ids()),
                         at<uint32_t>(B, p.far_keys()), at<uint32_t>(B, p.far_ids()),

@@ -40,6 +40,8 @@ enum StatusWord : int {
   ST_OPEN_UNITS = 5,   //   8x8 blocks left unfinished by the near slice
   ST_KH = 6,           //   far-slice instances on hinted tiles (binned with the near slice)
   ST_KNEAR = 7,        //   the near lists' instances (ST_KA + ST_KH)
+  ST_BIG_H = 31,       //   far items with many hinted / open tiles, emitted a wave each
+  ST_BIG_F = 32,
   ST_DTICKET = 8,      // 4 depth-sort tickets
   ST_TTICKET = 16,     // up to 4 tile-sort tickets
   ST_SCAN_TICKET = 24,
@@ -83,6 +85,21 @@ W2W_DEV ItemRef item_ref(const SceneDesc* scenes, uint32_t n_scenes, uint32_t i)
   return r;
 }
 
+// members of the tile rectangle [x0, x1) x [y0, y1) (non-empty) from the
+// prefix counts
+W2W_DEV uint32_t sat_rect(const uint32_t* __restrict__ sat, uint32_t t0, uint32_t tx, uint32_t x0, uint32_t x1,
+                          uint32_t y0, uint32_t y1) {
+  const uint32_t* r1 = sat + t0 + (y1 - 1) * tx;
+  uint32_t c = r1[x1 - 1];
+  if (x0 > 0) c -= r1[x0 - 1];
+  if (y0 > 0) {
+    const uint32_t* r0 = sat + t0 + (y0 - 1) * tx;
+    c -= r0[x1 - 1];
+    if (x0 > 0) c += r0[x0 - 1];
+  }
+  return c;
+}
+
 struct PreArgs {
   const void* table;
   uint32_t scenes_cap;
@@ -100,6 +117,10 @@ struct PreArgs {
   uint32_t* dkey;        // depth bits, CULLED_DEPTH_KEY when culled
   uint32_t* ids;         // iota, the values of the depth sort
   uint32_t* item_frame;
+  // depth-sliced binning: prefix counts of the hinted tiles (null: none)
+  // and each item's hinted tiles out
+  const uint32_t* hint_sat;
+  uint32_t* hcount;
 };
 
 __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
@@ -227,6 +248,7 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
     a.radii[i] = radius;
     a.rect[i] = make_uint2(x0 | (y0 << 16), x1 | (y1 << 16));
     a.counts[i] = cnt;
+    if (a.hint_sat) a.hcount[i] = cnt ? sat_rect(a.hint_sat, fd.tile_off, (uint32_t)fd.tiles_x, x0, x1, y0, y1) : 0u;
     a.dkey[i] = keep ? __float_as_uint(g.ndcz) : CULLED_DEPTH_KEY;
     a.ids[i] = i;
     a.item_frame[i] = ir.frame;
@@ -241,7 +263,9 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
 // Tiles are large (16384 values: few look-back links), loads and stores are
 // striped over the workgroup (coalesced) and transposed through LDS so that
 // each thread scans 16 consecutive values, and one wave looks back over 64
-// predecessors per step.
+// predecessors per step.  Optional: positions below *d_begin count as 0;
+// scatter[perm[i]] = out[i] (GATHER only); with *gate == 0 the scan is
+// skipped (total 0, out untouched).
 constexpr int SCAN_THREADS = 1024;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
@@ -284,12 +308,22 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(
     const uint32_t* __restrict__ in, const uint32_t* __restrict__ perm,
     uint32_t* __restrict__ out, uint32_t* __restrict__ lb, uint32_t* __restrict__ ticket,
     uint32_t* __restrict__ total_out, uint32_t* __restrict__ overflow_out,
-    const uint32_t* __restrict__ d_n, uint32_t n_cap, uint32_t capacity) {
+    const uint32_t* __restrict__ d_n, uint32_t n_cap, uint32_t capacity,
+    const uint32_t* __restrict__ d_begin = nullptr, uint32_t* __restrict__ scatter = nullptr,
+    const uint32_t* __restrict__ gate = nullptr) {
   __shared__ uint32_t s_v[SCAN_TILE + SCAN_TILE / 32];  // one pad word per 32: conflict-free transposes
   __shared__ uint32_t s_tmp[SCAN_THREADS / 64];
   __shared__ uint32_t s_tile, s_excl, s_tot;
   const int tid = threadIdx.x;
+  if (gate && *gate == 0u) {
+    if (blockIdx.x == 0 && tid == 0) {
+      *total_out = 0u;
+      if (overflow_out) *overflow_out = 0u;
+    }
+    return;
+  }
   const uint32_t n = sort_count(d_n, n_cap);
+  const uint32_t begin = d_begin ? *d_begin : 0u;
   const uint32_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
   auto pad = [](int i) { return i + (i >> 5); };
   for (;;) {
@@ -298,20 +332,23 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(
     const uint32_t tile = s_tile;
     if (tile >= ntiles) return;
     const uint32_t tb = tile * SCAN_TILE;
+    uint32_t pr[SCAN_ITEMS];  // GATHER: the permuted index of each striped position
     if constexpr (GATHER) {
-      uint32_t pr[SCAN_ITEMS];
 #pragma unroll
       for (int j = 0; j < SCAN_ITEMS; j++) {
         const uint32_t idx = tb + j * SCAN_THREADS + tid;
         pr[j] = idx < n ? perm[idx] : 0xFFFFFFFFu;
       }
 #pragma unroll
-      for (int j = 0; j < SCAN_ITEMS; j++) s_v[pad(j * SCAN_THREADS + tid)] = pr[j] != 0xFFFFFFFFu ? in[pr[j]] : 0u;
+      for (int j = 0; j < SCAN_ITEMS; j++) {
+        const uint32_t idx = tb + j * SCAN_THREADS + tid;
+        s_v[pad(j * SCAN_THREADS + tid)] = (pr[j] != 0xFFFFFFFFu && idx >= begin) ? in[pr[j]] : 0u;
+      }
     } else {
 #pragma unroll
       for (int j = 0; j < SCAN_ITEMS; j++) {
         const uint32_t idx = tb + j * SCAN_THREADS + tid;
-        s_v[pad(j * SCAN_THREADS + tid)] = idx < n ? in[idx] : 0u;
+        s_v[pad(j * SCAN_THREADS + tid)] = (idx < n && idx >= begin) ? in[idx] : 0u;
       }
     }
     __syncthreads();
@@ -346,7 +383,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; j++) {
       const uint32_t idx = tb + j * SCAN_THREADS + tid;
-      if (idx < n) out[idx] = e0 + s_v[pad(j * SCAN_THREADS + tid)];
+      if (idx < n) {
+        const uint32_t v = e0 + s_v[pad(j * SCAN_THREADS + tid)];
+        out[idx] = v;
+        if constexpr (GATHER) {
+          if (scatter) scatter[pr[j]] = v;
+        }
+      }
     }
     __syncthreads();  // s_v and s_tile are rewritten by the next tile
   }
@@ -419,21 +462,6 @@ __global__ __launch_bounds__(256) void k_tile_sat(const void* __restrict__ table
   }
 }
 
-// members of the tile rectangle [x0, x1) x [y0, y1) (non-empty) from the
-// prefix counts
-W2W_DEV uint32_t sat_rect(const uint32_t* __restrict__ sat, uint32_t t0, uint32_t tx, uint32_t x0, uint32_t x1,
-                          uint32_t y0, uint32_t y1) {
-  const uint32_t* r1 = sat + t0 + (y1 - 1) * tx;
-  uint32_t c = r1[x1 - 1];
-  if (x0 > 0) c -= r1[x0 - 1];
-  if (y0 > 0) {
-    const uint32_t* r0 = sat + t0 + (y0 - 1) * tx;
-    c -= r0[x1 - 1];
-    if (x0 > 0) c += r0[x0 - 1];
-  }
-  return c;
-}
-
 // The near lists' total: the near slice's instances and the far items'
 // instances on hinted tiles.
 __global__ void k_near_total(uint32_t* __restrict__ status, uint32_t capacity) {
@@ -441,59 +469,55 @@ __global__ void k_near_total(uint32_t* __restrict__ status, uint32_t capacity) {
 }
 
 // Per depth position of the far slice (0 in the near slice): how many tiles
-// of its item's rectangle are in the tile set (a minus b).  With item_off
-// given, far items also get their full-layout offset there (the backward's
-// record slots), whether or not a tile is counted.  With *gate == 0 the set
-// is known to be empty: every count is 0.  With sat given (the set's prefix
-// counts, b null) a rectangle costs four reads.
+// of its item's rectangle are in the tile set (a minus b).  With *gate == 0
+// the set is known to be empty and nothing is written (the scan and the
+// emission behind it are gated the same way).
 __global__ __launch_bounds__(256) void k_count_tiles(const void* __restrict__ table, uint32_t scenes_cap,
                                                      uint32_t items_cap, const uint32_t* __restrict__ status,
                                                      const uint32_t* __restrict__ perm,
-                                                     const uint32_t* __restrict__ offs,
                                                      const uint2* __restrict__ rect,
                                                      const uint32_t* __restrict__ item_frame,
                                                      const uint32_t* __restrict__ set_a,
                                                      const uint32_t* __restrict__ set_b,
                                                      uint32_t* __restrict__ count,
-                                                     uint32_t* __restrict__ item_off,
-                                                     const uint32_t* __restrict__ gate,
-                                                     const uint32_t* __restrict__ sat) {
+                                                     const uint32_t* __restrict__ gate) {
+  if (gate && *gate == 0u) return;
   const FrameDesc* frames = table_frames(table, scenes_cap);
   const uint32_t P = min(static_cast<const BatchHeader*>(table)->n_items, items_cap);
   const uint32_t S = status[ST_SPLIT];
-  const bool empty = gate && *gate == 0u;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
     uint32_t c = 0;
-    if (i >= S && !(empty && !item_off)) {
+    if (i >= S) {
       const uint32_t g = perm[i];
-      if (item_off) item_off[g] = offs[i];
       const uint2 r = rect[g];
       const uint32_t x0 = r.x & 0xffffu, y0 = r.x >> 16, x1 = r.y & 0xffffu, y1 = r.y >> 16;
-      if (!empty && x1 > x0 && y1 > y0) {
+      if (x1 > x0 && y1 > y0) {
         const FrameDesc& fd = frames[item_frame[g]];
-        if (sat) {
-          c = sat_rect(sat, fd.tile_off, (uint32_t)fd.tiles_x, x0, x1, y0, y1);
-        } else {
-          for (uint32_t ty = y0; ty < y1; ty++)
-            c += tileset_count(set_a, set_b, fd.tile_off + ty * (uint32_t)fd.tiles_x + x0, x1 - x0);
-        }
+        for (uint32_t ty = y0; ty < y1; ty++)
+          c += tileset_count(set_a, set_b, fd.tile_off + ty * (uint32_t)fd.tiles_x + x0, x1 - x0);
       }
     }
     count[i] = c;
   }
 }
 
+constexpr uint32_t kFarSerial = 32;
+
 // Far items' instances on the tiles of a set (a minus b): one thread per far
 // depth position (most have none and stop at once), the item's member tiles
-// in row-major order from *d_base (null: 0) + its compact offset.  An
+// in row-major order from *d_base (null: 0) + its compact offset coff[i]
+// (an exclusive scan whose total is *d_total: counts are differences).  An
 // instance's key is its tile's bit index.  Adds the keys' digits to the tile
-// sort's histogram (hist[pass][256]) like k_emit.
+// sort's histogram (hist[pass][256]) like k_emit.  Skipped when *gate == 0.
+// An item with more than kFarSerial instances is left to k_emit_far_big
+// (its depth position appended to big[], *n_big counts them): one thread
+// would hold up its wave.
 __global__ __launch_bounds__(256) void k_emit_far(const void* __restrict__ table, uint32_t scenes_cap,
                                                   uint32_t items_cap, uint32_t capacity,
                                                   const uint32_t* __restrict__ status,
                                                   const uint32_t* __restrict__ perm,
                                                   const uint32_t* __restrict__ coff,
-                                                  const uint32_t* __restrict__ count,
+                                                  const uint32_t* __restrict__ d_total,
                                                   const uint2* __restrict__ rect,
                                                   const uint32_t* __restrict__ item_frame,
                                                   const uint32_t* __restrict__ set_a,
@@ -502,7 +526,9 @@ __global__ __launch_bounds__(256) void k_emit_far(const void* __restrict__ table
                                                   uint32_t* __restrict__ tile_keys,
                                                   uint32_t* __restrict__ ids,
                                                   uint32_t* __restrict__ hist, int hist_passes,
-                                                  int hist_last_bits) {
+                                                  int hist_last_bits, const uint32_t* __restrict__ gate,
+                                                  uint32_t* __restrict__ big, uint32_t* __restrict__ n_big) {
+  if (gate && *gate == 0u) return;
   __shared__ uint32_t s_hist[4][256];
   for (int k = threadIdx.x; k < 4 * 256; k += blockDim.x) (&s_hist[0][0])[k] = 0u;
   __syncthreads();
@@ -510,9 +536,14 @@ __global__ __launch_bounds__(256) void k_emit_far(const void* __restrict__ table
   const uint32_t P = min(static_cast<const BatchHeader*>(table)->n_items, items_cap);
   const uint32_t S = status[ST_SPLIT];
   const uint32_t base = d_base ? *d_base : 0u;
+  const uint32_t total = *d_total;
   for (uint32_t i = S + blockIdx.x * blockDim.x + threadIdx.x; i < P; i += gridDim.x * blockDim.x) {
-    const uint32_t c = count[i];
+    const uint32_t c = (i + 1 < P ? coff[i + 1] : total) - coff[i];
     if (c == 0) continue;
+    if (c > kFarSerial) {
+      big[atomicAdd(n_big, 1u)] = i;
+      continue;
+    }
     const uint32_t g = perm[i];
     const uint2 r = rect[g];
     const uint32_t x0 = r.x & 0xffffu, y0 = r.x >> 16, x1 = r.y & 0xffffu, y1 = r.y >> 16;
@@ -548,6 +579,68 @@ __global__ __launch_bounds__(256) void k_emit_far(const void* __restrict__ table
   }
 }
 
+// The far items k_emit_far left (more than kFarSerial instances): a wave per
+// item walks its rectangle 64 tiles at a time in row-major order; member
+// tiles are placed by ballot prefix counts.  Same outputs as the serial walk.
+__global__ __launch_bounds__(256) void k_emit_far_big(const void* __restrict__ table, uint32_t scenes_cap,
+                                                      uint32_t capacity, const uint32_t* __restrict__ perm,
+                                                      const uint32_t* __restrict__ coff,
+                                                      const uint2* __restrict__ rect,
+                                                      const uint32_t* __restrict__ item_frame,
+                                                      const uint32_t* __restrict__ set_a,
+                                                      const uint32_t* __restrict__ set_b,
+                                                      const uint32_t* __restrict__ d_base,
+                                                      const uint32_t* __restrict__ big,
+                                                      const uint32_t* __restrict__ n_big, uint32_t big_cap,
+                                                      uint32_t* __restrict__ tile_keys,
+                                                      uint32_t* __restrict__ ids,
+                                                      uint32_t* __restrict__ hist, int hist_passes,
+                                                      int hist_last_bits, const uint32_t* __restrict__ gate) {
+  if (gate && *gate == 0u) return;
+  __shared__ uint32_t s_hist[4][256];
+  for (int k = threadIdx.x; k < 4 * 256; k += blockDim.x) (&s_hist[0][0])[k] = 0u;
+  __syncthreads();
+  const FrameDesc* frames = table_frames(table, scenes_cap);
+  const uint32_t nb = min(*n_big, big_cap);
+  const uint32_t base = d_base ? *d_base : 0u;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < nb; w += waves) {
+    const uint32_t i = big[w];
+    const uint32_t g = perm[i];
+    const uint2 r = rect[g];
+    const uint32_t x0 = r.x & 0xffffu, y0 = r.x >> 16, x1 = r.y & 0xffffu, y1 = r.y >> 16;
+    const uint32_t rw = x1 - x0, n = rw * (y1 - y0);
+    const FrameDesc& fd = frames[item_frame[g]];
+    uint32_t o = base + coff[i];
+    for (uint32_t t0 = 0; t0 < n; t0 += 64) {
+      const uint32_t t = t0 + lane;
+      bool member = false;
+      uint32_t key = 0;
+      if (t < n) {
+        key = fd.tile_off + (y0 + t / rw) * (uint32_t)fd.tiles_x + x0 + t % rw;
+        member = (tileset_word(set_a, set_b, key >> 5) >> (key & 31u)) & 1u;
+      }
+      const uint64_t m = __ballot(member);
+      const uint32_t at = o + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+      if (member && at < capacity) {
+        tile_keys[at] = key;
+        ids[at] = g;
+        for (int p = 0; p < hist_passes; p++) {
+          const int bits = (p == hist_passes - 1) ? hist_last_bits : 8;
+          atomicAdd(&s_hist[p][(key >> (8 * p)) & ((1u << bits) - 1u)], 1u);
+        }
+      }
+      o += (uint32_t)__popcll(m);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < hist_passes * 256; k += blockDim.x) {
+    const uint32_t v = (&s_hist[0][0])[k];
+    if (v) atomicAdd(hist + k, v);
+  }
+}
+
 // Instance emission in depth order.  A workgroup owns depth-sorted positions
 // [256b, 256b+256) and the contiguous instance range they cover; each thread
 // strides over instances and finds its owner by binary search in LDS.  Chunks
@@ -568,7 +661,6 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
                                               uint32_t* __restrict__ ticket,
                                               uint32_t* __restrict__ tile_keys,
                                               uint32_t* __restrict__ ids,
-                                              uint32_t* __restrict__ item_off,
                                               uint32_t* __restrict__ hist, int hist_passes,
                                               int hist_last_bits) {
   __shared__ uint32_t s_start[256], s_g[256], s_x0[256], s_y0[256], s_w[256], s_tx[256], s_toff[256];
@@ -596,7 +688,6 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
       const FrameDesc& fd = frames[item_frame[g]];
       s_start[tid] = offs[i];
       s_g[tid] = g;
-      item_off[g] = offs[i];  // the backward finds an item's records from here
       s_x0[tid] = r.x & 0xffffu;
       s_y0[tid] = r.x >> 16;
       s_w[tid] = (r.y & 0xffffu) - (r.x & 0xffffu);
