@@ -463,7 +463,9 @@ def bench_batched(args):
     V = sum(len(j.cameras) for j in jobs)
     log(f"batched: {len(jobs)} scenes, {G} Gaussians, {V} views")
     targets = _targets(jobs, dev, args.tile)
-    bt = BatchedTrainer([j.scene for j in jobs], [j.cameras for j in jobs], targets, dev, args.tile)
+    split = max(args.depth_split, 0.0)  # this workload is not swept: 0 unless given
+    bt = BatchedTrainer([j.scene for j in jobs], [j.cameras for j in jobs], targets, dev, args.tile,
+                        depth_split=split)
     for i in range(max(args.warmup, 1)):
         bt.step()
         torch.cuda.synchronize()
@@ -516,7 +518,7 @@ def bench_batched(args):
            "dtype": "fp32", "data": "synthetic",
            "config": {"workload": f"{len(jobs)} scenes, Gaussians log-uniform [10k,500k] (total {G}), SH3, "
                                   "4 views each at {256,512,1024}^2, fwd+bwd+Adam",
-                      "graph": not args.no_graph, "tile": args.tile},
+                      "graph": not args.no_graph, "tile": args.tile, "depth_split": split},
            "plain_views_per_s": V / tp, "speedup_vs_plain": tp / tb,
            "latency_ms": {"p50": float(np.percentile(lat_b, 50)), "p99": float(np.percentile(lat_b, 99))}}
     print(json.dumps(out), flush=True)
