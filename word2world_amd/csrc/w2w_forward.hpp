@@ -6,13 +6,21 @@
 //                  ((frame, Gaussian) pair)
 //   depth sort     (depth bits, item) over all items with the onesweep sort
 //   k_scan         tile counts gathered in depth order, single-pass decoupled
-//                  look-back exclusive scan -> offsets, K
+//                  look-back exclusive scan -> offsets, K, each item's offset
 //   k_emit         instance keys in depth order, block-level load balancing
 //                  (prepareSort.cu:239-301 emitted one Gaussian per thread
 //                  with a serial tile loop)
 //   tile sort      stable sort of the 32-bit global tile ids
 //   k_ranges       gaussianTiles.cu:6-33
 //   k_blend        pixelGaussiansFull.cu:3-98 / pixelGaussiansDev.cuh:20-79
+//
+// Depth-sliced binning (an option of the batched path; same outputs): the
+// emission, sort and ranges run first over the near lists (the nearest items,
+// plus the far items on the tiles the previous forward left unfinished), the
+// blend stores unfinished pixels' state (BLEND_NEAR), and only the tiles left
+// open get far lists (k_count_tiles, k_scan, k_emit_far, sort, ranges) and a
+// resumed blend (BLEND_FAR).  A tile's lists, near then far, are its full list
+// in depth order cut where every pixel is done.
 //
 // Sorting the depth keys first and then stably sorting the tile ids gives
 // exactly the order of the reference's stable sort of 64-bit keys
