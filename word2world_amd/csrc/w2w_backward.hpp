@@ -394,6 +394,8 @@ struct PreBwdArgs {
   const float* rec;          // [capacity][GR_FIELDS] per-instance partials
   const uint8_t* flag;       // [capacity]
   uint32_t capacity;
+  const uint32_t* item_frame;  // per item: its frame
+  float* item_grad;          // per item [8]: dL/dconic a, b, c, dL/dopacity, dL/ddepth, 3 unused
   float* dL_dmeans2D;    // per item [3]  (NDC units, z = 0)
   float* dL_dcolors;     // per item [3]
   float* dL_dopacity;    // per Gaussian
@@ -403,6 +405,53 @@ struct PreBwdArgs {
   float* dL_dscales;     // per Gaussian [3]
   float* dL_drots;       // per Gaussian [4]
 };
+
+// One thread per item: the sum of its instance records in emission order
+// (the flags are scanned eight at a time; the flag array is padded to 256
+// bytes).  Writes the item's dL/dmean2D and dL/dcolor outputs and the other
+// sums to item_grad for k_preprocess_bwd.  A light kernel with four times
+// the threads of the per-Gaussian one: the dependent flag -> record loads
+// are hidden by occupancy rather than serialised over a Gaussian's views.
+__global__ __launch_bounds__(256) void k_item_grads(PreBwdArgs a) {
+  const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
+  const uint32_t n_items = min(hdr->n_items, a.items_cap);
+  const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_items; i += gridDim.x * blockDim.x) {
+    const bool vis = a.radii[i] > 0 && (frames[a.item_frame[i]].flags & FRAME_BACKWARD);
+    float ga[GR_FIELDS];
+#pragma unroll
+    for (int f = 0; f < GR_FIELDS; f++) ga[f] = 0.0f;
+    if (vis) {
+      const uint32_t k0 = a.item_off[i];
+      const uint32_t k1 = min(k0 + a.counts[i], a.capacity);
+      for (uint32_t kw = k0 & ~7u; kw < k1; kw += 8) {
+        uint64_t f8 = *reinterpret_cast<const uint64_t*>(a.flag + kw);
+        if (kw < k0) f8 &= ~0ull << (8 * (k0 - kw));
+        if (k1 - kw < 8) f8 &= ~(~0ull << (8 * (k1 - kw)));
+        while (f8) {
+          const uint32_t b = (uint32_t)__builtin_ctzll(f8) >> 3;
+          f8 &= ~(0xFFull << (8 * b));
+          const float2* r = reinterpret_cast<const float2*>(a.rec + (size_t)(kw + b) * GR_FIELDS);
+#pragma unroll
+          for (int f = 0; f < GR_FIELDS / 2; f++) {
+            const float2 rv = r[f];
+            ga[2 * f] += rv.x;
+            ga[2 * f + 1] += rv.y;
+          }
+        }
+      }
+    }
+    a.dL_dmeans2D[3 * (size_t)i] = ga[GA_MX];
+    a.dL_dmeans2D[3 * (size_t)i + 1] = ga[GA_MY];
+    a.dL_dmeans2D[3 * (size_t)i + 2] = 0.0f;
+    a.dL_dcolors[3 * (size_t)i] = ga[GA_R];
+    a.dL_dcolors[3 * (size_t)i + 1] = ga[GA_G];
+    a.dL_dcolors[3 * (size_t)i + 2] = ga[GA_B];
+    float4* ig = reinterpret_cast<float4*>(a.item_grad + 8 * (size_t)i);
+    ig[0] = make_float4(ga[GA_CA], ga[GA_CB], ga[GA_CC], ga[GA_OP]);
+    ig[1] = make_float4(ga[GA_DEPTH], 0.0f, 0.0f, 0.0f);
+  }
+}
 
 // One thread per Gaussian: loops over the views of its scene in this batch,
 // recomputes each view's projection and sums the chain rule in registers, so
@@ -430,36 +479,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       if (i >= a.items_cap) break;
       const FrameDesc& fd = frames[sd.frame_off + v];
       const bool vis = a.radii[i] > 0 && (fd.flags & FRAME_BACKWARD);
-      // sum of the item's instance records, in emission order
+      // the item's record sums (k_item_grads)
       float ga[GR_FIELDS];
       for (int f = 0; f < GR_FIELDS; f++) ga[f] = 0.0f;
       if (vis) {
-        // flags eight at a time (the flag array is padded to 256 bytes)
-        const uint32_t k0 = a.item_off[i];
-        const uint32_t k1 = min(k0 + a.counts[i], a.capacity);
-        for (uint32_t kw = k0 & ~7u; kw < k1; kw += 8) {
-          uint64_t f8 = *reinterpret_cast<const uint64_t*>(a.flag + kw);
-          if (kw < k0) f8 &= ~0ull << (8 * (k0 - kw));
-          if (k1 - kw < 8) f8 &= ~(~0ull << (8 * (k1 - kw)));
-          while (f8) {
-            const uint32_t b = (uint32_t)__builtin_ctzll(f8) >> 3;
-            f8 &= ~(0xFFull << (8 * b));
-            const float2* r = reinterpret_cast<const float2*>(a.rec + (size_t)(kw + b) * GR_FIELDS);
-#pragma unroll
-            for (int f = 0; f < GR_FIELDS / 2; f++) {
-              const float2 rv = r[f];
-              ga[2 * f] += rv.x;
-              ga[2 * f + 1] += rv.y;
-            }
-          }
-        }
+        ga[GA_MX] = a.dL_dmeans2D[3 * (size_t)i];
+        ga[GA_MY] = a.dL_dmeans2D[3 * (size_t)i + 1];
+        const float4* ig = reinterpret_cast<const float4*>(a.item_grad + 8 * (size_t)i);
+        const float4 t0 = ig[0], t1 = ig[1];
+        ga[GA_CA] = t0.x;
+        ga[GA_CB] = t0.y;
+        ga[GA_CC] = t0.z;
+        ga[GA_OP] = t0.w;
+        ga[GA_DEPTH] = t1.x;
       }
-      a.dL_dmeans2D[3 * (size_t)i] = vis ? ga[GA_MX] : 0.0f;
-      a.dL_dmeans2D[3 * (size_t)i + 1] = vis ? ga[GA_MY] : 0.0f;
-      a.dL_dmeans2D[3 * (size_t)i + 2] = 0.0f;
-      a.dL_dcolors[3 * (size_t)i] = vis ? ga[GA_R] : 0.0f;
-      a.dL_dcolors[3 * (size_t)i + 1] = vis ? ga[GA_G] : 0.0f;
-      a.dL_dcolors[3 * (size_t)i + 2] = vis ? ga[GA_B] : 0.0f;
       if (!vis) continue;
       gop += ga[GA_OP];
       Cam c;

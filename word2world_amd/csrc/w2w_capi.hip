@@ -151,6 +151,7 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   p.dstatus = take((size_t)4 * p.dtiles * RADIX * 4);
   p.slb = take((size_t)p.stiles * 4 + 4);
   L.item_off = take(4 * I);
+  L.item_grad = take(32 * I);
   p.sliced = c.depth_split > 0 && c.depth_split < 1000;
   p.ocount = p.boff = p.slb_far = p.open_bm = p.hint_bm = p.open_units = p.hint_sat = p.big = 0;
   p.tid_far = p.ranges_far = 0;
@@ -585,11 +586,15 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
   pa.rec = rec;
   pa.flag = flag;
   pa.capacity = (uint32_t)c.capacity;
+  pa.item_frame = at<uint32_t>(G, p.L.item_frame);
+  pa.item_grad = at<float>(G, p.L.item_grad);
   pa.dL_dmeans2D = dL_dmeans2D; pa.dL_dcolors = dL_dcolors; pa.dL_dopacity = dL_dopacity;
   pa.dL_dmeans3D = dL_dmeans3D; pa.dL_dcov3D = dL_dcov3D; pa.dL_dsh = dL_dsh;
   pa.dL_dscales = dL_dscales; pa.dL_drots = dL_drotations;
   if (c.gaussians > 0) {
     ProfSpan ps(W2W_KID_PREPROCESS_BWD, s);
+    hipLaunchKernelGGL(k_item_grads, dim3(std::min<uint32_t>(cdiv((uint64_t)c.items, 256), 16384)), dim3(256), 0, s,
+                       pa);
     hipLaunchKernelGGL(k_preprocess_bwd, dim3(std::min<uint32_t>(cdiv((uint64_t)c.gaussians, 256), 8192)),
                        dim3(256), 0, s, pa);
   }
