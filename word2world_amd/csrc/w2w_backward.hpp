@@ -474,6 +474,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const float p[3] = {a.means[3 * gg], a.means[3 * gg + 1], a.means[3 * gg + 2]};
     const float s[3] = {a.scales[3 * gg], a.scales[3 * gg + 1], a.scales[3 * gg + 2]};
     const float q[4] = {a.rots[4 * gg], a.rots[4 * gg + 1], a.rots[4 * gg + 2], a.rots[4 * gg + 3]};
+    float sig[6];  // Sigma does not depend on the view: once per Gaussian
+    {
+      float R0[9], M0[9];
+      sigma_of(q, s, R0, M0, sig);
+    }
     for (uint32_t v = 0; v < sd.n_views; v++) {
       const uint32_t i = sd.item_off + g * sd.n_views + v;
       if (i >= a.items_cap) break;
@@ -498,7 +503,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       Cam c;
       load_cam(c, fd);
       Geom geo;
-      geometry(c, p, s, q, geo);
+      geometry_sig(c, p, sig, geo);
       if (!geo.visible) continue;
       const float* V = c.V;
       const float* F = c.F;

@@ -278,7 +278,28 @@ W2W_DEV void sh_visit(float x, float y, float z, F&& f) {
   }
 }
 
-W2W_DEV void geometry(const Cam& c, const float p[3], const float s[3], const float q[4], Geom& g) {
+// R, M = R diag(s) and Sigma = M M^T (xx, xy, xz, yy, yz, zz) of a Gaussian.
+W2W_DEV void sigma_of(const float q[4], const float s[3], float R[9], float Mm[9], float sig[6]) {
+  float w = q[0], x = q[1], y = q[2], z = q[3];
+  float xx = x * x, yy = y * y, zz = z * z;
+  float xy = x * y, xz = x * z, yz = y * z;
+  float wx = w * x, wy = w * y, wz = w * z;
+  R[0] = 1.0f - 2.0f * (yy + zz); R[1] = 2.0f * (xy - wz);        R[2] = 2.0f * (xz + wy);
+  R[3] = 2.0f * (xy + wz);        R[4] = 1.0f - 2.0f * (xx + zz); R[5] = 2.0f * (yz - wx);
+  R[6] = 2.0f * (xz - wy);        R[7] = 2.0f * (yz + wx);        R[8] = 1.0f - 2.0f * (xx + yy);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) Mm[3 * i + j] = R[3 * i + j] * s[j];
+#define W2W_SIGE(i, j) ((Mm[3 * (i) + 0] * Mm[3 * (j) + 0] + Mm[3 * (i) + 1] * Mm[3 * (j) + 1]) + Mm[3 * (i) + 2] * Mm[3 * (j) + 2])
+  sig[0] = W2W_SIGE(0, 0); sig[1] = W2W_SIGE(0, 1); sig[2] = W2W_SIGE(0, 2);
+  sig[3] = W2W_SIGE(1, 1); sig[4] = W2W_SIGE(1, 2); sig[5] = W2W_SIGE(2, 2);
+#undef W2W_SIGE
+}
+
+// geometry() with the view-independent Sigma (6 entries) given: everything
+// else of the projection, per view.  g.R and g.M are not set.
+W2W_DEV void geometry_sig(const Cam& c, const float p[3], const float sig[6], Geom& g) {
   const float* V = c.V;
   g.visible = false;
   float tx = ((V[0] * p[0] + V[4] * p[1]) + V[8] * p[2]) + V[12];
@@ -312,23 +333,8 @@ W2W_DEV void geometry(const Cam& c, const float p[3], const float s[3], const fl
     g.jx[k] = c.fx * (right * zi - (g.txc * fwd) * zi2);
     g.jy[k] = c.fy * (up * zi - (g.tyc * fwd) * zi2);
   }
-  float w = q[0], x = q[1], y = q[2], z = q[3];
-  float xx = x * x, yy = y * y, zz = z * z;
-  float xy = x * y, xz = x * z, yz = y * z;
-  float wx = w * x, wy = w * y, wz = w * z;
-  float* R = g.R;
-  R[0] = 1.0f - 2.0f * (yy + zz); R[1] = 2.0f * (xy - wz);        R[2] = 2.0f * (xz + wy);
-  R[3] = 2.0f * (xy + wz);        R[4] = 1.0f - 2.0f * (xx + zz); R[5] = 2.0f * (yz - wx);
-  R[6] = 2.0f * (xz - wy);        R[7] = 2.0f * (yz + wx);        R[8] = 1.0f - 2.0f * (xx + yy);
-  float* Mm = g.M;
 #pragma unroll
-  for (int i = 0; i < 3; i++)
-#pragma unroll
-    for (int j = 0; j < 3; j++) Mm[3 * i + j] = R[3 * i + j] * s[j];
-#define W2W_SIGE(i, j) ((Mm[3 * (i) + 0] * Mm[3 * (j) + 0] + Mm[3 * (i) + 1] * Mm[3 * (j) + 1]) + Mm[3 * (i) + 2] * Mm[3 * (j) + 2])
-  g.Sig[0] = W2W_SIGE(0, 0); g.Sig[1] = W2W_SIGE(0, 1); g.Sig[2] = W2W_SIGE(0, 2);
-  g.Sig[3] = W2W_SIGE(1, 1); g.Sig[4] = W2W_SIGE(1, 2); g.Sig[5] = W2W_SIGE(2, 2);
-#undef W2W_SIGE
+  for (int k = 0; k < 6; k++) g.Sig[k] = sig[k];
   const float* S6 = g.Sig;
   float v0 = (S6[0] * g.jx[0] + S6[1] * g.jx[1]) + S6[2] * g.jx[2];
   float v1 = (S6[1] * g.jx[0] + S6[3] * g.jx[1]) + S6[4] * g.jx[2];
@@ -357,6 +363,12 @@ W2W_DEV void geometry(const Cam& c, const float p[3], const float s[3], const fl
   g.px = ((ndcx + 1.0f) * 0.5f) * (float)c.W - 0.5f;
   g.py = ((ndcy + 1.0f) * 0.5f) * (float)c.H - 0.5f;
   g.visible = true;
+}
+
+W2W_DEV void geometry(const Cam& c, const float p[3], const float s[3], const float q[4], Geom& g) {
+  float sig[6];
+  sigma_of(q, s, g.R, g.M, sig);
+  geometry_sig(c, p, sig, g);
 }
 
 W2W_DEV uint32_t tile_coord(float v, uint32_t n) {
