@@ -927,6 +927,8 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
     const float x0 = (float)ux, x1 = (float)(ux + kUnit - 1);
     const float y0 = (float)uy, y1 = (float)(uy + kUnit - 1);
     const uint2 range_near = a.ranges[tile];
+    // the tile's hint bit, read now so the load is done by the block's end
+    const uint32_t hint_word = MODE == BLEND_NEAR ? a.hint_bm[tile >> 5] : 0u;
     const uint2 range = MODE == BLEND_FAR ? a.ranges_far[tile] : range_near;
     const uint32_t* const ids = MODE == BLEND_FAR ? a.ids_far : a.ids;
     const uint32_t lbase = MODE == BLEND_FAR ? range_near.y - range_near.x : 0u;
@@ -1026,7 +1028,7 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
       __builtin_amdgcn_wave_barrier();  // reads of this chunk precede the next chunk's writes
     }
     const bool open = MODE == BLEND_NEAR && !wave_all(done);
-    const bool unfinished = open && !((a.hint_bm[tile >> 5] >> (tile & 31u)) & 1u);
+    const bool unfinished = open && !((hint_word >> (tile & 31u)) & 1u);
     if (open && lane == 0) atomicOr(a.open_bm + (tile >> 5), 1u << (tile & 31u));
     if (unfinished) {
       if (inside) {
