@@ -81,6 +81,34 @@ W2W_DEV float wave_sum2(float a, float b) {
   return row_sum16(fold16(fold32(a, b), 0.0f));
 }
 
+// wave_sum4(a0, b0, c0, d0), wave_sum4(a1, b1, c1, d1) and wave_sum2(a2, b2)
+// with the same additions, but the five half-wave swaps issued back to back
+// and then the three row swaps: one pair of wait states per stage instead of
+// one per swap (the swaps of a stage read and write independent registers).
+W2W_DEV void wave_sum10(float a0, float b0, float c0, float d0, float a1, float b1, float c1, float d1, float a2,
+                        float b2, float& s0, float& s1, float& s2) {
+  asm volatile(
+      "s_nop 1\n\t"
+      "v_permlane32_swap_b32 %0, %1\n\t"
+      "v_permlane32_swap_b32 %2, %3\n\t"
+      "v_permlane32_swap_b32 %4, %5\n\t"
+      "v_permlane32_swap_b32 %6, %7\n\t"
+      "v_permlane32_swap_b32 %8, %9\n\t"
+      "s_nop 1"
+      : "+v"(a0), "+v"(b0), "+v"(c0), "+v"(d0), "+v"(a1), "+v"(b1), "+v"(c1), "+v"(d1), "+v"(a2), "+v"(b2));
+  float x0 = a0 + b0, y0 = c0 + d0, x1 = a1 + b1, y1 = c1 + d1, x2 = a2 + b2, y2 = 0.0f;
+  asm volatile(
+      "s_nop 1\n\t"
+      "v_permlane16_swap_b32 %0, %1\n\t"
+      "v_permlane16_swap_b32 %2, %3\n\t"
+      "v_permlane16_swap_b32 %4, %5\n\t"
+      "s_nop 1"
+      : "+v"(x0), "+v"(y0), "+v"(x1), "+v"(y1), "+v"(x2), "+v"(y2));
+  s0 = row_sum16(x0 + y0);
+  s1 = row_sum16(x1 + y1);
+  s2 = row_sum16(x2 + y2);
+}
+
 W2W_DEV float lane63(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
 }
@@ -150,7 +178,7 @@ void k_blend_bwd(BlendBwdArgs a) {
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: LDS addresses in scalars
   const int row = lane >> 4;
   const bool row_tail = (lane & 15) == 15;
-  // which field each row holds after wave_sum4 / wave_sum2
+  // which field each row holds after wave_sum4 / wave_sum2 (and wave_sum10)
   const int fa = (row & 1) ? (row >> 1) + 2 : (row >> 1);  // rows -> fields 0, 2, 1, 3
   const float a_min = 0x1.010102p-8f;
   uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
@@ -324,9 +352,9 @@ void k_blend_bwd(BlendBwdArgs a) {
           }
         }
         {
-          const float s0 = wave_sum4(g_mx, g_my, g_ca, g_cb);  // rows: mx, ca, my, cb
-          const float s1 = wave_sum4(g_cc, g_op, g_r, g_g);    // rows: cc, r, op, g
-          const float s2 = wave_sum2(g_b, g_d);                // rows: b, 0, depth, 0
+          // rows of s0: mx, ca, my, cb; s1: cc, r, op, g; s2: b, 0, depth, 0
+          float s0, s1, s2;
+          wave_sum10(g_mx, g_my, g_ca, g_cb, g_cc, g_op, g_r, g_g, g_b, g_d, s0, s1, s2);
           if (row_tail) {
             s_part[w][fa][j] = s0;
             s_part[w][4 + fa][j] = s1;
