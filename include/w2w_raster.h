@@ -96,6 +96,7 @@ typedef struct W2WLayout {
   size_t grad_rec;   /* binning: float[capacity][10] per-instance backward partials */
   size_t grad_flag;  /* binning: uint8[capacity] 1 where grad_rec holds a partial */
   size_t item_grad;  /* geom: float[items][8] per-item record sums of the backward */
+  size_t cullk;      /* geom: float[items][4] per-splat part of the blends' block test */
 } W2WLayout;
 
 /* Capacities of a batched workspace: every grid is sized by these, and every

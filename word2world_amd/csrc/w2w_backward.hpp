@@ -120,6 +120,7 @@ struct BlendBwdArgs {
   const uint2* ranges;
   const uint32_t* ids;
   const Splat* splats;
+  const float4* cullk;      // per item: the block test's per-splat part
   const uint2* rect;        // per item: tile rectangle (x0 | y0 << 16, x1 | y1 << 16)
   const uint32_t* item_off; // per item: emission position of its first instance
   unsigned long long* stats;  // optional [3]: evaluated pairs, contributing pairs, loaded instances
@@ -159,7 +160,7 @@ struct BlendBwdArgs {
 // the partials differ from the AUX variant's with zero inputs at most in the
 // sign of a zero.
 template <int TS, bool AUX>
-__global__ __launch_bounds__(64 * units_per_tile<TS>()) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__(64 * units_per_tile<TS>()) __attribute__((amdgpu_waves_per_eu(4, 4)))
 void k_blend_bwd(BlendBwdArgs a) {
   constexpr int UPT = (int)units_per_tile<TS>();
   constexpr int UX = TS / kUnit;
@@ -248,6 +249,7 @@ void k_blend_bwd(BlendBwdArgs a) {
     // gather splat 0 (present when nproc > 0).
     uint32_t id_n = 0;
     Splat cur = {};
+    float4 kcur = {};
     uint2 rect_cur = make_uint2(0u, 0u);
     uint32_t off_cur = 0;
     if (nproc > 0) {
@@ -255,6 +257,7 @@ void k_blend_bwd(BlendBwdArgs a) {
       id_n = k >= 0 ? id_at(k) : 0u;
       const uint32_t id0 = id_n < P ? id_n : 0u;
       cur = a.splats[id0];
+      kcur = a.cullk[id0];
       rect_cur = a.rect[id0];
       off_cur = a.item_off[id0];
       id_n = k >= 64 ? id_at(k - 64) : 0u;
@@ -262,11 +265,13 @@ void k_blend_bwd(BlendBwdArgs a) {
     for (int cend = nproc; cend > 0; cend -= 64) {
       const int cstart = cend - 64;  // may be negative: lanes below 0 are empty
       const Splat s = cur;
+      const float4 kq = kcur;
       const uint2 srect = rect_cur;
       const uint32_t soff = off_cur;
       if (cstart > 0) {
         const uint32_t idc = id_n < P ? id_n : 0u;
         cur = a.splats[idc];
+        kcur = a.cullk[idc];
         rect_cur = a.rect[idc];
         off_cur = a.item_off[idc];
         const int k2 = cstart - 128 + lane;
@@ -276,7 +281,7 @@ void k_blend_bwd(BlendBwdArgs a) {
       n_loaded += (uint32_t)min(64, cend);
       const bool valid = kk_l >= 0 && kk_l < wmax;
       const bool touch = valid && extent_touches(s.a.x, s.a.y, unpack_extent(s.c.w), x0, x1, y0, y1) &&
-                         ellipse_touches(s.a.x, s.a.y, s.a.z, s.a.w, s.b.x, s.b.w, x0, x1, y0, y1);
+                         ellipse_touches_k(s.a.x, s.a.y, s.a.z, s.a.w, s.b.x, kq, x0, x1, y0, y1);
       uint64_t m = __ballot(touch);
       uint64_t cm = 0;
       if constexpr (kStage) {

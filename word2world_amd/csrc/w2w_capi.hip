@@ -152,6 +152,7 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   p.slb = take((size_t)p.stiles * 4 + 4);
   L.item_off = take(4 * I);
   L.item_grad = take(32 * I);
+  L.cullk = take(16 * I);
   p.sliced = c.depth_split > 0 && c.depth_split < 1000;
   p.ocount = p.boff = p.slb_far = p.open_bm = p.hint_bm = p.open_units = p.hint_sat = p.big = 0;
   p.tid_far = p.ranges_far = 0;
@@ -362,6 +363,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
     pa.item_frame = at<uint32_t>(G, p.L.item_frame);
     pa.hint_sat = nullptr;
     pa.hcount = nullptr;
+    pa.cullk = at<float4>(G, p.L.cullk);
     if (p.sliced && cap > 0) {
       // depth-sliced binning: the hint is the last forward's set of tiles its
       // near lists left unfinished (a guess that only steers work; any
@@ -463,6 +465,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   ba.ranges = at<uint2>(I, p.L.ranges);
   ba.ids = at<uint32_t>(B, p.L.sorted_ids);
   ba.splats = at<Splat>(G, p.L.splats);
+  ba.cullk = at<float4>(G, p.L.cullk);
   ba.stats = g_stats;
   ba.color = out_color; ba.alpha = out_alpha; ba.depth = out_depth;
   ba.n_contrib = at<uint32_t>(I, p.L.n_contrib);
@@ -561,6 +564,7 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
   ba.ranges = at<uint2>(I, p.L.ranges);
   ba.ids = at<uint32_t>(B, p.L.sorted_ids);
   ba.splats = at<Splat>(G, p.L.splats);
+  ba.cullk = at<float4>(G, p.L.cullk);
   ba.rect = at<uint2>(G, p.L.rect);
   ba.item_off = at<uint32_t>(G, p.L.item_off);
   ba.rec = rec;

@@ -438,30 +438,47 @@ W2W_HD bool extent_touches(float mx, float my, float2 e, float x0, float x1, flo
 
 // Exact form of the same question: whether the ellipse Q(m - p) <= q1 meets
 // the rectangle [x0, x1] x [y0, y1] (pixel centres lie inside it), with q1 the
-// rounding-widened threshold of cull_extent.  In double: if the centre is
-// outside the rectangle the minimum of the convex Q over it lies on an edge,
-// and on each edge it is a clamped one-dimensional quadratic minimum.  Only a
+// rounding-widened threshold of cull_extent.  If the centre is outside the
+// rectangle the minimum of the convex Q over it lies on an edge, and on each
+// edge it is a clamped one-dimensional quadratic minimum (in double).  Only a
 // minimum above q1 (1 + 1e-6) counts as a miss; every degenerate case, NaN
 // included, counts as touching.
-W2W_HD bool ellipse_touches(float mx, float my, float ca, float cb, float cc, float skip, float x0,
-                            float x1, float y0, float y1) {
+//
+// Split in two: cull_consts, the per-splat part (threshold and the edge
+// minimisers' slopes -b/c and -b/a; formed once per splat by the
+// preprocess), and ellipse_touches_k per (splat, block).  The threshold is
+// rounded up to float; the slopes in float move the edge minimiser by
+// |b/c| 2^-24 |dx| at most, which raises the computed minimum by a factor
+// of at most 1 + (b^2 / det) 2^-48 < 1 + 1e-7 for every conic with a finite
+// threshold (8 u kappa <= 0.5 bounds b^2 / det), inside the 1e-6 widening.
+W2W_HD float4 cull_consts(float ca, float cb, float cc, float skip) {
+  const float inf = __builtin_inff();
   const double a = ca, b = cb, c = cc;
-  if (!(a > 0.0) || !(c > 0.0) || !(skip < 0.0f)) return true;
+  if (!(a > 0.0) || !(c > 0.0) || !(skip < 0.0f)) return make_float4(inf, 0.0f, 0.0f, 0.0f);
   const double det = a * c - b * b;
-  if (!(det > 0.0)) return true;
+  if (!(det > 0.0)) return make_float4(inf, 0.0f, 0.0f, 0.0f);
   const double h = 0.5 * (a - c);
   const double lmax = 0.5 * (a + c) + __builtin_sqrt(h * h + b * b);
   const double u = 0x1p-24;
   const double den = 1.0 - 8.0 * u * ((a + c) * lmax / det);
-  if (!(den > 0.5)) return true;
+  if (!(den > 0.5)) return make_float4(inf, 0.0f, 0.0f, 0.0f);
   const double q1 = (-2.0 * (double)skip) * (1.0 + 4.0 * u) / den * (1.0 + 1e-6);
+  float q1f = (float)q1;
+  if ((double)q1f < q1) q1f = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, q1f) + 1u);  // round up
+  return make_float4(q1f, (float)(-b / c), (float)(-b / a), 0.0f);
+}
+
+W2W_HD bool ellipse_touches_k(float mx, float my, float ca, float cb, float cc, float4 k, float x0, float x1,
+                              float y0, float y1) {
+  if (!(k.x < __builtin_inff())) return true;
   const double X = mx, Y = my;
   if (!(X < x0 || X > x1 || Y < y0 || Y > y1)) return true;  // centre inside (or NaN)
+  const double a = ca, b = cb, c = cc, kc = k.y, ka = k.z;
   double qmin = 1e300;
   // vertical edges p.x = E, offset dx = X - E, dy in [Y - y1, Y - y0]
   for (int e = 0; e < 2; e++) {
     const double dx = X - (double)(e ? x1 : x0);
-    double dy = -b * dx / c;
+    double dy = kc * dx;
     dy = dy < Y - y1 ? Y - y1 : (dy > Y - y0 ? Y - y0 : dy);
     const double q = (a * dx * dx + 2.0 * b * dx * dy) + c * dy * dy;
     qmin = q < qmin ? q : qmin;
@@ -469,12 +486,17 @@ W2W_HD bool ellipse_touches(float mx, float my, float ca, float cb, float cc, fl
   // horizontal edges p.y = E
   for (int e = 0; e < 2; e++) {
     const double dy = Y - (double)(e ? y1 : y0);
-    double dx = -b * dy / a;
+    double dx = ka * dy;
     dx = dx < X - x1 ? X - x1 : (dx > X - x0 ? X - x0 : dx);
     const double q = (a * dx * dx + 2.0 * b * dx * dy) + c * dy * dy;
     qmin = q < qmin ? q : qmin;
   }
-  return !(qmin > q1);
+  return !(qmin > (double)k.x);
+}
+
+W2W_HD bool ellipse_touches(float mx, float my, float ca, float cb, float cc, float skip, float x0,
+                            float x1, float y0, float y1) {
+  return ellipse_touches_k(mx, my, ca, cb, cc, cull_consts(ca, cb, cc, skip), x0, x1, y0, y1);
 }
 
 // ---- wave64 helpers ----

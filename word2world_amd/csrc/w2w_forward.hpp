@@ -129,6 +129,7 @@ struct PreArgs {
   // and each item's hinted tiles out
   const uint32_t* hint_sat;
   uint32_t* hcount;
+  float4* cullk;         // per item: the block test's per-splat part (cull_consts)
 };
 
 __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
@@ -253,6 +254,8 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
     }
     sp.c = make_float4(rgb[0], rgb[1], rgb[2], __builtin_bit_cast(float, ext));
     a.splats[i] = sp;
+    a.cullk[i] = keep ? cull_consts(g.ca, g.cb, g.cc, skip_below)
+                      : make_float4(__builtin_inff(), 0.0f, 0.0f, 0.0f);
     a.radii[i] = radius;
     a.rect[i] = make_uint2(x0 | (y0 << 16), x1 | (y1 << 16));
     a.counts[i] = cnt;
@@ -812,6 +815,7 @@ struct BlendArgs {
   const uint2* ranges;
   const uint32_t* ids;
   const Splat* splats;
+  const float4* cullk;        // per item: the block test's per-splat part
   unsigned long long* stats;  // optional [3]: evaluated pairs, contributing pairs, loaded instances
   float* color;               // per frame [3,H,W] at 3*pix_off
   float* alpha;               // per frame [H,W] at pix_off
@@ -885,7 +889,7 @@ W2W_DEV bool wave_all(bool p) { return __ballot(!p) == 0ull; }
 // processed.  A workgroup claims four consecutive blocks at a time, one per
 // wave; the waves are otherwise independent.
 template <int TS, int MODE>
-__global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_blend(BlendArgs a) {
   __shared__ Splat s_chunk[4][64];  // each wave's current chunk, for broadcast reads
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
   const uint32_t T = min(hdr->n_tiles, a.tiles_cap);
@@ -956,23 +960,27 @@ __global__ __launch_bounds__(256) void k_blend(BlendArgs a) {
     // (lanes past the end gather splat 0, which exists whenever n > 0)
     uint32_t id_n = 0;
     Splat cur = {};
+    float4 kcur = {};
     if (n > 0 && !wave_all(done)) {
       id_n = lane < n ? ids[range.x + lane] : 0u;
       cur = a.splats[id_n < P ? id_n : 0u];
+      kcur = a.cullk[id_n < P ? id_n : 0u];
       id_n = 64 + lane < n ? ids[range.x + 64 + lane] : 0u;
     }
     for (int base = 0; base < n; base += 64) {
       if (wave_all(done)) break;
       const Splat s = cur;
+      const float4 kq = kcur;
       if (base + 64 < n) {
         cur = a.splats[id_n < P ? id_n : 0u];
+        kcur = a.cullk[id_n < P ? id_n : 0u];
         const int k2 = base + 128 + lane;
         id_n = k2 < n ? ids[range.x + k2] : 0u;
       }
       n_loaded += (uint32_t)min(64, n - base);
       const bool valid = base + lane < n;
       const bool touch = valid && extent_touches(s.a.x, s.a.y, unpack_extent(s.c.w), x0, x1, y0, y1) &&
-                         ellipse_touches(s.a.x, s.a.y, s.a.z, s.a.w, s.b.x, s.b.w, x0, x1, y0, y1);
+                         ellipse_touches_k(s.a.x, s.a.y, s.a.z, s.a.w, s.b.x, kq, x0, x1, y0, y1);
       uint64_t m = __ballot(touch);
       if (m == 0ull) continue;
       // the wave's chunk goes to its LDS slot; survivors are then read back
