@@ -989,29 +989,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
       // order, so no barrier is needed.
       stage[lane] = s;
       __builtin_amdgcn_wave_barrier();
+      // straight-line per lane: every lane evaluates and selects commit the
+      // update (the tests of the reference's loop, in its order: skipped
+      // pair, alpha below 1/255, transmittance below 1e-4), so a survivor
+      // costs no exec-mask branches
       auto eval = [&](const Splat& q, int j) {
-        if (done) return;
         const float dx = q.a.x - pxf, dy = q.a.y - pyf;
         const float pw = power_of(q.a.z, q.a.w, q.b.x, dx, dy);
-        n_eval++;
-        if (pw > 0.0f || pw < q.b.w) return;
+        const bool live = !done;
+        n_eval += live ? 1u : 0u;
+        const bool in_range = live && !(pw > 0.0f || pw < q.b.w);
         const float G = exp_neg(pw);
         const float al = fminf(0.99f, q.b.y * G);
-        if (al < a_min) return;
+        const bool hit = in_range && !(al < a_min);
         const float nT = T_ * (1.0f - al);
-        if (nT < 1e-4f) {
-          done = true;
-          return;
-        }
+        const bool saturate = nT < 1e-4f;
+        const bool apply = hit && !saturate;
+        done = done || (hit && saturate);
         const float aT = al * T_;
-        C0 = __builtin_fmaf(q.c.x, aT, C0);
-        C1 = __builtin_fmaf(q.c.y, aT, C1);
-        C2 = __builtin_fmaf(q.c.z, aT, C2);
-        A = A + aT;
-        Dd = __builtin_fmaf(q.b.z, aT, Dd);
-        T_ = nT;
-        last = lbase + (uint32_t)(base + j + 1);
-        n_con++;
+        C0 = apply ? __builtin_fmaf(q.c.x, aT, C0) : C0;
+        C1 = apply ? __builtin_fmaf(q.c.y, aT, C1) : C1;
+        C2 = apply ? __builtin_fmaf(q.c.z, aT, C2) : C2;
+        A = apply ? A + aT : A;
+        Dd = apply ? __builtin_fmaf(q.b.z, aT, Dd) : Dd;
+        T_ = apply ? nT : T_;
+        last = apply ? lbase + (uint32_t)(base + j + 1) : last;
+        n_con += apply ? 1u : 0u;
       };
       // two survivors in flight: the next one's read overlaps this one's work
       int ja = __builtin_ctzll(m);
