@@ -227,5 +227,5 @@ def test_depth_sliced_binning_is_exact(gpu, dense, tile, split, forwards):
     assert torch.equal(ref_ws.image[o:o + 4 * n], ws.image[o:o + 4 * n])
     for name in ("means2D", "colors", "opacity", "means3D", "cov3D", "sh", "scales", "rotations"):
         assert torch.equal(getattr(ref_g, name), getattr(g, name)), name
-    if dense and split == 0.3:
+    if dense and split == 0.3 and forwards == 1:
         assert KN + KB < K  # tiles saturated by the near lists drop their far instances

@@ -366,12 +366,15 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
     pa.cullk = at<float4>(G, p.L.cullk);
     if (p.sliced && cap > 0) {
       // depth-sliced binning: the hint is the last forward's set of tiles its
-      // near lists left unfinished (a guess that only steers work; any
-      // content gives the same outputs); hinted tiles get their far items in
-      // their near lists too, behind the near items, so depth order is kept
+      // near lists left unfinished, grown by one tile (a guess that only
+      // steers work; any content gives the same outputs); hinted tiles get
+      // their far items in their near lists too, behind the near items, so
+      // depth order is kept
       const size_t words = cdiv((uint64_t)c.tiles, 32);
-      hipLaunchKernelGGL(k_move_u32, dim3((uint32_t)std::min<size_t>(cdiv(words, 256), 1024)), dim3(256), 0, s,
-                         at<uint32_t>(G, p.hint_bm), at<uint32_t>(G, p.open_bm), words);
+      hipLaunchKernelGGL(k_dilate_tiles, dim3((uint32_t)std::min<size_t>(cdiv(words, 256), 1024)), dim3(256), 0,
+                         s, table, (uint32_t)c.scenes, (uint32_t)c.tiles, at<uint32_t>(G, p.open_bm),
+                         at<uint32_t>(G, p.hint_bm));
+      fill_u32(at<uint32_t>(G, p.open_bm), 0u, words, s);
       hipLaunchKernelGGL(k_tile_sat, dim3((uint32_t)std::min<int32_t>(c.frames, 4096)), dim3(256), 0, s, table,
                          (uint32_t)c.scenes, (uint32_t)c.frames, at<uint32_t>(G, p.hint_bm),
                          at<uint32_t>(G, p.hint_sat));

@@ -67,15 +67,6 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
-// dst <- src, then src <- 0
-__global__ void k_move_u32(uint32_t* __restrict__ dst, uint32_t* __restrict__ src, size_t n) {
-  size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    dst[i] = src[i];
-    src[i] = 0u;
-  }
-}
-
 // Item -> (scene, Gaussian, view, frame).
 struct ItemRef {
   uint32_t scene, g, view, frame, gg;
@@ -806,6 +797,37 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ key
 // frame owning global tile t
 W2W_DEV uint32_t tile_frame(const FrameDesc* frames, uint32_t n_frames, uint32_t t) {
   return upper_index(n_frames, t, [&](uint32_t k) { return frames[k].tile_off; });
+}
+
+// Depth-sliced binning: the next forward's hint, the tiles its near lists
+// left open grown by one tile within their frame (a tile that stays open is
+// usually next to one that was): bit t = any open tile among t and its eight
+// neighbours.  One thread per 32-tile word.
+__global__ __launch_bounds__(256) void k_dilate_tiles(const void* __restrict__ table, uint32_t scenes_cap,
+                                                      uint32_t tiles_cap, const uint32_t* __restrict__ open_bm,
+                                                      uint32_t* __restrict__ hint_bm) {
+  const BatchHeader* hdr = static_cast<const BatchHeader*>(table);
+  const FrameDesc* frames = table_frames(table, scenes_cap);
+  const uint32_t T = min(hdr->n_tiles, tiles_cap);
+  const uint32_t nw = (T + 31) / 32;
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += gridDim.x * blockDim.x) {
+    uint32_t out = 0;
+    for (uint32_t b = 0; b < 32; b++) {
+      const uint32_t t = w * 32 + b;
+      if (t >= T) break;
+      const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, t)];
+      const int txn = fd.tiles_x, tyn = fd.tiles_y;
+      const int lt = (int)(t - fd.tile_off), tx = lt % txn, ty = lt / txn;
+      uint32_t any = 0;
+      for (int ny = max(ty - 1, 0); ny <= min(ty + 1, tyn - 1); ny++)
+        for (int nx = max(tx - 1, 0); nx <= min(tx + 1, txn - 1); nx++) {
+          const uint32_t nt = fd.tile_off + (uint32_t)(ny * txn + nx);
+          any |= (open_bm[nt >> 5] >> (nt & 31u)) & 1u;
+        }
+      out |= any << b;
+    }
+    hint_bm[w] = out;
+  }
 }
 
 struct BlendArgs {
