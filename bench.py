@@ -75,7 +75,7 @@ def parse():
 
 
 HBM_STAGES_NULL = ("fill",)  # buffer clears: no algorithmic bytes
-SPLIT_CANDIDATES = (0.0, 0.1, 0.15, 0.2, 0.3, 0.4)
+SPLIT_CANDIDATES = (0.0, 0.2, 0.25, 0.3, 0.35, 0.4)
 
 
 def stage_cost(name, P, M, V, K, T, HW, stats_view, tpasses, dpasses=4):
