@@ -79,7 +79,8 @@ typedef struct W2WWorkspace {
  * debugging read them; the layout is a pure function of the sizes). */
 typedef struct W2WLayout {
   size_t geom_bytes, binning_bytes, image_bytes;
-  size_t splats;     /* geom: 48-byte records (px,py,a,b | c,o,depth,skip | r,g,b,extent) */
+  size_t splats;     /* geom: 64-byte records (px,py,a,b | c,o,depth,skip | r,g,b,extent |
+                        the block test's per-splat constants) */
   size_t radii;      /* geom: int32[P] */
   size_t rect;       /* geom: uint32x2[P] (x0|y0<<16, x1|y1<<16) */
   size_t counts;     /* geom: uint32[P] tiles touched */
@@ -96,7 +97,6 @@ typedef struct W2WLayout {
   size_t grad_rec;   /* binning: float[capacity][10] per-instance backward partials */
   size_t grad_flag;  /* binning: uint8[capacity] 1 where grad_rec holds a partial */
   size_t item_grad;  /* geom: float[items][8] per-item record sums of the backward */
-  size_t cullk;      /* geom: float[items][4] per-splat part of the blends' block test */
 } W2WLayout;
 
 /* Capacities of a batched workspace: every grid is sized by these, and every

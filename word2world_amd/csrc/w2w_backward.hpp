@@ -119,8 +119,7 @@ struct BlendBwdArgs {
   uint32_t* ticket;         // zeroed; tiles are claimed in order through it
   const uint2* ranges;
   const uint32_t* ids;
-  const Splat* splats;
-  const float4* cullk;      // per item: the block test's per-splat part
+  const SplatRec* splats;   // per item: splat and block-test constants
   const uint2* rect;        // per item: tile rectangle (x0 | y0 << 16, x1 | y1 << 16)
   const uint32_t* item_off; // per item: emission position of its first instance
   unsigned long long* stats;  // optional [3]: evaluated pairs, contributing pairs, loaded instances
@@ -256,8 +255,9 @@ void k_blend_bwd(BlendBwdArgs a) {
       const int k = nproc - 64 + lane;
       id_n = k >= 0 ? id_at(k) : 0u;
       const uint32_t id0 = id_n < P ? id_n : 0u;
-      cur = a.splats[id0];
-      kcur = a.cullk[id0];
+      const SplatRec r = a.splats[id0];
+      cur = r.s;
+      kcur = r.k;
       rect_cur = a.rect[id0];
       off_cur = a.item_off[id0];
       id_n = k >= 64 ? id_at(k - 64) : 0u;
@@ -270,8 +270,9 @@ void k_blend_bwd(BlendBwdArgs a) {
       const uint32_t soff = off_cur;
       if (cstart > 0) {
         const uint32_t idc = id_n < P ? id_n : 0u;
-        cur = a.splats[idc];
-        kcur = a.cullk[idc];
+        const SplatRec r = a.splats[idc];
+        cur = r.s;
+        kcur = r.k;
         rect_cur = a.rect[idc];
         off_cur = a.item_off[idc];
         const int k2 = cstart - 128 + lane;

@@ -137,7 +137,7 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   const size_t I = (size_t)c.items;
   p.status = take(ST_WORDS * 4);
   p.table = take(table_bytes(1, 1));
-  L.splats = take(sizeof(Splat) * I);
+  L.splats = take(sizeof(SplatRec) * I);
   L.radii = take(4 * I);
   L.rect = take(8 * I);
   L.counts = take(4 * I);
@@ -152,7 +152,6 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   p.slb = take((size_t)p.stiles * 4 + 4);
   L.item_off = take(4 * I);
   L.item_grad = take(32 * I);
-  L.cullk = take(16 * I);
   p.sliced = c.depth_split > 0 && c.depth_split < 1000;
   p.ocount = p.boff = p.slb_far = p.open_bm = p.hint_bm = p.open_units = p.hint_sat = p.big = 0;
   p.tid_far = p.ranges_far = 0;
@@ -354,7 +353,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
     pa.table = table; pa.scenes_cap = (uint32_t)c.scenes; pa.items_cap = items_cap;
     pa.means = prm.means; pa.scales = prm.scales; pa.rots = prm.rots; pa.opac = prm.opac;
     pa.sh = prm.sh; pa.ts = c.tile;
-    pa.splats = at<Splat>(G, p.L.splats);
+    pa.splats = at<SplatRec>(G, p.L.splats);
     pa.radii = at<int32_t>(G, p.L.radii);
     pa.rect = at<uint2>(G, p.L.rect);
     pa.counts = at<uint32_t>(G, p.L.counts);
@@ -363,7 +362,6 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
     pa.item_frame = at<uint32_t>(G, p.L.item_frame);
     pa.hint_sat = nullptr;
     pa.hcount = nullptr;
-    pa.cullk = at<float4>(G, p.L.cullk);
     if (p.sliced && cap > 0) {
       // depth-sliced binning: the hint is the last forward's set of tiles its
       // near lists left unfinished, grown by one tile (a guess that only
@@ -467,8 +465,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   ba.ticket = status + ST_BLEND_TICKET;
   ba.ranges = at<uint2>(I, p.L.ranges);
   ba.ids = at<uint32_t>(B, p.L.sorted_ids);
-  ba.splats = at<Splat>(G, p.L.splats);
-  ba.cullk = at<float4>(G, p.L.cullk);
+  ba.splats = at<SplatRec>(G, p.L.splats);
   ba.stats = g_stats;
   ba.color = out_color; ba.alpha = out_alpha; ba.depth = out_depth;
   ba.n_contrib = at<uint32_t>(I, p.L.n_contrib);
@@ -566,8 +563,7 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
   ba.ticket = status + ST_BWD_TICKET;
   ba.ranges = at<uint2>(I, p.L.ranges);
   ba.ids = at<uint32_t>(B, p.L.sorted_ids);
-  ba.splats = at<Splat>(G, p.L.splats);
-  ba.cullk = at<float4>(G, p.L.cullk);
+  ba.splats = at<SplatRec>(G, p.L.splats);
   ba.rect = at<uint2>(G, p.L.rect);
   ba.item_off = at<uint32_t>(G, p.L.item_off);
   ba.rec = rec;
@@ -822,13 +818,13 @@ int w2w_backward(const W2WFrame* f, const W2WWorkspace* ws, const float* dL_dcol
 }
 
 __global__ void k_sorted_keys(const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ ids,
-                              const Splat* __restrict__ splats, const uint32_t* __restrict__ status,
+                              const SplatRec* __restrict__ splats, const uint32_t* __restrict__ status,
                               uint32_t cap, uint32_t P, uint64_t* __restrict__ out) {
   uint32_t n = status[ST_K];
   n = n < cap ? n : cap;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     uint32_t id = ids[i];
-    float z = id < P ? splats[id].b.z : 0.0f;
+    float z = id < P ? splats[id].s.b.z : 0.0f;
     out[i] = ((uint64_t)tiles[i] << 32) | (uint64_t)__float_as_uint(z);
   }
 }
@@ -846,7 +842,7 @@ int w2w_sorted_keys(const W2WFrame* f, const W2WWorkspace* ws, uint64_t* out_key
   const uint32_t grid = std::min<uint32_t>(cdiv((uint64_t)c.capacity, 256), 4096);
   hipLaunchKernelGGL(k_sorted_keys, dim3(grid), dim3(256), 0, s,
                      at<uint32_t>(ws->binning, p.L.sorted_tiles),
-                     at<uint32_t>(ws->binning, p.L.sorted_ids), at<Splat>(ws->geom, p.L.splats),
+                     at<uint32_t>(ws->binning, p.L.sorted_ids), at<SplatRec>(ws->geom, p.L.splats),
                      at<uint32_t>(ws->geom, p.status), (uint32_t)c.capacity, (uint32_t)f->P, out_keys);
   return launch_status();
 }

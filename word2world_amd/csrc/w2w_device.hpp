@@ -377,11 +377,20 @@ W2W_DEV uint32_t tile_coord(float v, uint32_t n) {
   return (uint32_t)v;
 }
 
-// Per-instance record the blend kernels gather (48 bytes, 16-byte aligned):
+// The splat fields the blend kernels evaluate (48 bytes, 16-byte aligned):
 //   a = (px, py, conic_a, conic_b), b = (conic_c, opacity, depth, skip_below),
 //   c = (r, g, b, cull extent: two binary16 half-widths packed in 32 bits).
 struct __align__(16) Splat {
   float4 a, b, c;
+};
+
+// What the preprocess stores per item and the blends gather per instance:
+// the splat and the per-splat part of the block test (cull_consts, below),
+// 64 bytes on a 64-byte boundary, so one gather is one 64-byte HBM segment
+// (a 48-byte splat beside a separate 16-byte array touches 2.5 on average).
+struct __align__(64) SplatRec {
+  Splat s;
+  float4 k;
 };
 
 constexpr uint32_t kExtentInf = 0x7C007C00u;  // (+inf, +inf) in binary16

@@ -109,7 +109,7 @@ struct PreArgs {
   const float* opac;     // [G] activated
   const float* sh;       // packed SH blocks (SceneDesc::sh_off)
   int ts;
-  Splat* splats;         // per item
+  SplatRec* splats;      // per item
   int32_t* radii;
   uint2* rect;           // (x0 | y0 << 16, x1 | y1 << 16)
   uint32_t* counts;
@@ -120,7 +120,6 @@ struct PreArgs {
   // and each item's hinted tiles out
   const uint32_t* hint_sat;
   uint32_t* hcount;
-  float4* cullk;         // per item: the block test's per-splat part (cull_consts)
 };
 
 __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
@@ -244,9 +243,10 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
       sp.b = make_float4(0.0f, o, 0.0f, 0.0f);
     }
     sp.c = make_float4(rgb[0], rgb[1], rgb[2], __builtin_bit_cast(float, ext));
-    a.splats[i] = sp;
-    a.cullk[i] = keep ? cull_consts(g.ca, g.cb, g.cc, skip_below)
-                      : make_float4(__builtin_inff(), 0.0f, 0.0f, 0.0f);
+    SplatRec rec;
+    rec.s = sp;
+    rec.k = keep ? cull_consts(g.ca, g.cb, g.cc, skip_below) : make_float4(__builtin_inff(), 0.0f, 0.0f, 0.0f);
+    a.splats[i] = rec;
     a.radii[i] = radius;
     a.rect[i] = make_uint2(x0 | (y0 << 16), x1 | (y1 << 16));
     a.counts[i] = cnt;
@@ -836,8 +836,7 @@ struct BlendArgs {
   uint32_t* ticket;           // zeroed; tiles are claimed in order through it
   const uint2* ranges;
   const uint32_t* ids;
-  const Splat* splats;
-  const float4* cullk;        // per item: the block test's per-splat part
+  const SplatRec* splats;     // per item: splat and block-test constants
   unsigned long long* stats;  // optional [3]: evaluated pairs, contributing pairs, loaded instances
   float* color;               // per frame [3,H,W] at 3*pix_off
   float* alpha;               // per frame [H,W] at pix_off
@@ -985,8 +984,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     float4 kcur = {};
     if (n > 0 && !wave_all(done)) {
       id_n = lane < n ? ids[range.x + lane] : 0u;
-      cur = a.splats[id_n < P ? id_n : 0u];
-      kcur = a.cullk[id_n < P ? id_n : 0u];
+      const SplatRec r = a.splats[id_n < P ? id_n : 0u];
+      cur = r.s;
+      kcur = r.k;
       id_n = 64 + lane < n ? ids[range.x + 64 + lane] : 0u;
     }
     for (int base = 0; base < n; base += 64) {
@@ -994,8 +994,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
       const Splat s = cur;
       const float4 kq = kcur;
       if (base + 64 < n) {
-        cur = a.splats[id_n < P ? id_n : 0u];
-        kcur = a.cullk[id_n < P ? id_n : 0u];
+        const SplatRec r = a.splats[id_n < P ? id_n : 0u];
+        cur = r.s;
+        kcur = r.k;
         const int k2 = base + 128 + lane;
         id_n = k2 < n ? ids[range.x + k2] : 0u;
       }

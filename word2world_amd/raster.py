@@ -91,7 +91,7 @@ class Workspace:
         return self._view(self.geom, self.layout.offsets, self.P, torch.int32)
 
     def splats(self):
-        return self._view(self.geom, self.layout.splats, 12 * self.P, torch.float32, (self.P, 12))
+        return self._view(self.geom, self.layout.splats, 16 * self.P, torch.float32, (self.P, 16))
 
     def sorted_ids(self, n: int):
         return self._view(self.binning, self.layout.sorted_ids, n, torch.int32)
