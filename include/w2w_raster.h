@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define W2W_ABI_VERSION 3
+#define W2W_ABI_VERSION 4
 
 enum {
   W2W_OK = 0,
