@@ -68,6 +68,28 @@ void fill_u32(uint32_t* ptr, uint32_t v, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, ptr, v, n);
 }
 
+// Collects fills and launches them together (k_fill_ranges); a full set is
+// launched before the next range is added.  Stream order: the fills run
+// where launch() is called.
+struct FillBatch {
+  FillRanges f{};
+  uint64_t longest = 0;
+  void add(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+    if (n == 0) return;
+    if (f.n == kFillRanges) launch(s);
+    f.ptr[f.n] = p; f.len[f.n] = n; f.val[f.n] = v; f.n++;
+    longest = std::max(longest, n);
+  }
+  void launch(hipStream_t s) {
+    if (f.n == 0) return;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((longest + 255) / 256, 4096);
+    ProfSpan ps(W2W_KID_FILL, s);
+    hipLaunchKernelGGL(k_fill_ranges, dim3(blocks), dim3(256), 0, s, f);
+    f = FillRanges{};
+    longest = 0;
+  }
+};
+
 int launch_status() {
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? W2W_OK : W2W_ERR_HIP;
@@ -342,12 +364,30 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   const uint32_t tiles_cap = (uint32_t)c.tiles;
   const uint32_t* d_items = &static_cast<const BatchHeader*>(table)->n_items;
 
-  fill_u32(status, 0u, ST_WORDS, s);
-  fill_u32(at<uint32_t>(I, p.L.ranges), 0u, 2 * (size_t)c.tiles, s);
+  // depth-sliced binning: the hint is the last forward's set of tiles its
+  // near lists left unfinished, grown by one tile (a guess that only steers
+  // work; any content gives the same outputs), taken before the open set is
+  // cleared below
+  const bool hinted = p.sliced && cap > 0 && items_cap > 0;
+  const size_t tile_words = cdiv((uint64_t)c.tiles, 32);
+  if (hinted)
+    hipLaunchKernelGGL(k_dilate_tiles, dim3((uint32_t)std::min<size_t>(cdiv(tile_words, 256), 1024)), dim3(256),
+                       0, s, table, (uint32_t)c.scenes, (uint32_t)c.tiles, at<uint32_t>(G, p.open_bm),
+                       at<uint32_t>(G, p.hint_bm));
+  FillBatch fb;
+  fb.add(status, ST_WORDS, 0u, s);
+  fb.add(at<uint32_t>(I, p.L.ranges), 2 * (uint64_t)c.tiles, 0u, s);
   if (items_cap > 0) {
-    fill_u32(at<uint32_t>(G, p.dhist), 0u, 4 * RADIX, s);
-    fill_u32(at<uint32_t>(G, p.slb), 0u, p.stiles + 1, s);
-    fill_u32(at<uint32_t>(B, p.thist), 0u, 4 * RADIX, s);
+    fb.add(at<uint32_t>(G, p.dhist), 4 * RADIX, 0u, s);
+    fb.add(at<uint32_t>(G, p.slb), p.stiles + 1, 0u, s);
+    fb.add(at<uint32_t>(B, p.thist), 4 * RADIX, 0u, s);
+  }
+  if (hinted) {
+    fb.add(at<uint32_t>(G, p.slb_far), p.stiles + 1, 0u, s);  // the hinted scan's look-back words
+    fb.add(at<uint32_t>(G, p.open_bm), tile_words, 0u, s);
+  }
+  fb.launch(s);
+  if (items_cap > 0) {
 
     PreArgs pa;
     pa.table = table; pa.scenes_cap = (uint32_t)c.scenes; pa.items_cap = items_cap;
@@ -362,17 +402,9 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
     pa.item_frame = at<uint32_t>(G, p.L.item_frame);
     pa.hint_sat = nullptr;
     pa.hcount = nullptr;
-    if (p.sliced && cap > 0) {
-      // depth-sliced binning: the hint is the last forward's set of tiles its
-      // near lists left unfinished, grown by one tile (a guess that only
-      // steers work; any content gives the same outputs); hinted tiles get
-      // their far items in their near lists too, behind the near items, so
-      // depth order is kept
-      const size_t words = cdiv((uint64_t)c.tiles, 32);
-      hipLaunchKernelGGL(k_dilate_tiles, dim3((uint32_t)std::min<size_t>(cdiv(words, 256), 1024)), dim3(256), 0,
-                         s, table, (uint32_t)c.scenes, (uint32_t)c.tiles, at<uint32_t>(G, p.open_bm),
-                         at<uint32_t>(G, p.hint_bm));
-      fill_u32(at<uint32_t>(G, p.open_bm), 0u, words, s);
+    if (hinted) {
+      // hinted tiles get their far items in their near lists too, behind the
+      // near items, so depth order is kept
       hipLaunchKernelGGL(k_tile_sat, dim3((uint32_t)std::min<int32_t>(c.frames, 4096)), dim3(256), 0, s, table,
                          (uint32_t)c.scenes, (uint32_t)c.frames, at<uint32_t>(G, p.hint_bm),
                          at<uint32_t>(G, p.hint_sat));
@@ -408,7 +440,6 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
         hipLaunchKernelGGL(k_split, dim3(1), dim3(1), 0, s, table, items_cap, (uint32_t)c.depth_split,
                            at<uint32_t>(G, p.L.offsets), status, cap);
         // far items' hinted tiles (counted by the preprocess) in depth order
-        fill_u32(at<uint32_t>(G, p.slb_far), 0u, p.stiles + 1, s);
         {
           ProfSpan ps(W2W_KID_SCAN, s);
           hipLaunchKernelGGL(k_scan<true>, dim3(std::min<uint32_t>(p.stiles, 2048)), dim3(SCAN_THREADS), 0, s,
@@ -498,7 +529,14 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
                        at<uint2>(G, p.L.rect), at<uint32_t>(G, p.L.item_frame), open_bm, hint_bm,
                        at<uint32_t>(G, p.ocount), status + ST_OPEN_UNITS);
   }
-  fill_u32(at<uint32_t>(G, p.slb_far), 0u, p.stiles + 1, s);
+  uint2* ranges_far = at<uint2>(I, p.ranges_far);
+  {
+    FillBatch fb;
+    fb.add(at<uint32_t>(G, p.slb_far), p.stiles + 1, 0u, s);
+    fb.add(at<uint32_t>(B, p.thist), 4 * RADIX, 0u, s);  // the near sort is done with it
+    fb.add(reinterpret_cast<uint32_t*>(ranges_far), 2 * (uint64_t)c.tiles, 0u, s);
+    fb.launch(s);
+  }
   {
     ProfSpan ps(W2W_KID_SCAN, s);
     hipLaunchKernelGGL(k_scan<false>, dim3(std::min<uint32_t>(p.stiles, 2048)), dim3(SCAN_THREADS), 0, s,
@@ -506,7 +544,6 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
                        at<uint32_t>(G, p.slb_far) + 1, status + ST_SCANB_TICKET, status + ST_KB, nullptr,
                        d_items, items_cap, cap, nullptr, nullptr, status + ST_OPEN_UNITS);
   }
-  fill_u32(at<uint32_t>(B, p.thist), 0u, 4 * RADIX, s);
   {
     ProfSpan ps(W2W_KID_EMIT_FAR, s);
     hipLaunchKernelGGL(k_emit_far, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 8192)), dim3(256), 0, s,
@@ -528,8 +565,6 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
                         status + ST_KB, cap, p.tpasses, p.tlast_bits, true, at<uint32_t>(B, p.thist),
                         at<uint32_t>(B, p.tstatus), status + ST_TTICKET, W2W_KID_TSORT_HIST,
                         W2W_KID_TSORT_PASS, s, /*hist_ready=*/true);
-  uint2* ranges_far = at<uint2>(I, p.ranges_far);
-  fill_u32(reinterpret_cast<uint32_t*>(ranges_far), 0u, 2 * (size_t)c.tiles, s);
   {
     ProfSpan ps(W2W_KID_RANGES, s);
     hipLaunchKernelGGL(k_ranges, dim3(std::min<uint32_t>(cdiv(cap, 256), 4096)), dim3(256), 0, s,
@@ -555,8 +590,12 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
   uint32_t* status = at<uint32_t>(G, p.status);
   float* rec = at<float>(B, p.L.grad_rec);
   uint8_t* flag = at<uint8_t>(B, p.L.grad_flag);
-  fill_u32(reinterpret_cast<uint32_t*>(flag), 0u, cdiv((uint64_t)c.capacity, 4), s);
-  fill_u32(status + ST_BWD_TICKET, 0u, 1, s);
+  {
+    FillBatch fb;
+    fb.add(reinterpret_cast<uint32_t*>(flag), cdiv((uint64_t)c.capacity, 4), 0u, s);
+    fb.add(status + ST_BWD_TICKET, 1, 0u, s);
+    fb.launch(s);
+  }
   BlendBwdArgs ba;
   ba.table = table; ba.scenes_cap = (uint32_t)c.scenes; ba.tiles_cap = (uint32_t)c.tiles;
   ba.items_cap = (uint32_t)c.items; ba.capacity = (uint32_t)c.capacity;

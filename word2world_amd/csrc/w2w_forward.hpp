@@ -67,6 +67,31 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
+// Several word ranges set in one launch (a step's status words, look-back
+// words, histograms and masks are many small fills; each launch of its own
+// costs more than its stores).  Ranges are filled one after the other by the
+// whole grid; the range index is uniform, so every field is a kernel
+// argument read.
+constexpr int kFillRanges = 8;
+struct FillRanges {
+  uint32_t* ptr[kFillRanges];
+  uint64_t len[kFillRanges];
+  uint32_t val[kFillRanges];
+  int n;
+};
+
+__global__ __launch_bounds__(256) void k_fill_ranges(FillRanges f) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < kFillRanges; r++) {
+    if (r >= f.n) break;
+    uint32_t* const p = f.ptr[r];
+    const uint32_t v = f.val[r];
+    for (uint64_t i = t; i < f.len[r]; i += stride) p[i] = v;
+  }
+}
+
 // Item -> (scene, Gaussian, view, frame).
 struct ItemRef {
   uint32_t scene, g, view, frame, gg;
