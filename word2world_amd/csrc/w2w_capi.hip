@@ -269,6 +269,11 @@ __global__ void k_make_single(void* table, const float* __restrict__ view,
 // persistent blend grids: enough workgroups to fill every CU several times
 // over; each loops claiming tiles from a ticket
 constexpr uint32_t kBlendGrid = 256 * 8;
+// Emission grids: every workgroup adds its LDS histogram of the tile keys to
+// the sort's global one at its end (device-scope atomics on a few hundred
+// words, executed memory-side), so the grids stay near one resident round.
+constexpr uint32_t kEmitGrid = 2048;
+constexpr uint32_t kEmitFarGrid = 1024;
 
 // workgroups of four independent waves, one 8x8 pixel block per wave at a time
 template <int TS>
@@ -453,7 +458,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
       }
       {
         ProfSpan ps(W2W_KID_EMIT, s);
-        hipLaunchKernelGGL(k_emit, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 4096)),
+        hipLaunchKernelGGL(k_emit, dim3(std::min<uint32_t>(cdiv(items_cap, 256), kEmitGrid)),
                            dim3(256), 0, s, table, (uint32_t)c.scenes, at<uint32_t>(G, p.L.perm),
                            at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.L.counts),
                            at<uint2>(G, p.L.rect), at<uint32_t>(G, p.L.item_frame), items_cap, cap,
@@ -463,7 +468,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
       }
       if (p.sliced) {
         ProfSpan ps(W2W_KID_EMIT_FAR, s);
-        hipLaunchKernelGGL(k_emit_far, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 8192)), dim3(256), 0, s,
+        hipLaunchKernelGGL(k_emit_far, dim3(std::min<uint32_t>(cdiv(items_cap, 256), kEmitFarGrid)), dim3(256), 0, s,
                            table, (uint32_t)c.scenes, items_cap, cap, status, at<uint32_t>(G, p.L.perm),
                            at<uint32_t>(G, p.boff), status + ST_KH, at<uint2>(G, p.L.rect),
                            at<uint32_t>(G, p.L.item_frame), at<uint32_t>(G, p.hint_bm), nullptr, status + ST_KA,
@@ -546,7 +551,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   }
   {
     ProfSpan ps(W2W_KID_EMIT_FAR, s);
-    hipLaunchKernelGGL(k_emit_far, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 8192)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_emit_far, dim3(std::min<uint32_t>(cdiv(items_cap, 256), kEmitFarGrid)), dim3(256), 0, s,
                        table, (uint32_t)c.scenes, items_cap, cap, status, at<uint32_t>(G, p.L.perm),
                        at<uint32_t>(G, p.boff), status + ST_KB, at<uint2>(G, p.L.rect),
                        at<uint32_t>(G, p.L.item_frame), open_bm, hint_bm, nullptr,
