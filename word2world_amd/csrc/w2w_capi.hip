@@ -320,7 +320,7 @@ void sort_passes(const KT* kin, const uint32_t* vin, KT* kA, uint32_t* vA, KT* k
                  hipStream_t s, bool hist_ready = false) {
   const uint32_t ntiles = cdiv(n_cap, SORT_TILE);
   if (ntiles == 0) return;
-  const uint32_t hgrid = std::min<uint32_t>(cdiv(n_cap, 256), 2048);
+  const uint32_t hgrid = std::min<uint32_t>(cdiv(n_cap, 256), 1024);
   {
     ProfSpan ps(kid_hist, s);
     hipLaunchKernelGGL(k_sort_hist<KT>, dim3(hgrid), dim3(256), 0, s, hist_ready ? nullptr : kin, d_n,
