@@ -96,22 +96,23 @@ __global__ __launch_bounds__(256) void k_activate_bwd(
 // L2 loss over the packed frame outputs; dL/dcolor for every pixel of every
 // frame (zero for frames without the backward flag), per-frame loss sums
 // accumulated into loss[frame] (pre-zeroed).
-// L2 image loss, one workgroup per tile: dL/dcolour of the tile's pixels
-// (zero where the clamp to [0, 1] is active or the frame has no backward) and
-// the tile's share of its frame's mean squared error, summed in a fixed
-// order (lane-strided, DPP across the wave, waves in order).
+// L2 image loss, one wave per tile: dL/dcolour of the tile's pixels (zero
+// where the clamp to [0, 1] is active or the frame has no backward) and the
+// tile's share of its frame's mean squared error, summed in a fixed order
+// (each lane's pixels in order, then DPP across the wave).  A wave keeps
+// several pixels' loads in flight and needs no barrier.
 __global__ __launch_bounds__(256) void k_l2_loss_tiles(const void* __restrict__ table,
                                                        uint32_t scenes_cap, uint32_t tiles_cap,
                                                        int ts, const float* __restrict__ color,
                                                        const float* __restrict__ target,
                                                        float* __restrict__ dL_dcolor,
                                                        float* __restrict__ tile_loss) {
-  __shared__ float s_w[4];
   const BatchHeader* hdr = static_cast<const BatchHeader*>(table);
   const FrameDesc* frames = table_frames(table, scenes_cap);
   const uint32_t T = min(hdr->n_tiles, tiles_cap);
-  const int tid = (int)threadIdx.x;
-  for (uint32_t t = blockIdx.x; t < T; t += gridDim.x) {
+  const int lane = (int)(threadIdx.x & 63u);
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); t < T; t += waves) {
     const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, t)];
     const int lt = (int)(t - fd.tile_off);
     const int x0 = (lt % fd.tiles_x) * ts, y0 = (lt / fd.tiles_x) * ts;
@@ -119,7 +120,8 @@ __global__ __launch_bounds__(256) void k_l2_loss_tiles(const void* __restrict__ 
     const float w = 1.0f / (3.0f * (float)HW);
     const bool on = (fd.flags & FRAME_BACKWARD) != 0;
     float part = 0.0f;
-    for (int k = tid; k < ts * ts; k += 256) {
+#pragma unroll 4
+    for (int k = lane; k < ts * ts; k += 64) {
       const int px = x0 + k % ts, py = y0 + k / ts;
       if (px >= fd.W || py >= fd.H) continue;
       const uint64_t lp = (uint64_t)py * fd.W + px;
@@ -136,10 +138,7 @@ __global__ __launch_bounds__(256) void k_l2_loss_tiles(const void* __restrict__ 
       part += sq * w;
     }
     part = wave_reduce_dpp(part);
-    if ((tid & 63) == 63) s_w[tid >> 6] = part;
-    __syncthreads();
-    if (tid == 0) tile_loss[t] = on ? ((s_w[0] + s_w[1]) + (s_w[2] + s_w[3])) : 0.0f;
-    __syncthreads();
+    if (lane == 63) tile_loss[t] = on ? part : 0.0f;
   }
 }
 
