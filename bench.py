@@ -298,7 +298,8 @@ def bench_scene(args):
         ev[i][1].record()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    if bt.overflow():
+    if bt.overflow() or bt.skipped_overflow() or bt.grow_log:
+        # any overflowed step inside the timed region did incomplete work
         raise RuntimeError("instance capacity overflowed inside the timed region")
     dens = {"every": args.densify_every, "cycles_timed": sum(1 for it, _ in bt.densify_log
                                                              if it > bt.it - args.steps),

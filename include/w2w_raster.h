@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define W2W_ABI_VERSION 4
+#define W2W_ABI_VERSION 5
 
 enum {
   W2W_OK = 0,
@@ -171,14 +171,17 @@ int w2w_backward(const W2WFrame* f, const W2WWorkspace* ws, const float* dL_dcol
  * totals (optional, 6 int64): Gaussians, items, frames, tiles, pixels, SH
  * floats of the encoded batch. */
 size_t w2w_batch_table_bytes(int32_t scenes_cap, int32_t frames_cap);
-int w2w_batch_encode(void* host_table, size_t table_bytes_avail, int32_t scenes_cap,
-                     int32_t frames_cap, int32_t tile, int32_t n_scenes, const int64_t* scene_n,
-                     const int32_t* scene_views, const int32_t* scene_M, const int32_t* scene_D,
-                     const float* frame_view, const float* frame_proj, const float* frame_campos,
-                     const float* frame_bg, const float* frame_focal, const int32_t* frame_wh,
-                     const uint32_t* frame_flags, const int64_t* scene_base,
-                     const int64_t* scene_sh_off, const int64_t* frame_target_off,
-                     int64_t* totals);
+/* Encodes a batch for the workspace capacities `caps` (scenes, frames, tile
+ * edge, and the item, Gaussian, tile and pixel totals): returns
+ * W2W_ERR_CAPACITY when the batch exceeds any of them, so that no kernel of a
+ * workspace sized by `caps` can be handed a batch larger than its buffers. */
+int w2w_batch_encode(void* host_table, size_t table_bytes_avail, const W2WCaps* caps,
+                     int32_t n_scenes, const int64_t* scene_n, const int32_t* scene_views,
+                     const int32_t* scene_M, const int32_t* scene_D, const float* frame_view,
+                     const float* frame_proj, const float* frame_campos, const float* frame_bg,
+                     const float* frame_focal, const int32_t* frame_wh, const uint32_t* frame_flags,
+                     const int64_t* scene_base, const int64_t* scene_sh_off,
+                     const int64_t* frame_target_off, int64_t* totals);
 int w2w_batch_workspace_layout(const W2WCaps* caps, W2WLayout* out);
 /* radii per item. */
 int w2w_forward_batch(const void* table, const W2WCaps* caps, const W2WWorkspace* ws,
@@ -249,11 +252,16 @@ int w2w_l2_loss(const void* table, const W2WCaps* caps, const float* color, cons
  * group, hyper[8..15] = the same for a group's second rate, hyper[16] =
  * 1/sqrt(1-b2^t), hyper[17..20] = 1-b1, b2, 1-b2, eps.  Group k may alternate
  * rates: element e uses the second rate when (e mod group_period[k]) >=
- * group_split[k] (period <= 0: one rate). */
+ * group_split[k] (period <= 0: one rate).  skip_if (optional, device): when
+ * the word is nonzero at run time -- e.g. the overflow word of the step's
+ * forward, whose images and gradients are then incomplete -- nothing is
+ * updated and bit 0 of *sticky (optional, device) is set; the caller clears
+ * *sticky after growing the workspace. */
 int w2w_adam_step_hyper(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
                         int64_t n, int32_t n_groups, const int64_t* group_end,
                         const int32_t* group_period, const int32_t* group_split,
-                        const float* hyper, void* stream);
+                        const float* hyper, const uint32_t* skip_if, uint32_t* sticky,
+                        void* stream);
 
 /* Range tables for stores whose scenes stay resident at fixed rows.  A row
  * range {int64 start, off, len} covers Gaussian rows [start, start+len) and

@@ -137,3 +137,29 @@ def test_depth_sliced_training_is_exact(gpu):
     S, KA, KB, n_open, KH, KN = (int(v) for v in sl.ws.sliced_counts().cpu())
     print(f"near {KA} hinted {KH} far {KB} of K {int(sl.ws.status()[0])}", flush=True)
     assert KN == KA + KH and KN + KB <= int(sl.ws.status()[0])
+
+
+def test_overflowed_step_updates_nothing_then_grows(gpu):
+    """A step whose instance capacity overflows (incomplete images and
+    gradients) leaves parameters and Adam state untouched and sets the sticky
+    overflow word; the next step grows the workspace and then trains exactly
+    as a trainer that had room from the start."""
+    jobs = _jobs()
+    tg = _targets(jobs, gpu)
+    ref = BatchedTrainer([j[0] for j in jobs], [j[2] for j in jobs], tg, gpu)
+    bt = BatchedTrainer([j[0] for j in jobs], [j[2] for j in jobs], tg, gpu, capacity=2000)
+    start = {name: getattr(bt.store, name).clone() for name in ("flat", "exp_avg", "exp_avg_sq")}
+    bt.step()
+    torch.cuda.synchronize()
+    assert bt.overflow() and bt.skipped_overflow()
+    for name, t in start.items():
+        assert torch.equal(getattr(bt.store, name), t), name
+    bt.it = 0  # the skipped step did not count
+    lb = bt.step().clone()
+    assert bt.grow_log and bt.grow_log[0][2] > 2000, bt.grow_log
+    lr_ = ref.step().clone()
+    torch.cuda.synchronize()
+    assert not bt.overflow() and not bt.skipped_overflow()
+    assert torch.equal(lb, lr_)
+    for name in ("grad", "flat", "exp_avg", "exp_avg_sq"):
+        assert torch.equal(getattr(ref.store, name), getattr(bt.store, name)), name

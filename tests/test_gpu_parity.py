@@ -175,7 +175,85 @@ def test_dropin_signature(gpu):
     assert len(outs) == 8
     shapes = [(1000, 3), (1000, 3), (1000, 1), (1000, 3), (1000, 6), (1000, 16, 3), (1000, 3), (1000, 4)]
     assert [tuple(t.shape) for t in outs] == shapes
-    assert all(torch.isfinite(t).all() for t in outs)
+    # values: the oracle's float32 hand backward on the same inputs (itself
+    # pinned against float64 autograd in the CPU tests)
+    m_, s_, r_, o_, sh_ = arrs
+    zero = np.zeros((96, 96), np.float32)
+    ob = oracle.backward(f, m_, s_, r_, sh_, *cam_args(cam), cam.focal_x, cam.focal_y,
+                         gC.cpu().numpy(), zero, zero)
+    names = ["dL_dmeans2D", "dL_dcolors", "dL_dopacity", "dL_dmeans3D", "dL_dcov3D", "dL_dsh",
+             "dL_dscales", "dL_drotations"]
+    for k, t in zip(names, outs):
+        got = t.cpu().numpy()
+        e = _rel(got.reshape(ob[k].shape), ob[k])
+        print(k, "drop-in vs oracle", e, flush=True)
+        assert e <= GRAD_TOL, (k, e)
+
+
+class _RendererFunc(torch.autograd.Function):
+    """The autograd Function of Renderer.py:43-104 around the drop-in calls:
+    the three buffers are saved in ctx and handed back to the backward."""
+
+    @staticmethod
+    def forward(ctx, means3D, sh, opacities, scales, rotations, vm, pm, cp, fx, fy, tanx, tany, W, H):
+        K, color, depth, alpha, radii, gb, bb, ib = raster.rasterize_gaussians(
+            sh, fy, scales, means3D, pm, 3, opacities, rotations, cp, vm, fx,
+            image_width=W, image_height=H)
+        ctx.K, ctx.cam = K, (vm, pm, cp, tanx, tany)
+        ctx.save_for_backward(means3D, radii, sh, scales, rotations, gb, bb, ib, alpha)
+        return color, radii, depth, alpha
+
+    @staticmethod
+    def backward(ctx, g_color, g_radii, g_depth, g_alpha):
+        means3D, radii, sh, scales, rotations, gb, bb, ib, alpha = ctx.saved_tensors
+        vm, pm, cp, tanx, tany = ctx.cam
+        e = torch.empty(0, device=means3D.device)
+        outs = raster.rasterize_gaussians_backward(
+            torch.ones(3, device=means3D.device), means3D, radii, e, scales, rotations, 1.0, e, vm,
+            pm, tanx, tany, g_color, g_depth, g_alpha, sh, 3, cp, gb, ctx.K, bb, ib, alpha, False)
+        g_m2d, g_col, g_op, g_m3d, g_cov, g_sh, g_sc, g_rot = outs
+        return (g_m3d, g_sh, g_op, g_sc, g_rot) + (None,) * 9
+
+
+def test_dropin_buffers_are_freed(gpu):
+    """A Renderer.py-style loop through the drop-in (GaussianTrainer.py:104-114:
+    forward, loss, backward, drop everything): no forward state outlives its
+    buffers and device memory returns to where it started."""
+    arrs = scene_np(2000, 3, 31, 2.0)
+    m, s, r, o, sh = to_dev(arrs, gpu)
+    params = [t.clone().requires_grad_(True) for t in (m, sh, o.reshape(-1, 1), s, r)]
+    cam = orbit_views(128, 128)[0]
+    vm, pm, cp = (cam.viewmatrix.to(gpu), cam.projmatrix.to(gpu), cam.campos.to(gpu))
+    args = (vm, pm, cp, cam.focal_x, cam.focal_y, cam.tanfovx, cam.tanfovy, 128, 128)
+
+    def one():
+        color, radii, depth, alpha = _RendererFunc.apply(*params, *args)
+        loss = (color ** 2).mean()
+        loss.backward()
+        for p in params:
+            p.grad = None
+
+    one()  # the first call sizes the capacity policy and torch's pools
+    torch.cuda.synchronize()
+    base = torch.cuda.memory_allocated()
+    for i in range(50):
+        one()
+        if i == 0:
+            torch.cuda.synchronize()
+            first = torch.cuda.memory_allocated()
+    torch.cuda.synchronize()
+    print("live states", raster.dropin_live_states(), "allocated", base, first,
+          torch.cuda.memory_allocated(), flush=True)
+    assert raster.dropin_live_states() == 0
+    assert torch.cuda.memory_allocated() == base
+    # a forward whose outputs are kept holds exactly one state
+    keep = raster.rasterize_gaussians(params[1].detach(), cam.focal_y, params[3].detach(),
+                                      params[0].detach(), pm, 3, params[2].detach(),
+                                      params[4].detach(), cp, vm, cam.focal_x,
+                                      image_width=128, image_height=128)
+    assert raster.dropin_live_states() == 1
+    del keep
+    assert raster.dropin_live_states() == 0
 
 
 SORT_SIZES = [1000, 5000, 10000, 15000, 20000, 1_000_003]

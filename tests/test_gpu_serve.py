@@ -79,3 +79,42 @@ def test_plain_server_completes(gpu):
     done = serve.PlainServer(pool, gpu).serve(reqs)
     st = serve.latency_stats(done)
     assert st["requests"] == 4 and st["p99_s"] >= st["p50_s"] > 0
+
+
+def test_budget_counts_the_workspace(gpu):
+    """for_budget: the batch workspace and the arena together stay within the
+    HBM budget, and the device memory the server takes matches its own
+    count."""
+    pool, sizes = _pool(gpu)
+    caps = _caps(pool)
+    ws = serve.workspace_bytes(caps)
+    budget = ws + 12_000 * serve.Arena.bytes_per_row(16)
+    torch.cuda.synchronize()
+    before = torch.cuda.memory_allocated()
+    srv = serve.ContinuousServer.for_budget(pool, budget, caps, gpu)
+    torch.cuda.synchronize()
+    took = torch.cuda.memory_allocated() - before
+    print("budget", budget, "server count", srv.hbm_bytes, "allocated", took, flush=True)
+    assert srv.arena.rows == 12_000 and srv.hbm_bytes <= budget
+    assert took <= srv.hbm_bytes + (1 << 20) and took >= 0.9 * srv.hbm_bytes
+    with pytest.raises(ValueError):
+        serve.ContinuousServer.for_budget(pool, ws // 2, caps, gpu)
+
+
+def test_requests_that_cannot_fit_raise(gpu):
+    """A request larger than the workspace or the arena is refused at
+    admission; a lone request that overflows the instance capacity raises
+    instead of being retried for ever."""
+    pool, sizes = _pool(gpu)
+    reqs = serve.make_requests(3, 1000.0, sizes, seed=9, opt_steps=1, resolutions=(96,))
+    pool.build_targets(reqs)
+    small = _caps(pool)
+    small.pixels = 96 * 96 - 1
+    with pytest.raises(ValueError, match="cannot be served"):
+        serve.ContinuousServer(pool, 20_000, small, gpu).serve(copy.deepcopy(reqs))
+    with pytest.raises(ValueError, match="arena"):
+        serve.ContinuousServer(pool, 100, _caps(pool), gpu).serve(copy.deepcopy(reqs))
+    tight = _caps(pool)
+    tight.capacity = 64
+    with pytest.raises(RuntimeError, match="alone needs more tile instances"):
+        serve.ContinuousServer(pool, 20_000, tight, gpu).serve(copy.deepcopy(reqs))

@@ -145,11 +145,32 @@ def test_adam_hyper_matches_per_element_reference(gpu, offset):
     P = P + ns * (Mo / den)
     st = L.w2w_adam_step_hyper(sl(p).data_ptr(), sl(g).data_ptr(), sl(m).data_ptr(), sl(v).data_ptr(), n,
                                len(sizes), ends.ctypes.data, period.ctypes.data, split.ctypes.data,
-                               hyper.data_ptr(), torch.cuda.current_stream().cuda_stream)
+                               hyper.data_ptr(), None, None, torch.cuda.current_stream().cuda_stream)
     assert st == 0
     torch.cuda.synchronize()
     assert torch.equal(sl(m), Mo) and torch.equal(sl(v), Vo)
     assert torch.equal(sl(p), P)
+    # gated: an overflowed step updates nothing and leaves its mark
+    gate = torch.tensor([0, 1], dtype=torch.int32, device=gpu)
+    sticky = torch.zeros(1, dtype=torch.int32, device=gpu)
+    before = [sl(t).clone() for t in (p, m, v)]
+    st = L.w2w_adam_step_hyper(sl(p).data_ptr(), sl(g).data_ptr(), sl(m).data_ptr(), sl(v).data_ptr(), n,
+                               len(sizes), ends.ctypes.data, period.ctypes.data, split.ctypes.data,
+                               hyper.data_ptr(), gate[1:].data_ptr(), sticky.data_ptr(),
+                               torch.cuda.current_stream().cuda_stream)
+    assert st == 0
+    torch.cuda.synchronize()
+    assert all(torch.equal(sl(t), b) for t, b in zip((p, m, v), before))
+    assert int(sticky[0]) == 1
+    # an open gate updates as before and leaves the mark alone
+    sticky.zero_()
+    st = L.w2w_adam_step_hyper(sl(p).data_ptr(), sl(g).data_ptr(), sl(m).data_ptr(), sl(v).data_ptr(), n,
+                               len(sizes), ends.ctypes.data, period.ctypes.data, split.ctypes.data,
+                               hyper.data_ptr(), gate[:1].data_ptr(), sticky.data_ptr(),
+                               torch.cuda.current_stream().cuda_stream)
+    assert st == 0
+    torch.cuda.synchronize()
+    assert not torch.equal(sl(p), before[0]) and int(sticky[0]) == 0
 
 
 @pytest.mark.gpu

@@ -75,6 +75,7 @@ KERNEL_IDS = {0: "fill", 1: "preprocess", 2: "depth_sort_hist", 3: "depth_sort_p
               20: "voxel", 21: "count_open", 22: "emit_far", 23: "blend_far"}
 
 _lib = None
+ABI_VERSION = 5  # W2W_ABI_VERSION of include/w2w_raster.h
 
 
 def load(path: str = LIB_PATH):
@@ -100,8 +101,7 @@ def load(path: str = LIB_PATH):
     L.w2w_batch_table_bytes.restype = ctypes.c_size_t
     L.w2w_batch_table_bytes.argtypes = [ctypes.c_int32, ctypes.c_int32]
     L.w2w_batch_encode.restype = ctypes.c_int
-    L.w2w_batch_encode.argtypes = [_vp, ctypes.c_size_t, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
-                                   ctypes.c_int32] + [_vp] * 14 + [_vp]
+    L.w2w_batch_encode.argtypes = [_vp, ctypes.c_size_t, P(W2WCaps), ctypes.c_int32] + [_vp] * 14 + [_vp]
     L.w2w_forward_batch.restype = ctypes.c_int
     L.w2w_forward_batch.argtypes = [_vp, P(W2WCaps), P(W2WWorkspace)] + [_vp] * 10
     L.w2w_backward_batch.restype = ctypes.c_int
@@ -140,7 +140,7 @@ def load(path: str = LIB_PATH):
     L.w2w_l2_loss.argtypes = [_vp, P(W2WCaps), _vp, _vp, _vp, _vp, _vp, _vp]
     L.w2w_adam_step_hyper.restype = ctypes.c_int
     L.w2w_adam_step_hyper.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32, _vp, _vp,
-                                      _vp, _vp, _vp]
+                                      _vp, _vp, _vp, _vp, _vp]
     L.w2w_activate_rows.restype = ctypes.c_int
     L.w2w_activate_rows.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int64, _vp, _vp, _vp, _vp]
     L.w2w_activate_backward_rows.restype = ctypes.c_int
@@ -162,6 +162,9 @@ def load(path: str = LIB_PATH):
                                   _vp, _vp, ctypes.c_size_t, _vp]
     L.w2w_densify_stats.restype = ctypes.c_int
     L.w2w_densify_stats.argtypes = [_vp] * 6 + [_vp]
+    if L.w2w_version() != ABI_VERSION:
+        raise RuntimeError(f"{path} has C ABI version {L.w2w_version()}, this binding expects "
+                           f"{ABI_VERSION}: rebuild it with word2world_amd.build.build()")
     _lib = L
     return L
 

@@ -103,8 +103,7 @@ def encode(scenes: list, caps: _lib.W2WCaps, bases=None, sh_offs=None,
     b = None if bases is None else np.ascontiguousarray(bases, np.int64)
     so = None if sh_offs is None else np.ascontiguousarray(sh_offs, np.int64)
     to = None if target_offs is None else np.ascontiguousarray(target_offs, np.int64)
-    st = L.w2w_batch_encode(buf.ctypes.data, nbytes, caps.scenes, caps.frames, caps.tile,
-                            len(scenes), sn.ctypes.data, sv.ctypes.data, sm.ctypes.data,
+    st = L.w2w_batch_encode(buf.ctypes.data, nbytes, ctypes.byref(caps), len(scenes), sn.ctypes.data, sv.ctypes.data, sm.ctypes.data,
                             sd.ctypes.data, view.ctypes.data, proj.ctypes.data, cp.ctypes.data,
                             bg.ctypes.data, foc.ctypes.data, wh.ctypes.data, flags.ctypes.data,
                             None if b is None else b.ctypes.data,

@@ -165,6 +165,13 @@ class BatchedTrainer:
         self.graph = None
         self._recapture = False
         self._fresh_ws, self._prev_overflow = False, False
+        # sticky overflow: the Adam launch skips an overflowed step (its
+        # images and gradients are incomplete) and sets this word; step()
+        # reads it back without waiting and grows the workspace when set
+        self.sticky = torch.zeros(1, dtype=torch.int32, device=device)
+        self._sticky_host = torch.zeros(1, dtype=torch.int32).pin_memory()
+        self._sticky_ev = None
+        self.grow_log: list = []
         self.it = 0
         st = self.store
         self._reset_stats()
@@ -248,7 +255,8 @@ class BatchedTrainer:
                                          st.exp_avg.data_ptr(), st.exp_avg_sq.data_ptr(),
                                          int(st.ends[-1]), len(GROUPS), st.ends.ctypes.data,
                                          self._period.ctypes.data, self._split.ctypes.data,
-                                         st.hyper.data_ptr(), s), "w2w_adam_step_hyper")
+                                         st.hyper.data_ptr(), ws.geom.data_ptr() + 4,
+                                         self.sticky.data_ptr(), s), "w2w_adam_step_hyper")
 
     def _upload_hyper(self):
         self.it += 1
@@ -259,7 +267,12 @@ class BatchedTrainer:
 
     def step(self):
         """One fwd+bwd+Adam step of every scene (graph replay when captured),
-        then the densification cycle when one is due."""
+        then the densification cycle when one is due.  A step whose instance
+        capacity overflowed updates no parameter; the trainer notices it a
+        step or two later (without waiting on the device) and grows the
+        workspace."""
+        if self._sticky_ev is not None and self._sticky_ev.query() and int(self._sticky_host[0]):
+            self.grow()
         self._upload_hyper()
         self._fresh_ws = False
         if self.graph is not None:
@@ -269,9 +282,34 @@ class BatchedTrainer:
             if self._recapture:
                 self._recapture = False
                 self.capture()
+        self._sticky_host.copy_(self.sticky, non_blocking=True)
+        self._sticky_ev = torch.cuda.Event()
+        self._sticky_ev.record()
         if self.densify_every and self.it % self.densify_every == 0:
             self.densify()
         return self.loss
+
+    def skipped_overflow(self) -> bool:
+        """Whether any step since the workspace last grew overflowed its
+        instance capacity (and so updated nothing).  Synchronises."""
+        return bool(int(self.sticky.cpu()[0]))
+
+    def grow(self):
+        """Re-size the instance capacity from a forward of the current scenes,
+        clear the sticky overflow word and re-capture the graph if there was
+        one."""
+        old = int(self.caps.capacity)
+        torch.cuda.synchronize()
+        self._size_workspace()
+        self.sticky.zero_()
+        self._sticky_host.zero_()
+        self._sticky_ev = None
+        self._fresh_ws = True
+        self._prev_overflow = False
+        if self.graph is not None:
+            self.graph = None
+            self._recapture = True
+        self.grow_log.append((self.it, old, int(self.caps.capacity)))
 
     def densify(self):
         """One densification cycle of every scene on the device; the store,

@@ -22,8 +22,16 @@ ROOT = os.path.dirname(HERE)
 LIB_NAME = "libw2w_raster.so"
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 SOURCES = ["csrc/w2w_capi.hip"]
-DEPS = ["csrc/w2w_capi.hip", "csrc/w2w_device.hpp", "csrc/w2w_sort.hpp", "csrc/w2w_forward.hpp",
-        "csrc/w2w_backward.hpp", "csrc/w2w_adam.hpp", "csrc/w2w_step.hpp", "../include/w2w_raster.h"]
+
+
+def deps() -> list[str]:
+    """Every file the library is built from: the sources, every kernel header
+    under csrc/ and the public header."""
+    csrc = os.path.join(HERE, "csrc")
+    names = sorted(f for f in os.listdir(csrc) if f.endswith((".hip", ".hpp", ".h")))
+    return [os.path.join("csrc", f) for f in names] + [os.path.join("..", "include", "w2w_raster.h")]
+
+
 ARCH = os.environ.get("W2W_OFFLOAD_ARCH", "gfx950")
 
 
@@ -46,7 +54,7 @@ def is_stale() -> bool:
     if not os.path.exists(LIB_PATH):
         return True
     t = os.path.getmtime(LIB_PATH)
-    return any(os.path.getmtime(os.path.join(HERE, d)) > t for d in DEPS)
+    return any(os.path.getmtime(os.path.join(HERE, d)) > t for d in deps())
 
 
 def build(force: bool = True, verbose: bool = False) -> str:

@@ -724,16 +724,17 @@ size_t w2w_batch_table_bytes(int32_t scenes_cap, int32_t frames_cap) {
   return table_bytes((uint32_t)scenes_cap, (uint32_t)frames_cap);
 }
 
-int w2w_batch_encode(void* host_table, size_t table_bytes_avail, int32_t scenes_cap,
-                     int32_t frames_cap, int32_t tile, int32_t n_scenes, const int64_t* scene_n,
-                     const int32_t* scene_views, const int32_t* scene_M, const int32_t* scene_D,
-                     const float* frame_view, const float* frame_proj, const float* frame_campos,
-                     const float* frame_bg, const float* frame_focal, const int32_t* frame_wh,
-                     const uint32_t* frame_flags, const int64_t* scene_base,
-                     const int64_t* scene_sh_off, const int64_t* frame_target_off,
-                     int64_t* totals) {
-  if (!host_table || scenes_cap < 1 || frames_cap < 1 || n_scenes < 0 || n_scenes > scenes_cap)
-    return W2W_ERR_INVALID;
+int w2w_batch_encode(void* host_table, size_t table_bytes_avail, const W2WCaps* caps,
+                     int32_t n_scenes, const int64_t* scene_n, const int32_t* scene_views,
+                     const int32_t* scene_M, const int32_t* scene_D, const float* frame_view,
+                     const float* frame_proj, const float* frame_campos, const float* frame_bg,
+                     const float* frame_focal, const int32_t* frame_wh, const uint32_t* frame_flags,
+                     const int64_t* scene_base, const int64_t* scene_sh_off,
+                     const int64_t* frame_target_off, int64_t* totals) {
+  if (!host_table || !caps) return W2W_ERR_INVALID;
+  const int32_t scenes_cap = caps->scenes, frames_cap = caps->frames, tile = caps->tile;
+  if (scenes_cap < 1 || frames_cap < 1 || n_scenes < 0) return W2W_ERR_INVALID;
+  if (n_scenes > scenes_cap) return W2W_ERR_CAPACITY;
   if (tile != 8 && tile != 16 && tile != 32) return W2W_ERR_INVALID;
   if (table_bytes_avail < table_bytes((uint32_t)scenes_cap, (uint32_t)frames_cap))
     return W2W_ERR_CAPACITY;
@@ -780,6 +781,11 @@ int w2w_batch_encode(void* host_table, size_t table_bytes_avail, int32_t scenes_
     shf += (uint64_t)scene_n[k] * scene_M[k] * 3;
   }
   if (items >= LB_VAL || tiles >= LB_VAL || g >= LB_VAL) return W2W_ERR_CAPACITY;
+  // every grid and buffer of a workspace is sized by its caps: a batch that
+  // exceeds any of them would be cut short or written past its buffers
+  if (items > (uint64_t)std::max<int64_t>(caps->items, 0) || g > (uint64_t)std::max<int64_t>(caps->gaussians, 0) ||
+      tiles > (uint64_t)std::max<int64_t>(caps->tiles, 0) || pix > (uint64_t)std::max<int64_t>(caps->pixels, 0))
+    return W2W_ERR_CAPACITY;
   h->n_scenes = (uint32_t)n_scenes; h->n_frames = (uint32_t)frames; h->n_items = (uint32_t)items;
   h->n_gaussians = (uint32_t)g; h->n_tiles = (uint32_t)tiles; h->n_pixels = pix;
   if (totals) {
@@ -1068,14 +1074,15 @@ int w2w_l2_loss(const void* table, const W2WCaps* caps, const float* color, cons
 int w2w_adam_step_hyper(float* params, const float* grads, float* exp_avg, float* exp_avg_sq,
                         int64_t n, int32_t n_groups, const int64_t* group_end,
                         const int32_t* group_period, const int32_t* group_split,
-                        const float* hyper, void* stream) {
+                        const float* hyper, const uint32_t* skip_if, uint32_t* sticky,
+                        void* stream) {
   if (n < 0 || n_groups < 1 || n_groups > ADAM_MAX_SEGMENTS || !group_end || !hyper)
     return W2W_ERR_INVALID;
   if (n == 0) return W2W_OK;
   if (!params || !grads || !exp_avg || !exp_avg_sq) return W2W_ERR_INVALID;
   AdamGraphArgs a;
   a.p = params; a.g = grads; a.m = exp_avg; a.v = exp_avg_sq; a.n = (uint64_t)n;
-  a.nseg = n_groups; a.hyper = hyper;
+  a.nseg = n_groups; a.hyper = hyper; a.skip_if = skip_if; a.sticky = sticky;
   int64_t prev = 0;
   for (int k = 0; k < ADAM_MAX_SEGMENTS; k++) {
     if (k < n_groups) {

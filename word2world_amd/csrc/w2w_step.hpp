@@ -179,6 +179,8 @@ struct AdamGraphArgs {
   uint32_t split[ADAM_MAX_SEGMENTS];
   int lr2[ADAM_MAX_SEGMENTS];  // index into hyper for the second rate, -1 if none
   const float* hyper;
+  const uint32_t* skip_if;  // optional: no update when *skip_if != 0 (an overflowed step)
+  uint32_t* sticky;         // optional: ORed with 1 when the update was skipped
 };
 
 W2W_DEV void adam_one(float& p, float g, float& m, float& v, float ns, float inv_bc2_sqrt, float one_m_b1,
@@ -217,6 +219,13 @@ struct AdamRate {
 // per thread and then advanced.
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_adam_hyper(AdamGraphArgs a) {
+  if (a.skip_if) {
+    const uint32_t skip = *a.skip_if;
+    if (skip) {
+      if (a.sticky && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(a.sticky, 1u);
+      return;
+    }
+  }
   const float inv_bc2_sqrt = a.hyper[16], one_m_b1 = a.hyper[17], b2 = a.hyper[18];
   const float one_m_b2 = a.hyper[19], eps = a.hyper[20];
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;

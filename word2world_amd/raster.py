@@ -58,6 +58,22 @@ class Workspace:
                                          L.binning_bytes, self.image.data_ptr(), L.image_bytes,
                                          self.capacity)
 
+    @classmethod
+    def from_buffers(cls, geom, binning, image, P, W, H, tile, capacity) -> "Workspace":
+        """A view of existing buffers (no allocation); their sizes must match
+        the layout of (P, W, H, tile, capacity)."""
+        ws = cls.__new__(cls)
+        ws.P, ws.W, ws.H, ws.tile, ws.capacity = P, W, H, tile, int(capacity)
+        ws.layout = L = _lib.layout(P, W, H, tile, ws.capacity)
+        for name, t, need in (("geomBuffer", geom, L.geom_bytes), ("binningBuffer", binning, L.binning_bytes),
+                              ("imgBuffer", image, L.image_bytes)):
+            if t.dtype != torch.uint8 or not t.is_cuda or t.numel() < need:
+                raise ValueError(f"{name} is not a workspace buffer of this size")
+        ws.geom, ws.binning, ws.image = geom, binning, image
+        ws._struct = _lib.W2WWorkspace(geom.data_ptr(), L.geom_bytes, binning.data_ptr(), L.binning_bytes,
+                                       image.data_ptr(), L.image_bytes, ws.capacity)
+        return ws
+
     def fits(self, P, W, H, tile, capacity) -> bool:
         return (self.P, self.W, self.H, self.tile) == (P, W, H, tile) and self.capacity >= capacity
 
@@ -291,7 +307,48 @@ def camera_from_tensors(viewmatrix, projmatrix, campos, width, height, focal_x, 
 # ---------------------------------------------------------------------------
 # Drop-in for the reference's `_C` module (ext.cpp:5-7)
 # ---------------------------------------------------------------------------
-_WS_REGISTRY: dict = {}
+class _DropInState:
+    """What the backward needs besides the three buffers: the sizes that carve
+    them and the forward's camera.  It is kept alive only by an attribute of
+    the binning buffer it describes, so it lives exactly as long as that
+    tensor (the reference's buffers die with the autograd ctx that saved
+    them, rasterize_points.cu:24-30, Renderer.py:78-79).  It holds no strong
+    reference to any of the buffers."""
+
+    __slots__ = ("P", "W", "H", "tile", "capacity", "cam", "ptrs", "__weakref__")
+
+    def __init__(self, ws: Workspace, cam: Camera):
+        self.P, self.W, self.H, self.tile, self.capacity = ws.P, ws.W, ws.H, ws.tile, ws.capacity
+        self.cam = cam
+        self.ptrs = (ws.geom.data_ptr(), ws.binning.data_ptr(), ws.image.data_ptr())
+
+
+# binning-buffer address -> state, for buffers whose Python object was
+# re-created (the attribute is the primary link).  Weak values: an entry
+# disappears with the binning buffer that owns its state.
+_DROPIN_STATE: "weakref.WeakValueDictionary[int, _DropInState]" = weakref.WeakValueDictionary()
+_STATE_ATTR = "_w2w_dropin_state"
+
+
+def _attach_state(ws: Workspace, cam: Camera) -> None:
+    st = _DropInState(ws, cam)
+    setattr(ws.binning, _STATE_ATTR, st)
+    _DROPIN_STATE[ws.binning.data_ptr()] = st
+
+
+def _find_state(geomBuffer, binningBuffer, imgBuffer) -> _DropInState:
+    st = getattr(binningBuffer, _STATE_ATTR, None)
+    if st is None:
+        st = _DROPIN_STATE.get(binningBuffer.data_ptr())
+    if st is None or st.ptrs != (geomBuffer.data_ptr(), binningBuffer.data_ptr(), imgBuffer.data_ptr()):
+        raise RuntimeError("geomBuffer / binningBuffer / imgBuffer do not come from one "
+                           "rasterize_gaussians call of this module")
+    return st
+
+
+def dropin_live_states() -> int:
+    """Number of drop-in forward states still alive (for leak checks)."""
+    return len(_DROPIN_STATE)
 
 
 def rasterize_gaussians(sh, focal_y, scales, means3D, projmatrix, sh_degree, opacities, rotations,
@@ -300,14 +357,13 @@ def rasterize_gaussians(sh, focal_y, scales, means3D, projmatrix, sh_degree, opa
     """``_C.rasterize_gaussians`` (rasterize_points.cu:31-100): returns
     ``(num_rendered, color[3,H,W], depth[1,H,W], alpha[1,H,W], radii[P],
     geomBuffer, binningBuffer, imgBuffer)``.  Reads K back, as the reference
-    does (prepareSort.cu:382-387)."""
+    does (prepareSort.cu:382-387).  The three buffers own the view's state:
+    nothing else keeps it, so dropping them frees it."""
     cam = camera_from_tensors(viewmatrix, projmatrix, campos, image_width, image_height, focal_x,
                               focal_y, background, device=means3D.device)
     K, color, depth, alpha, radii, ws = forward_checked(means3D, scales, rotations, opacities, sh,
                                                         sh_degree, cam, tile)
-    key = ws.binning.data_ptr()
-    _WS_REGISTRY[key] = (ws, cam)
-    weakref.finalize(ws.binning, _WS_REGISTRY.pop, key, None)
+    _attach_state(ws, cam)
     return (K, color, depth.unsqueeze(0), alpha.unsqueeze(0), radii, ws.geom, ws.binning, ws.image)
 
 
@@ -320,10 +376,10 @@ def rasterize_gaussians_backward(background, means3D, radii, colors_precomp, sca
     dL_dsh, dL_dscales, dL_drotations)``.  The opacities the forward used are
     read back from the forward's state; H, W come from dL_dout_color as in the
     reference."""
-    entry = _WS_REGISTRY.get(binningBuffer.data_ptr())
-    if entry is None:
-        raise RuntimeError("binningBuffer does not come from rasterize_gaussians")
-    ws, cam = entry
+    st = _find_state(geomBuffer, binningBuffer, imgBuffer)
+    ws = Workspace.from_buffers(geomBuffer, binningBuffer, imgBuffer, st.P, st.W, st.H, st.tile,
+                                st.capacity)
+    cam = st.cam
     H, W = int(dL_dout_color.shape[1]), int(dL_dout_color.shape[2])
     if (H, W) != (cam.height, cam.width):
         raise ValueError("dL_dout_color does not match the forward's image size")

@@ -201,9 +201,44 @@ def _est_instances(n: int, cams) -> int:
     return int(sum(48 * n * max(c.width * c.height / (1024.0 * 1024.0), 0.0625) for c in cams)) + 4096
 
 
+def workspace_bytes(caps: _lib.W2WCaps, max_jobs: int = 256) -> int:
+    """Device bytes of a continuous server's per-batch state for `caps`: the
+    batched workspace (geometry, binning, image buffers and table), the
+    per-frame outputs and loss gradient, the per-item gradients and the
+    job range tables."""
+    L = _lib.batch_layout(caps)
+    table = _lib.load().w2w_batch_table_bytes(caps.scenes, caps.frames)
+    items, pix, G = max(int(caps.items), 1), max(int(caps.pixels), 1), max(int(caps.gaussians), 1)
+    per_batch = (L.geom_bytes + L.binning_bytes + L.image_bytes + table
+                 + 4 * (3 * pix + pix + pix) + 4 * items      # colour, depth, alpha, radii
+                 + 4 * 3 * pix                                 # dL/dcolour
+                 + 4 * (6 * items + 3 * G + 6 * G + 3 * G + 4 * G + G)  # BatchGrads
+                 + 4 * (max(int(caps.frames), 1) + max(int(caps.tiles), 1)))  # losses
+    jobs = 4 * 4 * 5 * max_jobs + max_jobs * ROW_RANGE.itemsize + 5 * max_jobs * PARAM_RANGE.itemsize
+    return int(per_batch + jobs)
+
+
 class ContinuousServer:
+    @classmethod
+    def for_budget(cls, pool: ScenePool, hbm_bytes: int, caps: _lib.W2WCaps, device, tile: int = 16,
+                   max_jobs: int = 256, keep_results: bool = False) -> "ContinuousServer":
+        """A server whose per-batch workspace and resident-scene arena
+        together fit `hbm_bytes`: the workspace is sized by `caps`, the rest
+        becomes arena rows (parameters, gradients, Adam moments and
+        activations of resident scenes), so admission by arena rows is
+        admission by the HBM budget."""
+        M = pool.params[0]["sh"].shape[1]
+        ws = workspace_bytes(caps, max_jobs)
+        rows = (int(hbm_bytes) - ws) // Arena.bytes_per_row(M)
+        if rows < 1:
+            raise ValueError(f"an HBM budget of {hbm_bytes} bytes does not hold the batch workspace "
+                             f"({ws} bytes for these caps) and any scene")
+        return cls(pool, int(rows), caps, device, tile, max_jobs, keep_results)
+
     def __init__(self, pool: ScenePool, arena_rows: int, caps: _lib.W2WCaps, device, tile: int = 16,
                  max_jobs: int = 256, keep_results: bool = False):
+        if int(caps.tile) != tile:
+            raise ValueError(f"caps.tile {caps.tile} differs from tile {tile}")
         self.keep_results = keep_results
         self.pool, self.device, self.tile = pool, torch.device(device), tile
         self.M = pool.params[0]["sh"].shape[1]
@@ -223,12 +258,30 @@ class ContinuousServer:
         self.prm_dev = torch.zeros(5 * max_jobs * PARAM_RANGE.itemsize, dtype=torch.uint8, device=device)
         self.k_scale = 1.0
         self.ticks = 0
+        self.hbm_bytes = workspace_bytes(caps, max_jobs) + arena_rows * Arena.bytes_per_row(self.M)
+
+    def _fits_alone(self, r: Request) -> str | None:
+        """Why request r can never be served by this server (None: it can)."""
+        n = self.pool.scenes[r.scene].n
+        nv, ts, c = len(r.cameras), self.tile, self.caps
+        px = sum(cm.width * cm.height for cm in r.cameras)
+        tl = sum(((cm.width + ts - 1) // ts) * ((cm.height + ts - 1) // ts) for cm in r.cameras)
+        if n > self.arena.rows:
+            return f"its scene has {n} Gaussians, the arena holds {self.arena.rows}"
+        for what, need, cap in (("items", n * nv, c.items), ("Gaussians", n, c.gaussians),
+                                ("frames", nv, c.frames), ("pixels", px, c.pixels), ("tiles", tl, c.tiles)):
+            if need > cap:
+                return f"it needs {need} {what}, the workspace holds {cap}"
+        return None
 
     # -- admission --
     def _admit(self, queue: list, active: list, now: float):
         while queue and queue[0].arrival <= now:
             r = queue[0]
             n = self.pool.scenes[r.scene].n
+            why = self._fits_alone(r)
+            if why is not None:
+                raise ValueError(f"request {r.rid} cannot be served: {why}")
             if len(active) >= self.max_jobs:
                 break
             base = self.arena.alloc(n)
@@ -375,11 +428,13 @@ class ContinuousServer:
                 # idle until the next arrival
                 time.sleep(max(0.0, min(queue[0].arrival - now, 0.01)))
                 continue
-            work = self._batch(active)
-            if not work:
-                work = active[:1]
+            work = self._batch(active) or active[:1]  # a lone request always fits the caps
             if not self._run(work):
-                self.k_scale *= 1.5
+                if len(work) == 1:
+                    r = work[0]
+                    raise RuntimeError(f"request {r.rid} alone needs more tile instances than the "
+                                       f"workspace capacity ({self.caps.capacity})")
+                self.k_scale *= 1.5  # fewer requests per batch from now on
                 continue
             torch.cuda.synchronize()
             self.ticks += 1
