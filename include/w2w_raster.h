@@ -220,6 +220,20 @@ size_t w2w_scan_temp_bytes(int64_t n);
 int w2w_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n,
                            void* temp, size_t temp_bytes, void* stream);
 
+/* No counterpart in the reference: w2w_exclusive_scan_u32 for callers that
+ * cannot allocate (code under stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same kernels and the same results; no temp
+ * argument.  Its scratch (ticket and one look-back word per 16384 values) is
+ * a static device array of the library, 64 KiB on each GPU it runs on, held
+ * until the process ends, cleared by a fill kernel on `stream` at every call
+ * (no allocation, no memset).  n <= W2W_SCAN_NOALLOC_MAX_N, otherwise
+ * W2W_ERR_CAPACITY.  Every call on one GPU shares that GPU's scratch: calls
+ * that overlap on one GPU (other streams, other threads) race; calls in one
+ * stream are ordered and safe. */
+#define W2W_SCAN_NOALLOC_MAX_N (1 << 28)
+int w2w_exclusive_scan_u32_noalloc(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n,
+                                   void* stream);
+
 /* Fused Adam step over a flat parameter store split into `n_groups` (<= 8)
  * consecutive groups (group k ends at group_end[k], host array) with
  * learning rate group_lr[k] (host array): the torch.optim.Adam update the

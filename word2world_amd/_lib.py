@@ -63,7 +63,7 @@ class W2WDensifyConfig(ctypes.Structure):
 EXPORTS = ["w2w_version", "w2w_splat_extents", "w2w_block_touches", "w2w_workspace_layout", "w2w_batch_workspace_layout",
            "w2w_batch_table_bytes", "w2w_batch_encode", "w2w_forward_batch", "w2w_backward_batch", "w2w_forward", "w2w_backward", "w2w_sorted_keys",
            "w2w_sort_temp_bytes", "w2w_sort_pairs_u64", "w2w_scan_temp_bytes",
-           "w2w_exclusive_scan_u32", "w2w_prof_enable", "w2w_prof_reset", "w2w_prof_read", "w2w_stats_enable", "w2w_adam_step", "w2w_activate", "w2w_activate_backward",
+           "w2w_exclusive_scan_u32", "w2w_exclusive_scan_u32_noalloc", "w2w_prof_enable", "w2w_prof_reset", "w2w_prof_read", "w2w_stats_enable", "w2w_adam_step", "w2w_activate", "w2w_activate_backward",
            "w2w_l2_loss", "w2w_adam_step_hyper", "w2w_activate_rows", "w2w_activate_backward_rows",
            "w2w_adam_ranges", "w2w_densify_temp_bytes", "w2w_densify_plan", "w2w_densify_apply",
            "w2w_densify_stats", "w2w_voxel_temp_bytes", "w2w_voxel_field"]
@@ -121,6 +121,8 @@ def load(path: str = LIB_PATH):
     L.w2w_scan_temp_bytes.argtypes = [ctypes.c_int64]
     L.w2w_exclusive_scan_u32.restype = ctypes.c_int
     L.w2w_exclusive_scan_u32.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp, ctypes.c_size_t, _vp]
+    L.w2w_exclusive_scan_u32_noalloc.restype = ctypes.c_int
+    L.w2w_exclusive_scan_u32_noalloc.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp]
     L.w2w_prof_enable.restype = ctypes.c_int
     L.w2w_prof_enable.argtypes = [ctypes.c_int32, ctypes.c_uint32]
     L.w2w_prof_reset.restype = ctypes.c_int

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "../../include/w2w_raster.h"
 #include "w2w_adam.hpp"
@@ -940,6 +941,18 @@ size_t w2w_scan_temp_bytes(int64_t n) {
   return align_up(4 * ((size_t)cdiv((uint64_t)n, SCAN_TILE) + 2));
 }
 
+// The scan of n > 0 values with its scratch at t (tiles + 2 words: ticket,
+// spare, one look-back word per 16384-value tile), cleared here by a fill
+// kernel (no memset node when the caller captures the stream).
+static void scan_launch(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n, uint32_t* t,
+                        hipStream_t s) {
+  const uint32_t tiles = cdiv((uint64_t)n, SCAN_TILE);
+  fill_u32(t, 0u, (size_t)tiles + 2, s);
+  ProfSpan ps(W2W_KID_SCAN, s);
+  hipLaunchKernelGGL(k_scan<false>, dim3(std::min<uint32_t>(tiles, 2048)), dim3(SCAN_THREADS), 0, s,
+                     in, nullptr, out, t + 2, t, total, nullptr, nullptr, (uint32_t)n, 0xffffffffu);
+}
+
 int w2w_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n,
                            void* temp, size_t temp_bytes, void* stream) {
   if (n < 0 || n >= (int64_t)LB_VAL || !total) return W2W_ERR_INVALID;
@@ -951,12 +964,57 @@ int w2w_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t* total, i
   }
   if (!in || !out || !temp) return W2W_ERR_INVALID;
   if (temp_bytes < w2w_scan_temp_bytes(n)) return W2W_ERR_CAPACITY;
-  const uint32_t tiles = cdiv((uint64_t)n, SCAN_TILE);
-  uint32_t* t = static_cast<uint32_t*>(temp);
-  fill_u32(t, 0u, (size_t)tiles + 2, s);
-  ProfSpan ps(W2W_KID_SCAN, s);
-  hipLaunchKernelGGL(k_scan<false>, dim3(std::min<uint32_t>(tiles, 2048)), dim3(SCAN_THREADS), 0, s,
-                     in, nullptr, out, t + 2, t, total, nullptr, nullptr, (uint32_t)n, 0xffffffffu);
+  scan_launch(in, out, total, n, static_cast<uint32_t*>(temp), s);
+  return launch_status();
+}
+
+// ---- the scan for callers that cannot allocate ----
+// Its scratch is `scratch`, a file-scope device array: each GPU the library
+// runs on holds its own copy until the process ends.  In host code the name
+// is only a stand-in for the symbol, not a device address, so it is never
+// passed to a kernel or a copy: the first call on a device takes the array's
+// address there with hipGetSymbolAddress (that device current) and keeps it
+// per device; every later use goes through that pointer.
+constexpr size_t SCAN_SCRATCH_WORDS = (size_t)(W2W_SCAN_NOALLOC_MAX_N / SCAN_TILE) + 2;
+static_assert(W2W_SCAN_NOALLOC_MAX_N % SCAN_TILE == 0, "scratch sized in whole scan tiles");
+static_assert(W2W_SCAN_NOALLOC_MAX_N < LB_VAL, "within the scan's 30-bit look-back values");
+static __device__ __align__(256) uint32_t scratch[SCAN_SCRATCH_WORDS];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kScratchDevices] = {};
+
+int scratch_of_current_device(uint32_t** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return W2W_ERR_HIP;
+  if (dev < 0 || dev >= kScratchDevices) return W2W_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return W2W_ERR_HIP;
+    g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return W2W_OK;
+}
+}  // namespace
+
+int w2w_exclusive_scan_u32_noalloc(const uint32_t* in, uint32_t* out, uint32_t* total, int64_t n,
+                                   void* stream) {
+  if (n < 0 || !total) return W2W_ERR_INVALID;
+  if (n > (int64_t)W2W_SCAN_NOALLOC_MAX_N) return W2W_ERR_CAPACITY;
+  if (n > 0 && (!in || !out)) return W2W_ERR_INVALID;
+  uint32_t* t = nullptr;
+  const int err = scratch_of_current_device(&t);
+  if (err != W2W_OK) return err;
+  hipStream_t s = (hipStream_t)stream;
+  (void)hipGetLastError();
+  if (n == 0) {
+    fill_u32(total, 0u, 1, s);
+    return launch_status();
+  }
+  scan_launch(in, out, total, n, t, s);
   return launch_status();
 }
 
