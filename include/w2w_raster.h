@@ -391,6 +391,7 @@ enum {
   W2W_KID_ADAM = 12, W2W_KID_SORT_HIST = 13, W2W_KID_SORT_PASS = 14, W2W_KID_ACTIVATE = 15,
   W2W_KID_LOSS = 16, W2W_KID_ACTIVATE_BWD = 17, W2W_KID_SH_BWD = 18, W2W_KID_DENSIFY = 19,
   W2W_KID_VOXEL = 20, W2W_KID_COUNT_OPEN = 21, W2W_KID_EMIT_FAR = 22, W2W_KID_BLEND_FAR = 23,
+  W2W_KID_TILE_ORDER = 24,
 };
 int w2w_prof_enable(int32_t capacity, uint32_t kernel_mask);
 int w2w_prof_reset(void);

@@ -72,7 +72,8 @@ KERNEL_IDS = {0: "fill", 1: "preprocess", 2: "depth_sort_hist", 3: "depth_sort_p
               5: "emit", 6: "tile_sort_hist", 7: "tile_sort_pass", 8: "ranges", 9: "blend",
               10: "blend_bwd", 11: "preprocess_bwd", 12: "adam", 13: "sort_hist", 14: "sort_pass",
               15: "activate", 16: "loss", 17: "activate_bwd", 18: "sh_bwd", 19: "densify",
-              20: "voxel", 21: "count_open", 22: "emit_far", 23: "blend_far"}
+              20: "voxel", 21: "count_open", 22: "emit_far", 23: "blend_far",
+              24: "tile_order"}
 
 _lib = None
 ABI_VERSION = 5  # W2W_ABI_VERSION of include/w2w_raster.h

@@ -21,13 +21,22 @@
 
 namespace w2w {
 
-// per-instance gradient record (floats)
+// Per-instance backward record (floats): sums over the instance's pixels of
+// q = G dL/dalpha (zero where the alpha was clamped) weighted by the
+// pixel-to-mean offsets (dx, dy) = mean2D - pixel, and the colour and depth
+// partials.  Every gradient of the splat's 2D quantities is a splat constant
+// times one of these sums (dL/dopacity = S, dL/dconic a = -o Sxx / 2, b = -o
+// Sxy, c = -o Syy / 2, dL/dmean2D = -o (a Sx + b Sy, b Sx + c Sy) (W/2,
+// H/2)), so the constants are applied once per item after the records are
+// summed (k_item_grads) rather than once per pixel.
 enum GradField : int {
-  GA_MX = 0, GA_MY = 1,            // dL/dmean2D in NDC units
-  GA_CA = 2, GA_CB = 3, GA_CC = 4, // dL/dconic (a, b, c of the power form)
-  GA_OP = 5,                       // dL/dopacity
-  GA_R = 6, GA_G = 7, GA_B = 8,    // dL/drgb
-  GA_DEPTH = 9,                    // dL/d(NDC depth)
+  GA_SX = 0, GA_SY = 1,                // S q dx, S q dy
+  GA_SXX = 2, GA_SXY = 3, GA_SYY = 4,  // S q dx dx, S q dx dy, S q dy dy
+  GA_S = 5,                            // S q
+  GA_R = 6, GA_G = 7, GA_B = 8,        // dL/drgb
+  GA_DEPTH = 9,                        // dL/d(NDC depth)
+  // the item-level results k_item_grads forms from the summed moments
+  GA_MX = 0, GA_MY = 1, GA_CA = 2, GA_CB = 3, GA_CC = 4, GA_OP = 5,
 };
 
 constexpr int GR_FIELDS = 10;
@@ -134,6 +143,9 @@ struct BlendBwdArgs {
   // list (null: one list)
   const uint2* ranges_far;
   const uint32_t* ids_far;
+  // the tiles with backward work, heaviest first (k_tile_order), and their count
+  const uint32_t* order;
+  const uint32_t* n_order;
 };
 
 // Backward blend (backward.cu:396-520 semantics, with our forward's
@@ -141,7 +153,9 @@ struct BlendBwdArgs {
 // contributor, recovering T by T <- T / (1 - alpha) from the stored final T
 // (the reciprocal of 1 - alpha >= 0.01 is taken with the hardware rcp, 1 ulp).
 //
-// A workgroup claims a tile; wave w owns 8x8 block w.  The waves walk the
+// Used for tile edge 32 (k_blend_bwd_tile below covers edges 8 and 16).
+// A workgroup claims a tile (heaviest first, through the order of
+// k_tile_order); wave w owns 8x8 block w.  The waves walk the
 // tile's list together in chunks of 64 (down from the tile's largest last
 // contributor), each gathering the chunk one instance per lane, dropping by
 // ballot the splats whose cull extent misses its block (every pixel of the
@@ -182,9 +196,11 @@ void k_blend_bwd(BlendBwdArgs a) {
   const int fa = (row & 1) ? (row >> 1) + 2 : (row >> 1);  // rows -> fields 0, 2, 1, 3
   const float a_min = 0x1.010102p-8f;
   uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
+  const uint32_t n_order = min(*a.n_order, Tn);
   for (;;) {
     if (tid == 0) {
-      s_tile = atomicAdd(a.ticket, 1u);
+      const uint32_t t = atomicAdd(a.ticket, 1u);
+      s_tile = t < n_order ? a.order[t] : 0xffffffffu;
       s_max = 0;
     }
     __syncthreads();
@@ -226,7 +242,6 @@ void k_blend_bwd(BlendBwdArgs a) {
       }
     }
     const float bgdot = (fd.bg[0] * gC0 + fd.bg[1] * gC1) + fd.bg[2] * gC2;
-    const float hw = 0.5f * (float)fd.W, hh = 0.5f * (float)fd.H;
 
     int wmax = last;
 #pragma unroll
@@ -334,13 +349,9 @@ void k_blend_bwd(BlendBwdArgs a) {
         const float g_b = aTm * gC2;
         const float g_d = AUX ? aTm * gD : 0.0f;
         const bool free_alpha = contrib && oG <= 0.99f;  // a clamped alpha passes no gradient
-        const float g_op = free_alpha ? G * dLda : 0.0f;
-        const float dLdp = free_alpha ? (o * dLda) * G : 0.0f;
-        const float g_ca = (-0.5f * dx * dx) * dLdp;
-        const float g_cb = (-dx * dy) * dLdp;
-        const float g_cc = (-0.5f * dy * dy) * dLdp;
-        const float g_mx = (-(ca * dx + cb * dy) * dLdp) * hw;
-        const float g_my = (-(cb * dx + cc * dy) * dLdp) * hh;
+        const float g_op = free_alpha ? G * dLda : 0.0f;  // the record's moments
+        const float g_mx = g_op * dx, g_my = g_op * dy;
+        const float g_ca = g_mx * dx, g_cb = g_mx * dy, g_cc = g_my * dy;
         {  // selects, not branches: the state of non-contributing lanes is kept
           const float nl = 1.0f - al;
           const float n0 = al * cr + nl * acc0;
@@ -358,7 +369,7 @@ void k_blend_bwd(BlendBwdArgs a) {
           }
         }
         {
-          // rows of s0: mx, ca, my, cb; s1: cc, r, op, g; s2: b, 0, depth, 0
+          // rows of s0: Sx, Sxx, Sy, Sxy; s1: Syy, r, S, g; s2: b, 0, depth, 0
           float s0, s1, s2;
           wave_sum10(g_mx, g_my, g_ca, g_cb, g_cc, g_op, g_r, g_g, g_b, g_d, s0, s1, s2);
           if (row_tail) {
@@ -415,6 +426,318 @@ void k_blend_bwd(BlendBwdArgs a) {
   }
 }
 
+// ---- the backward's tile order: heaviest tiles first ----
+// The forward blend leaves per tile the walk length of its backward (the sum
+// over its 8x8 blocks of the block's largest last contributor).  Tiles are
+// bucketed by its binary logarithm and listed from the heaviest bucket down;
+// tiles without work (no contributor, or a frame without the backward) are
+// left out.  Taken in this order the long tiles start first instead of
+// finishing last (the order within a bucket is whatever the atomics give;
+// it changes which wave takes a tile, never a result).
+constexpr int kCostBuckets = 33;  // 0: no work; b >= 1: cost in [2^(b-1), 2^b)
+
+W2W_DEV uint32_t cost_bucket(uint32_t c) { return c ? 32u - (uint32_t)__builtin_clz(c) : 0u; }
+
+struct TileOrderArgs {
+  const void* table;
+  uint32_t tiles_cap;
+  const uint32_t* cost;  // per tile
+  uint32_t* hist;        // [kCostBuckets], zeroed
+  uint32_t* cursor;      // [kCostBuckets], zeroed
+  uint32_t* n_order;     // tiles listed
+  uint32_t* order;       // [tiles]
+};
+
+__global__ __launch_bounds__(256) void k_tile_cost_hist(TileOrderArgs a) {
+  __shared__ uint32_t h[kCostBuckets];
+  if (threadIdx.x < kCostBuckets) h[threadIdx.x] = 0u;
+  __syncthreads();
+  const uint32_t T = min(static_cast<const BatchHeader*>(a.table)->n_tiles, a.tiles_cap);
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x)
+    atomicAdd(&h[cost_bucket(a.cost[t])], 1u);
+  __syncthreads();
+  if (threadIdx.x < kCostBuckets && h[threadIdx.x]) atomicAdd(a.hist + threadIdx.x, h[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_tile_order(TileOrderArgs a) {
+  __shared__ uint32_t off[kCostBuckets], cnt[kCostBuckets], base[kCostBuckets];
+  const uint32_t T = min(static_cast<const BatchHeader*>(a.table)->n_tiles, a.tiles_cap);
+  if (threadIdx.x == 0) {
+    uint32_t o = 0;
+    for (int b = kCostBuckets - 1; b >= 1; b--) {
+      off[b] = o;
+      o += a.hist[b];
+    }
+    off[0] = o;
+    if (blockIdx.x == 0) *a.n_order = o;
+  }
+  for (uint32_t r0 = blockIdx.x * blockDim.x; r0 < T; r0 += gridDim.x * blockDim.x) {  // uniform
+    if (threadIdx.x < kCostBuckets) cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t t = r0 + threadIdx.x;
+    const uint32_t b = t < T ? cost_bucket(a.cost[t]) : 0u;
+    const uint32_t rank = b ? atomicAdd(&cnt[b], 1u) : 0u;
+    __syncthreads();
+    if (threadIdx.x >= 1 && threadIdx.x < kCostBuckets && cnt[threadIdx.x])
+      base[threadIdx.x] = atomicAdd(a.cursor + threadIdx.x, cnt[threadIdx.x]);
+    __syncthreads();
+    if (b) a.order[off[b] + base[b] + rank] = t;
+    __syncthreads();  // cnt and base are rewritten by the next round
+  }
+}
+
+// Backward blend for tile edges 8 and 16: one wave per tile, lane l holding
+// pixel (l mod 8, l / 8) of each of the tile's NB 8x8 blocks (NB = 1 or 4).
+// Waves claim tiles on their own, heaviest first, through the order of
+// k_tile_order; no barriers, no LDS merge.  The wave walks the tile's list
+// back to front in chunks of 64 from the tile's largest last contributor:
+// lane l gathers instance base + l and tests it against each block (cull
+// extent, then the exact ellipse test); for each instance that touches some
+// block, every touched block with a pixel still behind its last contributor
+// is evaluated (the forward's alpha recomputed bit for bit, T <- T / (1 -
+// alpha) from the stored final T, the hardware rcp), each lane adding its
+// pixels' moments (GradField) in block order, and one wave sum over the
+// lanes gives the instance's record, stored straight from the wave.  The
+// colour of the contributors behind each pixel enters only through its dot
+// product with dL/dcolour, one running sum per pixel.  Every sum has a fixed
+// order: deterministic.
+template <int TS, bool AUX>
+__global__ __launch_bounds__(256) void k_blend_bwd_tile(BlendBwdArgs a) {
+  constexpr int BX = TS / kUnit;
+  constexpr int NB = BX * BX;
+  // records move as native 4-float vectors (copies of HIP's float4 struct
+  // into LDS lower to memcpys that stay in scratch)
+  using v4f = float __attribute__((ext_vector_type(4)));
+  __shared__ v4f s_stage[4][3][64];  // each wave's chunk (splat a, b, c), read back by broadcast
+  const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
+  const uint32_t Tn = min(hdr->n_tiles, a.tiles_cap);
+  const uint32_t P = min(hdr->n_items, a.items_cap);
+  const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
+  const int lane = (int)(threadIdx.x & 63u);
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  v4f(*const stage)[64] = s_stage[wv];
+  const int row = lane >> 4;
+  const bool row_tail = (lane & 15) == 15;
+  const int fa = (row & 1) ? (row >> 1) + 2 : (row >> 1);  // wave_sum10 rows -> fields 0, 2, 1, 3
+  const float a_min = 0x1.010102p-8f;
+  const uint32_t n_order = min(*a.n_order, Tn);
+  uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
+  for (;;) {
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(a.ticket, 1u);
+    t = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)t, 0, 64));
+    if (t >= n_order) break;
+    const uint32_t tile = a.order[t];
+    if (tile >= Tn) continue;
+    const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, tile)];
+    const int lt = (int)(tile - fd.tile_off);
+    const int tx = lt % fd.tiles_x, ty = lt / fd.tiles_x;
+    const int ox = tx * TS, oy = ty * TS;  // the tile's first pixel
+    const float pxf = (float)(ox + (lane & 7)), pyf = (float)(oy + (lane >> 3));
+    const uint2 range = a.ranges[tile];
+    const uint2 range_far = a.ranges_far ? a.ranges_far[tile] : make_uint2(0u, 0u);
+    const int n_near = (int)(range.y - range.x);
+    const int n_list = n_near + (int)(range_far.y - range_far.x);
+    auto id_at = [&](int k) -> uint32_t {
+      return k < n_near ? a.ids[range.x + k] : a.ids_far[range_far.x + (uint32_t)(k - n_near)];
+    };
+    const size_t HW = (size_t)fd.W * fd.H;
+    const float* gcol = a.dL_dcolor + 3 * fd.pix_off;
+
+    // per pixel of each block: last contributor, running T, the running
+    // dot product of the colour behind it with dL/dcolour (and the depth /
+    // alpha sums), dL/dcolour, final T times background . dL/dcolour
+    int last[NB], wmax_b[NB];
+    float T[NB], accG[NB], gC0[NB], gC1[NB], gC2[NB], Tfbg[NB];
+    float gA[NB], gD[NB], accD[NB], accA[NB];
+    int wmax = 0;
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const int px = ox + (b % BX) * kUnit + (lane & 7), py = oy + (b / BX) * kUnit + (lane >> 3);
+      const bool inside = px < fd.W && py < fd.H;
+      const size_t lp = (size_t)py * fd.W + px;
+      const size_t pid = fd.pix_off + lp;
+      last[b] = inside ? min((int)a.n_contrib[pid], n_list) : 0;
+      const float Tf = inside ? a.final_T[pid] : 0.0f;
+      gC0[b] = inside ? gcol[lp] : 0.0f;
+      gC1[b] = inside ? gcol[HW + lp] : 0.0f;
+      gC2[b] = inside ? gcol[2 * HW + lp] : 0.0f;
+      gA[b] = gD[b] = 0.0f;
+      if constexpr (AUX) {
+        gA[b] = inside && a.dL_dalpha ? a.dL_dalpha[pid] : 0.0f;
+        gD[b] = inside && a.dL_ddepth ? a.dL_ddepth[pid] : 0.0f;
+      }
+      const float bgdot = (fd.bg[0] * gC0[b] + fd.bg[1] * gC1[b]) + fd.bg[2] * gC2[b];
+      Tfbg[b] = Tf * bgdot;
+      T[b] = Tf;
+      accG[b] = accD[b] = accA[b] = 0.0f;
+      int wm = last[b];
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) wm = max(wm, __shfl_xor(wm, off, 64));
+      wmax_b[b] = __builtin_amdgcn_readfirstlane(wm);
+      wmax = max(wmax, wmax_b[b]);
+    }
+    if (wmax == 0) continue;
+
+    // chunk c covers [wmax - 64 (c + 1), wmax - 64 c) clipped at 0; lane l
+    // holds instance start + l.  Ids two chunks ahead; records, rectangles
+    // and record offsets one chunk ahead.  Lanes before the start gather
+    // splat 0 (present when wmax > 0).
+    // the next chunk's record of this lane: splat a, b, c and block-test
+    // constants, its rectangle and record offset
+    const v4f* const rec4 = reinterpret_cast<const v4f*>(a.splats);
+    uint32_t id_n;
+    v4f ra, rb, rc, rk;
+    uint2 rect_cur;
+    uint32_t off_cur;
+    {
+      const int k = wmax - 64 + lane;
+      id_n = k >= 0 ? id_at(k) : 0u;
+      const uint32_t id0 = id_n < P ? id_n : 0u;
+      ra = rec4[4 * (size_t)id0];
+      rb = rec4[4 * (size_t)id0 + 1];
+      rc = rec4[4 * (size_t)id0 + 2];
+      rk = rec4[4 * (size_t)id0 + 3];
+      rect_cur = a.rect[id0];
+      off_cur = a.item_off[id0];
+      id_n = k >= 64 ? id_at(k - 64) : 0u;
+    }
+    for (int cend = wmax; cend > 0; cend -= 64) {
+      const int cstart = cend - 64;  // may be negative: lanes below 0 are empty
+      const v4f sa = ra, sb = rb, sc = rc;
+      const float4 kq = make_float4(rk.x, rk.y, rk.z, rk.w);
+      const uint2 srect = rect_cur;
+      const uint32_t soff = off_cur;
+      if (cstart > 0) {
+        const uint32_t idc = id_n < P ? id_n : 0u;
+        ra = rec4[4 * (size_t)idc];
+        rb = rec4[4 * (size_t)idc + 1];
+        rc = rec4[4 * (size_t)idc + 2];
+        rk = rec4[4 * (size_t)idc + 3];
+        rect_cur = a.rect[idc];
+        off_cur = a.item_off[idc];
+        const int k2 = cstart - 128 + lane;
+        id_n = k2 >= 0 ? id_at(k2) : 0u;
+      }
+      n_loaded += (uint32_t)min(64, cend);
+      const int kk_l = cstart + lane;
+      const float2 ext = unpack_extent(sc.w);
+      uint64_t mb[NB];
+      uint64_t m = 0;
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        const float x0 = (float)(ox + (b % BX) * kUnit), y0 = (float)(oy + (b / BX) * kUnit);
+        const float x1 = x0 + (float)(kUnit - 1), y1 = y0 + (float)(kUnit - 1);
+        const bool touch = kk_l >= 0 && kk_l < wmax_b[b] && extent_touches(sa.x, sa.y, ext, x0, x1, y0, y1) &&
+                           ellipse_touches_k(sa.x, sa.y, sa.z, sa.w, sb.x, kq, x0, x1, y0, y1);
+        mb[b] = __ballot(touch);
+        m |= mb[b];
+      }
+      if (m == 0ull) continue;
+      stage[0][lane] = sa;  // read back by this wave only: LDS keeps its order
+      stage[1][lane] = sb;
+      stage[2][lane] = sc;
+      __builtin_amdgcn_wave_barrier();
+      // the record position of each lane's instance: the item's first
+      // instance plus the tile's index in the item's rectangle
+      const uint32_t rx0 = srect.x & 0xFFFFu, ry0 = srect.x >> 16, rx1 = srect.y & 0xFFFFu;
+      const uint32_t pos_l = soff + ((uint32_t)ty - ry0) * (rx1 - rx0) + ((uint32_t)tx - rx0);
+      while (m) {
+        const int j = 63 - __builtin_clzll(m);
+        m &= ~(1ull << j);
+        const int kk = cstart + j;
+        const v4f qa = stage[0][j], qb = stage[1][j], qc = stage[2][j];
+        const float mx = qa.x, my = qa.y, ca = qa.z, cb = qa.w;
+        const float cc = qb.x, o = qb.y, dz = qb.z, sk = qb.w;
+        const float cr = qc.x, cg = qc.y, cbl = qc.z;
+        float F0 = 0.0f, F1 = 0.0f, F2 = 0.0f, F3 = 0.0f, F4 = 0.0f, F5 = 0.0f;
+        float F6 = 0.0f, F7 = 0.0f, F8 = 0.0f, F9 = 0.0f;
+        bool any = false;
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+          if (!((mb[b] >> j) & 1ull)) continue;  // uniform
+          // straight-line per lane: selects keep the state of lanes that do
+          // not contribute
+          const bool act = kk < last[b];
+          const float dx = mx - (pxf + (float)((b % BX) * kUnit));
+          const float dy = my - (pyf + (float)((b / BX) * kUnit));
+          const float pw = power_of(ca, cb, cc, dx, dy);
+          const float G = exp_neg(pw);
+          const float oG = o * G;
+          const float al = fminf(0.99f, oG);
+          const bool contrib = act && (pw <= 0.0f && pw >= sk) && al >= a_min;
+          n_eval += act ? 1u : 0u;
+          n_con += contrib ? 1u : 0u;
+          if (__ballot(contrib) == 0ull) continue;
+          any = true;
+          const float r1 = __builtin_amdgcn_rcpf(1.0f - al);
+          const float Tn = T[b] * r1;
+          const float aT = al * Tn;
+          const float cdg = (cr * gC0[b] + cg * gC1[b]) + cbl * gC2[b];
+          float dLda = cdg - accG[b];
+          if constexpr (AUX) {
+            dLda = dLda + (dz - accD[b]) * gD[b];
+            dLda = dLda + (1.0f - accA[b]) * gA[b];
+          }
+          dLda = dLda * Tn - Tfbg[b] * r1;
+          const bool free_alpha = contrib && oG <= 0.99f;  // a clamped alpha passes no gradient
+          const float qq = free_alpha ? G * dLda : 0.0f;
+          const float aTm = contrib ? aT : 0.0f;
+          const float qdx = qq * dx, qdy = qq * dy;
+          F0 += qdx;
+          F1 += qdy;
+          F2 = __builtin_fmaf(qdx, dx, F2);
+          F3 = __builtin_fmaf(qdx, dy, F3);
+          F4 = __builtin_fmaf(qdy, dy, F4);
+          F5 += qq;
+          F6 = __builtin_fmaf(aTm, gC0[b], F6);
+          F7 = __builtin_fmaf(aTm, gC1[b], F7);
+          F8 = __builtin_fmaf(aTm, gC2[b], F8);
+          if constexpr (AUX) F9 = __builtin_fmaf(aTm, gD[b], F9);
+          const float nl = 1.0f - al;
+          const float nG = __builtin_fmaf(al, cdg, nl * accG[b]);
+          T[b] = contrib ? Tn : T[b];
+          accG[b] = contrib ? nG : accG[b];
+          if constexpr (AUX) {
+            const float nD = __builtin_fmaf(al, dz, nl * accD[b]);
+            const float nA = __builtin_fmaf(nl, accA[b], al);
+            accD[b] = contrib ? nD : accD[b];
+            accA[b] = contrib ? nA : accA[b];
+          }
+        }
+        if (!any) continue;
+        // rows of s0: Sx, Sxx, Sy, Sxy; s1: Syy, r, S, g; s2: b, 0, depth, 0
+        float s0, s1, s2;
+        wave_sum10(F0, F1, F2, F3, F4, F5, F6, F7, F8, F9, s0, s1, s2);
+        const uint32_t pos = (uint32_t)__builtin_amdgcn_readlane((int)pos_l, j);
+        if (pos < a.capacity) {
+          float* r = a.rec + (size_t)pos * GR_FIELDS;
+          if (row_tail) {
+            r[fa] = s0;
+            r[4 + fa] = s1;
+            if ((row & 1) == 0) r[8 + (row >> 1)] = s2;
+          }
+          if (lane == 0) a.flag[pos] = 1;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();  // this chunk's reads precede the next chunk's writes
+    }
+  }
+  if (a.stats) {
+    unsigned long long e = n_eval, c = n_con, kf = (lane == 0) ? n_loaded : 0u;
+    for (int off = 32; off >= 1; off >>= 1) {
+      e += __shfl_xor(e, off, 64);
+      c += __shfl_xor(c, off, 64);
+      kf += __shfl_xor(kf, off, 64);
+    }
+    if (lane == 0) {
+      atomicAdd(a.stats + 0, e);
+      atomicAdd(a.stats + 1, c);
+      if (kf) atomicAdd(a.stats + 2, kf);
+    }
+  }
+}
+
 struct PreBwdArgs {
   const void* table;
   uint32_t scenes_cap, items_cap, gaussians_cap;
@@ -428,6 +751,7 @@ struct PreBwdArgs {
   const float* rec;          // [capacity][GR_FIELDS] per-instance partials
   const uint8_t* flag;       // [capacity]
   uint32_t capacity;
+  const SplatRec* splats;    // per item: mean2D, conic and opacity of the forward
   const uint32_t* item_frame;  // per item: its frame
   float* item_grad;          // per item [8]: dL/dconic a, b, c, dL/dopacity, dL/ddepth, 3 unused
   float* dL_dmeans2D;    // per item [3]  (NDC units, z = 0)
@@ -442,16 +766,19 @@ struct PreBwdArgs {
 
 // One thread per item: the sum of its instance records in emission order
 // (the flags are scanned eight at a time; the flag array is padded to 256
-// bytes).  Writes the item's dL/dmean2D and dL/dcolor outputs and the other
-// sums to item_grad for k_preprocess_bwd.  A light kernel with four times
-// the threads of the per-Gaussian one: the dependent flag -> record loads
-// are hidden by occupancy rather than serialised over a Gaussian's views.
+// bytes), then the splat constants applied to the summed moments (GradField).
+// Writes the item's dL/dmean2D and dL/dcolor outputs and the other
+// gradients to item_grad for k_preprocess_bwd.  A light kernel with four
+// times the threads of the per-Gaussian one: the dependent flag -> record
+// loads are hidden by occupancy rather than serialised over a Gaussian's
+// views.
 __global__ __launch_bounds__(256) void k_item_grads(PreBwdArgs a) {
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
   const uint32_t n_items = min(hdr->n_items, a.items_cap);
   const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_items; i += gridDim.x * blockDim.x) {
-    const bool vis = a.radii[i] > 0 && (frames[a.item_frame[i]].flags & FRAME_BACKWARD);
+    const FrameDesc& fd = frames[a.item_frame[i]];
+    const bool vis = a.radii[i] > 0 && (fd.flags & FRAME_BACKWARD);
     float ga[GR_FIELDS];
 #pragma unroll
     for (int f = 0; f < GR_FIELDS; f++) ga[f] = 0.0f;
@@ -474,6 +801,18 @@ __global__ __launch_bounds__(256) void k_item_grads(PreBwdArgs a) {
           }
         }
       }
+      // moments -> gradients of the 2D splat (dL/dmean2D in NDC units:
+      // d pixel / d ndc = W/2, H/2)
+      const Splat sp = a.splats[i].s;
+      const float ca = sp.a.z, cb = sp.a.w, cc = sp.b.x, o = sp.b.y;
+      const float sx = ga[GA_SX], sy = ga[GA_SY];
+      const float sxx = ga[GA_SXX], sxy = ga[GA_SXY], syy = ga[GA_SYY];
+      ga[GA_MX] = (-(ca * sx + cb * sy) * o) * (0.5f * (float)fd.W);
+      ga[GA_MY] = (-(cb * sx + cc * sy) * o) * (0.5f * (float)fd.H);
+      ga[GA_CA] = (-0.5f * o) * sxx;
+      ga[GA_CB] = -o * sxy;
+      ga[GA_CC] = (-0.5f * o) * syy;
+      // ga[GA_OP] = S is dL/dopacity as it stands
     }
     a.dL_dmeans2D[3 * (size_t)i] = ga[GA_MX];
     a.dL_dmeans2D[3 * (size_t)i + 1] = ga[GA_MY];

@@ -123,6 +123,7 @@ struct Plan {
   size_t ocount, boff, slb_far, open_bm, hint_bm, open_units, hint_sat, big;  // geom
   size_t tid_far;                         // binning: the far sort's second id buffer
   size_t ranges_far;                      // image
+  size_t tile_cost, tile_order;           // image: the backward's tile order
   // the near tile sort ends in (L.sorted_tiles, L.sorted_ids); its other
   // buffer pair is free afterwards and takes the far slice's emission
   size_t scratch_keys() const { return (tpasses & 1) ? tkey0 : tkey1; }
@@ -208,6 +209,8 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   L.n_contrib = take(4 * (size_t)c.pixels);
   L.final_T = take(4 * (size_t)c.pixels);
   if (p.sliced) p.ranges_far = take(8 * (size_t)c.tiles);
+  p.tile_cost = take(4 * (size_t)c.tiles);
+  p.tile_order = take(4 * (size_t)c.tiles);
   L.image_bytes = o;
   L.table = p.table;
   return true;
@@ -303,15 +306,25 @@ void launch_blend_tile(const BlendArgs& a, int mode, int tile, uint32_t tiles_ca
   }
 }
 
-// one tile per workgroup at a time, one wave per 8x8 block of it
+// tile edges 8 and 16: one wave per tile (workgroups of four independent
+// waves); edge 32: one workgroup per tile, one wave per 8x8 block of it
 template <int TS>
 void launch_blend_bwd(const BlendBwdArgs& a, uint32_t tiles_cap, hipStream_t s) {
   ProfSpan ps(W2W_KID_BLEND_BWD, s);
-  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(tiles_cap, kBlendGrid));
-  if (a.dL_dalpha || a.dL_ddepth)
-    hipLaunchKernelGGL((k_blend_bwd<TS, true>), dim3(grid), dim3(64 * units_per_tile<TS>()), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_blend_bwd<TS, false>), dim3(grid), dim3(64 * units_per_tile<TS>()), 0, s, a);
+  const bool aux = a.dL_dalpha || a.dL_ddepth;
+  if constexpr (TS <= 16) {
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((tiles_cap + 3) / 4, kBlendGrid));
+    if (aux)
+      hipLaunchKernelGGL((k_blend_bwd_tile<TS, true>), dim3(grid), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_blend_bwd_tile<TS, false>), dim3(grid), dim3(256), 0, s, a);
+  } else {
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(tiles_cap, kBlendGrid));
+    if (aux)
+      hipLaunchKernelGGL((k_blend_bwd<TS, true>), dim3(grid), dim3(64 * units_per_tile<TS>()), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_blend_bwd<TS, false>), dim3(grid), dim3(64 * units_per_tile<TS>()), 0, s, a);
+  }
 }
 
 template <typename KT>
@@ -383,6 +396,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   FillBatch fb;
   fb.add(status, ST_WORDS, 0u, s);
   fb.add(at<uint32_t>(I, p.L.ranges), 2 * (uint64_t)c.tiles, 0u, s);
+  fb.add(at<uint32_t>(I, p.tile_cost), (uint64_t)c.tiles, 0u, s);
   if (items_cap > 0) {
     fb.add(at<uint32_t>(G, p.dhist), 4 * RADIX, 0u, s);
     fb.add(at<uint32_t>(G, p.slb), p.stiles + 1, 0u, s);
@@ -513,6 +527,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   ba.hint_bm = nullptr;
   ba.open_units = nullptr;
   ba.n_open_units = nullptr;
+  ba.tile_cost = at<uint32_t>(I, p.tile_cost);
   if (!p.sliced || items_cap == 0 || cap == 0) {
     launch_blend_tile(ba, BLEND_FULL, c.tile, tiles_cap, s);
     return launch_status();
@@ -600,7 +615,22 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
     FillBatch fb;
     fb.add(reinterpret_cast<uint32_t*>(flag), cdiv((uint64_t)c.capacity, 4), 0u, s);
     fb.add(status + ST_BWD_TICKET, 1, 0u, s);
+    fb.add(status + ST_ORDER_N, 1 + 2 * kCostBuckets, 0u, s);  // count, histogram, cursors
     fb.launch(s);
+  }
+  {
+    // the tiles with work, heaviest first
+    TileOrderArgs oa;
+    oa.table = table; oa.tiles_cap = (uint32_t)c.tiles;
+    oa.cost = at<uint32_t>(I, p.tile_cost);
+    oa.hist = status + ST_COST_HIST;
+    oa.cursor = status + ST_COST_CURSOR;
+    oa.n_order = status + ST_ORDER_N;
+    oa.order = at<uint32_t>(I, p.tile_order);
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(cdiv((uint64_t)c.tiles, 256), 512));
+    ProfSpan ps(W2W_KID_TILE_ORDER, s);
+    hipLaunchKernelGGL(k_tile_cost_hist, dim3(grid), dim3(256), 0, s, oa);
+    hipLaunchKernelGGL(k_tile_order, dim3(grid), dim3(256), 0, s, oa);
   }
   BlendBwdArgs ba;
   ba.table = table; ba.scenes_cap = (uint32_t)c.scenes; ba.tiles_cap = (uint32_t)c.tiles;
@@ -619,6 +649,8 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
   ba.dL_dcolor = dL_dcolor; ba.dL_dalpha = dL_dalpha; ba.dL_ddepth = dL_ddepth;
   ba.ranges_far = p.sliced ? at<uint2>(I, p.ranges_far) : nullptr;
   ba.ids_far = p.sliced ? at<uint32_t>(B, p.far_ids()) : nullptr;
+  ba.order = at<uint32_t>(I, p.tile_order);
+  ba.n_order = status + ST_ORDER_N;
   switch (c.tile) {
     case 8: launch_blend_bwd<8>(ba, (uint32_t)c.tiles, s); break;
     case 16: launch_blend_bwd<16>(ba, (uint32_t)c.tiles, s); break;
@@ -634,6 +666,7 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
   pa.rec = rec;
   pa.flag = flag;
   pa.capacity = (uint32_t)c.capacity;
+  pa.splats = at<SplatRec>(G, p.L.splats);
   pa.item_frame = at<uint32_t>(G, p.L.item_frame);
   pa.item_grad = at<float>(G, p.L.item_grad);
   pa.dL_dmeans2D = dL_dmeans2D; pa.dL_dcolors = dL_dcolors; pa.dL_dopacity = dL_dopacity;

@@ -59,7 +59,10 @@ enum StatusWord : int {
   ST_SCANB_TICKET = 28,
   ST_SCANH_TICKET = 29,
   ST_BLENDB_TICKET = 30,
-  ST_WORDS = 64,
+  ST_ORDER_N = 33,     // backward: tiles with work, in the heaviest-first order
+  ST_COST_HIST = 34,   //   tiles per cost bucket (kCostBuckets words)
+  ST_COST_CURSOR = 67, //   slots taken per cost bucket (kCostBuckets words)
+  ST_WORDS = 128,
 };
 
 __global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t n) {
@@ -876,6 +879,10 @@ struct BlendArgs {
   const uint32_t* hint_bm;  // tiles whose near list is their whole list
   uint32_t* open_units;     // list of the unfinished 8x8 blocks (units) of other tiles, any order
   uint32_t* n_open_units;
+  // optional, zeroed: per tile, the backward's walk length -- the sum over
+  // its 8x8 blocks of the block's largest last contributor -- for frames
+  // that take part in the backward (the backward visits tiles heaviest first)
+  uint32_t* tile_cost;
 };
 
 // BLEND_FULL: one list per tile, final outputs.  BLEND_NEAR: the near lists;
@@ -1088,6 +1095,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     }
     const bool open = MODE == BLEND_NEAR && !wave_all(done);
     const bool unfinished = open && !((hint_word >> (tile & 31u)) & 1u);
+    if (a.tile_cost && !unfinished && (fd.flags & FRAME_BACKWARD)) {
+      // final outputs: this block's share of the tile's backward work
+      uint32_t wm = last;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) wm = max(wm, (uint32_t)__shfl_xor((int)wm, off, 64));
+      if (lane == 0 && wm) atomicAdd(a.tile_cost + tile, min(wm, 1u << 27));
+    }
     if (open && lane == 0) atomicOr(a.open_bm + (tile >> 5), 1u << (tile & 31u));
     if (unfinished) {
       if (inside) {
