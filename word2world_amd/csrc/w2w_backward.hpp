@@ -1,14 +1,16 @@
 // Backward rasterizer kernels for gfx950.
 //
-//   k_blend_bwd       per-pixel reverse traversal (the role of backward.cu:521-665):
-//                     starts from the stored final T and last-contributor index,
-//                     recomputes alpha with the forward's exact arithmetic and
-//                     T <- T / (1 - alpha); no per-(pixel, Gaussian) state.  A
-//                     workgroup owns a tile, one wave per 8x8 block; each wave
-//                     reduces a splat's partials over its pixels with DPP, the
-//                     waves' sums are added in LDS in a fixed order and written
-//                     once per (tile, splat) instance to that instance's slot in
-//                     emission order.  No atomics: the result is deterministic.
+//   k_blend_bwd_tile  per-pixel reverse traversal (the role of backward.cu:521-665)
+//                     for tile edges 8 and 16: starts from the stored final T and
+//                     last-contributor index, recomputes alpha with the forward's
+//                     exact arithmetic and T <- T / (1 - alpha); no
+//                     per-(pixel, Gaussian) state.  One wave owns a tile (every
+//                     lane a pixel of each 8x8 block), tiles are taken heaviest
+//                     first, and a splat's moments are summed over the wave and
+//                     written once per (tile, splat) instance to that instance's
+//                     slot in emission order.  No atomics: deterministic.
+//   k_blend_bwd       the same for tile edge 32: a workgroup per tile, one wave
+//                     per 8x8 block, the waves' sums added in LDS in a fixed order.
 //   k_preprocess_bwd  per-Gaussian chain rule (the role of backward.cu:27-227 and
 //                     :229-519), differentiating the forward exactly as defined
 //                     in k_preprocess (radial clamp, degree-aware SH with the 0.3
@@ -128,9 +130,7 @@ struct BlendBwdArgs {
   uint32_t* ticket;         // zeroed; tiles are claimed in order through it
   const uint2* ranges;
   const uint32_t* ids;
-  const SplatRec* splats;   // per item: splat and block-test constants
-  const uint2* rect;        // per item: tile rectangle (x0 | y0 << 16, x1 | y1 << 16)
-  const uint32_t* item_off; // per item: emission position of its first instance
+  const SplatRec* splats;   // per item: splat, block-test threshold, rectangle, record offset
   unsigned long long* stats;  // optional [3]: evaluated pairs, contributing pairs, loaded instances
   const uint32_t* n_contrib;
   const float* final_T;
@@ -263,33 +263,33 @@ void k_blend_bwd(BlendBwdArgs a) {
     // gather splat 0 (present when nproc > 0).
     uint32_t id_n = 0;
     Splat cur = {};
-    float4 kcur = {};
+    float qcur = 0.0f;
     uint2 rect_cur = make_uint2(0u, 0u);
     uint32_t off_cur = 0;
     if (nproc > 0) {
       const int k = nproc - 64 + lane;
       id_n = k >= 0 ? id_at(k) : 0u;
       const uint32_t id0 = id_n < P ? id_n : 0u;
-      const SplatRec r = a.splats[id0];
+      const SplatRec& r = a.splats[id0];
       cur = r.s;
-      kcur = r.k;
-      rect_cur = a.rect[id0];
-      off_cur = a.item_off[id0];
+      qcur = r.q1;
+      rect_cur = make_uint2(r.rect0, r.rect1);
+      off_cur = r.item_off;
       id_n = k >= 64 ? id_at(k - 64) : 0u;
     }
     for (int cend = nproc; cend > 0; cend -= 64) {
       const int cstart = cend - 64;  // may be negative: lanes below 0 are empty
       const Splat s = cur;
-      const float4 kq = kcur;
+      const float kq = qcur;
       const uint2 srect = rect_cur;
       const uint32_t soff = off_cur;
       if (cstart > 0) {
         const uint32_t idc = id_n < P ? id_n : 0u;
-        const SplatRec r = a.splats[idc];
+        const SplatRec& r = a.splats[idc];
         cur = r.s;
-        kcur = r.k;
-        rect_cur = a.rect[idc];
-        off_cur = a.item_off[idc];
+        qcur = r.q1;
+        rect_cur = make_uint2(r.rect0, r.rect1);
+        off_cur = r.item_off;
         const int k2 = cstart - 128 + lane;
         id_n = k2 >= 0 ? id_at(k2) : 0u;
       }
@@ -297,7 +297,7 @@ void k_blend_bwd(BlendBwdArgs a) {
       n_loaded += (uint32_t)min(64, cend);
       const bool valid = kk_l >= 0 && kk_l < wmax;
       const bool touch = valid && extent_touches(s.a.x, s.a.y, unpack_extent(s.c.w), x0, x1, y0, y1) &&
-                         ellipse_touches_k(s.a.x, s.a.y, s.a.z, s.a.w, s.b.x, kq, x0, x1, y0, y1);
+                         ellipse_touches_q(s.a.x, s.a.y, s.a.z, s.a.w, s.b.x, kq, x0, x1, y0, y1);
       uint64_t m = __ballot(touch);
       uint64_t cm = 0;
       if constexpr (kStage) {
@@ -427,13 +427,13 @@ void k_blend_bwd(BlendBwdArgs a) {
 }
 
 // ---- the backward's tile order: heaviest tiles first ----
-// The forward blend leaves per tile the walk length of its backward (the sum
-// over its 8x8 blocks of the block's largest last contributor).  Tiles are
-// bucketed by its binary logarithm and listed from the heaviest bucket down;
-// tiles without work (no contributor, or a frame without the backward) are
-// left out.  Taken in this order the long tiles start first instead of
-// finishing last (the order within a bucket is whatever the atomics give;
-// it changes which wave takes a tile, never a result).
+// The forward blend leaves per tile its number of contributing (pixel,
+// splat) pairs, the measure of its backward work.  Tiles are bucketed by its
+// binary logarithm and listed from the heaviest bucket down; tiles without
+// work (no contributor, or a frame without the backward) are left out.
+// Taken in this order the long tiles start first instead of finishing last
+// (the order within a bucket is whatever the atomics give; it changes which
+// wave takes a tile, never a result).
 constexpr int kCostBuckets = 33;  // 0: no work; b >= 1: cost in [2^(b-1), 2^b)
 
 W2W_DEV uint32_t cost_bucket(uint32_t c) { return c ? 32u - (uint32_t)__builtin_clz(c) : 0u; }
@@ -487,39 +487,278 @@ __global__ __launch_bounds__(256) void k_tile_order(TileOrderArgs a) {
 }
 
 // Backward blend for tile edges 8 and 16: one wave per tile, lane l holding
-// pixel (l mod 8, l / 8) of each of the tile's NB 8x8 blocks (NB = 1 or 4).
-// Waves claim tiles on their own, heaviest first, through the order of
-// k_tile_order; no barriers, no LDS merge.  The wave walks the tile's list
-// back to front in chunks of 64 from the tile's largest last contributor:
-// lane l gathers instance base + l and tests it against each block (cull
-// extent, then the exact ellipse test); for each instance that touches some
-// block, every touched block with a pixel still behind its last contributor
-// is evaluated (the forward's alpha recomputed bit for bit, T <- T / (1 -
-// alpha) from the stored final T, the hardware rcp), each lane adding its
-// pixels' moments (GradField) in block order, and one wave sum over the
-// lanes gives the instance's record, stored straight from the wave.  The
-// colour of the contributors behind each pixel enters only through its dot
-// product with dL/dcolour, one running sum per pixel.  Every sum has a fixed
-// order: deterministic.
-template <int TS, bool AUX>
-__global__ __launch_bounds__(256) void k_blend_bwd_tile(BlendBwdArgs a) {
+// pixel (l mod 8, l / 8) of each of the tile's NB 8x8 blocks (NB = 1 or 4);
+// waves claim tiles on their own, heaviest first, through the order of
+// k_tile_order; no barriers, no LDS merge.
+//
+// Per chunk of 64 instances (walking the list back to front from the
+// tile's largest last contributor) lane l gathers instance base + l and
+// tests it against each block (cull extent, then the exact ellipse test);
+// for each instance that touches a block with a pixel still behind its last
+// contributor, the pixels recompute the forward's alpha bit for bit and
+// T <- T / (1 - alpha) from the stored final T (the hardware rcp), each lane
+// adding its pixels' moments (GradField) in block order; the colour of the
+// contributors behind each pixel enters only through its dot product with
+// dL/dcolour, one running sum per pixel.  One wave sum over the lanes gives
+// the instance's record, stored straight from the wave.  Every sum has a
+// fixed order: deterministic.
+// per-pixel state of the backward walk
+struct BwdPix {
+  int last;                            // 1-based last contributor (0: none)
+  float T;                             // transmittance in front of the current instance
+  float accG;                          // colour behind the pixel . dL/dcolour
+  float accD, accA;                    // depth and alpha behind (AUX)
+  float gC0, gC1, gC2, gA, gD;         // dL/dcolour, dL/dalpha, dL/ddepth
+  float Tfbg;                          // final T * (background . dL/dcolour)
+};
+
+template <bool AUX>
+W2W_DEV void bwd_pix_load(const BlendBwdArgs& a, const FrameDesc& fd, int px, int py, int n_list, BwdPix& q) {
+  const bool inside = px < fd.W && py < fd.H;
+  const size_t HW = (size_t)fd.W * fd.H;
+  const size_t lp = (size_t)py * fd.W + px;
+  const size_t pid = fd.pix_off + lp;
+  const float* gcol = a.dL_dcolor + 3 * fd.pix_off;
+  q.last = inside ? min((int)a.n_contrib[pid], n_list) : 0;
+  const float Tf = inside ? a.final_T[pid] : 0.0f;
+  q.gC0 = inside ? gcol[lp] : 0.0f;
+  q.gC1 = inside ? gcol[HW + lp] : 0.0f;
+  q.gC2 = inside ? gcol[2 * HW + lp] : 0.0f;
+  q.gA = q.gD = 0.0f;
+  if constexpr (AUX) {
+    q.gA = inside && a.dL_dalpha ? a.dL_dalpha[pid] : 0.0f;
+    q.gD = inside && a.dL_ddepth ? a.dL_ddepth[pid] : 0.0f;
+  }
+  const float bgdot = (fd.bg[0] * q.gC0 + fd.bg[1] * q.gC1) + fd.bg[2] * q.gC2;
+  q.Tfbg = Tf * bgdot;
+  q.T = Tf;
+  q.accG = q.accD = q.accA = 0.0f;
+}
+
+// One (instance, pixel) pair of the walk: if the instance contributes to
+// the pixel, add its moments to F and step the pixel's state back over it.
+// Returns whether any lane of the wave contributed.
+template <bool AUX, bool STATS>
+W2W_DEV bool bwd_pair(float mx, float my, float ca, float cb, float cc, float o, float dz, float sk, float cr,
+                      float cg, float cbl, float pxf, float pyf, int kk, BwdPix& q, float (&F)[GR_FIELDS],
+                      uint32_t& n_eval, uint32_t& n_con) {
+  const float a_min = 0x1.010102p-8f;
+  // straight-line per lane: selects keep the state of lanes that do not
+  // contribute
+  const bool act = kk < q.last;
+  const float dx = mx - pxf, dy = my - pyf;
+  const float pw = power_of(ca, cb, cc, dx, dy);
+  const float G = exp_neg(pw);
+  const float oG = o * G;
+  const float al = fminf(0.99f, oG);
+  const bool contrib = act && (pw <= 0.0f && pw >= sk) && al >= a_min;
+  if constexpr (STATS) {
+    n_eval += act ? 1u : 0u;
+    n_con += contrib ? 1u : 0u;
+  }
+  if (__ballot(contrib) == 0ull) return false;
+  const float r1 = __builtin_amdgcn_rcpf(1.0f - al);
+  const float Tn = q.T * r1;
+  const float aT = al * Tn;
+  const float cdg = (cr * q.gC0 + cg * q.gC1) + cbl * q.gC2;
+  float dLda = cdg - q.accG;
+  if constexpr (AUX) {
+    dLda = dLda + (dz - q.accD) * q.gD;
+    dLda = dLda + (1.0f - q.accA) * q.gA;
+  }
+  dLda = dLda * Tn - q.Tfbg * r1;
+  const bool free_alpha = contrib && oG <= 0.99f;  // a clamped alpha passes no gradient
+  const float qq = free_alpha ? G * dLda : 0.0f;
+  const float aTm = contrib ? aT : 0.0f;
+  const float qdx = qq * dx, qdy = qq * dy;
+  F[GA_SX] += qdx;
+  F[GA_SY] += qdy;
+  F[GA_SXX] = __builtin_fmaf(qdx, dx, F[GA_SXX]);
+  F[GA_SXY] = __builtin_fmaf(qdx, dy, F[GA_SXY]);
+  F[GA_SYY] = __builtin_fmaf(qdy, dy, F[GA_SYY]);
+  F[GA_S] += qq;
+  F[GA_R] = __builtin_fmaf(aTm, q.gC0, F[GA_R]);
+  F[GA_G] = __builtin_fmaf(aTm, q.gC1, F[GA_G]);
+  F[GA_B] = __builtin_fmaf(aTm, q.gC2, F[GA_B]);
+  if constexpr (AUX) F[GA_DEPTH] = __builtin_fmaf(aTm, q.gD, F[GA_DEPTH]);
+  const float nl = 1.0f - al;
+  const float nG = __builtin_fmaf(al, cdg, nl * q.accG);
+  q.T = contrib ? Tn : q.T;
+  q.accG = contrib ? nG : q.accG;
+  if constexpr (AUX) {
+    const float nD = __builtin_fmaf(al, dz, nl * q.accD);
+    const float nA = __builtin_fmaf(nl, q.accA, al);
+    q.accD = contrib ? nD : q.accD;
+    q.accA = contrib ? nA : q.accA;
+  }
+  return true;
+}
+
+// records move as native 4-element vectors (copies of HIP's float4 struct into
+// LDS lower to memcpys that stay in scratch); the record's last 16 bytes are
+// read as integers (threshold bits, rectangle, record offset): a bit cast of
+// one element of a float vector here read element 0's bits
+using v4f = float __attribute__((ext_vector_type(4)));
+using v4u = uint32_t __attribute__((ext_vector_type(4)));
+
+struct BwdShared {
+  v4f stage[4][3][64];  // each wave's chunk (splat a, b, c), read back by broadcast
+};
+
+// One tile, walked by this wave: blocks b = 0 .. NB-1 of every lane.
+template <int TS, bool AUX, bool STATS>
+W2W_DEV void bwd_tile(const BlendBwdArgs& a, BwdShared& sh, const FrameDesc& fd, uint32_t tile, uint32_t P,
+                      int lane, int wv, uint32_t& n_eval, uint32_t& n_con, uint32_t& n_loaded) {
   constexpr int BX = TS / kUnit;
-  constexpr int NB = BX * BX;
-  // records move as native 4-float vectors (copies of HIP's float4 struct
-  // into LDS lower to memcpys that stay in scratch)
-  using v4f = float __attribute__((ext_vector_type(4)));
-  __shared__ v4f s_stage[4][3][64];  // each wave's chunk (splat a, b, c), read back by broadcast
+  constexpr int NBL = BX * BX;  // blocks per lane
+  v4f(*const stage)[64] = sh.stage[wv];
+  const int row = lane >> 4;
+  const bool row_tail = (lane & 15) == 15;
+  const int fa = (row & 1) ? (row >> 1) + 2 : (row >> 1);  // wave_sum10 rows -> fields 0, 2, 1, 3
+  const int lt = (int)(tile - fd.tile_off);
+  const int tx = lt % fd.tiles_x, ty = lt / fd.tiles_x;
+  const uint2 range = a.ranges[tile];
+  const uint2 range_far = a.ranges_far ? a.ranges_far[tile] : make_uint2(0u, 0u);
+  const int n_near = (int)(range.y - range.x);
+  const int n_list = n_near + (int)(range_far.y - range_far.x);
+  auto id_at = [&](int k) -> uint32_t {
+    return k < n_near ? a.ids[range.x + k] : a.ids_far[range_far.x + (uint32_t)(k - n_near)];
+  };
+  // block b of this lane: origin (bx0[b], by0[b]) in pixels
+  int bx0[NBL], by0[NBL], wmax_b[NBL];
+  BwdPix q[NBL];
+  int wmax = 0;
+#pragma unroll
+  for (int b = 0; b < NBL; b++) {
+    bx0[b] = tx * TS + (b % BX) * kUnit;
+    by0[b] = ty * TS + (b / BX) * kUnit;
+    bwd_pix_load<AUX>(a, fd, bx0[b] + (lane & 7), by0[b] + (lane >> 3), n_list, q[b]);
+    int wm = q[b].last;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) wm = max(wm, __shfl_xor(wm, off, 64));
+    wmax_b[b] = __builtin_amdgcn_readfirstlane(wm);
+    wmax = max(wmax, wmax_b[b]);
+  }
+  if (wmax == 0) return;
+  const float pxf = (float)(lane & 7), pyf = (float)(lane >> 3);  // offsets inside a block
+
+  // chunk c covers [wmax - 64 (c + 1), wmax - 64 c) clipped at 0; lane l
+  // holds instance start + l.  Ids two chunks ahead; records, rectangles and
+  // record offsets one chunk ahead.  Lanes before the start gather splat 0
+  // (present when wmax > 0).
+  // one 64-byte record per instance: splat a, b, c and (threshold,
+  // rectangle, record offset)
+  const v4f* const rec4 = reinterpret_cast<const v4f*>(a.splats);
+  const v4u* const rec4u = reinterpret_cast<const v4u*>(a.splats);
+  uint32_t id_n;
+  v4f ra, rb, rc;
+  v4u rk;
+  {
+    const int k = wmax - 64 + lane;
+    id_n = k >= 0 ? id_at(k) : 0u;
+    const uint32_t id0 = id_n < P ? id_n : 0u;
+    ra = rec4[4 * (size_t)id0];
+    rb = rec4[4 * (size_t)id0 + 1];
+    rc = rec4[4 * (size_t)id0 + 2];
+    rk = rec4u[4 * (size_t)id0 + 3];
+    id_n = k >= 64 ? id_at(k - 64) : 0u;
+  }
+  for (int cend = wmax; cend > 0; cend -= 64) {
+    const int cstart = cend - 64;  // may be negative: lanes below 0 are empty
+    const v4f sa = ra, sb = rb, sc = rc;
+    const uint32_t kq_bits = rk.x;
+    const float kq = __uint_as_float(kq_bits);
+    const uint2 srect = make_uint2(rk.y, rk.z);
+    const uint32_t soff = rk.w;
+    if (cstart > 0) {
+      const uint32_t idc = id_n < P ? id_n : 0u;
+      ra = rec4[4 * (size_t)idc];
+      rb = rec4[4 * (size_t)idc + 1];
+      rc = rec4[4 * (size_t)idc + 2];
+      rk = rec4u[4 * (size_t)idc + 3];
+      const int k2 = cstart - 128 + lane;
+      id_n = k2 >= 0 ? id_at(k2) : 0u;
+    }
+    if constexpr (STATS) n_loaded += (uint32_t)min(64, cend);
+    const int kk_l = cstart + lane;
+    const float2 ext = unpack_extent(sc.w);
+    uint64_t mb[NBL];
+    uint64_t m = 0;
+#pragma unroll
+    for (int b = 0; b < NBL; b++) {
+      const float x0 = (float)bx0[b], y0 = (float)by0[b];
+      const float x1 = x0 + (float)(kUnit - 1), y1 = y0 + (float)(kUnit - 1);
+      const bool touch = kk_l >= 0 && kk_l < wmax_b[b] && extent_touches(sa.x, sa.y, ext, x0, x1, y0, y1) &&
+                         ellipse_touches_q(sa.x, sa.y, sa.z, sa.w, sb.x, kq, x0, x1, y0, y1);
+      mb[b] = __ballot(touch);
+      m |= mb[b];
+    }
+    // the record position of each lane's instance: the item's first
+    // instance plus the tile's index in the item's rectangle
+    const uint32_t rx0 = srect.x & 0xFFFFu, ry0 = srect.x >> 16, rx1 = srect.y & 0xFFFFu;
+    const uint32_t pos_l = soff + ((uint32_t)ty - ry0) * (rx1 - rx0) + ((uint32_t)tx - rx0);
+    if (m) {
+      stage[0][lane] = sa;  // read back by this wave only: LDS keeps its order
+      stage[1][lane] = sb;
+      stage[2][lane] = sc;
+      __builtin_amdgcn_wave_barrier();
+      // survivors back to front; the next one's LDS read is in flight while
+      // the current one is evaluated
+      int jn = 63 - __builtin_clzll(m);
+      m &= ~(1ull << jn);
+      v4f na = stage[0][jn], nb = stage[1][jn], nc = stage[2][jn];
+      for (;;) {
+        const int j = jn;
+        const v4f qa = na, qb = nb, qc = nc;
+        jn = m ? 63 - __builtin_clzll(m) : -1;
+        if (jn >= 0) {
+          m &= ~(1ull << jn);
+          na = stage[0][jn];
+          nb = stage[1][jn];
+          nc = stage[2][jn];
+        }
+        const int kk = cstart + j;
+        float F[GR_FIELDS];
+#pragma unroll
+        for (int f = 0; f < GR_FIELDS; f++) F[f] = 0.0f;
+        bool any = false;
+#pragma unroll
+        for (int b = 0; b < NBL; b++) {
+          if (!((mb[b] >> j) & 1ull)) continue;  // uniform
+          any |= bwd_pair<AUX, STATS>(qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w, qc.x, qc.y, qc.z,
+                                      pxf + (float)bx0[b], pyf + (float)by0[b], kk, q[b], F, n_eval, n_con);
+        }
+        if (any) {
+          // rows of s0: Sx, Sxx, Sy, Sxy; s1: Syy, r, S, g; s2: b, 0, depth, 0
+          float s0, s1, s2;
+          wave_sum10(F[0], F[1], F[2], F[3], F[4], F[5], F[6], F[7], F[8], F[9], s0, s1, s2);
+          const uint32_t pos = (uint32_t)__builtin_amdgcn_readlane((int)pos_l, j);
+          if (pos < a.capacity) {
+            float* r = a.rec + (size_t)pos * GR_FIELDS;
+            if (row_tail) {
+              r[fa] = s0;
+              r[4 + fa] = s1;
+              if ((row & 1) == 0) r[8 + (row >> 1)] = s2;
+            }
+            if (lane == 0) a.flag[pos] = 1;
+          }
+        }
+        if (jn < 0) break;
+      }
+      __builtin_amdgcn_wave_barrier();  // this chunk's reads precede the next chunk's writes
+    }
+  }
+}
+
+template <int TS, bool AUX, bool STATS>
+__global__ __launch_bounds__(256) void k_blend_bwd_tile(BlendBwdArgs a) {
+  __shared__ BwdShared sh;
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
   const uint32_t Tn = min(hdr->n_tiles, a.tiles_cap);
   const uint32_t P = min(hdr->n_items, a.items_cap);
   const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
   const int lane = (int)(threadIdx.x & 63u);
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  v4f(*const stage)[64] = s_stage[wv];
-  const int row = lane >> 4;
-  const bool row_tail = (lane & 15) == 15;
-  const int fa = (row & 1) ? (row >> 1) + 2 : (row >> 1);  // wave_sum10 rows -> fields 0, 2, 1, 3
-  const float a_min = 0x1.010102p-8f;
   const uint32_t n_order = min(*a.n_order, Tn);
   uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
   for (;;) {
@@ -529,201 +768,17 @@ __global__ __launch_bounds__(256) void k_blend_bwd_tile(BlendBwdArgs a) {
     if (t >= n_order) break;
     const uint32_t tile = a.order[t];
     if (tile >= Tn) continue;
+    // the heaviest tiles set the kernel's length: their waves take issue
+    // priority over the waves of lighter tiles sharing the SIMD
+    if (t < n_order / 16u)
+      __builtin_amdgcn_s_setprio(2);
+    else
+      __builtin_amdgcn_s_setprio(0);
     const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, tile)];
-    const int lt = (int)(tile - fd.tile_off);
-    const int tx = lt % fd.tiles_x, ty = lt / fd.tiles_x;
-    const int ox = tx * TS, oy = ty * TS;  // the tile's first pixel
-    const float pxf = (float)(ox + (lane & 7)), pyf = (float)(oy + (lane >> 3));
-    const uint2 range = a.ranges[tile];
-    const uint2 range_far = a.ranges_far ? a.ranges_far[tile] : make_uint2(0u, 0u);
-    const int n_near = (int)(range.y - range.x);
-    const int n_list = n_near + (int)(range_far.y - range_far.x);
-    auto id_at = [&](int k) -> uint32_t {
-      return k < n_near ? a.ids[range.x + k] : a.ids_far[range_far.x + (uint32_t)(k - n_near)];
-    };
-    const size_t HW = (size_t)fd.W * fd.H;
-    const float* gcol = a.dL_dcolor + 3 * fd.pix_off;
-
-    // per pixel of each block: last contributor, running T, the running
-    // dot product of the colour behind it with dL/dcolour (and the depth /
-    // alpha sums), dL/dcolour, final T times background . dL/dcolour
-    int last[NB], wmax_b[NB];
-    float T[NB], accG[NB], gC0[NB], gC1[NB], gC2[NB], Tfbg[NB];
-    float gA[NB], gD[NB], accD[NB], accA[NB];
-    int wmax = 0;
-#pragma unroll
-    for (int b = 0; b < NB; b++) {
-      const int px = ox + (b % BX) * kUnit + (lane & 7), py = oy + (b / BX) * kUnit + (lane >> 3);
-      const bool inside = px < fd.W && py < fd.H;
-      const size_t lp = (size_t)py * fd.W + px;
-      const size_t pid = fd.pix_off + lp;
-      last[b] = inside ? min((int)a.n_contrib[pid], n_list) : 0;
-      const float Tf = inside ? a.final_T[pid] : 0.0f;
-      gC0[b] = inside ? gcol[lp] : 0.0f;
-      gC1[b] = inside ? gcol[HW + lp] : 0.0f;
-      gC2[b] = inside ? gcol[2 * HW + lp] : 0.0f;
-      gA[b] = gD[b] = 0.0f;
-      if constexpr (AUX) {
-        gA[b] = inside && a.dL_dalpha ? a.dL_dalpha[pid] : 0.0f;
-        gD[b] = inside && a.dL_ddepth ? a.dL_ddepth[pid] : 0.0f;
-      }
-      const float bgdot = (fd.bg[0] * gC0[b] + fd.bg[1] * gC1[b]) + fd.bg[2] * gC2[b];
-      Tfbg[b] = Tf * bgdot;
-      T[b] = Tf;
-      accG[b] = accD[b] = accA[b] = 0.0f;
-      int wm = last[b];
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) wm = max(wm, __shfl_xor(wm, off, 64));
-      wmax_b[b] = __builtin_amdgcn_readfirstlane(wm);
-      wmax = max(wmax, wmax_b[b]);
-    }
-    if (wmax == 0) continue;
-
-    // chunk c covers [wmax - 64 (c + 1), wmax - 64 c) clipped at 0; lane l
-    // holds instance start + l.  Ids two chunks ahead; records, rectangles
-    // and record offsets one chunk ahead.  Lanes before the start gather
-    // splat 0 (present when wmax > 0).
-    // the next chunk's record of this lane: splat a, b, c and block-test
-    // constants, its rectangle and record offset
-    const v4f* const rec4 = reinterpret_cast<const v4f*>(a.splats);
-    uint32_t id_n;
-    v4f ra, rb, rc, rk;
-    uint2 rect_cur;
-    uint32_t off_cur;
-    {
-      const int k = wmax - 64 + lane;
-      id_n = k >= 0 ? id_at(k) : 0u;
-      const uint32_t id0 = id_n < P ? id_n : 0u;
-      ra = rec4[4 * (size_t)id0];
-      rb = rec4[4 * (size_t)id0 + 1];
-      rc = rec4[4 * (size_t)id0 + 2];
-      rk = rec4[4 * (size_t)id0 + 3];
-      rect_cur = a.rect[id0];
-      off_cur = a.item_off[id0];
-      id_n = k >= 64 ? id_at(k - 64) : 0u;
-    }
-    for (int cend = wmax; cend > 0; cend -= 64) {
-      const int cstart = cend - 64;  // may be negative: lanes below 0 are empty
-      const v4f sa = ra, sb = rb, sc = rc;
-      const float4 kq = make_float4(rk.x, rk.y, rk.z, rk.w);
-      const uint2 srect = rect_cur;
-      const uint32_t soff = off_cur;
-      if (cstart > 0) {
-        const uint32_t idc = id_n < P ? id_n : 0u;
-        ra = rec4[4 * (size_t)idc];
-        rb = rec4[4 * (size_t)idc + 1];
-        rc = rec4[4 * (size_t)idc + 2];
-        rk = rec4[4 * (size_t)idc + 3];
-        rect_cur = a.rect[idc];
-        off_cur = a.item_off[idc];
-        const int k2 = cstart - 128 + lane;
-        id_n = k2 >= 0 ? id_at(k2) : 0u;
-      }
-      n_loaded += (uint32_t)min(64, cend);
-      const int kk_l = cstart + lane;
-      const float2 ext = unpack_extent(sc.w);
-      uint64_t mb[NB];
-      uint64_t m = 0;
-#pragma unroll
-      for (int b = 0; b < NB; b++) {
-        const float x0 = (float)(ox + (b % BX) * kUnit), y0 = (float)(oy + (b / BX) * kUnit);
-        const float x1 = x0 + (float)(kUnit - 1), y1 = y0 + (float)(kUnit - 1);
-        const bool touch = kk_l >= 0 && kk_l < wmax_b[b] && extent_touches(sa.x, sa.y, ext, x0, x1, y0, y1) &&
-                           ellipse_touches_k(sa.x, sa.y, sa.z, sa.w, sb.x, kq, x0, x1, y0, y1);
-        mb[b] = __ballot(touch);
-        m |= mb[b];
-      }
-      if (m == 0ull) continue;
-      stage[0][lane] = sa;  // read back by this wave only: LDS keeps its order
-      stage[1][lane] = sb;
-      stage[2][lane] = sc;
-      __builtin_amdgcn_wave_barrier();
-      // the record position of each lane's instance: the item's first
-      // instance plus the tile's index in the item's rectangle
-      const uint32_t rx0 = srect.x & 0xFFFFu, ry0 = srect.x >> 16, rx1 = srect.y & 0xFFFFu;
-      const uint32_t pos_l = soff + ((uint32_t)ty - ry0) * (rx1 - rx0) + ((uint32_t)tx - rx0);
-      while (m) {
-        const int j = 63 - __builtin_clzll(m);
-        m &= ~(1ull << j);
-        const int kk = cstart + j;
-        const v4f qa = stage[0][j], qb = stage[1][j], qc = stage[2][j];
-        const float mx = qa.x, my = qa.y, ca = qa.z, cb = qa.w;
-        const float cc = qb.x, o = qb.y, dz = qb.z, sk = qb.w;
-        const float cr = qc.x, cg = qc.y, cbl = qc.z;
-        float F0 = 0.0f, F1 = 0.0f, F2 = 0.0f, F3 = 0.0f, F4 = 0.0f, F5 = 0.0f;
-        float F6 = 0.0f, F7 = 0.0f, F8 = 0.0f, F9 = 0.0f;
-        bool any = false;
-#pragma unroll
-        for (int b = 0; b < NB; b++) {
-          if (!((mb[b] >> j) & 1ull)) continue;  // uniform
-          // straight-line per lane: selects keep the state of lanes that do
-          // not contribute
-          const bool act = kk < last[b];
-          const float dx = mx - (pxf + (float)((b % BX) * kUnit));
-          const float dy = my - (pyf + (float)((b / BX) * kUnit));
-          const float pw = power_of(ca, cb, cc, dx, dy);
-          const float G = exp_neg(pw);
-          const float oG = o * G;
-          const float al = fminf(0.99f, oG);
-          const bool contrib = act && (pw <= 0.0f && pw >= sk) && al >= a_min;
-          n_eval += act ? 1u : 0u;
-          n_con += contrib ? 1u : 0u;
-          if (__ballot(contrib) == 0ull) continue;
-          any = true;
-          const float r1 = __builtin_amdgcn_rcpf(1.0f - al);
-          const float Tn = T[b] * r1;
-          const float aT = al * Tn;
-          const float cdg = (cr * gC0[b] + cg * gC1[b]) + cbl * gC2[b];
-          float dLda = cdg - accG[b];
-          if constexpr (AUX) {
-            dLda = dLda + (dz - accD[b]) * gD[b];
-            dLda = dLda + (1.0f - accA[b]) * gA[b];
-          }
-          dLda = dLda * Tn - Tfbg[b] * r1;
-          const bool free_alpha = contrib && oG <= 0.99f;  // a clamped alpha passes no gradient
-          const float qq = free_alpha ? G * dLda : 0.0f;
-          const float aTm = contrib ? aT : 0.0f;
-          const float qdx = qq * dx, qdy = qq * dy;
-          F0 += qdx;
-          F1 += qdy;
-          F2 = __builtin_fmaf(qdx, dx, F2);
-          F3 = __builtin_fmaf(qdx, dy, F3);
-          F4 = __builtin_fmaf(qdy, dy, F4);
-          F5 += qq;
-          F6 = __builtin_fmaf(aTm, gC0[b], F6);
-          F7 = __builtin_fmaf(aTm, gC1[b], F7);
-          F8 = __builtin_fmaf(aTm, gC2[b], F8);
-          if constexpr (AUX) F9 = __builtin_fmaf(aTm, gD[b], F9);
-          const float nl = 1.0f - al;
-          const float nG = __builtin_fmaf(al, cdg, nl * accG[b]);
-          T[b] = contrib ? Tn : T[b];
-          accG[b] = contrib ? nG : accG[b];
-          if constexpr (AUX) {
-            const float nD = __builtin_fmaf(al, dz, nl * accD[b]);
-            const float nA = __builtin_fmaf(nl, accA[b], al);
-            accD[b] = contrib ? nD : accD[b];
-            accA[b] = contrib ? nA : accA[b];
-          }
-        }
-        if (!any) continue;
-        // rows of s0: Sx, Sxx, Sy, Sxy; s1: Syy, r, S, g; s2: b, 0, depth, 0
-        float s0, s1, s2;
-        wave_sum10(F0, F1, F2, F3, F4, F5, F6, F7, F8, F9, s0, s1, s2);
-        const uint32_t pos = (uint32_t)__builtin_amdgcn_readlane((int)pos_l, j);
-        if (pos < a.capacity) {
-          float* r = a.rec + (size_t)pos * GR_FIELDS;
-          if (row_tail) {
-            r[fa] = s0;
-            r[4 + fa] = s1;
-            if ((row & 1) == 0) r[8 + (row >> 1)] = s2;
-          }
-          if (lane == 0) a.flag[pos] = 1;
-        }
-      }
-      __builtin_amdgcn_wave_barrier();  // this chunk's reads precede the next chunk's writes
-    }
+    bwd_tile<TS, AUX, STATS>(a, sh, fd, tile, P, lane, wv, n_eval, n_con, n_loaded);
   }
-  if (a.stats) {
+  __builtin_amdgcn_s_setprio(0);
+  if (STATS && a.stats) {
     unsigned long long e = n_eval, c = n_con, kf = (lane == 0) ? n_loaded : 0u;
     for (int off = 32; off >= 1; off >>= 1) {
       e += __shfl_xor(e, off, 64);
@@ -747,7 +802,6 @@ struct PreBwdArgs {
   const float* sh;       // packed SH blocks
   const int32_t* radii;  // per item
   const uint32_t* counts;    // per item: instances emitted
-  const uint32_t* item_off;  // per item: emission position of the first
   const float* rec;          // [capacity][GR_FIELDS] per-instance partials
   const uint8_t* flag;       // [capacity]
   uint32_t capacity;
@@ -783,7 +837,7 @@ __global__ __launch_bounds__(256) void k_item_grads(PreBwdArgs a) {
 #pragma unroll
     for (int f = 0; f < GR_FIELDS; f++) ga[f] = 0.0f;
     if (vis) {
-      const uint32_t k0 = a.item_off[i];
+      const uint32_t k0 = a.splats[i].item_off;  // emission position of the item's first instance
       const uint32_t k1 = min(k0 + a.counts[i], a.capacity);
       for (uint32_t kw = k0 & ~7u; kw < k1; kw += 8) {
         uint64_t f8 = *reinterpret_cast<const uint64_t*>(a.flag + kw);

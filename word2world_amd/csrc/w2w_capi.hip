@@ -124,6 +124,7 @@ struct Plan {
   size_t tid_far;                         // binning: the far sort's second id buffer
   size_t ranges_far;                      // image
   size_t tile_cost, tile_order;           // image: the backward's tile order
+  size_t rlb;                             // geom: the records' item-order scan
   // the near tile sort ends in (L.sorted_tiles, L.sorted_ids); its other
   // buffer pair is free afterwards and takes the far slice's emission
   size_t scratch_keys() const { return (tpasses & 1) ? tkey0 : tkey1; }
@@ -174,7 +175,8 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   p.dhist = take(4 * 4 * RADIX);
   p.dstatus = take((size_t)4 * p.dtiles * RADIX * 4);
   p.slb = take((size_t)p.stiles * 4 + 4);
-  L.item_off = take(4 * I);
+  p.rlb = take((size_t)p.stiles * 4 + 4);  // look-back words of the records' item-order scan
+  L.item_off = 0;  // the offset lives in the splat record
   L.item_grad = take(32 * I);
   p.sliced = c.depth_split > 0 && c.depth_split < 1000;
   p.ocount = p.boff = p.slb_far = p.open_bm = p.hint_bm = p.open_units = p.hint_sat = p.big = 0;
@@ -314,10 +316,14 @@ void launch_blend_bwd(const BlendBwdArgs& a, uint32_t tiles_cap, hipStream_t s) 
   const bool aux = a.dL_dalpha || a.dL_ddepth;
   if constexpr (TS <= 16) {
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((tiles_cap + 3) / 4, kBlendGrid));
-    if (aux)
-      hipLaunchKernelGGL((k_blend_bwd_tile<TS, true>), dim3(grid), dim3(256), 0, s, a);
+    if (aux && a.stats)
+      hipLaunchKernelGGL((k_blend_bwd_tile<TS, true, true>), dim3(grid), dim3(256), 0, s, a);
+    else if (aux)
+      hipLaunchKernelGGL((k_blend_bwd_tile<TS, true, false>), dim3(grid), dim3(256), 0, s, a);
+    else if (a.stats)
+      hipLaunchKernelGGL((k_blend_bwd_tile<TS, false, true>), dim3(grid), dim3(256), 0, s, a);
     else
-      hipLaunchKernelGGL((k_blend_bwd_tile<TS, false>), dim3(grid), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_blend_bwd_tile<TS, false, false>), dim3(grid), dim3(256), 0, s, a);
   } else {
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(tiles_cap, kBlendGrid));
     if (aux)
@@ -398,6 +404,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   fb.add(at<uint32_t>(I, p.L.ranges), 2 * (uint64_t)c.tiles, 0u, s);
   fb.add(at<uint32_t>(I, p.tile_cost), (uint64_t)c.tiles, 0u, s);
   if (items_cap > 0) {
+    fb.add(at<uint32_t>(G, p.rlb), p.stiles + 1, 0u, s);
     fb.add(at<uint32_t>(G, p.dhist), 4 * RADIX, 0u, s);
     fb.add(at<uint32_t>(G, p.slb), p.stiles + 1, 0u, s);
     fb.add(at<uint32_t>(B, p.thist), 4 * RADIX, 0u, s);
@@ -436,6 +443,15 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
       hipLaunchKernelGGL(k_preprocess, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 8192)),
                          dim3(256), 0, s, pa);
     }
+    {
+      // each item's first backward record: the tile counts scanned in item
+      // order (an item's records are contiguous), into its splat record
+      ProfSpan ps(W2W_KID_SCAN, s);
+      hipLaunchKernelGGL(k_scan<false>, dim3(std::min<uint32_t>(p.stiles, 2048)), dim3(SCAN_THREADS), 0, s,
+                         at<uint32_t>(G, p.L.counts), nullptr, nullptr, at<uint32_t>(G, p.rlb) + 1,
+                         status + ST_REC_TICKET, status + ST_REC_TOTAL, nullptr, d_items, items_cap, cap,
+                         nullptr, nullptr, nullptr, at<SplatRec>(G, p.L.splats));
+    }
     // depth sort: (depth bits, item) over the items, 4 passes, result in dkey0 / perm
     sort_passes<uint32_t>(at<uint32_t>(G, p.dkey0), at<uint32_t>(G, p.L.perm),
                           at<uint32_t>(G, p.dkey0), at<uint32_t>(G, p.L.perm),
@@ -448,8 +464,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
                          0, s, at<uint32_t>(G, p.L.counts), at<uint32_t>(G, p.L.perm),
                          at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.slb) + 1,
                          status + ST_SCAN_TICKET, status + ST_K, status + ST_OVERFLOW, d_items,
-                         items_cap, cap, nullptr,
-                         at<uint32_t>(G, p.L.item_off));  // the backward finds an item's records from here
+                         items_cap, cap);
     }
     if (cap > 0) {
       // depth-sliced binning: the near slice is binned here, the far slice
@@ -639,8 +654,6 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
   ba.ranges = at<uint2>(I, p.L.ranges);
   ba.ids = at<uint32_t>(B, p.L.sorted_ids);
   ba.splats = at<SplatRec>(G, p.L.splats);
-  ba.rect = at<uint2>(G, p.L.rect);
-  ba.item_off = at<uint32_t>(G, p.L.item_off);
   ba.rec = rec;
   ba.flag = flag;
   ba.stats = g_stats ? g_stats + 3 : nullptr;
@@ -662,7 +675,6 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
   pa.means = prm.means; pa.scales = prm.scales; pa.rots = prm.rots; pa.sh = prm.sh;
   pa.radii = at<int32_t>(G, p.L.radii);
   pa.counts = at<uint32_t>(G, p.L.counts);
-  pa.item_off = at<uint32_t>(G, p.L.item_off);
   pa.rec = rec;
   pa.flag = flag;
   pa.capacity = (uint32_t)c.capacity;

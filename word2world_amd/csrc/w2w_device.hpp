@@ -384,13 +384,19 @@ struct __align__(16) Splat {
   float4 a, b, c;
 };
 
-// What the preprocess stores per item and the blends gather per instance:
-// the splat and the per-splat part of the block test (cull_consts, below),
-// 64 bytes on a 64-byte boundary, so one gather is one 64-byte HBM segment
-// (a 48-byte splat beside a separate 16-byte array touches 2.5 on average).
+// What the preprocess stores per item and the blends gather per instance,
+// 64 bytes on a 64-byte boundary so that one gather is one 64-byte HBM
+// segment: the splat, the block test's threshold (cull_threshold, below),
+// the item's tile rectangle (x0 | y0 << 16, x1 | y1 << 16) and the slot of
+// its first backward record (an item-order scan of the tile counts; the
+// item's records are contiguous, one per tile of its rectangle in row-major
+// order), which together place an instance's backward record without
+// further gathers.
 struct __align__(64) SplatRec {
   Splat s;
-  float4 k;
+  float q1;           // block-test threshold
+  uint32_t rect0, rect1;
+  uint32_t item_off;
 };
 
 constexpr uint32_t kExtentInf = 0x7C007C00u;  // (+inf, +inf) in binary16
@@ -453,36 +459,37 @@ W2W_HD bool extent_touches(float mx, float my, float2 e, float x0, float x1, flo
 // minimum above q1 (1 + 1e-6) counts as a miss; every degenerate case, NaN
 // included, counts as touching.
 //
-// Split in two: cull_consts, the per-splat part (threshold and the edge
-// minimisers' slopes -b/c and -b/a; formed once per splat by the
-// preprocess), and ellipse_touches_k per (splat, block).  The threshold is
-// rounded up to float; the slopes in float move the edge minimiser by
-// |b/c| 2^-24 |dx| at most, which raises the computed minimum by a factor
-// of at most 1 + (b^2 / det) 2^-48 < 1 + 1e-7 for every conic with a finite
-// threshold (8 u kappa <= 0.5 bounds b^2 / det), inside the 1e-6 widening.
-W2W_HD float4 cull_consts(float ca, float cb, float cc, float skip) {
+// Split in two: cull_threshold, the per-splat threshold formed once by the
+// preprocess (rounded up to float), and ellipse_touches_q per (splat,
+// block), which forms the edge minimisers' slopes -b/c and -b/a by float
+// division: they move the edge minimiser by |b/c| 2^-24 |dx| at most, which
+// raises the computed minimum by a factor of at most 1 + (b^2 / det) 2^-48 <
+// 1 + 1e-7 for every conic with a finite threshold (8 u kappa <= 0.5 bounds
+// b^2 / det), inside the 1e-6 widening.
+W2W_HD float cull_threshold(float ca, float cb, float cc, float skip) {
   const float inf = __builtin_inff();
   const double a = ca, b = cb, c = cc;
-  if (!(a > 0.0) || !(c > 0.0) || !(skip < 0.0f)) return make_float4(inf, 0.0f, 0.0f, 0.0f);
+  if (!(a > 0.0) || !(c > 0.0) || !(skip < 0.0f)) return inf;
   const double det = a * c - b * b;
-  if (!(det > 0.0)) return make_float4(inf, 0.0f, 0.0f, 0.0f);
+  if (!(det > 0.0)) return inf;
   const double h = 0.5 * (a - c);
   const double lmax = 0.5 * (a + c) + __builtin_sqrt(h * h + b * b);
   const double u = 0x1p-24;
   const double den = 1.0 - 8.0 * u * ((a + c) * lmax / det);
-  if (!(den > 0.5)) return make_float4(inf, 0.0f, 0.0f, 0.0f);
+  if (!(den > 0.5)) return inf;
   const double q1 = (-2.0 * (double)skip) * (1.0 + 4.0 * u) / den * (1.0 + 1e-6);
   float q1f = (float)q1;
   if ((double)q1f < q1) q1f = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, q1f) + 1u);  // round up
-  return make_float4(q1f, (float)(-b / c), (float)(-b / a), 0.0f);
+  return q1f;
 }
 
-W2W_HD bool ellipse_touches_k(float mx, float my, float ca, float cb, float cc, float4 k, float x0, float x1,
+W2W_HD bool ellipse_touches_q(float mx, float my, float ca, float cb, float cc, float q1, float x0, float x1,
                               float y0, float y1) {
-  if (!(k.x < __builtin_inff())) return true;
+  if (!(q1 < __builtin_inff())) return true;
   const double X = mx, Y = my;
   if (!(X < x0 || X > x1 || Y < y0 || Y > y1)) return true;  // centre inside (or NaN)
-  const double a = ca, b = cb, c = cc, kc = k.y, ka = k.z;
+  // a finite threshold implies a > 0, c > 0
+  const double a = ca, b = cb, c = cc, kc = (double)(-cb / cc), ka = (double)(-cb / ca);
   double qmin = 1e300;
   // vertical edges p.x = E, offset dx = X - E, dy in [Y - y1, Y - y0]
   for (int e = 0; e < 2; e++) {
@@ -500,12 +507,12 @@ W2W_HD bool ellipse_touches_k(float mx, float my, float ca, float cb, float cc, 
     const double q = (a * dx * dx + 2.0 * b * dx * dy) + c * dy * dy;
     qmin = q < qmin ? q : qmin;
   }
-  return !(qmin > (double)k.x);
+  return !(qmin > (double)q1);
 }
 
 W2W_HD bool ellipse_touches(float mx, float my, float ca, float cb, float cc, float skip, float x0,
                             float x1, float y0, float y1) {
-  return ellipse_touches_k(mx, my, ca, cb, cc, cull_consts(ca, cb, cc, skip), x0, x1, y0, y1);
+  return ellipse_touches_q(mx, my, ca, cb, cc, cull_threshold(ca, cb, cc, skip), x0, x1, y0, y1);
 }
 
 // ---- wave64 helpers ----

@@ -62,6 +62,8 @@ enum StatusWord : int {
   ST_ORDER_N = 33,     // backward: tiles with work, in the heaviest-first order
   ST_COST_HIST = 34,   //   tiles per cost bucket (kCostBuckets words)
   ST_COST_CURSOR = 67, //   slots taken per cost bucket (kCostBuckets words)
+  ST_REC_TICKET = 100, // the records' item-order scan
+  ST_REC_TOTAL = 101,
   ST_WORDS = 128,
 };
 
@@ -127,6 +129,47 @@ W2W_DEV uint32_t sat_rect(const uint32_t* __restrict__ sat, uint32_t t0, uint32_
   return c;
 }
 
+// Look-back by a whole wave: lane l reads the word of tile p - l (a tile
+// before the first counts as inclusive 0) and the wave adds the words up to
+// the nearest inclusive one once all of them are published.  Sums saturate
+// at LB_VAL (above every capacity), so a total too large for the 30-bit
+// field still reads as an overflow.
+// lb_publish: the tile's own total, as soon as it is known (tile 0's is
+// already its inclusive prefix); lb_resolve: the look-back proper, then the
+// tile's inclusive prefix.  scan_lookback_wave does both in a row.
+W2W_DEV void lb_publish(uint32_t* lb, uint32_t tile, uint32_t tot, int lane) {
+  if (lane == 0) st_release_dev(lb + tile, (tile == 0 ? LB_INC : LB_AGG) | tot);
+}
+
+W2W_DEV uint32_t lb_resolve(uint32_t* lb, uint32_t tile, uint32_t tot, int lane) {
+  if (tile == 0) return 0u;
+  uint32_t excl = 0;
+  int p = (int)tile - 1;
+  for (;;) {
+    const int q = p - lane;
+    const uint32_t w = q >= 0 ? ld_acquire_dev(lb + q) : LB_INC;
+    const uint32_t flag = w & ~LB_VAL;
+    const uint64_t inc = __ballot(flag == LB_INC);
+    const uint64_t unready = __ballot(flag == 0u);
+    const int first = inc ? __builtin_ctzll(inc) : 64;
+    const uint64_t need = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+    if (unready & need) continue;  // a word up to the nearest inclusive one is not published yet
+    uint64_t v = lane <= first ? (uint64_t)(w & LB_VAL) : 0ull;  // 64 words of up to 30 bits
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    excl = (uint32_t)min<uint64_t>((uint64_t)excl + v, LB_VAL);
+    if (first < 64) break;
+    p -= 64;
+  }
+  if (lane == 0) st_release_dev(lb + tile, LB_INC | min(excl + tot, LB_VAL));
+  return excl;
+}
+
+W2W_DEV uint32_t scan_lookback_wave(uint32_t* lb, uint32_t tile, uint32_t tot, int lane) {
+  lb_publish(lb, tile, tot, lane);
+  return lb_resolve(lb, tile, tot, lane);
+}
+
 struct PreArgs {
   const void* table;
   uint32_t scenes_cap;
@@ -150,6 +193,9 @@ struct PreArgs {
   uint32_t* hcount;
 };
 
+// One thread per item: the splat record (its record offset is filled in by
+// k_scan over the tile counts in item order), radius, rectangle, tile count,
+// depth key.
 __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
   const uint32_t n_items = min(hdr->n_items, a.items_cap);
@@ -158,17 +204,78 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
   const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_items; i += stride) {
-    const ItemRef ir = item_ref(scenes, n_scenes, i);
-    const SceneDesc& sd = scenes[ir.scene];
-    const FrameDesc& fd = frames[ir.frame];
+    ItemRef ir{};
     Cam c;
+    float p[3] = {0.0f, 0.0f, 0.0f};
+    float o = 0.0f;
+    Geom g{};
+    bool keep = false;
+    float skip_below = 0.0f;
+    uint32_t ext = kExtentInf;
+    uint32_t x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    int32_t radius = 0;
+    uint32_t cnt = 0;
+    {
+    ir = item_ref(scenes, n_scenes, i);
+    const FrameDesc& fd = frames[ir.frame];
     load_cam(c, fd);
     const uint32_t gi = ir.gg;
-    const float p[3] = {a.means[3 * gi], a.means[3 * gi + 1], a.means[3 * gi + 2]};
+    p[0] = a.means[3 * gi]; p[1] = a.means[3 * gi + 1]; p[2] = a.means[3 * gi + 2];
     const float s[3] = {a.scales[3 * gi], a.scales[3 * gi + 1], a.scales[3 * gi + 2]};
     const float q[4] = {a.rots[4 * gi], a.rots[4 * gi + 1], a.rots[4 * gi + 2], a.rots[4 * gi + 3]};
-    const float o = a.opac[gi];
-
+    o = a.opac[gi];
+    geometry(c, p, s, q, g);
+    keep = g.visible;
+    float r = 0.0f;
+    if (keep) {
+      float half_tr = (g.cxx + g.cyy) * 0.5f;
+      float ds = half_tr * half_tr - g.det;
+      float root = __builtin_sqrtf(fmaxf(0.1f, ds));
+      float lmax = fmaxf(half_tr + root, half_tr - root);
+      float r3 = 3.0f * __builtin_sqrtf(lmax);
+      keep = o > 0.0039f;
+      if (keep) {
+        float lnr = log_pos(o * 255.0f);
+        // exponents below ln(1/(255 o)) - 1e-5 give o*exp(power) < 1/255 with a
+        // margin far above the rounding of exp and of this log, so the blend
+        // kernels may skip such pairs before evaluating the exponential
+        skip_below = -lnr - 1e-5f;
+        float radp = __builtin_sqrtf((2.0f * lmax) * lnr);
+        r = fminf(radp, r3);
+        keep = r > 0.0f;
+      }
+    }
+    ext = keep ? cull_extent(g.ca, g.cb, g.cc, skip_below) : kExtentInf;
+    if (keep) {
+      const float tsf = (float)a.ts, tsm1 = (float)(a.ts - 1);
+      x0 = tile_coord((g.px - r) / tsf, (uint32_t)fd.tiles_x);
+      y0 = tile_coord((g.py - r) / tsf, (uint32_t)fd.tiles_y);
+      x1 = tile_coord(((g.px + r) + tsm1) / tsf, (uint32_t)fd.tiles_x);
+      y1 = tile_coord(((g.py + r) + tsm1) / tsf, (uint32_t)fd.tiles_y);
+      if (fd.flags & FRAME_TIGHT) {
+        // tiles holding a pixel centre within the extent box (half a pixel of
+        // slack covers the rounding of these quotients)
+        const float2 e = unpack_extent(__builtin_bit_cast(float, ext));
+        if (e.x < 60000.0f && e.y < 60000.0f) {
+          const uint32_t bx0 = tile_coord(((g.px - e.x) - 0.5f) / tsf, (uint32_t)fd.tiles_x);
+          const uint32_t by0 = tile_coord(((g.py - e.y) - 0.5f) / tsf, (uint32_t)fd.tiles_y);
+          const uint32_t bx1 = min(tile_coord(((g.px + e.x) + 0.5f) / tsf, (uint32_t)fd.tiles_x) + 1u,
+                                   (uint32_t)fd.tiles_x);
+          const uint32_t by1 = min(tile_coord(((g.py + e.y) + 0.5f) / tsf, (uint32_t)fd.tiles_y) + 1u,
+                                   (uint32_t)fd.tiles_y);
+          x0 = max(x0, bx0); y0 = max(y0, by0);
+          x1 = min(x1, bx1); y1 = min(y1, by1);
+        }
+      }
+      cnt = (x1 > x0 && y1 > y0) ? (x1 - x0) * (y1 - y0) : 0u;
+      radius = (int32_t)__builtin_ceilf(r);
+    }
+    }
+    SplatRec rec;
+    uint32_t hcnt = 0, dkey = CULLED_DEPTH_KEY;
+    {
+    const SceneDesc& sd = scenes[ir.scene];
+    const FrameDesc& fd = frames[ir.frame];
     // SH colour, direction normalize(mean - campos), +0.5, unclamped
     float rgb[3];
     {
@@ -212,56 +319,6 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
       rgb[2] = r2 + 0.5f;
     }
 
-    Geom g;
-    geometry(c, p, s, q, g);
-    int32_t radius = 0;
-    uint32_t cnt = 0, x0 = 0, y0 = 0, x1 = 0, y1 = 0;
-    bool keep = g.visible;
-    float r = 0.0f;
-    float skip_below = 0.0f;
-    if (keep) {
-      float half_tr = (g.cxx + g.cyy) * 0.5f;
-      float ds = half_tr * half_tr - g.det;
-      float root = __builtin_sqrtf(fmaxf(0.1f, ds));
-      float lmax = fmaxf(half_tr + root, half_tr - root);
-      float r3 = 3.0f * __builtin_sqrtf(lmax);
-      keep = o > 0.0039f;
-      if (keep) {
-        float lnr = log_pos(o * 255.0f);
-        // exponents below ln(1/(255 o)) - 1e-5 give o*exp(power) < 1/255 with a
-        // margin far above the rounding of exp and of this log, so the blend
-        // kernels may skip such pairs before evaluating the exponential
-        skip_below = -lnr - 1e-5f;
-        float radp = __builtin_sqrtf((2.0f * lmax) * lnr);
-        r = fminf(radp, r3);
-        keep = r > 0.0f;
-      }
-    }
-    const uint32_t ext = keep ? cull_extent(g.ca, g.cb, g.cc, skip_below) : kExtentInf;
-    if (keep) {
-      const float tsf = (float)a.ts, tsm1 = (float)(a.ts - 1);
-      x0 = tile_coord((g.px - r) / tsf, (uint32_t)fd.tiles_x);
-      y0 = tile_coord((g.py - r) / tsf, (uint32_t)fd.tiles_y);
-      x1 = tile_coord(((g.px + r) + tsm1) / tsf, (uint32_t)fd.tiles_x);
-      y1 = tile_coord(((g.py + r) + tsm1) / tsf, (uint32_t)fd.tiles_y);
-      if (fd.flags & FRAME_TIGHT) {
-        // tiles holding a pixel centre within the extent box (half a pixel of
-        // slack covers the rounding of these quotients)
-        const float2 e = unpack_extent(__builtin_bit_cast(float, ext));
-        if (e.x < 60000.0f && e.y < 60000.0f) {
-          const uint32_t bx0 = tile_coord(((g.px - e.x) - 0.5f) / tsf, (uint32_t)fd.tiles_x);
-          const uint32_t by0 = tile_coord(((g.py - e.y) - 0.5f) / tsf, (uint32_t)fd.tiles_y);
-          const uint32_t bx1 = min(tile_coord(((g.px + e.x) + 0.5f) / tsf, (uint32_t)fd.tiles_x) + 1u,
-                                   (uint32_t)fd.tiles_x);
-          const uint32_t by1 = min(tile_coord(((g.py + e.y) + 0.5f) / tsf, (uint32_t)fd.tiles_y) + 1u,
-                                   (uint32_t)fd.tiles_y);
-          x0 = max(x0, bx0); y0 = max(y0, by0);
-          x1 = min(x1, bx1); y1 = min(y1, by1);
-        }
-      }
-      cnt = (x1 > x0 && y1 > y0) ? (x1 - x0) * (y1 - y0) : 0u;
-      radius = (int32_t)__builtin_ceilf(r);
-    }
     Splat sp;
     if (keep) {
       sp.a = make_float4(g.px, g.py, g.ca, g.cb);
@@ -271,15 +328,20 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
       sp.b = make_float4(0.0f, o, 0.0f, 0.0f);
     }
     sp.c = make_float4(rgb[0], rgb[1], rgb[2], __builtin_bit_cast(float, ext));
-    SplatRec rec;
     rec.s = sp;
-    rec.k = keep ? cull_consts(g.ca, g.cb, g.cc, skip_below) : make_float4(__builtin_inff(), 0.0f, 0.0f, 0.0f);
+    rec.q1 = keep ? cull_threshold(g.ca, g.cb, g.cc, skip_below) : __builtin_inff();
+    rec.rect0 = x0 | (y0 << 16);
+    rec.rect1 = x1 | (y1 << 16);
+    if (a.hint_sat) hcnt = cnt ? sat_rect(a.hint_sat, fd.tile_off, (uint32_t)fd.tiles_x, x0, x1, y0, y1) : 0u;
+    dkey = keep ? __float_as_uint(g.ndcz) : CULLED_DEPTH_KEY;
+    }
+    rec.item_off = 0u;  // k_scan over the counts fills it in
     a.splats[i] = rec;
     a.radii[i] = radius;
-    a.rect[i] = make_uint2(x0 | (y0 << 16), x1 | (y1 << 16));
+    a.rect[i] = make_uint2(rec.rect0, rec.rect1);
     a.counts[i] = cnt;
-    if (a.hint_sat) a.hcount[i] = cnt ? sat_rect(a.hint_sat, fd.tile_off, (uint32_t)fd.tiles_x, x0, x1, y0, y1) : 0u;
-    a.dkey[i] = keep ? __float_as_uint(g.ndcz) : CULLED_DEPTH_KEY;
+    if (a.hint_sat) a.hcount[i] = hcnt;
+    a.dkey[i] = dkey;
     a.ids[i] = i;
     a.item_frame[i] = ir.frame;
   }
@@ -300,39 +362,6 @@ constexpr int SCAN_THREADS = 1024;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 
-// Look-back by a whole wave: lane l reads the word of tile p - l (a tile
-// before the first counts as inclusive 0) and the wave adds the words up to
-// the nearest inclusive one once all of them are published.  Sums saturate
-// at LB_VAL (above every capacity), so a total too large for the 30-bit
-// field still reads as an overflow.
-W2W_DEV uint32_t scan_lookback_wave(uint32_t* lb, uint32_t tile, uint32_t tot, int lane) {
-  if (tile == 0) {
-    if (lane == 0) st_release_dev(lb, LB_INC | tot);
-    return 0u;
-  }
-  if (lane == 0) st_release_dev(lb + tile, LB_AGG | tot);
-  uint32_t excl = 0;
-  int p = (int)tile - 1;
-  for (;;) {
-    const int q = p - lane;
-    const uint32_t w = q >= 0 ? ld_acquire_dev(lb + q) : LB_INC;
-    const uint32_t flag = w & ~LB_VAL;
-    const uint64_t inc = __ballot(flag == LB_INC);
-    const uint64_t unready = __ballot(flag == 0u);
-    const int first = inc ? __builtin_ctzll(inc) : 64;
-    const uint64_t need = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
-    if (unready & need) continue;  // a word up to the nearest inclusive one is not published yet
-    uint64_t v = lane <= first ? (uint64_t)(w & LB_VAL) : 0ull;  // 64 words of up to 30 bits
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    excl = (uint32_t)min<uint64_t>((uint64_t)excl + v, LB_VAL);
-    if (first < 64) break;
-    p -= 64;
-  }
-  if (lane == 0) st_release_dev(lb + tile, LB_INC | min(excl + tot, LB_VAL));
-  return excl;
-}
-
 template <bool GATHER>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan(
     const uint32_t* __restrict__ in, const uint32_t* __restrict__ perm,
@@ -340,7 +369,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(
     uint32_t* __restrict__ total_out, uint32_t* __restrict__ overflow_out,
     const uint32_t* __restrict__ d_n, uint32_t n_cap, uint32_t capacity,
     const uint32_t* __restrict__ d_begin = nullptr, uint32_t* __restrict__ scatter = nullptr,
-    const uint32_t* __restrict__ gate = nullptr) {
+    const uint32_t* __restrict__ gate = nullptr, SplatRec* __restrict__ rec_out = nullptr) {
   __shared__ uint32_t s_v[SCAN_TILE + SCAN_TILE / 32];  // one pad word per 32: conflict-free transposes
   __shared__ uint32_t s_tmp[SCAN_THREADS / 64];
   __shared__ uint32_t s_tile, s_excl, s_tot;
@@ -415,7 +444,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(
       const uint32_t idx = tb + j * SCAN_THREADS + tid;
       if (idx < n) {
         const uint32_t v = e0 + s_v[pad(j * SCAN_THREADS + tid)];
-        out[idx] = v;
+        if (out) out[idx] = v;
+        if (rec_out) rec_out[idx].item_off = v;
         if constexpr (GATHER) {
           if (scatter) scatter[pr[j]] = v;
         }
@@ -879,9 +909,10 @@ struct BlendArgs {
   const uint32_t* hint_bm;  // tiles whose near list is their whole list
   uint32_t* open_units;     // list of the unfinished 8x8 blocks (units) of other tiles, any order
   uint32_t* n_open_units;
-  // optional, zeroed: per tile, the backward's walk length -- the sum over
-  // its 8x8 blocks of the block's largest last contributor -- for frames
-  // that take part in the backward (the backward visits tiles heaviest first)
+  // optional, zeroed: per tile, the number of contributing (pixel, splat)
+  // pairs, for frames that take part in the backward (which visits tiles
+  // heaviest first).  The same in every emission and binning mode: each
+  // pixel's sequence of contributors is.
   uint32_t* tile_cost;
 };
 
@@ -993,6 +1024,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     bool done = !inside;
     float T_ = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, A = 0.0f, Dd = 0.0f;
     uint32_t last = 0;
+    uint32_t pairs = 0;  // contributing (pixel, splat) pairs of this pass (wave-uniform)
     const size_t HW = (size_t)fd.W * fd.H;
     const size_t lp = (size_t)py * fd.W + px;
     const size_t pid = fd.pix_off + lp;
@@ -1013,29 +1045,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     // (lanes past the end gather splat 0, which exists whenever n > 0)
     uint32_t id_n = 0;
     Splat cur = {};
-    float4 kcur = {};
+    float qcur = 0.0f;
     if (n > 0 && !wave_all(done)) {
       id_n = lane < n ? ids[range.x + lane] : 0u;
-      const SplatRec r = a.splats[id_n < P ? id_n : 0u];
+      const SplatRec& r = a.splats[id_n < P ? id_n : 0u];
       cur = r.s;
-      kcur = r.k;
+      qcur = r.q1;
       id_n = 64 + lane < n ? ids[range.x + 64 + lane] : 0u;
     }
     for (int base = 0; base < n; base += 64) {
       if (wave_all(done)) break;
       const Splat s = cur;
-      const float4 kq = kcur;
+      const float kq = qcur;
       if (base + 64 < n) {
-        const SplatRec r = a.splats[id_n < P ? id_n : 0u];
+        const SplatRec& r = a.splats[id_n < P ? id_n : 0u];
         cur = r.s;
-        kcur = r.k;
+        qcur = r.q1;
         const int k2 = base + 128 + lane;
         id_n = k2 < n ? ids[range.x + k2] : 0u;
       }
       n_loaded += (uint32_t)min(64, n - base);
       const bool valid = base + lane < n;
       const bool touch = valid && extent_touches(s.a.x, s.a.y, unpack_extent(s.c.w), x0, x1, y0, y1) &&
-                         ellipse_touches_k(s.a.x, s.a.y, s.a.z, s.a.w, s.b.x, kq, x0, x1, y0, y1);
+                         ellipse_touches_q(s.a.x, s.a.y, s.a.z, s.a.w, s.b.x, kq, x0, x1, y0, y1);
       uint64_t m = __ballot(touch);
       if (m == 0ull) continue;
       // the wave's chunk goes to its LDS slot; survivors are then read back
@@ -1070,6 +1102,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
         T_ = apply ? nT : T_;
         last = apply ? lbase + (uint32_t)(base + j + 1) : last;
         n_con += apply ? 1u : 0u;
+        pairs += (uint32_t)__builtin_popcountll(__ballot(apply));  // scalar
       };
       // two survivors in flight: the next one's read overlaps this one's work
       int ja = __builtin_ctzll(m);
@@ -1095,13 +1128,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
     }
     const bool open = MODE == BLEND_NEAR && !wave_all(done);
     const bool unfinished = open && !((hint_word >> (tile & 31u)) & 1u);
-    if (a.tile_cost && !unfinished && (fd.flags & FRAME_BACKWARD)) {
-      // final outputs: this block's share of the tile's backward work
-      uint32_t wm = last;
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) wm = max(wm, (uint32_t)__shfl_xor((int)wm, off, 64));
-      if (lane == 0 && wm) atomicAdd(a.tile_cost + tile, min(wm, 1u << 27));
-    }
+    if (a.tile_cost && pairs && lane == 0 && (fd.flags & FRAME_BACKWARD))
+      atomicAdd(a.tile_cost + tile, pairs);  // this pass's contributing pairs of the block
     if (open && lane == 0) atomicOr(a.open_bm + (tile >> 5), 1u << (tile & 31u));
     if (unfinished) {
       if (inside) {
