@@ -7,56 +7,13 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import (BG, cam_args, gpu_camera, oracle_forward, scene_np, to_dev, unpack_rect)
+from helpers import (GRAD_TOL, IMG_TOL, cam_args, compare_forward as _compare_forward, gpu_camera,
+                     oracle_forward, rel_err as _rel, run_gpu as _run_gpu, scene_np, to_dev)
 from oracle import oracle, torch_ref
 from word2world_amd import raster
 from word2world_amd.camera import orbit_views
 
 pytestmark = pytest.mark.gpu
-
-IMG_TOL = 1e-4
-GRAD_TOL = 1e-3
-
-
-def _run_gpu(arrs, deg, cam, dev, ts):
-    m, s, r, o, sh = to_dev(arrs, dev)
-    gcam = gpu_camera(cam, dev)
-    K, color, depth, alpha, radii, ws = raster.forward_checked(m, s, r, o, sh, deg, gcam, ts)
-    torch.cuda.synchronize()
-    keys = raster.sorted_keys(m, s, r, o, sh, deg, gcam, ts, ws, K).cpu().numpy().view(np.uint64)
-    return dict(K=K, color=color.cpu().numpy(), depth=depth.cpu().numpy(),
-                alpha=alpha.cpu().numpy(), radii=radii.cpu().numpy(),
-                counts=ws.counts().cpu().numpy().view(np.uint32),
-                rect=unpack_rect(ws.rect().cpu().numpy()), keys=keys,
-                ids=ws.sorted_ids(K).cpu().numpy().view(np.uint32),
-                ranges=ws.ranges().cpu().numpy().view(np.uint32),
-                n_contrib=ws.n_contrib().cpu().numpy().view(np.uint32),
-                final_T=ws.final_T().cpu().numpy(), splats=ws.splats().cpu().numpy(),
-                ws=ws, gcam=gcam, dev_arrs=(m, s, r, o, sh))
-
-
-def _compare_forward(f, g, ts):
-    print(f"K oracle={f['K']} gpu={g['K']}", flush=True)
-    np.testing.assert_array_equal(g["radii"], f["radii"])
-    np.testing.assert_array_equal(g["counts"], f["counts"])
-    vis = f["counts"] > 0
-    np.testing.assert_array_equal(g["rect"][vis], f["rect"][vis])
-    assert g["K"] == f["K"]
-    np.testing.assert_array_equal(g["keys"], f["keys"])
-    np.testing.assert_array_equal(g["ids"], f["vals"])
-    np.testing.assert_array_equal(g["ranges"], f["ranges"])
-    np.testing.assert_array_equal(g["n_contrib"], f["n_contrib"])
-    np.testing.assert_array_equal(g["final_T"], f["final_T"])
-    # per-Gaussian preprocess outputs of visible Gaussians: bit-exact too
-    sp = g["splats"]
-    np.testing.assert_array_equal(sp[vis, 0:2], f["m2d"][vis])
-    np.testing.assert_array_equal(sp[vis, 2:4], f["conic"][vis, 0:2])
-    np.testing.assert_array_equal(sp[vis, 4], f["conic"][vis, 2])
-    np.testing.assert_array_equal(sp[vis, 6], f["depth"][vis])
-    assert np.abs(g["color"] - f["color"]).max() <= IMG_TOL
-    assert np.abs(g["alpha"] - f["alpha"]).max() <= IMG_TOL
-    assert np.abs(g["depth"] - f["depth_img"]).max() <= IMG_TOL
-
 
 @pytest.mark.parametrize("n,deg,res,ts,scale_mul,seed", [
     (4096, 0, 256, 16, 1.0, 0),     # the CPU-path workload, one view per case below
@@ -82,10 +39,6 @@ def test_forward_large_scene_properties(gpu):
     f = oracle_forward(arrs, 3, cam, 16)
     g = _run_gpu(arrs, 3, cam, gpu, 16)
     _compare_forward(f, g, 16)
-
-
-def _rel(a, b):
-    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
 
 
 @pytest.mark.parametrize("n,deg,res,ts,scale_mul,seed", [

@@ -126,7 +126,8 @@ class BatchedTrainer:
     def __init__(self, scenes: list, cameras: list, targets: list | None, device, tile: int = 16,
                  capacity: int | None = None, tight: bool = True, densify_every: int = 0,
                  densify_grad: float = 2e-4, densify_min_opacity: float = 0.005,
-                 generator: torch.Generator | None = None, depth_split: float = 0.0):
+                 generator: torch.Generator | None = None, depth_split: float = 0.0,
+                 densify_options: dict | None = None):
         """cameras[k]: word2world_amd.camera.Camera views of scene k;
         targets[k][v]: [3,H,W] target image (or None for render-only);
         tight: emit instances only where a Gaussian's cull extent reaches
@@ -137,7 +138,11 @@ class BatchedTrainer:
         on the device each step (0 = never); `generator` draws the split
         samples; depth_split in (0, 1): depth-sliced binning (the nearest
         fraction of the items binned and blended first, the rest only on
-        tiles with unsaturated pixels; same images and gradients)."""
+        tiles with unsaturated pixels; same images and gradients);
+        densify_options: the cycle's other thresholds (clone_scale_max,
+        split_scale_min, prune_scale_max, split_factor, split_shrink of
+        word2world_amd.densify.config; the reference trainer's values when
+        absent)."""
         self.store = BatchedStore(scenes, device)
         self.depth_split = float(depth_split)
         self.device = torch.device(device)
@@ -147,6 +152,7 @@ class BatchedTrainer:
                       for s, cams in zip(scenes, cameras)]
         self.densify_every = int(densify_every)
         self.densify_grad, self.densify_min_opacity = densify_grad, densify_min_opacity
+        self.densify_options = dict(densify_options or {})
         self.generator = generator
         self.densify_log: list = []
         self._size_workspace(capacity)
@@ -311,17 +317,19 @@ class BatchedTrainer:
             self._recapture = True
         self.grow_log.append((self.it, old, int(self.caps.capacity)))
 
-    def densify(self):
+    def densify(self, normals: torch.Tensor | None = None):
         """One densification cycle of every scene on the device; the store,
         the workspace (re-sized for the new Gaussian counts) and, when one was
-        captured, the graph (re-captured after the next eager step) follow."""
+        captured, the graph (re-captured after the next eager step) follow.
+        normals: the split children's standard normal samples, scene by scene
+        (drawn with the trainer's generator when None)."""
         from . import densify as dens
         st = self.store
         self._prev_overflow = self.overflow()
-        cfg = dens.config(self.densify_grad, self.densify_min_opacity)
+        cfg = dens.config(self.densify_grad, self.densify_min_opacity, **self.densify_options)
         p, m, v, ns, counts = dens.densify_segments([3, 3 * st.M, 1, 3, 4], (0, 2, 3, 4), st.flat,
                                                     st.exp_avg, st.exp_avg_sq, st.ns, self.accum,
-                                                    self.count, cfg, self.generator)
+                                                    self.count, cfg, self.generator, normals)
         st.rebuild(p, m, v, ns)
         for sp, n in zip(self.specs, ns):
             sp.n = n

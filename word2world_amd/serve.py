@@ -157,7 +157,6 @@ class ScenePool:
         o = 0
         cache = {}
         for r, c in need:
-            key = (r.scene, id(c))
             if r.scene not in cache:
                 m, s, q, op, sh = (t.to(self.device).contiguous() for t in activate(self.targets[r.scene].params()))
                 cache = {r.scene: (m, s, q, op, sh)}
@@ -169,7 +168,6 @@ class ScenePool:
             self.target_buf[o:o + n] = color.clamp(0, 1).reshape(-1)
             r.target_offs.append(o)
             o += n
-            del key
 
 
 def make_requests(n: int, rate: float, pool_sizes: list, seed: int = 0, opt_frac: float = 0.3,
