@@ -16,8 +16,13 @@ Other workloads: --workload batched64 (64 scenes, batched vs plain) and
 
 Prints ONE JSON line.  `value` = views/s over the timed steps; `roofline` is
 the dominant kernel, timed live with HIP events on its launch stream;
-`cpu_baseline` is the CPU oracle (single-threaded C) on one view of the same
-scene.
+`cpu_baseline` is the CPU oracle (C, one thread per view) on the CPU-path
+configuration (4,096 Gaussians, SH 0, 4 views at 256^2), median of five
+steps, with the GPU's rate on that configuration beside it and a 1M-scene
+sample scaled to the full scene (labelled extrapolated).  The default line
+also carries the 64-scene batch (`batched64`: batched vs plain, step
+latency) and the served stream (`served`: continuous batching vs a FIFO
+plain server, p50/p99 latency).
 """
 from __future__ import annotations
 
@@ -68,6 +73,10 @@ def parse():
                     help="depth-sliced binning: fraction of the items (nearest first) binned and "
                          "blended first, the rest only on tiles with unsaturated pixels (same "
                          "outputs); 0 = one binning pass, negative = pick by timing a few steps")
+    ap.add_argument("--no-extra-workloads", action="store_true",
+                    help="scene1m: skip the 64-scene and served legs that the default line carries")
+    ap.add_argument("--cpu-steps", type=int, default=5,
+                    help="steps of the CPU-path configuration the CPU baseline takes the median of")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="after the benchmark, run a plain device copy of this many bytes three "
                          "times (a reference kernel for hardware byte counters)")
@@ -397,8 +406,26 @@ def bench_scene(args):
         del src, dst
 
     cpu = None
+    cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args, cams_host, P)
+        cpu = cpu_baseline_small(args, dev)
+        cpu["extrapolated_1m"] = cpu_baseline(args, cams_host, P)
+
+    # the 64-scene batch and the served stream, as the metric names them
+    extra = {}
+    if not args.no_extra_workloads:
+        import copy
+        torch.cuda.empty_cache()
+        for name, fn, kw in (("batched64", bench_batched, dict(depth_split=0.0)),
+                             ("served", bench_served, {})):
+            a = copy.copy(args)
+            a.tile, a.steps, a.warmup = 16, 10, 3
+            for k, v in kw.items():
+                setattr(a, k, v)
+            r = fn(a, emit=False)
+            extra[name] = {k: r[k] for k in r if k not in ("metric", "n_gpus", "higher_is_better", "scaling",
+                                                          "vs_baseline", "dtype", "data")}
+            torch.cuda.empty_cache()
 
     out = {
         "metric": METRIC, "value": views_per_s, "unit": "views/s", "n_gpus": 1,
@@ -427,6 +454,7 @@ def bench_scene(args):
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 
@@ -442,12 +470,13 @@ def _targets(jobs, dev, tile):
             for c in j.cameras:
                 cam = raster.camera_from_tensors(c.viewmatrix, c.projmatrix, c.campos, c.width, c.height,
                                                  c.focal_x, c.focal_y, None, device=dev)
-                tl.append(raster.forward_checked(m, s, r, o, sh, 3, cam, tile)[1].clamp(0, 1))
+                tl.append(raster.forward_checked(m, s, r, o, sh, j.target_scene.sh_degree, cam,
+                                                 tile)[1].clamp(0, 1))
             out.append(tl)
     return out
 
 
-def bench_batched(args):
+def bench_batched(args, emit: bool = True):
     """64 scenes (10k-500k Gaussians, log-uniform), 4 views each at a
     resolution drawn from {256,512,1024}^2, fwd+bwd+Adam per step: the batched
     trainer (one launch sequence, HIP-graph replay) against the plain path
@@ -506,13 +535,17 @@ def bench_batched(args):
                 raise RuntimeError("plain trainer workspace overflowed twice")
     torch.cuda.synchronize()
     psteps = max(1, min(args.steps, 3))
+    plain_ms = []
     t0 = time.perf_counter()
     for _ in range(psteps):
+        t_s = time.perf_counter()
         for p in plains:
             p.step()
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        plain_ms.append((time.perf_counter() - t_s) * 1e3)
     tp = (time.perf_counter() - t0) / psteps
     log(f"plain: {tp * 1e3:.1f} ms/step, {V / tp:.1f} views/s")
+    plain_ms = np.array(plain_ms)
     out = {"metric": "fwd+bwd views/sec, 64-scene mixed batch (batched vs plain)", "value": V / tb,
            "unit": "views/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": tb * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -521,8 +554,14 @@ def bench_batched(args):
                                   "4 views each at {256,512,1024}^2, fwd+bwd+Adam",
                       "graph": not args.no_graph, "tile": args.tile, "depth_split": split},
            "plain_views_per_s": V / tp, "speedup_vs_plain": tp / tb,
-           "latency_ms": {"p50": float(np.percentile(lat_b, 50)), "p99": float(np.percentile(lat_b, 99))}}
-    print(json.dumps(out), flush=True)
+           "latency_ms": {"p50": float(np.percentile(lat_b, 50)), "p99": float(np.percentile(lat_b, 99))},
+           "plain_latency_ms": {"p50": float(np.percentile(plain_ms, 50)),
+                                "p99": float(np.percentile(plain_ms, 99))}}
+    del plains
+    torch.cuda.empty_cache()
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_voxel(args):
@@ -571,7 +610,7 @@ def bench_voxel(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_served(args):
+def bench_served(args, emit: bool = True):
     """Poisson request stream: 70 % renders (1-8 views), 30 % optimisation jobs
     (fwd+bwd+Adam steps on a resident scene), scenes log-uniform [5k, 1M];
     continuous batching against the plain FIFO server."""
@@ -610,13 +649,88 @@ def bench_served(args):
            "config": {"workload": f"{len(reqs)} requests, Poisson {args.rate}/s, 30% {args.opt_steps}-step "
                                   "optimisation jobs (4 views), 70% renders of 1-8 views, scenes "
                                   "log-uniform [5k,1M] SH3, views at {256,512,1024}^2"},
-           "served": st_c, "plain": st_p}
-    print(json.dumps(out), flush=True)
+           "served": st_c, "plain": st_p,
+           "speedup_vs_plain": st_c["views_per_s"] / st_p["views_per_s"] if st_p["views_per_s"] else None}
+    del pool
+    torch.cuda.empty_cache()
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
+
+
+def cpu_baseline_small(args, dev):
+    """The CPU-path configuration (4,096 Gaussians, SH 0, 4 views at 256^2,
+    one fwd+bwd step): the CPU oracle in C, one host thread per view, median
+    of --cpu-steps steps after one warm-up step.  The GPU's rate on the same
+    configuration (batched trainer, fwd+bwd+Adam, graph replay) is timed
+    beside it."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import torch
+    from oracle import oracle
+    from word2world_amd import workloads
+    from word2world_amd.batched import BatchedTrainer
+    from word2world_amd.scene import activate
+
+    job = workloads.cpu_small()
+    m, s, r, o, sh = [t.numpy().copy() for t in activate(job.scene.params())]
+    o = o.reshape(-1)
+    bg = np.ones(3, np.float32)
+    V, R = len(job.cameras), job.cameras[0].width
+    rng = np.random.default_rng(0)
+    gC = [rng.standard_normal((3, R, R)).astype(np.float32) * 1e-3 for _ in range(V)]
+    z = np.zeros((R, R), np.float32)
+
+    def one_view(k):
+        cam = job.cameras[k]
+        f = oracle.forward(m, s, r, o, sh, 0, cam.view_flat(), cam.proj_flat(), cam.campos_np(), bg, R, R, 16,
+                           cam.focal_x, cam.focal_y)
+        oracle.backward(f, m, s, r, sh, cam.view_flat(), cam.proj_flat(), cam.campos_np(), bg, cam.focal_x,
+                        cam.focal_y, gC[k], z, z)
+        return f["K"]
+
+    one_view(0)  # loads the library
+    steps = []
+    with ThreadPoolExecutor(V) as ex:
+        list(ex.map(one_view, range(V)))
+        for _ in range(max(args.cpu_steps, 1)):
+            t0 = time.perf_counter()
+            Ks = list(ex.map(one_view, range(V)))
+            steps.append(time.perf_counter() - t0)
+    med = float(np.median(steps))
+    log(f"cpu oracle, 4096/SH0/4x256^2: step times {[round(t, 3) for t in steps]} s, median {med:.3f} s")
+    targets = _targets([job], dev, 16)
+    # a fixed capacity well above the ~18k instances per view this scene
+    # needs, so that no timed step can overflow as it trains
+    bt = BatchedTrainer([job.scene], [job.cameras], targets, dev, 16, capacity=1 << 20)
+    for _ in range(3):
+        bt.step()
+    bt.capture()
+    bt.step()
+    torch.cuda.synchronize()
+    n = 100
+    t0 = time.perf_counter()
+    for _ in range(n):
+        bt.step()
+    torch.cuda.synchronize()
+    g = (time.perf_counter() - t0) / n
+    if bt.overflow() or bt.skipped_overflow():
+        raise RuntimeError("CPU-path configuration overflowed on the GPU")
+    del bt
+    return {"value": V / med, "unit": "views/s", "cores": V, "kind": "port",
+            "sample": f"the CPU-path configuration in full: 4,096 Gaussians, SH0, {V} views at {R}x{R}, one "
+                      f"fwd+bwd step (K per view {Ks}); CPU oracle in C, one thread per view; median of "
+                      f"{len(steps)} steps after a warm-up step",
+            "step_s": steps, "median_step_s": med, "cpu": _cpu_model(),
+            "gpu_same_config": {"value": V / g, "unit": "views/s", "ms_per_step": g * 1e3,
+                                "path": "batched trainer fwd+bwd+Adam, graph replay, tile 16"}}
 
 
 def cpu_baseline(args, cams_host, P):
-    """The CPU oracle (single-threaded C restatement) on one view of the same
-    scene: forward + backward, timed on this host."""
+    """The CPU oracle (single-threaded C restatement) on one view of the
+    benchmark scene, restricted to a sample of its Gaussians: forward +
+    backward, timed on this host and scaled to the full scene (an
+    extrapolation: the work is not exactly linear in the Gaussian count)."""
     from oracle import oracle
     from word2world_amd.scene import activate, make_scene
 
@@ -639,7 +753,7 @@ def cpu_baseline(args, cams_host, P):
     t2 = time.perf_counter()
     log(f"cpu oracle: forward {t1 - t0:.2f} s, backward {t2 - t1:.2f} s (K={f['K']})")
     frac = n / P
-    return {"value": frac / (t2 - t0), "unit": "views/s", "cores": 1, "kind": "port",
+    return {"value": frac / (t2 - t0), "unit": "views/s", "cores": 1, "kind": "port", "extrapolated": True,
             "sample": f"fwd+bwd of view 0 of the benchmark scene restricted to its first {n} of {P} "
                       f"(i.i.d.) Gaussians, {R}x{R}, SH3, CPU oracle in C on 1 thread: forward "
                       f"{t1 - t0:.2f} s, backward {t2 - t1:.2f} s, K={f['K']}; views/s scaled by "
