@@ -57,7 +57,10 @@ def parse():
     ap.add_argument("--res", type=int, default=1024)
     ap.add_argument("--views", type=int, default=4)
     ap.add_argument("--tile", type=int, default=0,
-                    help="tile edge 8, 16 or 32; 0 = pick by the autotuner (single-view sweep)")
+                    help="tile edge 8, 16 or 32; 0 = pick tile and chunk by the autotuner")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="instances per forward-blend chunk, 64, 128 or 256 (with --tile given; "
+                         "0 = 64)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-gaussians", type=int, default=250_000)
     ap.add_argument("--workload", choices=["scene1m", "batched64", "served", "voxel"], default="scene1m")
@@ -149,8 +152,8 @@ def stage_table(per, counts, P, M, V, K, T, HW, stats_view, tpasses):
 
 def main():
     args = parse()
-    if args.tile not in (0, 8, 16, 32):
-        print("--tile must be 0 (autotune), 8, 16 or 32", file=sys.stderr)
+    if args.tile not in (0, 8, 16, 32) or args.chunk not in (0, 64, 128, 256):
+        print("--tile must be 0 (autotune), 8, 16 or 32; --chunk 0, 64, 128 or 256", file=sys.stderr)
         sys.exit(2)
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the rasterizer's "
@@ -159,7 +162,6 @@ def main():
     if args.workload == "voxel":
         return bench_voxel(args)
     if args.workload in ("batched64", "served"):
-        args.tile = args.tile or 16  # the reference's tile edge
         args.steps = 10 if args.steps is None else args.steps
         return bench_batched(args) if args.workload == "batched64" else bench_served(args)
     return bench_scene(args)
@@ -235,14 +237,15 @@ def bench_scene(args):
         del tm, ts_, tr, to, tsh
     torch.cuda.empty_cache()
 
-    tune = None
+    tune, chunk = None, args.chunk
     if args.tile == 0:
         from word2world_amd import autotune
         m, s_, r, o, sh = (t.to(dev).contiguous() for t in activate(scene.params()))
-        tune = autotune.sweep(m, s_, r, o, sh, 3, cams[0], reps=2)
-        ts = min(tune, key=tune.get)
-        log("autotune (ms per single-view fwd+bwd): " + ", ".join(f"tile {k}: {v:.2f}" for k, v in tune.items())
-            + f" -> tile {ts}")
+        t = autotune.sweep(m, s_, r, o, sh, 3, cams[0], reps=2)
+        ts, chunk = min(t, key=t.get)
+        tune = {autotune.label(k): v for k, v in t.items()}
+        log("autotune (ms per single-view fwd+bwd): " + ", ".join(f"{k}: {v:.2f}" for k, v in tune.items())
+            + f" -> tile {ts}, chunk {chunk}")
         del m, s_, r, o, sh
         torch.cuda.empty_cache()
     split_ms = None
@@ -251,7 +254,7 @@ def bench_scene(args):
         # the same for every choice)
         split_ms = {}
         for frac in SPLIT_CANDIDATES:
-            t = BatchedTrainer([scene], [cams_host], [targets], dev, ts, depth_split=frac)
+            t = BatchedTrainer([scene], [cams_host], [targets], dev, ts, depth_split=frac, chunk=chunk)
             for _ in range(30):  # past the first steps, where the scene changes fastest
                 t.step()
             torch.cuda.synchronize()
@@ -265,7 +268,7 @@ def bench_scene(args):
         args.depth_split = min(split_ms, key=split_ms.get)
         log("depth split (ms per step): " + ", ".join(f"{k}: {v:.3f}" for k, v in split_ms.items())
             + f" -> {args.depth_split}")
-    bt = BatchedTrainer([scene], [cams_host], [targets], dev, ts, densify_every=args.densify_every,
+    bt = BatchedTrainer([scene], [cams_host], [targets], dev, ts, densify_every=args.densify_every, chunk=chunk,
                         densify_grad=2e-4, densify_min_opacity=0.005,
                         generator=torch.Generator(device=dev).manual_seed(0), depth_split=args.depth_split)
     if args.steps is None:
@@ -419,7 +422,7 @@ def bench_scene(args):
         for name, fn, kw in (("batched64", bench_batched, dict(depth_split=0.0)),
                              ("served", bench_served, {})):
             a = copy.copy(args)
-            a.tile, a.steps, a.warmup = 16, 10, 3
+            a.tile, a.chunk, a.steps, a.warmup = 0, 0, 10, 3
             for k, v in kw.items():
                 setattr(a, k, v)
             r = fn(a, emit=False)
@@ -435,7 +438,7 @@ def bench_scene(args):
         "config": {"workload": f"{P} Gaussians SH3 (seeded synthetic), {V} orbital views at {R}x{R} "
                                "(elevation U[-30,30]), fwd+bwd+Adam per step, L2 to a second "
                                "seed's renders; all views of a step in one launch per stage",
-                   "gaussians": P, "views_per_step": V, "resolution": R, "tile": ts,
+                   "gaussians": P, "views_per_step": V, "resolution": R, "tile": ts, "chunk": chunk or 64,
                    "densify_every": args.densify_every},
         "densify": dens,
         "autotune_ms": tune,
@@ -492,10 +495,21 @@ def bench_batched(args, emit: bool = True):
     G = sum(j.scene.n for j in jobs)
     V = sum(len(j.cameras) for j in jobs)
     log(f"batched: {len(jobs)} scenes, {G} Gaussians, {V} views")
-    targets = _targets(jobs, dev, args.tile)
+    targets = _targets(jobs, dev, 16)
     split = max(args.depth_split, 0.0)  # this workload is not swept: 0 unless given
-    bt = BatchedTrainer([j.scene for j in jobs], [j.cameras for j in jobs], targets, dev, args.tile,
-                        depth_split=split)
+    scenes, cameras = [j.scene for j in jobs], [j.cameras for j in jobs]
+    tile, chunk, tune = args.tile, args.chunk, None
+    if tile == 0:
+        # one (tile, chunk) serves every scene of the batch: time the trainer itself
+        from word2world_amd import autotune
+        t = autotune.sweep_trainer(lambda ts, ch: BatchedTrainer(scenes, cameras, targets, dev, ts,
+                                                                 depth_split=split, chunk=ch),
+                                   warmup=2, steps=3)
+        tile, chunk = min(t, key=t.get)
+        tune = {autotune.label(k): v for k, v in t.items()}
+        log("batched autotune (ms per step): " + ", ".join(f"{k}: {v:.1f}" for k, v in tune.items())
+            + f" -> tile {tile}, chunk {chunk}")
+    bt = BatchedTrainer(scenes, cameras, targets, dev, tile, depth_split=split, chunk=chunk)
     for i in range(max(args.warmup, 1)):
         bt.step()
         torch.cuda.synchronize()
@@ -523,7 +537,7 @@ def bench_batched(args, emit: bool = True):
     # plain: one scene at a time, one view at a time
     cams = [[raster.camera_from_tensors(c.viewmatrix, c.projmatrix, c.campos, c.width, c.height, c.focal_x,
                                         c.focal_y, None, device=dev) for c in j.cameras] for j in jobs]
-    plains = [PlainTrainer(j.scene, cs, t, TrainConfig(tile=args.tile), dev)
+    plains = [PlainTrainer(j.scene, cs, t, TrainConfig(tile=tile), dev)
               for j, cs, t in zip(jobs, cams, targets)]
     for p in plains:
         # a step whose workspace overflowed sized the next one; its update is
@@ -552,7 +566,8 @@ def bench_batched(args, emit: bool = True):
            "dtype": "fp32", "data": "synthetic",
            "config": {"workload": f"{len(jobs)} scenes, Gaussians log-uniform [10k,500k] (total {G}), SH3, "
                                   "4 views each at {256,512,1024}^2, fwd+bwd+Adam",
-                      "graph": not args.no_graph, "tile": args.tile, "depth_split": split},
+                      "graph": not args.no_graph, "tile": tile, "chunk": chunk or 64, "depth_split": split},
+           "autotune_ms": tune,
            "plain_views_per_s": V / tp, "speedup_vs_plain": tp / tb,
            "latency_ms": {"p50": float(np.percentile(lat_b, 50)), "p99": float(np.percentile(lat_b, 99))},
            "plain_latency_ms": {"p50": float(np.percentile(plain_ms, 50)),
@@ -627,11 +642,31 @@ def bench_served(args, emit: bool = True):
                            [make_scene(n, 3, 600 + k) for k, n in enumerate(sizes)], dev)
     reqs = serve.make_requests(args.requests, args.rate, sizes, seed=1, opt_steps=args.opt_steps)
     pool.build_targets(reqs)
+    tile, chunk, tune = args.tile, args.chunk, None
+    if tile == 0:
+        # requests of every size and resolution share the server's choice:
+        # tune on the heaviest request shape (the largest pool scene at
+        # 1024^2), where the time goes and whose instance count must fit
+        from word2world_amd import autotune, raster
+        from word2world_amd.camera import orbit_camera
+        from word2world_amd.scene import activate
+        k = int(np.argmax(sizes))
+        m, s_, r, o, sh = (t.to(dev).contiguous() for t in activate(pool.scenes[k].params()))
+        oc = orbit_camera(0.0, 0.0, 2.5, 1024, 1024)
+        cam = raster.camera_from_tensors(oc.viewmatrix, oc.projmatrix, oc.campos, 1024, 1024, oc.focal_x,
+                                         oc.focal_y, None, device=dev)
+        t = autotune.sweep(m, s_, r, o, sh, 3, cam, reps=2)
+        tile, chunk = min(t, key=t.get)
+        tune = {autotune.label(key): v for key, v in t.items()}
+        log(f"served autotune on {sizes[k]} Gaussians at 1024^2 (ms per fwd+bwd): "
+            + ", ".join(f"{key}: {v:.2f}" for key, v in tune.items()) + f" -> tile {tile}, chunk {chunk}")
+        del m, s_, r, o, sh
     c = serve._lib.W2WCaps()
-    c.scenes, c.frames, c.tile, c.depth_split = 128, 512, args.tile, 0
+    c.scenes, c.frames, c.tile, c.depth_split = 128, 512, tile, 0
+    c.chunk = chunk
     c.items, c.gaussians = 48_000_000, 12_000_000
     c.tiles, c.pixels, c.capacity = 2_000_000, 160_000_000, 400_000_000
-    srv = serve.ContinuousServer(pool, 12_000_000, c, dev, args.tile)
+    srv = serve.ContinuousServer(pool, 12_000_000, c, dev, tile)
     # warm up the kernels on a copy of the first requests
     srv.serve(copy.deepcopy(reqs[:4]))
     done_c = srv.serve(copy.deepcopy(reqs), log=log)
@@ -639,7 +674,7 @@ def bench_served(args, emit: bool = True):
     log(f"continuous: {st_c}")
     del srv
     torch.cuda.empty_cache()
-    done_p = serve.PlainServer(pool, dev, args.tile).serve(copy.deepcopy(reqs), log=log)
+    done_p = serve.PlainServer(pool, dev, tile).serve(copy.deepcopy(reqs), log=log)
     st_p = serve.latency_stats(done_p)
     log(f"plain: {st_p}")
     out = {"metric": "served fwd+bwd views/sec; p50/p99 latency, served vs plain", "value": st_c["views_per_s"],
@@ -648,7 +683,9 @@ def bench_served(args, emit: bool = True):
            "data": "synthetic",
            "config": {"workload": f"{len(reqs)} requests, Poisson {args.rate}/s, 30% {args.opt_steps}-step "
                                   "optimisation jobs (4 views), 70% renders of 1-8 views, scenes "
-                                  "log-uniform [5k,1M] SH3, views at {256,512,1024}^2"},
+                                  "log-uniform [5k,1M] SH3, views at {256,512,1024}^2",
+                      "tile": tile, "chunk": chunk or 64},
+           "autotune_ms": tune,
            "served": st_c, "plain": st_p,
            "speedup_vs_plain": st_c["views_per_s"] / st_p["views_per_s"] if st_p["views_per_s"] else None}
     del pool

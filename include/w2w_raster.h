@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define W2W_ABI_VERSION 6
+#define W2W_ABI_VERSION 7
 
 enum {
   W2W_OK = 0,
@@ -49,6 +49,10 @@ typedef struct W2WFrame {
   int32_t D;            /* active SH degree, 0..3, (D+1)^2 <= M */
   int32_t W, H;         /* image size in pixels */
   int32_t tile;         /* tile edge: 8, 16 or 32 */
+  int32_t chunk;        /* instances per forward-blend chunk: 64, 128 or 256
+                         * (0: 64); the counterpart of the reference's blend
+                         * batch (pixelGaussiansFull.cu:41-76); same outputs */
+  int32_t reserved;     /* 0 */
   float focal_x, focal_y;
   const float* means;       /* [P,3] */
   const float* scales;      /* [P,3] activated (exp) */
@@ -106,6 +110,9 @@ typedef struct W2WLayout {
 typedef struct W2WCaps {
   int32_t scenes, frames;  /* descriptor slots in the batch table */
   int32_t tile;            /* tile edge: 8, 16 or 32 */
+  int32_t chunk;           /* instances per forward-blend chunk: 64, 128 or
+                            * 256 (0: 64); same outputs */
+  int32_t reserved;        /* 0 */
   int32_t depth_split;     /* 0: one binning pass.  1..999: depth-sliced
                             * binning (no reference counterpart; same
                             * outputs): the nearest depth_split/1000 of the

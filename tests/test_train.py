@@ -79,9 +79,11 @@ def test_plain_trainer_reduces_loss(gpu):
 
 @pytest.mark.gpu
 def test_autotuner_picks_a_tile_and_tiles_agree(gpu):
-    """The sweep times every tile edge; the images at different edges agree
-    (the per-pixel contributor sequence only changes where the reference's
-    non-conservative rect rule drops a tile, a handful of pixels)."""
+    """The sweep times every (tile edge, chunk); the images at different
+    edges agree (the per-pixel contributor sequence only changes where the
+    reference's non-conservative rect rule drops a tile, a handful of
+    pixels), and at one edge every chunk gives the same image, depth, alpha,
+    contributor counts and final T bit for bit."""
     from word2world_amd import autotune, raster
     from word2world_amd.camera import orbit_camera
     from word2world_amd.scene import activate, make_scene
@@ -91,13 +93,23 @@ def test_autotuner_picks_a_tile_and_tiles_agree(gpu):
     cam = raster.camera_from_tensors(c.viewmatrix, c.projmatrix, c.campos, 256, 192, c.focal_x, c.focal_y,
                                      None, device=gpu)
     tuner = autotune.Autotuner(reps=1)
-    best = tuner.best_tile(m, s, r, o, sh, 3, cam)
+    best = tuner.best(m, s, r, o, sh, 3, cam)
     t = tuner.timings[tuner.key(20000, 256, 192)]
-    assert best in (8, 16, 32) and set(t) == {8, 16, 32} and all(v > 0 for v in t.values())
+    assert best in autotune.candidates() and set(t) == set(autotune.candidates())
+    assert all(v > 0 for v in t.values()) and tuner.best_tile(m, s, r, o, sh, 3, cam) == best[0]
     imgs = {ts: raster.forward_checked(m, s, r, o, sh, 3, cam, ts)[1] for ts in (8, 16, 32)}
     for ts in (8, 32):
         d = (imgs[ts] - imgs[16]).abs()
         assert float((d > 1e-5).float().mean()) < 0.01 and float(d.max()) < 0.5
+    for ts in (8, 16, 32):
+        ref = None
+        for ch in (64, 128, 256):
+            K, color, depth, alpha, radii, ws = raster.forward_checked(m, s, r, o, sh, 3, cam, ts, chunk=ch)
+            got = (color, depth, alpha, ws.n_contrib().clone(), ws.final_T().clone())
+            if ref is None:
+                ref = got
+            else:
+                assert all(torch.equal(a, b) for a, b in zip(ref, got)), (ts, ch)
 
 
 @pytest.mark.gpu

@@ -21,7 +21,7 @@ _vp = ctypes.c_void_p
 class W2WFrame(ctypes.Structure):
     _fields_ = [("P", ctypes.c_int32), ("M", ctypes.c_int32), ("D", ctypes.c_int32),
                 ("W", ctypes.c_int32), ("H", ctypes.c_int32), ("tile", ctypes.c_int32),
-                ("focal_x", ctypes.c_float), ("focal_y", ctypes.c_float),
+                ("chunk", ctypes.c_int32), ("reserved", ctypes.c_int32), ("focal_x", ctypes.c_float), ("focal_y", ctypes.c_float),
                 ("means", _vp), ("scales", _vp), ("rotations", _vp), ("opacities", _vp),
                 ("sh", _vp), ("viewmatrix", _vp), ("projmatrix", _vp), ("campos", _vp),
                 ("background", _vp)]
@@ -42,7 +42,7 @@ class W2WLayout(ctypes.Structure):
 
 class W2WCaps(ctypes.Structure):
     _fields_ = [("scenes", ctypes.c_int32), ("frames", ctypes.c_int32), ("tile", ctypes.c_int32),
-                ("depth_split", ctypes.c_int32), ("items", ctypes.c_int64), ("gaussians", ctypes.c_int64),
+                ("chunk", ctypes.c_int32), ("reserved", ctypes.c_int32), ("depth_split", ctypes.c_int32), ("items", ctypes.c_int64), ("gaussians", ctypes.c_int64),
                 ("tiles", ctypes.c_int64), ("pixels", ctypes.c_int64), ("capacity", ctypes.c_int64)]
 
 
@@ -76,7 +76,7 @@ KERNEL_IDS = {0: "fill", 1: "preprocess", 2: "depth_sort_hist", 3: "depth_sort_p
               24: "tile_order"}
 
 _lib = None
-ABI_VERSION = 6  # W2W_ABI_VERSION of include/w2w_raster.h
+ABI_VERSION = 7  # W2W_ABI_VERSION of include/w2w_raster.h
 
 
 def load(path: str = LIB_PATH):

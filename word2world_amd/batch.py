@@ -54,9 +54,10 @@ def frame_from_camera(cam, background=None, backward=True, tight=False) -> Frame
 
 
 def caps_for(scenes: list, tile: int, capacity: int, scenes_cap: int | None = None,
-             frames_cap: int | None = None, depth_split: float = 0.0) -> _lib.W2WCaps:
+             frames_cap: int | None = None, depth_split: float = 0.0, chunk: int = 0) -> _lib.W2WCaps:
     """Capacities of a batch.  depth_split in (0, 1): depth-sliced binning
-    with that fraction of the items (nearest first) in the near slice."""
+    with that fraction of the items (nearest first) in the near slice; chunk:
+    instances per forward-blend chunk (64, 128 or 256; 0 = 64)."""
     g = sum(s.n for s in scenes)
     items = sum(s.n * len(s.frames) for s in scenes)
     frames = [f for s in scenes for f in s.frames]
@@ -66,6 +67,7 @@ def caps_for(scenes: list, tile: int, capacity: int, scenes_cap: int | None = No
     c.scenes = scenes_cap or max(len(scenes), 1)
     c.frames = frames_cap or max(len(frames), 1)
     c.tile = tile
+    c.chunk = int(chunk)
     c.depth_split = int(round(1000 * depth_split)) if 0.0 < depth_split < 1.0 else 0
     if 0.0 < depth_split < 1.0:
         c.depth_split = min(max(c.depth_split, 1), 999)

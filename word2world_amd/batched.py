@@ -127,7 +127,7 @@ class BatchedTrainer:
                  capacity: int | None = None, tight: bool = True, densify_every: int = 0,
                  densify_grad: float = 2e-4, densify_min_opacity: float = 0.005,
                  generator: torch.Generator | None = None, depth_split: float = 0.0,
-                 densify_options: dict | None = None):
+                 densify_options: dict | None = None, chunk: int = 0):
         """cameras[k]: word2world_amd.camera.Camera views of scene k;
         targets[k][v]: [3,H,W] target image (or None for render-only);
         tight: emit instances only where a Gaussian's cull extent reaches
@@ -142,8 +142,10 @@ class BatchedTrainer:
         densify_options: the cycle's other thresholds (clone_scale_max,
         split_scale_min, prune_scale_max, split_factor, split_shrink of
         word2world_amd.densify.config; the reference trainer's values when
-        absent)."""
+        absent); chunk: instances per forward-blend chunk (64, 128 or 256;
+        0 = 64), a speed knob with the same outputs."""
         self.store = BatchedStore(scenes, device)
+        self.chunk = int(chunk)
         self.depth_split = float(depth_split)
         self.device = torch.device(device)
         self.tile = tile
@@ -206,7 +208,8 @@ class BatchedTrainer:
         self.count = torch.zeros(G, dtype=torch.float32, device=self.device)
 
     def _alloc(self, capacity: int):
-        self.caps = batch.caps_for(self.specs, self.tile, capacity, depth_split=self.depth_split)
+        self.caps = batch.caps_for(self.specs, self.tile, capacity, depth_split=self.depth_split,
+                                   chunk=self.chunk)
         self.ws = None
         self.grads = None
         self.ws = batch.BatchWorkspace(self.caps, self.device)

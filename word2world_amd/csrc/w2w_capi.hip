@@ -144,6 +144,7 @@ bool make_plan(const W2WCaps& c, Plan& p) {
       c.pixels < 1 || c.capacity < 0)
     return false;
   if (c.tile != 8 && c.tile != 16 && c.tile != 32) return false;
+  if (c.chunk != 0 && c.chunk != 64 && c.chunk != 128 && c.chunk != 256) return false;
   if (c.depth_split < 0) return false;
   if (c.items >= (int64_t)LB_VAL || c.capacity >= (int64_t)LB_VAL || c.tiles >= (int64_t)LB_VAL ||
       c.gaussians >= (int64_t)LB_VAL)
@@ -218,11 +219,11 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   return true;
 }
 
-bool single_caps(int32_t P, int32_t W, int32_t H, int32_t ts, int64_t cap, W2WCaps& c) {
+bool single_caps(int32_t P, int32_t W, int32_t H, int32_t ts, int64_t cap, W2WCaps& c, int32_t chunk = 0) {
   if (P < 0 || W <= 0 || H <= 0 || (ts != 8 && ts != 16 && ts != 32)) return false;
   const int64_t tx = (W + ts - 1) / ts, ty = (H + ts - 1) / ts;
   if (tx > 65535 || ty > 65535) return false;
-  c.scenes = 1; c.frames = 1; c.tile = ts; c.depth_split = 0;
+  c.scenes = 1; c.frames = 1; c.tile = ts; c.chunk = chunk; c.reserved = 0; c.depth_split = 0;
   c.items = P; c.gaussians = P; c.tiles = tx * ty; c.pixels = (int64_t)W * H; c.capacity = cap;
   return true;
 }
@@ -236,6 +237,7 @@ bool ws_ok(const W2WWorkspace* ws, const Plan& p) {
 bool frame_ok(const W2WFrame* f) {
   if (!f) return false;
   if (f->D < 0 || f->D > 3 || f->M < (f->D + 1) * (f->D + 1)) return false;
+  if (f->chunk != 0 && f->chunk != 64 && f->chunk != 128 && f->chunk != 256) return false;
   if (!(f->focal_x > 0.0f) || !(f->focal_y > 0.0f)) return false;
   if (!f->viewmatrix || !f->projmatrix || !f->campos || !f->background) return false;
   if (f->P > 0 && (!f->means || !f->scales || !f->rotations || !f->opacities || !f->sh)) return false;
@@ -288,23 +290,32 @@ uint32_t blend_grid(uint32_t tiles_cap) {
   return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((units + 3) / 4, kBlendGrid));
 }
 
-template <int TS>
+template <int TS, int CW>
 void launch_blend(const BlendArgs& a, int mode, uint32_t tiles_cap, hipStream_t s) {
   ProfSpan ps(mode == BLEND_FAR ? W2W_KID_BLEND_FAR : W2W_KID_BLEND, s);
   const dim3 grid(blend_grid<TS>(tiles_cap));
   if (mode == BLEND_NEAR)
-    hipLaunchKernelGGL((k_blend<TS, BLEND_NEAR>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_blend<TS, BLEND_NEAR, CW>), grid, dim3(256), 0, s, a);
   else if (mode == BLEND_FAR)
-    hipLaunchKernelGGL((k_blend<TS, BLEND_FAR>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_blend<TS, BLEND_FAR, CW>), grid, dim3(256), 0, s, a);
   else
-    hipLaunchKernelGGL((k_blend<TS, BLEND_FULL>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_blend<TS, BLEND_FULL, CW>), grid, dim3(256), 0, s, a);
 }
 
-void launch_blend_tile(const BlendArgs& a, int mode, int tile, uint32_t tiles_cap, hipStream_t s) {
-  switch (tile) {
-    case 8: launch_blend<8>(a, mode, tiles_cap, s); break;
-    case 16: launch_blend<16>(a, mode, tiles_cap, s); break;
-    default: launch_blend<32>(a, mode, tiles_cap, s); break;
+template <int TS>
+void launch_blend_chunk(const BlendArgs& a, int mode, int chunk, uint32_t tiles_cap, hipStream_t s) {
+  switch (chunk) {
+    case 128: launch_blend<TS, 2>(a, mode, tiles_cap, s); break;
+    case 256: launch_blend<TS, 4>(a, mode, tiles_cap, s); break;
+    default: launch_blend<TS, 1>(a, mode, tiles_cap, s); break;
+  }
+}
+
+void launch_blend_tile(const BlendArgs& a, int mode, const W2WCaps& c, uint32_t tiles_cap, hipStream_t s) {
+  switch (c.tile) {
+    case 8: launch_blend_chunk<8>(a, mode, c.chunk, tiles_cap, s); break;
+    case 16: launch_blend_chunk<16>(a, mode, c.chunk, tiles_cap, s); break;
+    default: launch_blend_chunk<32>(a, mode, c.chunk, tiles_cap, s); break;
   }
 }
 
@@ -544,7 +555,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   ba.n_open_units = nullptr;
   ba.tile_cost = at<uint32_t>(I, p.tile_cost);
   if (!p.sliced || items_cap == 0 || cap == 0) {
-    launch_blend_tile(ba, BLEND_FULL, c.tile, tiles_cap, s);
+    launch_blend_tile(ba, BLEND_FULL, c, tiles_cap, s);
     return launch_status();
   }
   // near lists: final outputs of finished blocks and hinted tiles, unfinished
@@ -555,7 +566,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   ba.hint_bm = hint_bm;
   ba.open_units = at<uint32_t>(G, p.open_units);
   ba.n_open_units = status + ST_OPEN_UNITS;  // zeroed with the status block
-  launch_blend_tile(ba, BLEND_NEAR, c.tile, tiles_cap, s);
+  launch_blend_tile(ba, BLEND_NEAR, c, tiles_cap, s);
   // far lists on the open tiles that were not hinted: counts, compact
   // offsets, emission, sort, ranges
   {
@@ -609,7 +620,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   ba.ticket = status + ST_BLENDB_TICKET;
   ba.ranges_far = ranges_far;
   ba.ids_far = at<uint32_t>(B, p.far_ids());
-  launch_blend_tile(ba, BLEND_FAR, c.tile, tiles_cap, s);
+  launch_blend_tile(ba, BLEND_FAR, c, tiles_cap, s);
   return launch_status();
 }
 
@@ -706,7 +717,7 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
 
 int single_setup(const W2WFrame* f, const W2WWorkspace* ws, Plan& p, hipStream_t s) {
   W2WCaps c;
-  if (!ws || !single_caps(f->P, f->W, f->H, f->tile, ws->capacity, c) || !make_plan(c, p))
+  if (!ws || !single_caps(f->P, f->W, f->H, f->tile, ws->capacity, c, f->chunk) || !make_plan(c, p))
     return W2W_ERR_INVALID;
   if (!ws_ok(ws, p)) return W2W_ERR_CAPACITY;
   (void)hipGetLastError();

@@ -962,19 +962,25 @@ W2W_DEV bool wave_all(bool p) { return __ballot(!p) == 0ull; }
 // its tile's depth-ordered list front to back and stops before the splat that
 // would take its transmittance below 1e-4.
 //
-// Each wave owns one 8x8 pixel block and walks the list in chunks of 64: lane
-// l gathers instance base + l, tests its cull extent against the block and a
-// ballot leaves the splats that may touch any pixel of the block; the wave
-// then visits those in order, broadcasting each from its lane.  A dropped
-// splat is one every pixel of the block would skip (its exponent is below
-// skip_below), so each pixel sees the same sequence of contributing splats,
-// in the same order and with the same arithmetic, as a walk over the whole
-// list.  The next chunk's ids and splat records are in flight while a chunk is
-// processed.  A workgroup claims four consecutive blocks at a time, one per
-// wave; the waves are otherwise independent.
-template <int TS, int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_blend(BlendArgs a) {
-  __shared__ Splat s_chunk[4][64];  // each wave's current chunk, for broadcast reads
+// Each wave owns one 8x8 pixel block and walks the list in chunks of 64 * CW
+// instances (CW = 1, 2, 4: the counterpart of the reference's 64/128/256
+// batch, pixelGaussiansFull.cu:41-76): lane l gathers instances base + l,
+// base + 64 + l, ..., and per 64-instance sub-chunk tests each one's cull
+// extent against the block; a ballot leaves the splats that may touch any
+// pixel of the block, and the wave visits those in order, broadcasting each
+// from its lane.  A dropped splat is one every pixel of the block would skip
+// (its exponent is below skip_below), so each pixel sees the same sequence of
+// contributing splats, in the same order and with the same arithmetic, as a
+// walk over the whole list, whatever CW is.  The next chunk's splat records
+// and the chunk after's ids are in flight while a chunk is processed: CW
+// sets how many gathers each lane keeps in flight.  A workgroup claims four
+// consecutive blocks at a time, one per wave; the waves are otherwise
+// independent.
+template <int TS, int MODE, int CW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, CW == 4 ? 2 : 4))) void k_blend(
+    BlendArgs a) {
+  __shared__ Splat s_chunk[4][64];  // each wave's current sub-chunk, for broadcast reads
+  constexpr int CH = 64 * CW;
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
   const uint32_t T = min(hdr->n_tiles, a.tiles_cap);
   const uint32_t P = min(hdr->n_items, a.items_cap);
@@ -1043,88 +1049,114 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) voi
 
     // software pipeline: ids two chunks ahead, splat records one chunk ahead
     // (lanes past the end gather splat 0, which exists whenever n > 0)
-    uint32_t id_n = 0;
-    Splat cur = {};
-    float qcur = 0.0f;
-    if (n > 0 && !wave_all(done)) {
-      id_n = lane < n ? ids[range.x + lane] : 0u;
-      const SplatRec& r = a.splats[id_n < P ? id_n : 0u];
-      cur = r.s;
-      qcur = r.q1;
-      id_n = 64 + lane < n ? ids[range.x + 64 + lane] : 0u;
+    uint32_t id_n[CW];
+    Splat cur[CW];
+    float qcur[CW];
+#pragma unroll
+    for (int c = 0; c < CW; c++) {
+      id_n[c] = 0u;
+      cur[c] = Splat{};
+      qcur[c] = 0.0f;
     }
-    for (int base = 0; base < n; base += 64) {
+    if (n > 0 && !wave_all(done)) {
+#pragma unroll
+      for (int c = 0; c < CW; c++) {
+        const int k = 64 * c + lane;
+        const uint32_t id = k < n ? ids[range.x + k] : 0u;
+        const SplatRec& r = a.splats[id < P ? id : 0u];
+        cur[c] = r.s;
+        qcur[c] = r.q1;
+        const int k1 = CH + k;
+        id_n[c] = k1 < n ? ids[range.x + k1] : 0u;
+      }
+    }
+    for (int base = 0; base < n; base += CH) {
       if (wave_all(done)) break;
-      const Splat s = cur;
-      const float kq = qcur;
-      if (base + 64 < n) {
-        const SplatRec& r = a.splats[id_n < P ? id_n : 0u];
-        cur = r.s;
-        qcur = r.q1;
-        const int k2 = base + 128 + lane;
-        id_n = k2 < n ? ids[range.x + k2] : 0u;
+      Splat sc[CW];
+      float kq[CW];
+#pragma unroll
+      for (int c = 0; c < CW; c++) {
+        sc[c] = cur[c];
+        kq[c] = qcur[c];
       }
-      n_loaded += (uint32_t)min(64, n - base);
-      const bool valid = base + lane < n;
-      const bool touch = valid && extent_touches(s.a.x, s.a.y, unpack_extent(s.c.w), x0, x1, y0, y1) &&
-                         ellipse_touches_q(s.a.x, s.a.y, s.a.z, s.a.w, s.b.x, kq, x0, x1, y0, y1);
-      uint64_t m = __ballot(touch);
-      if (m == 0ull) continue;
-      // the wave's chunk goes to its LDS slot; survivors are then read back
-      // with one uniform address (a broadcast), the next one while the
-      // current one is evaluated.  LDS operations of one wave complete in
-      // order, so no barrier is needed.
-      stage[lane] = s;
-      __builtin_amdgcn_wave_barrier();
-      // straight-line per lane: every lane evaluates and selects commit the
-      // update (the tests of the reference's loop, in its order: skipped
-      // pair, alpha below 1/255, transmittance below 1e-4), so a survivor
-      // costs no exec-mask branches
-      auto eval = [&](const Splat& q, int j) {
-        const float dx = q.a.x - pxf, dy = q.a.y - pyf;
-        const float pw = power_of(q.a.z, q.a.w, q.b.x, dx, dy);
-        const bool live = !done;
-        n_eval += live ? 1u : 0u;
-        const bool in_range = live && !(pw > 0.0f || pw < q.b.w);
-        const float G = exp_neg(pw);
-        const float al = fminf(0.99f, q.b.y * G);
-        const bool hit = in_range && !(al < a_min);
-        const float nT = T_ * (1.0f - al);
-        const bool saturate = nT < 1e-4f;
-        const bool apply = hit && !saturate;
-        done = done || (hit && saturate);
-        const float aT = al * T_;
-        C0 = apply ? __builtin_fmaf(q.c.x, aT, C0) : C0;
-        C1 = apply ? __builtin_fmaf(q.c.y, aT, C1) : C1;
-        C2 = apply ? __builtin_fmaf(q.c.z, aT, C2) : C2;
-        A = apply ? A + aT : A;
-        Dd = apply ? __builtin_fmaf(q.b.z, aT, Dd) : Dd;
-        T_ = apply ? nT : T_;
-        last = apply ? lbase + (uint32_t)(base + j + 1) : last;
-        n_con += apply ? 1u : 0u;
-        pairs += (uint32_t)__builtin_popcountll(__ballot(apply));  // scalar
-      };
-      // two survivors in flight: the next one's read overlaps this one's work
-      int ja = __builtin_ctzll(m);
-      m &= m - 1;
-      Splat qa = stage[ja], qb;
-      for (;;) {
-        const int jb = m ? __builtin_ctzll(m) : -1;
-        if (jb >= 0) {
-          qb = stage[jb];
-          m &= m - 1;
+      if (base + CH < n) {
+#pragma unroll
+        for (int c = 0; c < CW; c++) {
+          const SplatRec& r = a.splats[id_n[c] < P ? id_n[c] : 0u];
+          cur[c] = r.s;
+          qcur[c] = r.q1;
+          const int k2 = base + 2 * CH + 64 * c + lane;
+          id_n[c] = k2 < n ? ids[range.x + k2] : 0u;
         }
-        eval(qa, ja);
-        if (jb < 0 || wave_all(done)) break;
-        ja = m ? __builtin_ctzll(m) : -1;
-        if (ja >= 0) {
-          qa = stage[ja];
-          m &= m - 1;
-        }
-        eval(qb, jb);
-        if (ja < 0 || wave_all(done)) break;
       }
-      __builtin_amdgcn_wave_barrier();  // reads of this chunk precede the next chunk's writes
+#pragma unroll
+      for (int c = 0; c < CW; c++) {
+        const int sb = base + 64 * c;  // this sub-chunk's first instance (uniform)
+        if (sb >= n) break;
+        if (c > 0 && wave_all(done)) break;
+        const Splat& s = sc[c];
+        n_loaded += (uint32_t)min(64, n - sb);
+        const bool valid = sb + lane < n;
+        const bool touch = valid && extent_touches(s.a.x, s.a.y, unpack_extent(s.c.w), x0, x1, y0, y1) &&
+                           ellipse_touches_q(s.a.x, s.a.y, s.a.z, s.a.w, s.b.x, kq[c], x0, x1, y0, y1);
+        uint64_t m = __ballot(touch);
+        if (m == 0ull) continue;
+        // the wave's sub-chunk goes to its LDS slot; survivors are then read
+        // back with one uniform address (a broadcast), the next one while the
+        // current one is evaluated.  LDS operations of one wave complete in
+        // order, so no barrier is needed.
+        stage[lane] = s;
+        __builtin_amdgcn_wave_barrier();
+        // straight-line per lane: every lane evaluates and selects commit the
+        // update (the tests of the reference's loop, in its order: skipped
+        // pair, alpha below 1/255, transmittance below 1e-4), so a survivor
+        // costs no exec-mask branches
+        auto eval = [&](const Splat& q, int j) {
+          const float dx = q.a.x - pxf, dy = q.a.y - pyf;
+          const float pw = power_of(q.a.z, q.a.w, q.b.x, dx, dy);
+          const bool live = !done;
+          n_eval += live ? 1u : 0u;
+          const bool in_range = live && !(pw > 0.0f || pw < q.b.w);
+          const float G = exp_neg(pw);
+          const float al = fminf(0.99f, q.b.y * G);
+          const bool hit = in_range && !(al < a_min);
+          const float nT = T_ * (1.0f - al);
+          const bool saturate = nT < 1e-4f;
+          const bool apply = hit && !saturate;
+          done = done || (hit && saturate);
+          const float aT = al * T_;
+          C0 = apply ? __builtin_fmaf(q.c.x, aT, C0) : C0;
+          C1 = apply ? __builtin_fmaf(q.c.y, aT, C1) : C1;
+          C2 = apply ? __builtin_fmaf(q.c.z, aT, C2) : C2;
+          A = apply ? A + aT : A;
+          Dd = apply ? __builtin_fmaf(q.b.z, aT, Dd) : Dd;
+          T_ = apply ? nT : T_;
+          last = apply ? lbase + (uint32_t)(sb + j + 1) : last;
+          n_con += apply ? 1u : 0u;
+          pairs += (uint32_t)__builtin_popcountll(__ballot(apply));  // scalar
+        };
+        // two survivors in flight: the next one's read overlaps this one's work
+        int ja = __builtin_ctzll(m);
+        m &= m - 1;
+        Splat qa = stage[ja], qb;
+        for (;;) {
+          const int jb = m ? __builtin_ctzll(m) : -1;
+          if (jb >= 0) {
+            qb = stage[jb];
+            m &= m - 1;
+          }
+          eval(qa, ja);
+          if (jb < 0 || wave_all(done)) break;
+          ja = m ? __builtin_ctzll(m) : -1;
+          if (ja >= 0) {
+            qa = stage[ja];
+            m &= m - 1;
+          }
+          eval(qb, jb);
+          if (ja < 0 || wave_all(done)) break;
+        }
+        __builtin_amdgcn_wave_barrier();  // reads of this sub-chunk precede the next one's writes
+      }
     }
     const bool open = MODE == BLEND_NEAR && !wave_all(done);
     const bool unfinished = open && !((hint_word >> (tile & 31u)) & 1u);

@@ -139,10 +139,11 @@ class Camera:
     focal_y: float
 
 
-def make_frame(means, scales, rotations, opacities, sh, sh_degree: int, cam: Camera, tile: int):
+def make_frame(means, scales, rotations, opacities, sh, sh_degree: int, cam: Camera, tile: int,
+               chunk: int = 0):
     P = int(means.shape[0])
     M = int(sh.shape[1]) if sh.numel() else 1
-    f = _lib.W2WFrame(P, M, int(sh_degree), int(cam.width), int(cam.height), int(tile),
+    f = _lib.W2WFrame(P, M, int(sh_degree), int(cam.width), int(cam.height), int(tile), int(chunk), 0,
                       float(cam.focal_x), float(cam.focal_y), _ptr(means), _ptr(scales),
                       _ptr(rotations), _ptr(opacities), _ptr(sh), _ptr(cam.viewmatrix),
                       _ptr(cam.projmatrix), _ptr(cam.campos), _ptr(cam.background))
@@ -183,10 +184,11 @@ def _inputs(means, scales, rotations, opacities, sh):
 
 
 def forward(means, scales, rotations, opacities, sh, sh_degree: int, cam: Camera, tile: int = 16,
-            workspace: Workspace | None = None, capacity: int | None = None):
+            workspace: Workspace | None = None, capacity: int | None = None, chunk: int = 0):
     """One view.  Returns (color [3,H,W], depth [H,W], alpha [H,W], radii [P],
     workspace).  K and the overflow flag stay on the device
-    (``workspace.status()``); nothing here synchronises."""
+    (``workspace.status()``); nothing here synchronises.  chunk: instances
+    per forward-blend chunk (64, 128 or 256; 0 = 64), a speed knob only."""
     means, scales, rotations, opacities, sh = _inputs(means, scales, rotations, opacities, sh)
     P, W, H = int(means.shape[0]), int(cam.width), int(cam.height)
     key = (P, W, H, tile)
@@ -198,7 +200,7 @@ def forward(means, scales, rotations, opacities, sh, sh_degree: int, cam: Camera
     depth = torch.empty(H, W, dtype=torch.float32, device=dev)
     alpha = torch.empty(H, W, dtype=torch.float32, device=dev)
     radii = torch.empty(P, dtype=torch.int32, device=dev)
-    frame = make_frame(means, scales, rotations, opacities, sh, sh_degree, cam, tile)
+    frame = make_frame(means, scales, rotations, opacities, sh, sh_degree, cam, tile, chunk)
     st = _lib.load().w2w_forward(ctypes.byref(frame), ctypes.byref(workspace.struct), color.data_ptr(),
                                  depth.data_ptr(), alpha.data_ptr(), _ptr(radii) if P else None,
                                  _stream())
@@ -207,16 +209,16 @@ def forward(means, scales, rotations, opacities, sh, sh_degree: int, cam: Camera
 
 
 def forward_checked(means, scales, rotations, opacities, sh, sh_degree: int, cam: Camera,
-                    tile: int = 16, workspace: Workspace | None = None):
+                    tile: int = 16, workspace: Workspace | None = None, chunk: int = 0):
     """``forward`` that reads K back (one synchronisation) and re-runs with a
     larger workspace when the instance capacity overflowed."""
     P = int(means.shape[0])
     key = (P, int(cam.width), int(cam.height), tile)
-    out = forward(means, scales, rotations, opacities, sh, sh_degree, cam, tile, workspace)
+    out = forward(means, scales, rotations, opacities, sh, sh_degree, cam, tile, workspace, chunk=chunk)
     K, ovf = (int(v) for v in out[4].status().cpu())
     if ovf:
         cap = POLICY.observe(key, K)
-        out = forward(means, scales, rotations, opacities, sh, sh_degree, cam, tile, None, cap)
+        out = forward(means, scales, rotations, opacities, sh, sh_degree, cam, tile, None, cap, chunk)
         K2, ovf2 = (int(v) for v in out[4].status().cpu())
         assert not ovf2 and K2 == K, "instance count changed between identical runs"
     else:
