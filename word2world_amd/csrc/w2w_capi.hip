@@ -407,7 +407,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   const bool hinted = p.sliced && cap > 0 && items_cap > 0;
   const size_t tile_words = cdiv((uint64_t)c.tiles, 32);
   if (hinted)
-    hipLaunchKernelGGL(k_dilate_tiles, dim3((uint32_t)std::min<size_t>(cdiv(tile_words, 256), 1024)), dim3(256),
+    hipLaunchKernelGGL(k_dilate_tiles, dim3((uint32_t)std::min<uint64_t>(cdiv((uint64_t)c.tiles, 256), 4096)), dim3(256),
                        0, s, table, (uint32_t)c.scenes, (uint32_t)c.tiles, at<uint32_t>(G, p.open_bm),
                        at<uint32_t>(G, p.hint_bm));
   FillBatch fb;

@@ -491,34 +491,65 @@ __global__ void k_split(const void* __restrict__ table, uint32_t items_cap, uint
 
 // Inclusive 2-D prefix counts of a tile set per frame, indexed like the
 // tiles: sat[tile_off + y tiles_x + x] = members in columns 0..x of rows
-// 0..y.  One workgroup per frame: row prefixes, then column prefixes.
+// 0..y.  One workgroup per frame: row prefixes, then column prefixes, in LDS
+// when the frame fits (rows padded by one word: the row pass reads a column
+// per step, conflict-free), in global memory otherwise.
+constexpr uint32_t kSatLds = 16384;
 __global__ __launch_bounds__(256) void k_tile_sat(const void* __restrict__ table, uint32_t scenes_cap,
                                                   uint32_t frames_cap, const uint32_t* __restrict__ set,
                                                   uint32_t* __restrict__ sat) {
+  __shared__ uint32_t s[kSatLds];
   const BatchHeader* hdr = static_cast<const BatchHeader*>(table);
   const FrameDesc* frames = table_frames(table, scenes_cap);
   const uint32_t nf = min(hdr->n_frames, frames_cap);
   for (uint32_t f = blockIdx.x; f < nf; f += gridDim.x) {
     const FrameDesc& fd = frames[f];
     const uint32_t tx = (uint32_t)fd.tiles_x, ty = (uint32_t)fd.tiles_y, t0 = fd.tile_off;
-    for (uint32_t y = threadIdx.x; y < ty; y += blockDim.x) {
-      uint32_t run = 0;
-      for (uint32_t x = 0; x < tx; x++) {
-        const uint32_t t = t0 + y * tx + x;
-        run += (set[t >> 5] >> (t & 31u)) & 1u;
-        sat[t] = run;
+    const uint32_t rs = tx + 1u;  // padded row stride
+    if ((uint64_t)rs * ty <= kSatLds) {
+      for (uint32_t t = threadIdx.x; t < tx * ty; t += blockDim.x) {
+        const uint32_t g = t0 + t;
+        s[(t / tx) * rs + t % tx] = (set[g >> 5] >> (g & 31u)) & 1u;
       }
-    }
-    __syncthreads();
-    for (uint32_t x = threadIdx.x; x < tx; x += blockDim.x) {
-      uint32_t run = 0;
-      for (uint32_t y = 0; y < ty; y++) {
-        const uint32_t t = t0 + y * tx + x;
-        run += sat[t];
-        sat[t] = run;
+      __syncthreads();
+      for (uint32_t y = threadIdx.x; y < ty; y += blockDim.x) {
+        uint32_t run = 0;
+        for (uint32_t x = 0; x < tx; x++) {
+          run += s[y * rs + x];
+          s[y * rs + x] = run;
+        }
       }
+      __syncthreads();
+      for (uint32_t x = threadIdx.x; x < tx; x += blockDim.x) {
+        uint32_t run = 0;
+        for (uint32_t y = 0; y < ty; y++) {
+          run += s[y * rs + x];
+          s[y * rs + x] = run;
+        }
+      }
+      __syncthreads();
+      for (uint32_t t = threadIdx.x; t < tx * ty; t += blockDim.x) sat[t0 + t] = s[(t / tx) * rs + t % tx];
+      __syncthreads();  // s is rewritten by the next frame
+    } else {
+      for (uint32_t y = threadIdx.x; y < ty; y += blockDim.x) {
+        uint32_t run = 0;
+        for (uint32_t x = 0; x < tx; x++) {
+          const uint32_t t = t0 + y * tx + x;
+          run += (set[t >> 5] >> (t & 31u)) & 1u;
+          sat[t] = run;
+        }
+      }
+      __syncthreads();
+      for (uint32_t x = threadIdx.x; x < tx; x += blockDim.x) {
+        uint32_t run = 0;
+        for (uint32_t y = 0; y < ty; y++) {
+          const uint32_t t = t0 + y * tx + x;
+          run += sat[t];
+          sat[t] = run;
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
@@ -860,31 +891,30 @@ W2W_DEV uint32_t tile_frame(const FrameDesc* frames, uint32_t n_frames, uint32_t
 // Depth-sliced binning: the next forward's hint, the tiles its near lists
 // left open grown by one tile within their frame (a tile that stays open is
 // usually next to one that was): bit t = any open tile among t and its eight
-// neighbours.  One thread per 32-tile word.
+// neighbours.  One thread per tile; a wave's ballot gives two 32-tile words.
 __global__ __launch_bounds__(256) void k_dilate_tiles(const void* __restrict__ table, uint32_t scenes_cap,
                                                       uint32_t tiles_cap, const uint32_t* __restrict__ open_bm,
                                                       uint32_t* __restrict__ hint_bm) {
   const BatchHeader* hdr = static_cast<const BatchHeader*>(table);
   const FrameDesc* frames = table_frames(table, scenes_cap);
   const uint32_t T = min(hdr->n_tiles, tiles_cap);
-  const uint32_t nw = (T + 31) / 32;
-  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += gridDim.x * blockDim.x) {
-    uint32_t out = 0;
-    for (uint32_t b = 0; b < 32; b++) {
-      const uint32_t t = w * 32 + b;
-      if (t >= T) break;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t Tw = (T + 63u) & ~63u;  // whole waves: every lane reaches the ballot
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < Tw; t += gridDim.x * blockDim.x) {
+    uint32_t any = 0;
+    if (t < T) {
       const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, t)];
       const int txn = fd.tiles_x, tyn = fd.tiles_y;
       const int lt = (int)(t - fd.tile_off), tx = lt % txn, ty = lt / txn;
-      uint32_t any = 0;
       for (int ny = max(ty - 1, 0); ny <= min(ty + 1, tyn - 1); ny++)
         for (int nx = max(tx - 1, 0); nx <= min(tx + 1, txn - 1); nx++) {
           const uint32_t nt = fd.tile_off + (uint32_t)(ny * txn + nx);
           any |= (open_bm[nt >> 5] >> (nt & 31u)) & 1u;
         }
-      out |= any << b;
     }
-    hint_bm[w] = out;
+    const uint64_t b = __ballot(any != 0u);
+    const uint32_t w = t >> 5;  // lanes 0 and 32 hold the first tile of a word
+    if ((lane & 31u) == 0u && w < (T + 31u) / 32u) hint_bm[w] = (uint32_t)(b >> lane);
   }
 }
 
