@@ -49,10 +49,18 @@ def test_forward_large_scene_properties(gpu):
 ])
 def test_backward_vs_autograd(gpu, n, deg, res, ts, scale_mul, seed):
     arrs = scene_np(n, deg, seed, scale_mul)
-    m, s, r, o, sh = arrs
     cam = orbit_views(res, res)[1]
     print(f"case n={n} deg={deg} res={res} ts={ts}", flush=True)
-    f = oracle_forward(arrs, deg, cam, ts, record=1024)
+    _check_backward(gpu, arrs, deg, cam, ts, seed)
+
+
+def _check_backward(gpu, arrs, deg, cam, ts, seed, record=1024):
+    """GPU backward against float64 autograd over the oracle's recorded
+    contributor lists, and the intermediate gradients against the oracle's
+    float32 hand backward; returns the oracle forward."""
+    m, s, r, o, sh = arrs
+    res = cam.width
+    f = oracle_forward(arrs, deg, cam, ts, record=record)
     rng = np.random.default_rng(seed)
     gC = rng.standard_normal((3, res, res)).astype(np.float32)
     gA = rng.standard_normal((res, res)).astype(np.float32)
@@ -82,6 +90,21 @@ def test_backward_vs_autograd(gpu, n, deg, res, ts, scale_mul, seed):
         e = _rel(got[k].reshape(ob[k].shape), ob[k])
         print(k, "vs oracle", e, flush=True)
         assert e <= GRAD_TOL, (k, e)
+    return f
+
+
+def test_dense_faint_scene_backward_vs_autograd(gpu):
+    """A dense, faint scene (8,000 large Gaussians of opacity 0.03 at 64^2:
+    up to ~500 contributors per pixel, tiles of 2,000 to 83,000 contributing
+    pairs) against float64 autograd and the oracle's hand backward: long
+    per-pixel walks, the heaviest tiles first in the backward's order."""
+    arrs = list(scene_np(8000, 1, 41, 3.0))
+    arrs[3] = np.full_like(arrs[3], 0.03)
+    cam = orbit_views(64, 64)[1]
+    f = _check_backward(gpu, arrs, 1, cam, 16, 41)
+    cost = f["rec_count"].reshape(4, 16, 4, 16).sum(axis=(1, 3))
+    print("tile costs", cost.ravel(), flush=True)
+    assert f["rec_count"].max() >= 400 and cost.max() >= 1 << 16
 
 
 def test_empty_and_culled(gpu):

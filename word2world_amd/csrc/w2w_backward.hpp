@@ -43,6 +43,10 @@ enum GradField : int {
 
 constexpr int GR_FIELDS = 10;
 
+// An entry of the backward's tile order (k_tile_order): the tile in bits 0-29,
+// its wave's issue priority in bits 30-31.
+constexpr uint32_t kOrderTile = (1u << 30) - 1u;
+
 // Sum over the 64 lanes of a wave, result valid in lane 63: quad_perm
 // butterflies, row rotations, then the GFX9 row broadcasts.  All lanes must
 // be active.
@@ -200,7 +204,7 @@ void k_blend_bwd(BlendBwdArgs a) {
   for (;;) {
     if (tid == 0) {
       const uint32_t t = atomicAdd(a.ticket, 1u);
-      s_tile = t < n_order ? a.order[t] : 0xffffffffu;
+      s_tile = t < n_order ? (a.order[t] & kOrderTile) : 0xffffffffu;
       s_max = 0;
     }
     __syncthreads();
@@ -433,7 +437,10 @@ void k_blend_bwd(BlendBwdArgs a) {
 // work (no contributor, or a frame without the backward) are left out.
 // Taken in this order the long tiles start first instead of finishing last
 // (the order within a bucket is whatever the atomics give; it changes which
-// wave takes a tile, never a result).
+// wave takes a tile, never a result).  Each entry also carries an issue
+// priority for the wave that takes it (bits 30-31): 3 for the heaviest
+// nonempty bucket, 2 and 1 for the next two, 0 below: the heaviest tiles set
+// the kernel's length, and their waves share a SIMD with lighter ones.
 constexpr int kCostBuckets = 33;  // 0: no work; b >= 1: cost in [2^(b-1), 2^b)
 
 W2W_DEV uint32_t cost_bucket(uint32_t c) { return c ? 32u - (uint32_t)__builtin_clz(c) : 0u; }
@@ -461,14 +468,17 @@ __global__ __launch_bounds__(256) void k_tile_cost_hist(TileOrderArgs a) {
 
 __global__ __launch_bounds__(256) void k_tile_order(TileOrderArgs a) {
   __shared__ uint32_t off[kCostBuckets], cnt[kCostBuckets], base[kCostBuckets];
+  __shared__ uint32_t s_top;
   const uint32_t T = min(static_cast<const BatchHeader*>(a.table)->n_tiles, a.tiles_cap);
   if (threadIdx.x == 0) {
-    uint32_t o = 0;
+    uint32_t o = 0, top = 0;
     for (int b = kCostBuckets - 1; b >= 1; b--) {
       off[b] = o;
       o += a.hist[b];
+      if (!top && a.hist[b]) top = (uint32_t)b;
     }
     off[0] = o;
+    s_top = top;
     if (blockIdx.x == 0) *a.n_order = o;
   }
   for (uint32_t r0 = blockIdx.x * blockDim.x; r0 < T; r0 += gridDim.x * blockDim.x) {  // uniform
@@ -481,7 +491,10 @@ __global__ __launch_bounds__(256) void k_tile_order(TileOrderArgs a) {
     if (threadIdx.x >= 1 && threadIdx.x < kCostBuckets && cnt[threadIdx.x])
       base[threadIdx.x] = atomicAdd(a.cursor + threadIdx.x, cnt[threadIdx.x]);
     __syncthreads();
-    if (b) a.order[off[b] + base[b] + rank] = t;
+    if (b) {
+      const uint32_t prio = b + 3u > s_top ? 3u - (s_top - b) : 0u;
+      a.order[off[b] + base[b] + rank] = t | (prio << 30);
+    }
     __syncthreads();  // cnt and base are rewritten by the next round
   }
 }
@@ -766,14 +779,17 @@ __global__ __launch_bounds__(256) void k_blend_bwd_tile(BlendBwdArgs a) {
     if (lane == 0) t = atomicAdd(a.ticket, 1u);
     t = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)t, 0, 64));
     if (t >= n_order) break;
-    const uint32_t tile = a.order[t];
+    const uint32_t e = a.order[t];
+    const uint32_t tile = e & kOrderTile;
     if (tile >= Tn) continue;
     // the heaviest tiles set the kernel's length: their waves take issue
     // priority over the waves of lighter tiles sharing the SIMD
-    if (t < n_order / 16u)
-      __builtin_amdgcn_s_setprio(2);
-    else
-      __builtin_amdgcn_s_setprio(0);
+    switch (e >> 30) {
+      case 3: __builtin_amdgcn_s_setprio(3); break;
+      case 2: __builtin_amdgcn_s_setprio(2); break;
+      case 1: __builtin_amdgcn_s_setprio(1); break;
+      default: __builtin_amdgcn_s_setprio(0); break;
+    }
     const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, tile)];
     bwd_tile<TS, AUX, STATS>(a, sh, fd, tile, P, lane, wv, n_eval, n_con, n_loaded);
   }
