@@ -145,6 +145,8 @@ bool make_plan(const W2WCaps& c, Plan& p) {
     return false;
   if (c.tile != 8 && c.tile != 16 && c.tile != 32) return false;
   if (c.chunk != 0 && c.chunk != 64 && c.chunk != 128 && c.chunk != 256) return false;
+  // the forward blend numbers 8x8 blocks (and quads of them) in 30 bits
+  if ((uint64_t)c.tiles * (uint64_t)(c.tile / 8) * (uint64_t)(c.tile / 8) >= (1ull << 30)) return false;
   if (c.depth_split < 0) return false;
   if (c.items >= (int64_t)LB_VAL || c.capacity >= (int64_t)LB_VAL || c.tiles >= (int64_t)LB_VAL ||
       c.gaussians >= (int64_t)LB_VAL)

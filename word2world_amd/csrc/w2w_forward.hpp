@@ -1004,8 +1004,8 @@ W2W_DEV bool wave_all(bool p) { return __ballot(!p) == 0ull; }
 // walk over the whole list, whatever CW is.  The next chunk's splat records
 // and the chunk after's ids are in flight while a chunk is processed: CW
 // sets how many gathers each lane keeps in flight.  A workgroup claims four
-// consecutive blocks at a time, one per wave; the waves are otherwise
-// independent.
+// consecutive blocks (a quad) at a time and its waves take them one by one,
+// each moving on to the next quad on its own (no workgroup barrier).
 template <int TS, int MODE, int CW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, CW == 4 ? 2 : 4))) void k_blend(
     BlendArgs a) {
@@ -1020,7 +1020,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, CW == 4 
   Splat* const stage = s_chunk[threadIdx.x >> 6];
   const float a_min = 0x1.010102p-8f;
   uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
-  __shared__ uint32_t s_quad;
+  // the workgroup's current quad of four consecutive blocks (one tile of
+  // edge 16) and how many times its waves asked for a block since it was
+  // claimed: (quad << 4) | count.  Counts 0-3 are its blocks; the wave that
+  // reads 4 claims the next quad, waves reading 5 or more wait for it.  Each
+  // of the four waves asks at most once past 4 before the claim resets the
+  // count to 1, so the count stays below 16.  Starts "used up", so the first
+  // wave claims a quad.
+  __shared__ uint32_t s_claim;
+  if (MODE != BLEND_FAR && threadIdx.x == 0) s_claim = 4u;
+  if (MODE != BLEND_FAR) __syncthreads();
+  const uint32_t nquads = (U + 3) / 4;
   const uint32_t n_open = MODE == BLEND_FAR ? min(*a.n_open_units, U) : 0u;
   uint32_t far_next = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);  // this wave's next list entry
   for (;;) {
@@ -1031,15 +1041,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, CW == 4 
       if (w >= n_open) break;
       unit = a.open_units[w];
     } else {
-      // the workgroup claims four consecutive blocks (one tile of edge 16),
-      // one per wave: the waves gather the same tile list through one CU's
-      // caches
-      if (threadIdx.x == 0) s_quad = atomicAdd(a.ticket, 1u);
-      __syncthreads();
-      const uint32_t quad = s_quad;
-      __syncthreads();
-      if (quad >= (U + 3) / 4) break;
-      unit = quad * 4 + (threadIdx.x >> 6);
+      // each wave takes the next block of its workgroup's quad (so the
+      // blocks of a tile gather its list through one CU's caches, without
+      // waiting for each other); the wave that finds the quad used up
+      // claims the next one for the workgroup, and waves that find it being
+      // replaced wait for the new one
+      uint32_t got = 0;
+      if ((threadIdx.x & 63u) == 0) {
+        for (;;) {
+          const uint32_t v = atomicAdd(&s_claim, 1u);
+          const uint32_t k = v & 15u;
+          if (k < 4u) {
+            got = (v >> 4) < nquads ? (v >> 4) * 4u + k : 0xFFFFFFFFu;
+            break;
+          }
+          if (k == 4u) {
+            const uint32_t qn = atomicAdd(a.ticket, 1u);
+            atomicExch(&s_claim, (min(qn, 0x0FFFFFFFu) << 4) | 1u);  // block 0 of it is this wave's
+            got = qn < nquads ? qn * 4u : 0xFFFFFFFFu;
+            break;
+          }
+          while ((__atomic_load_n(&s_claim, __ATOMIC_RELAXED) & 15u) >= 4u) {
+          }
+        }
+      }
+      unit = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)got, 0, 64));
+      if (unit == 0xFFFFFFFFu) break;
       if (unit >= U) continue;
     }
     uint32_t tile;

@@ -190,14 +190,29 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(
         st_release_dev(st + tid, LB_INC | total);
       } else {
         st_release_dev(st + tid, LB_AGG | total);
+        // the preceding tiles' words, kLookAhead at a time (independent
+        // loads), consumed in order up to the first inclusive one; a word not
+        // yet published is read again from there
+        constexpr int kLookAhead = 8;
         int p = (int)tile - 1;
         for (;;) {
-          uint32_t v = ld_acquire_dev(status + (size_t)p * RADIX + tid);
-          uint32_t flag = v & ~LB_VAL;
-          if (flag == 0) continue;
-          excl += v & LB_VAL;
-          if (flag == LB_INC) break;
-          p--;
+          uint32_t w[kLookAhead];
+#pragma unroll
+          for (int k = 0; k < kLookAhead; k++)
+            w[k] = p - k >= 0 ? ld_acquire_dev(status + (size_t)(p - k) * RADIX + tid) : LB_INC;
+          bool found = false;
+          int k = 0;
+          for (; k < kLookAhead; k++) {
+            const uint32_t flag = w[k] & ~LB_VAL;
+            if (flag == 0) break;
+            excl += w[k] & LB_VAL;
+            if (flag == LB_INC) {
+              found = true;
+              break;
+            }
+          }
+          if (found) break;
+          p -= k;
         }
         st_release_dev(st + tid, LB_INC | (excl + total));
       }
