@@ -43,9 +43,6 @@ enum GradField : int {
 
 constexpr int GR_FIELDS = 10;
 
-// An entry of the backward's tile order (k_tile_order): the tile in bits 0-29,
-// its wave's issue priority in bits 30-31.
-constexpr uint32_t kOrderTile = (1u << 30) - 1u;
 
 // Sum over the 64 lanes of a wave, result valid in lane 63: quad_perm
 // butterflies, row rotations, then the GFX9 row broadcasts.  All lanes must
@@ -453,6 +450,7 @@ struct TileOrderArgs {
   uint32_t* cursor;      // [kCostBuckets], zeroed
   uint32_t* n_order;     // tiles listed
   uint32_t* order;       // [tiles]
+  uint32_t include_zero; // list the tiles without cost too (last): the forward blend's order
 };
 
 __global__ __launch_bounds__(256) void k_tile_cost_hist(TileOrderArgs a) {
@@ -479,19 +477,20 @@ __global__ __launch_bounds__(256) void k_tile_order(TileOrderArgs a) {
     }
     off[0] = o;
     s_top = top;
-    if (blockIdx.x == 0) *a.n_order = o;
+    if (blockIdx.x == 0) *a.n_order = o + (a.include_zero ? a.hist[0] : 0u);
   }
   for (uint32_t r0 = blockIdx.x * blockDim.x; r0 < T; r0 += gridDim.x * blockDim.x) {  // uniform
     if (threadIdx.x < kCostBuckets) cnt[threadIdx.x] = 0u;
     __syncthreads();
     const uint32_t t = r0 + threadIdx.x;
     const uint32_t b = t < T ? cost_bucket(a.cost[t]) : 0u;
-    const uint32_t rank = b ? atomicAdd(&cnt[b], 1u) : 0u;
+    const bool listed = t < T && (b || a.include_zero);
+    const uint32_t rank = listed ? atomicAdd(&cnt[b], 1u) : 0u;
     __syncthreads();
-    if (threadIdx.x >= 1 && threadIdx.x < kCostBuckets && cnt[threadIdx.x])
+    if (threadIdx.x < kCostBuckets && cnt[threadIdx.x])
       base[threadIdx.x] = atomicAdd(a.cursor + threadIdx.x, cnt[threadIdx.x]);
     __syncthreads();
-    if (b) {
+    if (listed) {
       const uint32_t prio = b + 3u > s_top ? 3u - (s_top - b) : 0u;
       a.order[off[b] + base[b] + rank] = t | (prio << 30);
     }

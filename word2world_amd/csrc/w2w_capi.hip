@@ -415,7 +415,6 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   FillBatch fb;
   fb.add(status, ST_WORDS, 0u, s);
   fb.add(at<uint32_t>(I, p.L.ranges), 2 * (uint64_t)c.tiles, 0u, s);
-  fb.add(at<uint32_t>(I, p.tile_cost), (uint64_t)c.tiles, 0u, s);
   if (items_cap > 0) {
     fb.add(at<uint32_t>(G, p.rlb), p.stiles + 1, 0u, s);
     fb.add(at<uint32_t>(G, p.dhist), 4 * RADIX, 0u, s);
@@ -427,6 +426,29 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
     fb.add(at<uint32_t>(G, p.open_bm), tile_words, 0u, s);
   }
   fb.launch(s);
+  // the forward blend's block order: every tile, heaviest first by the last
+  // forward's per-tile costs (any content is a valid order), then this
+  // forward's costs are cleared for its blend to count
+  const bool ordered = items_cap > 0 && cap > 0;
+  if (ordered) {
+    TileOrderArgs oa;
+    oa.table = table; oa.tiles_cap = tiles_cap;
+    oa.cost = at<uint32_t>(I, p.tile_cost);
+    oa.hist = status + ST_COST_HIST;
+    oa.cursor = status + ST_COST_CURSOR;
+    oa.n_order = status + ST_ORDER_N;
+    oa.order = at<uint32_t>(I, p.tile_order);
+    oa.include_zero = 1u;
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(cdiv((uint64_t)c.tiles, 256), 512));
+    ProfSpan ps(W2W_KID_TILE_ORDER, s);
+    hipLaunchKernelGGL(k_tile_cost_hist, dim3(grid), dim3(256), 0, s, oa);
+    hipLaunchKernelGGL(k_tile_order, dim3(grid), dim3(256), 0, s, oa);
+  }
+  {
+    FillBatch fc;
+    fc.add(at<uint32_t>(I, p.tile_cost), (uint64_t)c.tiles, 0u, s);
+    fc.launch(s);
+  }
   if (items_cap > 0) {
 
     PreArgs pa;
@@ -556,6 +578,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   ba.open_units = nullptr;
   ba.n_open_units = nullptr;
   ba.tile_cost = at<uint32_t>(I, p.tile_cost);
+  ba.order = ordered ? at<uint32_t>(I, p.tile_order) : nullptr;
   if (!p.sliced || items_cap == 0 || cap == 0) {
     launch_blend_tile(ba, BLEND_FULL, c, tiles_cap, s);
     return launch_status();
@@ -655,6 +678,7 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
     oa.cursor = status + ST_COST_CURSOR;
     oa.n_order = status + ST_ORDER_N;
     oa.order = at<uint32_t>(I, p.tile_order);
+    oa.include_zero = 0u;
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(cdiv((uint64_t)c.tiles, 256), 512));
     ProfSpan ps(W2W_KID_TILE_ORDER, s);
     hipLaunchKernelGGL(k_tile_cost_hist, dim3(grid), dim3(256), 0, s, oa);

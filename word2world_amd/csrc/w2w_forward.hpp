@@ -918,6 +918,11 @@ __global__ __launch_bounds__(256) void k_dilate_tiles(const void* __restrict__ t
   }
 }
 
+// An entry of a tile order (k_tile_order: heaviest tiles first by a per-tile
+// cost): the tile in bits 0-29, the issue priority of the wave that takes it
+// in bits 30-31.
+constexpr uint32_t kOrderTile = (1u << 30) - 1u;
+
 struct BlendArgs {
   const void* table;
   uint32_t scenes_cap, tiles_cap, items_cap;
@@ -944,6 +949,9 @@ struct BlendArgs {
   // heaviest first).  The same in every emission and binning mode: each
   // pixel's sequence of contributors is.
   uint32_t* tile_cost;
+  // optional (modes FULL and NEAR): every tile, heaviest first by the last
+  // forward's costs; a quad of blocks is then four blocks in this order
+  const uint32_t* order;
 };
 
 // BLEND_FULL: one list per tile, final outputs.  BLEND_NEAR: the near lists;
@@ -1068,6 +1076,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, CW == 4 
       unit = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)got, 0, 64));
       if (unit == 0xFFFFFFFFu) break;
       if (unit >= U) continue;
+      if (a.order) {
+        // position unit / UPT of the order, block unit % UPT of its tile; the
+        // heaviest tiles' waves take issue priority
+        constexpr uint32_t UPT = units_per_tile<TS>();
+        const uint32_t e = a.order[unit / UPT];
+        unit = (e & kOrderTile) * UPT + unit % UPT;
+        switch (e >> 30) {
+          case 3: __builtin_amdgcn_s_setprio(3); break;
+          case 2: __builtin_amdgcn_s_setprio(2); break;
+          case 1: __builtin_amdgcn_s_setprio(1); break;
+          default: __builtin_amdgcn_s_setprio(0); break;
+        }
+        if (unit >= U) continue;
+      }
     }
     uint32_t tile;
     int ux, uy;
@@ -1241,6 +1263,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, CW == 4 
       a.final_T[pid] = T_;
     }
   }
+  __builtin_amdgcn_s_setprio(0);
   if (a.stats) {  // evaluated / contributing pairs and loaded instances
     unsigned long long e = n_eval, c = n_con, kf = (lane == 0) ? n_loaded : 0u;
     for (int off = 32; off >= 1; off >>= 1) {
