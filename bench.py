@@ -239,14 +239,18 @@ def bench_scene(args):
 
     tune, chunk = None, args.chunk
     if args.tile == 0:
+        # time the trainer step itself (all views in one launch per stage, at
+        # the middle depth-split candidate): a single view's fwd+bwd ranks the
+        # chunks differently from the four-view launch
         from word2world_amd import autotune
-        m, s_, r, o, sh = (t.to(dev).contiguous() for t in activate(scene.params()))
-        t = autotune.sweep(m, s_, r, o, sh, 3, cams[0], reps=2)
+        split0 = args.depth_split if args.depth_split >= 0 else 0.3
+        t = autotune.sweep_trainer(lambda ts_, ch: BatchedTrainer([scene], [cams_host], [targets], dev, ts_,
+                                                                  depth_split=split0, chunk=ch),
+                                   warmup=10, steps=10)
         ts, chunk = min(t, key=t.get)
         tune = {autotune.label(k): v for k, v in t.items()}
-        log("autotune (ms per single-view fwd+bwd): " + ", ".join(f"{k}: {v:.2f}" for k, v in tune.items())
+        log("autotune (ms per trainer step): " + ", ".join(f"{k}: {v:.2f}" for k, v in tune.items())
             + f" -> tile {ts}, chunk {chunk}")
-        del m, s_, r, o, sh
         torch.cuda.empty_cache()
     split_ms = None
     if args.depth_split < 0:
@@ -644,23 +648,27 @@ def bench_served(args, emit: bool = True):
     pool.build_targets(reqs)
     tile, chunk, tune = args.tile, args.chunk, None
     if tile == 0:
-        # requests of every size and resolution share the server's choice:
-        # tune on the heaviest request shape (the largest pool scene at
-        # 1024^2), where the time goes and whose instance count must fit
-        from word2world_amd import autotune, raster
-        from word2world_amd.camera import orbit_camera
-        from word2world_amd.scene import activate
-        k = int(np.argmax(sizes))
-        m, s_, r, o, sh = (t.to(dev).contiguous() for t in activate(pool.scenes[k].params()))
-        oc = orbit_camera(0.0, 0.0, 2.5, 1024, 1024)
-        cam = raster.camera_from_tensors(oc.viewmatrix, oc.projmatrix, oc.campos, 1024, 1024, oc.focal_x,
-                                         oc.focal_y, None, device=dev)
-        t = autotune.sweep(m, s_, r, o, sh, 3, cam, reps=2)
+        # requests of every size and resolution share the server's choice,
+        # and the server runs them as one batch per launch: tune on a batch
+        # like the ones it forms (the eight largest pool scenes, 4 views each
+        # at 256, 512 and 1024^2 in turn; fwd+bwd+Adam steps)
+        from word2world_amd import autotune
+        from word2world_amd.batched import BatchedTrainer
+        from word2world_amd.workloads import SceneJob, orbit4
+        pick = [int(k) for k in np.argsort(sizes)[::-1][:8]]
+        jobs = [SceneJob(pool.scenes[k], pool.targets[k], orbit4((256, 512, 1024)[j % 3], azimuth0=37.0 * j))
+                for j, k in enumerate(pick)]
+        tt = _targets(jobs, dev, 16)
+        t = autotune.sweep_trainer(lambda ts_, ch: BatchedTrainer([j.scene for j in jobs], [j.cameras for j in jobs],
+                                                                  tt, dev, ts_, depth_split=0.0, chunk=ch),
+                                   warmup=3, steps=5)
         tile, chunk = min(t, key=t.get)
         tune = {autotune.label(key): v for key, v in t.items()}
-        log(f"served autotune on {sizes[k]} Gaussians at 1024^2 (ms per fwd+bwd): "
-            + ", ".join(f"{key}: {v:.2f}" for key, v in tune.items()) + f" -> tile {tile}, chunk {chunk}")
-        del m, s_, r, o, sh
+        log(f"served autotune on {len(jobs)} pool scenes ({sum(sizes[k] for k in pick)} Gaussians, 4 views each; "
+            "ms per step): " + ", ".join(f"{key}: {v:.2f}" for key, v in tune.items())
+            + f" -> tile {tile}, chunk {chunk}")
+        del jobs, tt
+        torch.cuda.empty_cache()
     c = serve._lib.W2WCaps()
     c.scenes, c.frames, c.tile, c.depth_split = 128, 512, tile, 0
     c.chunk = chunk

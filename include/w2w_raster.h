@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define W2W_ABI_VERSION 7
+#define W2W_ABI_VERSION 8
 
 enum {
   W2W_OK = 0,
@@ -84,14 +84,16 @@ typedef struct W2WWorkspace {
 typedef struct W2WLayout {
   size_t geom_bytes, binning_bytes, image_bytes;
   size_t splats;     /* geom: 64-byte records (px,py,a,b | c,o,depth,skip | r,g,b,extent |
-                        block-test threshold, tile rectangle (2 words as in rect), emission
-                        position of the item's first instance) */
+                        block-test threshold, tile rectangle (2 words as in rect), first
+                        backward-record position of the item relative to item_off) */
   size_t radii;      /* geom: int32[P] */
   size_t rect;       /* geom: uint32x2[P] (x0|y0<<16, x1|y1<<16) */
   size_t counts;     /* geom: uint32[P] tiles touched */
   size_t perm;       /* geom: uint32[P] Gaussian ids in (depth, id) order */
   size_t offsets;    /* geom: uint32[P] exclusive scan of counts in depth order */
-  size_t item_off;   /* 0: each item's record offset is the last word of its splat record */
+  size_t item_off;   /* geom: uint32[ceil(P/256)] first backward-record position of items
+                        256 k .. 256 k + 255 (the tile counts scanned in item order); an
+                        item's own position adds the last word of its splat record */
   size_t sorted_ids; /* binning: uint32[capacity] Gaussian id per sorted instance */
   size_t sorted_tiles; /* binning: uint32[capacity] tile id per sorted instance */
   size_t ranges;     /* image: uint32x2[T] per-tile [start,end) */

@@ -132,6 +132,7 @@ struct BlendBwdArgs {
   const uint2* ranges;
   const uint32_t* ids;
   const SplatRec* splats;   // per item: splat, block-test threshold, rectangle, record offset
+  const uint32_t* rec_base; // per PRE_TILE items: first record position (the splat record's offset adds to it)
   unsigned long long* stats;  // optional [3]: evaluated pairs, contributing pairs, loaded instances
   const uint32_t* n_contrib;
   const float* final_T;
@@ -275,7 +276,7 @@ void k_blend_bwd(BlendBwdArgs a) {
       cur = r.s;
       qcur = r.q1;
       rect_cur = make_uint2(r.rect0, r.rect1);
-      off_cur = r.item_off;
+      off_cur = r.item_off + a.rec_base[id0 / PRE_TILE];
       id_n = k >= 64 ? id_at(k - 64) : 0u;
     }
     for (int cend = nproc; cend > 0; cend -= 64) {
@@ -290,7 +291,7 @@ void k_blend_bwd(BlendBwdArgs a) {
         cur = r.s;
         qcur = r.q1;
         rect_cur = make_uint2(r.rect0, r.rect1);
-        off_cur = r.item_off;
+        off_cur = r.item_off + a.rec_base[idc / PRE_TILE];
         const int k2 = cstart - 128 + lane;
         id_n = k2 >= 0 ? id_at(k2) : 0u;
       }
@@ -662,7 +663,7 @@ W2W_DEV void bwd_tile(const BlendBwdArgs& a, BwdShared& sh, const FrameDesc& fd,
   // rectangle, record offset)
   const v4f* const rec4 = reinterpret_cast<const v4f*>(a.splats);
   const v4u* const rec4u = reinterpret_cast<const v4u*>(a.splats);
-  uint32_t id_n;
+  uint32_t id_n, base;
   v4f ra, rb, rc;
   v4u rk;
   {
@@ -673,6 +674,7 @@ W2W_DEV void bwd_tile(const BlendBwdArgs& a, BwdShared& sh, const FrameDesc& fd,
     rb = rec4[4 * (size_t)id0 + 1];
     rc = rec4[4 * (size_t)id0 + 2];
     rk = rec4u[4 * (size_t)id0 + 3];
+    base = a.rec_base[id0 / PRE_TILE];
     id_n = k >= 64 ? id_at(k - 64) : 0u;
   }
   for (int cend = wmax; cend > 0; cend -= 64) {
@@ -681,13 +683,14 @@ W2W_DEV void bwd_tile(const BlendBwdArgs& a, BwdShared& sh, const FrameDesc& fd,
     const uint32_t kq_bits = rk.x;
     const float kq = __uint_as_float(kq_bits);
     const uint2 srect = make_uint2(rk.y, rk.z);
-    const uint32_t soff = rk.w;
+    const uint32_t soff = rk.w + base;
     if (cstart > 0) {
       const uint32_t idc = id_n < P ? id_n : 0u;
       ra = rec4[4 * (size_t)idc];
       rb = rec4[4 * (size_t)idc + 1];
       rc = rec4[4 * (size_t)idc + 2];
       rk = rec4u[4 * (size_t)idc + 3];
+      base = a.rec_base[idc / PRE_TILE];
       const int k2 = cstart - 128 + lane;
       id_n = k2 >= 0 ? id_at(k2) : 0u;
     }
@@ -821,6 +824,7 @@ struct PreBwdArgs {
   const uint8_t* flag;       // [capacity]
   uint32_t capacity;
   const SplatRec* splats;    // per item: mean2D, conic and opacity of the forward
+  const uint32_t* rec_base;  // per PRE_TILE items: first record position (plus the splat record's offset)
   const uint32_t* item_frame;  // per item: its frame
   float* item_grad;          // per item [8]: dL/dconic a, b, c, dL/dopacity, dL/ddepth, 3 unused
   float* dL_dmeans2D;    // per item [3]  (NDC units, z = 0)
@@ -852,7 +856,7 @@ __global__ __launch_bounds__(256) void k_item_grads(PreBwdArgs a) {
 #pragma unroll
     for (int f = 0; f < GR_FIELDS; f++) ga[f] = 0.0f;
     if (vis) {
-      const uint32_t k0 = a.splats[i].item_off;  // emission position of the item's first instance
+      const uint32_t k0 = a.rec_base[i / PRE_TILE] + a.splats[i].item_off;  // the item's first record
       const uint32_t k1 = min(k0 + a.counts[i], a.capacity);
       for (uint32_t kw = k0 & ~7u; kw < k1; kw += 8) {
         uint64_t f8 = *reinterpret_cast<const uint64_t*>(a.flag + kw);

@@ -124,7 +124,8 @@ struct Plan {
   size_t tid_far;                         // binning: the far sort's second id buffer
   size_t ranges_far;                      // image
   size_t tile_cost, tile_order;           // image: the backward's tile order
-  size_t rlb;                             // geom: the records' item-order scan
+  size_t rtot, rlb;                       // geom: k_preprocess's tile sums, their scan's look-back words
+  uint32_t rtiles;                        //   tiles of PRE_TILE items
   // the near tile sort ends in (L.sorted_tiles, L.sorted_ids); its other
   // buffer pair is free afterwards and takes the far slice's emission
   size_t scratch_keys() const { return (tpasses & 1) ? tkey0 : tkey1; }
@@ -178,8 +179,12 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   p.dhist = take(4 * 4 * RADIX);
   p.dstatus = take((size_t)4 * p.dtiles * RADIX * 4);
   p.slb = take((size_t)p.stiles * 4 + 4);
-  p.rlb = take((size_t)p.stiles * 4 + 4);  // look-back words of the records' item-order scan
-  L.item_off = 0;  // the offset lives in the splat record
+  // each item's first backward record: the scanned tile sums of
+  // k_preprocess's item tiles (L.item_off) plus the word in its splat record
+  p.rtiles = cdiv((uint64_t)c.items, PRE_TILE);
+  p.rtot = take((size_t)p.rtiles * 4);
+  L.item_off = take((size_t)p.rtiles * 4);
+  p.rlb = take((size_t)cdiv(p.rtiles, SCAN_TILE) * 4 + 4);
   L.item_grad = take(32 * I);
   p.sliced = c.depth_split > 0 && c.depth_split < 1000;
   p.ocount = p.boff = p.slb_far = p.open_bm = p.hint_bm = p.open_units = p.hint_sat = p.big = 0;
@@ -416,15 +421,12 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   fb.add(status, ST_WORDS, 0u, s);
   fb.add(at<uint32_t>(I, p.L.ranges), 2 * (uint64_t)c.tiles, 0u, s);
   if (items_cap > 0) {
-    fb.add(at<uint32_t>(G, p.rlb), p.stiles + 1, 0u, s);
+    fb.add(at<uint32_t>(G, p.rlb), cdiv(p.rtiles, SCAN_TILE) + 1, 0u, s);
     fb.add(at<uint32_t>(G, p.dhist), 4 * RADIX, 0u, s);
     fb.add(at<uint32_t>(G, p.slb), p.stiles + 1, 0u, s);
     fb.add(at<uint32_t>(B, p.thist), 4 * RADIX, 0u, s);
   }
-  if (hinted) {
-    fb.add(at<uint32_t>(G, p.slb_far), p.stiles + 1, 0u, s);  // the hinted scan's look-back words
-    fb.add(at<uint32_t>(G, p.open_bm), tile_words, 0u, s);
-  }
+  if (hinted) fb.add(at<uint32_t>(G, p.open_bm), tile_words, 0u, s);
   fb.launch(s);
   // the forward blend's block order: every tile, heaviest first by the last
   // forward's per-tile costs (any content is a valid order), then this
@@ -473,19 +475,21 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
       pa.hint_sat = at<uint32_t>(G, p.hint_sat);
       pa.hcount = at<uint32_t>(G, p.ocount);
     }
+    pa.tile_tot = at<uint32_t>(G, p.rtot);
+    pa.status = status;
+    pa.split_permille = p.sliced ? (uint32_t)c.depth_split : 0u;
     {
       ProfSpan ps(W2W_KID_PREPROCESS, s);
-      hipLaunchKernelGGL(k_preprocess, dim3(std::min<uint32_t>(cdiv(items_cap, 256), 8192)),
-                         dim3(256), 0, s, pa);
+      hipLaunchKernelGGL(k_preprocess, dim3(std::min<uint32_t>(p.rtiles, 8192)), dim3(PRE_TILE), 0, s, pa);
     }
     {
-      // each item's first backward record: the tile counts scanned in item
-      // order (an item's records are contiguous), into its splat record
+      // the tiles' first backward-record positions, K (every tile count
+      // summed) and the capacity-overflow flag
       ProfSpan ps(W2W_KID_SCAN, s);
-      hipLaunchKernelGGL(k_scan<false>, dim3(std::min<uint32_t>(p.stiles, 2048)), dim3(SCAN_THREADS), 0, s,
-                         at<uint32_t>(G, p.L.counts), nullptr, nullptr, at<uint32_t>(G, p.rlb) + 1,
-                         status + ST_REC_TICKET, status + ST_REC_TOTAL, nullptr, d_items, items_cap, cap,
-                         nullptr, nullptr, nullptr, at<SplatRec>(G, p.L.splats));
+      hipLaunchKernelGGL(k_scan<false>, dim3(std::min<uint32_t>(cdiv(p.rtiles, SCAN_TILE), 2048)),
+                         dim3(SCAN_THREADS), 0, s, at<uint32_t>(G, p.rtot), nullptr,
+                         at<uint32_t>(G, p.L.item_off), at<uint32_t>(G, p.rlb) + 1, status + ST_REC_TICKET,
+                         status + ST_K, status + ST_OVERFLOW, nullptr, p.rtiles, cap);
     }
     // depth sort: (depth bits, item) over the items, 4 passes, result in dkey0 / perm
     sort_passes<uint32_t>(at<uint32_t>(G, p.dkey0), at<uint32_t>(G, p.L.perm),
@@ -493,31 +497,26 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
                           at<uint32_t>(G, p.dkey1), at<uint32_t>(G, p.id1), d_items, items_cap, 4,
                           8, false, at<uint32_t>(G, p.dhist), at<uint32_t>(G, p.dstatus),
                           status + ST_DTICKET, W2W_KID_DSORT_HIST, W2W_KID_DSORT_PASS, s);
-    {
-      ProfSpan ps(W2W_KID_SCAN, s);
-      hipLaunchKernelGGL(k_scan<true>, dim3(std::min<uint32_t>(p.stiles, 2048)), dim3(SCAN_THREADS),
-                         0, s, at<uint32_t>(G, p.L.counts), at<uint32_t>(G, p.L.perm),
-                         at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.slb) + 1,
-                         status + ST_SCAN_TICKET, status + ST_K, status + ST_OVERFLOW, d_items,
-                         items_cap, cap);
-    }
     if (cap > 0) {
-      // depth-sliced binning: the near slice is binned here, the far slice
-      // after the near blend (only on the tiles it leaves open)
+      // emission offsets: the tile counts gathered in depth order and
+      // scanned.  Depth-sliced binning: the near slice is binned here, the
+      // far slice after the near blend (only on the tiles it leaves open);
+      // far items are counted on the hinted tiles only, which places them in
+      // the near lists behind the near items (depth order kept)
       const uint32_t* d_near_end = nullptr;
       const uint32_t* d_kn = status + ST_K;
+      {
+        ProfSpan ps(W2W_KID_SCAN, s);
+        hipLaunchKernelGGL(k_scan<true>, dim3(std::min<uint32_t>(p.stiles, 2048)), dim3(SCAN_THREADS),
+                           0, s, at<uint32_t>(G, p.L.counts), at<uint32_t>(G, p.L.perm),
+                           at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.slb) + 1,
+                           status + ST_SCAN_TICKET, status + (p.sliced ? ST_KNEAR_RAW : ST_K), nullptr,
+                           d_items, items_cap, cap, p.sliced ? at<uint32_t>(G, p.ocount) : nullptr,
+                           p.sliced ? status + ST_SPLIT : nullptr);
+      }
       if (p.sliced) {
-        hipLaunchKernelGGL(k_split, dim3(1), dim3(1), 0, s, table, items_cap, (uint32_t)c.depth_split,
-                           at<uint32_t>(G, p.L.offsets), status, cap);
-        // far items' hinted tiles (counted by the preprocess) in depth order
-        {
-          ProfSpan ps(W2W_KID_SCAN, s);
-          hipLaunchKernelGGL(k_scan<true>, dim3(std::min<uint32_t>(p.stiles, 2048)), dim3(SCAN_THREADS), 0, s,
-                             at<uint32_t>(G, p.ocount), at<uint32_t>(G, p.L.perm), at<uint32_t>(G, p.boff),
-                             at<uint32_t>(G, p.slb_far) + 1, status + ST_SCANH_TICKET, status + ST_KH, nullptr,
-                             d_items, items_cap, cap, status + ST_SPLIT, nullptr, nullptr);
-        }
-        hipLaunchKernelGGL(k_near_total, dim3(1), dim3(1), 0, s, status, cap);
+        hipLaunchKernelGGL(k_near_total, dim3(1), dim3(1), 0, s, table, items_cap, at<uint32_t>(G, p.L.offsets),
+                           status, cap);
         d_near_end = status + ST_SPLIT;
         d_kn = status + ST_KNEAR;
       }
@@ -535,13 +534,13 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
         ProfSpan ps(W2W_KID_EMIT_FAR, s);
         hipLaunchKernelGGL(k_emit_far, dim3(std::min<uint32_t>(cdiv(items_cap, 256), kEmitFarGrid)), dim3(256), 0, s,
                            table, (uint32_t)c.scenes, items_cap, cap, status, at<uint32_t>(G, p.L.perm),
-                           at<uint32_t>(G, p.boff), status + ST_KH, at<uint2>(G, p.L.rect),
-                           at<uint32_t>(G, p.L.item_frame), at<uint32_t>(G, p.hint_bm), nullptr, status + ST_KA,
+                           at<uint32_t>(G, p.L.offsets), status + ST_KNEAR_RAW, at<uint2>(G, p.L.rect),
+                           at<uint32_t>(G, p.L.item_frame), at<uint32_t>(G, p.hint_bm), nullptr, nullptr,
                            at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0), at<uint32_t>(B, p.thist),
                            p.tpasses, p.tlast_bits, nullptr, at<uint32_t>(G, p.big), status + ST_BIG_H);
         hipLaunchKernelGGL(k_emit_far_big, dim3(1024), dim3(256), 0, s, table, (uint32_t)c.scenes, cap,
-                           at<uint32_t>(G, p.L.perm), at<uint32_t>(G, p.boff), at<uint2>(G, p.L.rect),
-                           at<uint32_t>(G, p.L.item_frame), at<uint32_t>(G, p.hint_bm), nullptr, status + ST_KA,
+                           at<uint32_t>(G, p.L.perm), at<uint32_t>(G, p.L.offsets), at<uint2>(G, p.L.rect),
+                           at<uint32_t>(G, p.L.item_frame), at<uint32_t>(G, p.hint_bm), nullptr, nullptr,
                            at<uint32_t>(G, p.big), status + ST_BIG_H, items_cap, at<uint32_t>(B, p.tkey0),
                            at<uint32_t>(B, p.tid0), at<uint32_t>(B, p.thist), p.tpasses, p.tlast_bits, nullptr);
       }
@@ -691,6 +690,7 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
   ba.ranges = at<uint2>(I, p.L.ranges);
   ba.ids = at<uint32_t>(B, p.L.sorted_ids);
   ba.splats = at<SplatRec>(G, p.L.splats);
+  ba.rec_base = at<uint32_t>(G, p.L.item_off);
   ba.rec = rec;
   ba.flag = flag;
   ba.stats = g_stats ? g_stats + 3 : nullptr;
@@ -716,6 +716,7 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
   pa.flag = flag;
   pa.capacity = (uint32_t)c.capacity;
   pa.splats = at<SplatRec>(G, p.L.splats);
+  pa.rec_base = at<uint32_t>(G, p.L.item_off);
   pa.item_frame = at<uint32_t>(G, p.L.item_frame);
   pa.item_grad = at<float>(G, p.L.item_grad);
   pa.dL_dmeans2D = dL_dmeans2D; pa.dL_dcolors = dL_dcolors; pa.dL_dopacity = dL_dopacity;

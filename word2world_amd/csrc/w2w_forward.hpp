@@ -57,13 +57,12 @@ enum StatusWord : int {
   ST_BLEND_TICKET = 26,
   ST_BWD_TICKET = 27,
   ST_SCANB_TICKET = 28,
-  ST_SCANH_TICKET = 29,
   ST_BLENDB_TICKET = 30,
   ST_ORDER_N = 33,     // backward: tiles with work, in the heaviest-first order
   ST_COST_HIST = 34,   //   tiles per cost bucket (kCostBuckets words)
   ST_COST_CURSOR = 67, //   slots taken per cost bucket (kCostBuckets words)
-  ST_REC_TICKET = 100, // the records' item-order scan
-  ST_REC_TOTAL = 101,
+  ST_REC_TICKET = 100, // the scan of k_preprocess's tile sums
+  ST_KNEAR_RAW = 101,  // depth-sliced binning: the near lists' instances before the capacity cap
   ST_WORDS = 128,
 };
 
@@ -191,19 +190,38 @@ struct PreArgs {
   // and each item's hinted tiles out
   const uint32_t* hint_sat;
   uint32_t* hcount;
+  // each tile of PRE_TILE items: its tile counts summed (scanned after this
+  // kernel into the tiles' first backward-record positions)
+  uint32_t* tile_tot;
+  // depth-sliced binning: the near slice's depth positions are written to
+  // status[ST_SPLIT] when split_permille > 0
+  uint32_t* status;
+  uint32_t split_permille;
 };
 
-// One thread per item: the splat record (its record offset is filled in by
-// k_scan over the tile counts in item order), radius, rectangle, tile count,
-// depth key.
-__global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
+constexpr uint32_t PRE_TILE = 256;
+
+// One thread per item: the splat record, radius, rectangle, tile count,
+// depth key.  A workgroup takes tiles of PRE_TILE consecutive items and scans
+// their tile counts: the record holds the exclusive prefix inside the tile,
+// and tile_tot the tile's sum (saturating at LB_VAL), so an item's first
+// backward-record position (an item's records are contiguous, in item
+// order) is the scanned tile_tot of its tile plus the record's word.  Every
+// tile of the capacity is written (zero past the batch's items).
+__global__ __launch_bounds__(PRE_TILE) void k_preprocess(PreArgs a) {
+  __shared__ uint32_t s_tmp[PRE_TILE / 64];
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
   const uint32_t n_items = min(hdr->n_items, a.items_cap);
   const uint32_t n_scenes = min(hdr->n_scenes, a.scenes_cap);
   const SceneDesc* scenes = table_scenes(a.table);
   const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_items; i += stride) {
+  const int tid = (int)threadIdx.x;
+  const uint32_t ntiles = (a.items_cap + PRE_TILE - 1) / PRE_TILE;
+  if (a.split_permille && blockIdx.x == 0 && tid == 0)
+    a.status[ST_SPLIT] = (uint32_t)min<uint64_t>(((uint64_t)n_items * a.split_permille + 999u) / 1000u, n_items);
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint32_t i = tile * PRE_TILE + (uint32_t)tid;
+    const bool live = i < n_items;
     ItemRef ir{};
     Cam c;
     float p[3] = {0.0f, 0.0f, 0.0f};
@@ -215,7 +233,7 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
     uint32_t x0 = 0, y0 = 0, x1 = 0, y1 = 0;
     int32_t radius = 0;
     uint32_t cnt = 0;
-    {
+    if (live) {
     ir = item_ref(scenes, n_scenes, i);
     const FrameDesc& fd = frames[ir.frame];
     load_cam(c, fd);
@@ -273,7 +291,7 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
     }
     SplatRec rec;
     uint32_t hcnt = 0, dkey = CULLED_DEPTH_KEY;
-    {
+    if (live) {
     const SceneDesc& sd = scenes[ir.scene];
     const FrameDesc& fd = frames[ir.frame];
     // SH colour, direction normalize(mean - campos), +0.5, unclamped
@@ -335,15 +353,20 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
     if (a.hint_sat) hcnt = cnt ? sat_rect(a.hint_sat, fd.tile_off, (uint32_t)fd.tiles_x, x0, x1, y0, y1) : 0u;
     dkey = keep ? __float_as_uint(g.ndcz) : CULLED_DEPTH_KEY;
     }
-    rec.item_off = 0u;  // k_scan over the counts fills it in
-    a.splats[i] = rec;
-    a.radii[i] = radius;
-    a.rect[i] = make_uint2(rec.rect0, rec.rect1);
-    a.counts[i] = cnt;
-    if (a.hint_sat) a.hcount[i] = hcnt;
-    a.dkey[i] = dkey;
-    a.ids[i] = i;
-    a.item_frame[i] = ir.frame;
+    uint32_t btot;
+    const uint32_t bexcl = block_excl_scan_256(cnt, s_tmp, &btot);
+    if (tid == 0) a.tile_tot[tile] = min(btot, LB_VAL);
+    if (live) {
+      rec.item_off = bexcl;
+      a.splats[i] = rec;
+      a.radii[i] = radius;
+      a.rect[i] = make_uint2(rec.rect0, rec.rect1);
+      a.counts[i] = cnt;
+      if (a.hint_sat) a.hcount[i] = hcnt;
+      a.dkey[i] = dkey;
+      a.ids[i] = i;
+      a.item_frame[i] = ir.frame;
+    }
   }
 }
 
@@ -355,9 +378,9 @@ __global__ __launch_bounds__(256) void k_preprocess(PreArgs a) {
 // Tiles are large (16384 values: few look-back links), loads and stores are
 // striped over the workgroup (coalesced) and transposed through LDS so that
 // each thread scans 16 consecutive values, and one wave looks back over 64
-// predecessors per step.  Optional: positions below *d_begin count as 0;
-// scatter[perm[i]] = out[i] (GATHER only); with *gate == 0 the scan is
-// skipped (total 0, out untouched).
+// predecessors per step.  Optional (GATHER only): with in_far, positions at
+// or past *d_split take in_far[perm[i]] instead of in[perm[i]]; with
+// *gate == 0 the scan is skipped (total 0, out untouched).
 constexpr int SCAN_THREADS = 1024;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
@@ -368,8 +391,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(
     uint32_t* __restrict__ out, uint32_t* __restrict__ lb, uint32_t* __restrict__ ticket,
     uint32_t* __restrict__ total_out, uint32_t* __restrict__ overflow_out,
     const uint32_t* __restrict__ d_n, uint32_t n_cap, uint32_t capacity,
-    const uint32_t* __restrict__ d_begin = nullptr, uint32_t* __restrict__ scatter = nullptr,
-    const uint32_t* __restrict__ gate = nullptr, SplatRec* __restrict__ rec_out = nullptr) {
+    const uint32_t* __restrict__ in_far = nullptr, const uint32_t* __restrict__ d_split = nullptr,
+    const uint32_t* __restrict__ gate = nullptr) {
   __shared__ uint32_t s_v[SCAN_TILE + SCAN_TILE / 32];  // one pad word per 32: conflict-free transposes
   __shared__ uint32_t s_tmp[SCAN_THREADS / 64];
   __shared__ uint32_t s_tile, s_excl, s_tot;
@@ -382,7 +405,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(
     return;
   }
   const uint32_t n = sort_count(d_n, n_cap);
-  const uint32_t begin = d_begin ? *d_begin : 0u;
+  const uint32_t split = (GATHER && in_far) ? *d_split : 0xFFFFFFFFu;
   const uint32_t ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
   auto pad = [](int i) { return i + (i >> 5); };
   for (;;) {
@@ -401,13 +424,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(
 #pragma unroll
       for (int j = 0; j < SCAN_ITEMS; j++) {
         const uint32_t idx = tb + j * SCAN_THREADS + tid;
-        s_v[pad(j * SCAN_THREADS + tid)] = (pr[j] != 0xFFFFFFFFu && idx >= begin) ? in[pr[j]] : 0u;
+        s_v[pad(j * SCAN_THREADS + tid)] = pr[j] != 0xFFFFFFFFu ? (idx < split ? in : in_far)[pr[j]] : 0u;
       }
     } else {
 #pragma unroll
       for (int j = 0; j < SCAN_ITEMS; j++) {
         const uint32_t idx = tb + j * SCAN_THREADS + tid;
-        s_v[pad(j * SCAN_THREADS + tid)] = (idx < n && idx >= begin) ? in[idx] : 0u;
+        s_v[pad(j * SCAN_THREADS + tid)] = idx < n ? in[idx] : 0u;
       }
     }
     __syncthreads();
@@ -442,14 +465,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; j++) {
       const uint32_t idx = tb + j * SCAN_THREADS + tid;
-      if (idx < n) {
-        const uint32_t v = e0 + s_v[pad(j * SCAN_THREADS + tid)];
-        if (out) out[idx] = v;
-        if (rec_out) rec_out[idx].item_off = v;
-        if constexpr (GATHER) {
-          if (scatter) scatter[pr[j]] = v;
-        }
-      }
+      if (idx < n) out[idx] = e0 + s_v[pad(j * SCAN_THREADS + tid)];
     }
     __syncthreads();  // s_v and s_tile are rewritten by the next tile
   }
@@ -475,18 +491,6 @@ W2W_DEV uint32_t tileset_count(const uint32_t* __restrict__ a, const uint32_t* _
     n -= k;
   }
   return c;
-}
-
-// Depth-sliced binning: the near slice is the first S = ceil(P * permille /
-// 1000) depth positions; K_A (its instances) is the scan's offset at S.
-__global__ void k_split(const void* __restrict__ table, uint32_t items_cap, uint32_t permille,
-                        const uint32_t* __restrict__ offs, uint32_t* __restrict__ status, uint32_t capacity) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const uint32_t P = min(static_cast<const BatchHeader*>(table)->n_items, items_cap);
-  const uint32_t S = (uint32_t)min<uint64_t>(((uint64_t)P * permille + 999u) / 1000u, P);
-  const uint32_t KA = S < P ? offs[S] : status[ST_K];
-  status[ST_SPLIT] = S;
-  status[ST_KA] = min(KA, capacity);
 }
 
 // Inclusive 2-D prefix counts of a tile set per frame, indexed like the
@@ -553,10 +557,23 @@ __global__ __launch_bounds__(256) void k_tile_sat(const void* __restrict__ table
   }
 }
 
-// The near lists' total: the near slice's instances and the far items'
-// instances on hinted tiles.
-__global__ void k_near_total(uint32_t* __restrict__ status, uint32_t capacity) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) status[ST_KNEAR] = min(status[ST_KA] + status[ST_KH], capacity);
+// Depth-sliced binning: the near slice is the first S = ceil(P * permille /
+// 1000) depth positions (set by k_preprocess).  One scan in depth order took
+// each near item's tile count and each far item's hinted-tile count, so its
+// offsets place both in the near lists; its total is the near lists' size.
+// Split into the near slice's instances (the offset at S) and the far
+// items' hinted instances.
+__global__ void k_near_total(const void* __restrict__ table, uint32_t items_cap,
+                             const uint32_t* __restrict__ offs, uint32_t* __restrict__ status,
+                             uint32_t capacity) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t P = min(static_cast<const BatchHeader*>(table)->n_items, items_cap);
+  const uint32_t S = status[ST_SPLIT];
+  const uint32_t raw = status[ST_KNEAR_RAW];
+  const uint32_t KA = S < P ? offs[S] : raw;
+  status[ST_KA] = min(KA, capacity);
+  status[ST_KH] = raw - KA;
+  status[ST_KNEAR] = min(raw, capacity);
 }
 
 // Per depth position of the far slice (0 in the near slice): how many tiles
