@@ -97,7 +97,9 @@ def test_dense_faint_scene_backward_vs_autograd(gpu):
     """A dense, faint scene (8,000 large Gaussians of opacity 0.03 at 64^2:
     up to ~500 contributors per pixel, tiles of 2,000 to 83,000 contributing
     pairs) against float64 autograd and the oracle's hand backward: long
-    per-pixel walks, the heaviest tiles first in the backward's order."""
+    per-pixel walks, the heaviest tiles first in the backward's order, and
+    tiles above 2^15 pairs walked by a whole workgroup (one wave per 8x8
+    block, sums merged in LDS)."""
     arrs = list(scene_np(8000, 1, 41, 3.0))
     arrs[3] = np.full_like(arrs[3], 0.03)
     cam = orbit_views(64, 64)[1]

@@ -148,6 +148,9 @@ struct BlendBwdArgs {
   // the tiles with backward work, heaviest first (k_tile_order), and their count
   const uint32_t* order;
   const uint32_t* n_order;
+  // per tile: contributing (pixel, splat) pairs counted by the forward blend
+  // (tile edge 16: tiles from kCoopCost up are walked by a whole workgroup)
+  const uint32_t* tile_cost;
 };
 
 // Backward blend (backward.cu:396-520 semantics, with our forward's
@@ -765,9 +768,164 @@ W2W_DEV void bwd_tile(const BlendBwdArgs& a, BwdShared& sh, const FrameDesc& fd,
   }
 }
 
+// Heavy tiles of edge 16 (at least kCoopCost contributing pairs): a single
+// wave's walk of such a tile outlasts the rest of the kernel, so the
+// workgroup walks it together.  Wave w holds 8x8 block w (lane l: pixel
+// (l mod 8, l / 8)); the four waves step through the same chunks of the
+// tile's list (down from the tile's largest last contributor), each sums an
+// instance's ten fields over its own lanes, and after every chunk the
+// threads add the waves' sums in wave order and write the records.  Fixed
+// order: deterministic (the sums round differently from the one-wave walk,
+// which adds a lane's four blocks first; whether a tile is heavy depends
+// only on the tile's own contributors).
+constexpr uint32_t kCoopCost = 1u << 15;
+
+struct BwdCoopShared {
+  float part[4][GR_FIELDS][64];  // per wave, per field, per instance of the chunk
+  uint64_t cm[4];                // instances with a sum, per wave
+  int wmax[4];                   // each block's largest last contributor
+};
+
+template <bool AUX, bool STATS>
+W2W_DEV void bwd_tile_coop(const BlendBwdArgs& a, BwdShared& sh, BwdCoopShared& cs, const FrameDesc& fd,
+                           uint32_t tile, uint32_t P, int lane, int wv, uint32_t& n_eval, uint32_t& n_con,
+                           uint32_t& n_loaded) {
+  constexpr int TS = 16;
+  v4f(*const stage)[64] = sh.stage[wv];
+  const int tid = wv * 64 + lane;
+  const int row = lane >> 4;
+  const bool row_tail = (lane & 15) == 15;
+  const int fa = (row & 1) ? (row >> 1) + 2 : (row >> 1);  // wave_sum10 rows -> fields 0, 2, 1, 3
+  const int lt = (int)(tile - fd.tile_off);
+  const int tx = lt % fd.tiles_x, ty = lt / fd.tiles_x;
+  const uint2 range = a.ranges[tile];
+  const uint2 range_far = a.ranges_far ? a.ranges_far[tile] : make_uint2(0u, 0u);
+  const int n_near = (int)(range.y - range.x);
+  const int n_list = n_near + (int)(range_far.y - range_far.x);
+  auto id_at = [&](int k) -> uint32_t {
+    return k < n_near ? a.ids[range.x + k] : a.ids_far[range_far.x + (uint32_t)(k - n_near)];
+  };
+  const int bx0 = tx * TS + (wv & 1) * kUnit, by0 = ty * TS + (wv >> 1) * kUnit;
+  BwdPix q;
+  bwd_pix_load<AUX>(a, fd, bx0 + (lane & 7), by0 + (lane >> 3), n_list, q);
+  int wm = q.last;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) wm = max(wm, __shfl_xor(wm, off, 64));
+  wm = __builtin_amdgcn_readfirstlane(wm);
+  if (lane == 0) cs.wmax[wv] = wm;
+  __syncthreads();
+  const int wmax = max(max(cs.wmax[0], cs.wmax[1]), max(cs.wmax[2], cs.wmax[3]));
+  const float pxf = (float)(bx0 + (lane & 7)), pyf = (float)(by0 + (lane >> 3));
+  const float x0 = (float)bx0, y0 = (float)by0;
+  const float x1 = x0 + (float)(kUnit - 1), y1 = y0 + (float)(kUnit - 1);
+  const v4f* const rec4 = reinterpret_cast<const v4f*>(a.splats);
+  const v4u* const rec4u = reinterpret_cast<const v4u*>(a.splats);
+  uint32_t id_n = 0, base = 0;
+  v4f ra = {}, rb = {}, rc = {};
+  v4u rk = {};
+  if (wmax > 0) {
+    const int k = wmax - 64 + lane;
+    id_n = k >= 0 ? id_at(k) : 0u;
+    const uint32_t id0 = id_n < P ? id_n : 0u;
+    ra = rec4[4 * (size_t)id0];
+    rb = rec4[4 * (size_t)id0 + 1];
+    rc = rec4[4 * (size_t)id0 + 2];
+    rk = rec4u[4 * (size_t)id0 + 3];
+    base = a.rec_base[id0 / PRE_TILE];
+    id_n = k >= 64 ? id_at(k - 64) : 0u;
+  }
+  for (int cend = wmax; cend > 0; cend -= 64) {  // the same trip count in every wave
+    const int cstart = cend - 64;
+    const v4f sa = ra, sb = rb, sc = rc;
+    const float kq = __uint_as_float(rk.x);
+    const uint2 srect = make_uint2(rk.y, rk.z);
+    const uint32_t soff = rk.w + base;
+    if (cstart > 0) {
+      const uint32_t idc = id_n < P ? id_n : 0u;
+      ra = rec4[4 * (size_t)idc];
+      rb = rec4[4 * (size_t)idc + 1];
+      rc = rec4[4 * (size_t)idc + 2];
+      rk = rec4u[4 * (size_t)idc + 3];
+      base = a.rec_base[idc / PRE_TILE];
+      const int k2 = cstart - 128 + lane;
+      id_n = k2 >= 0 ? id_at(k2) : 0u;
+    }
+    if constexpr (STATS) {
+      if (wv == 0) n_loaded += (uint32_t)min(64, cend);
+    }
+    const int kk_l = cstart + lane;
+    const bool touch = kk_l >= 0 && kk_l < wm && extent_touches(sa.x, sa.y, unpack_extent(sc.w), x0, x1, y0, y1) &&
+                       ellipse_touches_q(sa.x, sa.y, sa.z, sa.w, sb.x, kq, x0, x1, y0, y1);
+    uint64_t m = __ballot(touch);
+    uint64_t cm = 0;
+    const uint32_t rx0 = srect.x & 0xFFFFu, ry0 = srect.x >> 16, rx1 = srect.y & 0xFFFFu;
+    const uint32_t pos_l = soff + ((uint32_t)ty - ry0) * (rx1 - rx0) + ((uint32_t)tx - rx0);
+    if (m) {
+      stage[0][lane] = sa;  // read back by this wave only: LDS keeps its order
+      stage[1][lane] = sb;
+      stage[2][lane] = sc;
+      __builtin_amdgcn_wave_barrier();
+      int jn = 63 - __builtin_clzll(m);
+      m &= ~(1ull << jn);
+      v4f na = stage[0][jn], nb = stage[1][jn], nc = stage[2][jn];
+      for (;;) {
+        const int j = jn;
+        const v4f qa = na, qb = nb, qc = nc;
+        jn = m ? 63 - __builtin_clzll(m) : -1;
+        if (jn >= 0) {
+          m &= ~(1ull << jn);
+          na = stage[0][jn];
+          nb = stage[1][jn];
+          nc = stage[2][jn];
+        }
+        float F[GR_FIELDS];
+#pragma unroll
+        for (int f = 0; f < GR_FIELDS; f++) F[f] = 0.0f;
+        if (bwd_pair<AUX, STATS>(qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w, qc.x, qc.y, qc.z, pxf, pyf,
+                                 cstart + j, q, F, n_eval, n_con)) {
+          float s0, s1, s2;
+          wave_sum10(F[0], F[1], F[2], F[3], F[4], F[5], F[6], F[7], F[8], F[9], s0, s1, s2);
+          if (row_tail) {
+            cs.part[wv][fa][j] = s0;
+            cs.part[wv][4 + fa][j] = s1;
+            if ((row & 1) == 0) cs.part[wv][8 + (row >> 1)][j] = s2;
+          }
+          cm |= 1ull << j;
+        }
+        if (jn < 0) break;
+      }
+    }
+    if (lane == 0) cs.cm[wv] = cm;
+    __syncthreads();  // also orders this chunk's staged reads before the next chunk's writes
+    // instance j's record: its waves' sums added in wave order, five 8-byte
+    // pairs (pair p: fields 2 (p mod 5), +1 of instance p / 5)
+    const uint64_t any = cs.cm[0] | cs.cm[1] | cs.cm[2] | cs.cm[3];
+    for (int pr = tid; pr < 64 * (GR_FIELDS / 2); pr += 256) {  // uniform per wave
+      const int j = pr / (GR_FIELDS / 2), fp = pr - j * (GR_FIELDS / 2);
+      const uint32_t pos = (uint32_t)__shfl((int)pos_l, j, 64);
+      if (((any >> j) & 1ull) && pos < a.capacity) {
+        float v0 = 0.0f, v1 = 0.0f;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          if ((cs.cm[v] >> j) & 1ull) {
+            v0 += cs.part[v][2 * fp][j];
+            v1 += cs.part[v][2 * fp + 1][j];
+          }
+        }
+        *reinterpret_cast<float2*>(a.rec + (size_t)pos * GR_FIELDS + 2 * fp) = make_float2(v0, v1);
+        if (fp == 0) a.flag[pos] = 1;
+      }
+    }
+    __syncthreads();  // the sums and masks are rewritten by the next chunk
+  }
+  __syncthreads();  // cs.wmax is rewritten by the next tile
+}
+
 template <int TS, bool AUX, bool STATS>
 __global__ __launch_bounds__(256) void k_blend_bwd_tile(BlendBwdArgs a) {
   __shared__ BwdShared sh;
+  __shared__ BwdCoopShared cs;
+  __shared__ uint32_t s_claim;
   const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
   const uint32_t Tn = min(hdr->n_tiles, a.tiles_cap);
   const uint32_t P = min(hdr->n_items, a.items_cap);
@@ -776,10 +934,38 @@ __global__ __launch_bounds__(256) void k_blend_bwd_tile(BlendBwdArgs a) {
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const uint32_t n_order = min(*a.n_order, Tn);
   uint32_t n_eval = 0, n_con = 0, n_loaded = 0;
+  // heavy tiles lead the order: the workgroup claims entries together while
+  // they are heavy; the first light one goes to wave 0, and from then on
+  // every wave claims its own
+  uint32_t pending = 0xffffffffu;
+  if constexpr (TS == 16) {
+    for (;;) {
+      if (threadIdx.x == 0) s_claim = atomicAdd(a.ticket, 1u);
+      __syncthreads();
+      const uint32_t t = s_claim;
+      __syncthreads();  // s_claim is rewritten by the next claim
+      if (t >= n_order) break;
+      const uint32_t e = a.order[t];
+      const uint32_t tile = e & kOrderTile;
+      if (tile >= Tn) continue;
+      if (a.tile_cost[tile] < kCoopCost) {
+        pending = t;
+        break;
+      }
+      __builtin_amdgcn_s_setprio(3);
+      const FrameDesc& fd = frames[tile_frame(frames, hdr->n_frames, tile)];
+      bwd_tile_coop<AUX, STATS>(a, sh, cs, fd, tile, P, lane, wv, n_eval, n_con, n_loaded);
+    }
+  }
   for (;;) {
     uint32_t t = 0;
-    if (lane == 0) t = atomicAdd(a.ticket, 1u);
-    t = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)t, 0, 64));
+    if (wv == 0 && pending != 0xffffffffu) {
+      t = pending;
+      pending = 0xffffffffu;
+    } else {
+      if (lane == 0) t = atomicAdd(a.ticket, 1u);
+      t = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)t, 0, 64));
+    }
     if (t >= n_order) break;
     const uint32_t e = a.order[t];
     const uint32_t tile = e & kOrderTile;

@@ -533,7 +533,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
       if (p.sliced) {
         ProfSpan ps(W2W_KID_EMIT_FAR, s);
         hipLaunchKernelGGL(k_emit_far, dim3(std::min<uint32_t>(cdiv(items_cap, 256), kEmitFarGrid)), dim3(256), 0, s,
-                           table, (uint32_t)c.scenes, items_cap, cap, status, at<uint32_t>(G, p.L.perm),
+                           table, (uint32_t)c.scenes, tiles_cap, items_cap, cap, status, at<uint32_t>(G, p.L.perm),
                            at<uint32_t>(G, p.L.offsets), status + ST_KNEAR_RAW, at<uint2>(G, p.L.rect),
                            at<uint32_t>(G, p.L.item_frame), at<uint32_t>(G, p.hint_bm), nullptr, nullptr,
                            at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0), at<uint32_t>(B, p.thist),
@@ -618,7 +618,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   {
     ProfSpan ps(W2W_KID_EMIT_FAR, s);
     hipLaunchKernelGGL(k_emit_far, dim3(std::min<uint32_t>(cdiv(items_cap, 256), kEmitFarGrid)), dim3(256), 0, s,
-                       table, (uint32_t)c.scenes, items_cap, cap, status, at<uint32_t>(G, p.L.perm),
+                       table, (uint32_t)c.scenes, tiles_cap, items_cap, cap, status, at<uint32_t>(G, p.L.perm),
                        at<uint32_t>(G, p.boff), status + ST_KB, at<uint2>(G, p.L.rect),
                        at<uint32_t>(G, p.L.item_frame), open_bm, hint_bm, nullptr,
                        at<uint32_t>(B, p.scratch_keys()), at<uint32_t>(B, p.scratch_ids()), at<uint32_t>(B, p.thist),
@@ -701,6 +701,7 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
   ba.ids_far = p.sliced ? at<uint32_t>(B, p.far_ids()) : nullptr;
   ba.order = at<uint32_t>(I, p.tile_order);
   ba.n_order = status + ST_ORDER_N;
+  ba.tile_cost = at<uint32_t>(I, p.tile_cost);
   switch (c.tile) {
     case 8: launch_blend_bwd<8>(ba, (uint32_t)c.tiles, s); break;
     case 16: launch_blend_bwd<16>(ba, (uint32_t)c.tiles, s); break;

@@ -610,6 +610,7 @@ __global__ __launch_bounds__(256) void k_count_tiles(const void* __restrict__ ta
 }
 
 constexpr uint32_t kFarSerial = 32;
+constexpr uint32_t kSetLds = 4096;  // tile-set words k_emit_far keeps in LDS (131072 tiles)
 
 // Far items' instances on the tiles of a set (a minus b): one thread per far
 // depth position (most have none and stop at once), the item's member tiles
@@ -621,7 +622,7 @@ constexpr uint32_t kFarSerial = 32;
 // (its depth position appended to big[], *n_big counts them): one thread
 // would hold up its wave.
 __global__ __launch_bounds__(256) void k_emit_far(const void* __restrict__ table, uint32_t scenes_cap,
-                                                  uint32_t items_cap, uint32_t capacity,
+                                                  uint32_t tiles_cap, uint32_t items_cap, uint32_t capacity,
                                                   const uint32_t* __restrict__ status,
                                                   const uint32_t* __restrict__ perm,
                                                   const uint32_t* __restrict__ coff,
@@ -638,10 +639,18 @@ __global__ __launch_bounds__(256) void k_emit_far(const void* __restrict__ table
                                                   uint32_t* __restrict__ big, uint32_t* __restrict__ n_big) {
   if (gate && *gate == 0u) return;
   __shared__ uint32_t s_hist[4][256];
+  __shared__ uint32_t s_set[kSetLds];
+  const BatchHeader* hdr = static_cast<const BatchHeader*>(table);
+  // the tile set in LDS when it fits (the walk below reads a word per row
+  // of each item's rectangle, one dependent load after another)
+  const uint32_t set_words = (min(hdr->n_tiles, tiles_cap) + 31u) / 32u;
+  const bool in_lds = set_words <= kSetLds;
   for (int k = threadIdx.x; k < 4 * 256; k += blockDim.x) (&s_hist[0][0])[k] = 0u;
+  if (in_lds)
+    for (uint32_t k = threadIdx.x; k < set_words; k += blockDim.x) s_set[k] = tileset_word(set_a, set_b, k);
   __syncthreads();
   const FrameDesc* frames = table_frames(table, scenes_cap);
-  const uint32_t P = min(static_cast<const BatchHeader*>(table)->n_items, items_cap);
+  const uint32_t P = min(hdr->n_items, items_cap);
   const uint32_t S = status[ST_SPLIT];
   const uint32_t base = d_base ? *d_base : 0u;
   const uint32_t total = *d_total;
@@ -662,7 +671,7 @@ __global__ __launch_bounds__(256) void k_emit_far(const void* __restrict__ table
       uint32_t pos = fd.tile_off + ty * (uint32_t)fd.tiles_x + x0, rem = x1 - x0;
       while (rem > 0 && o < oe) {
         const uint32_t sh = pos & 31u, k = min(32u - sh, rem);
-        uint32_t m = tileset_word(set_a, set_b, pos >> 5) >> sh;
+        uint32_t m = (in_lds ? s_set[pos >> 5] : tileset_word(set_a, set_b, pos >> 5)) >> sh;
         if (k < 32u) m &= (1u << k) - 1u;
         while (m && o < oe) {
           const uint32_t key = pos + (uint32_t)__builtin_ctz(m);
