@@ -95,7 +95,7 @@ typedef struct W2WLayout {
                         256 k .. 256 k + 255 (the tile counts scanned in item order); an
                         item's own position adds the last word of its splat record */
   size_t sorted_ids; /* binning: uint32[capacity] Gaussian id per sorted instance */
-  size_t sorted_tiles; /* binning: uint32[capacity] tile id per sorted instance */
+  size_t sorted_tiles; /* binning: tile id per sorted instance, tile_key_bytes wide */
   size_t ranges;     /* image: uint32x2[T] per-tile [start,end) */
   size_t n_contrib;  /* image: uint32[H*W] 1-based last contributor */
   size_t final_T;    /* image: float[H*W] final transmittance */
@@ -104,6 +104,7 @@ typedef struct W2WLayout {
   size_t grad_rec;   /* binning: float[capacity][10] per-instance backward partials */
   size_t grad_flag;  /* binning: uint8[capacity] 1 where grad_rec holds a partial */
   size_t item_grad;  /* geom: float[items][8] per-item record sums of the backward */
+  size_t tile_key_bytes; /* 2 when the batch has at most 65536 tiles (uint16 keys), else 4 */
 } W2WLayout;
 
 /* Capacities of a batched workspace: every grid is sized by these, and every

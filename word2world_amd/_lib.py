@@ -37,7 +37,7 @@ class W2WLayout(ctypes.Structure):
     _fields_ = [(n, ctypes.c_size_t) for n in (
         "geom_bytes", "binning_bytes", "image_bytes", "splats", "radii", "rect", "counts", "perm",
         "offsets", "item_off", "sorted_ids", "sorted_tiles", "ranges", "n_contrib", "final_T",
-        "item_frame", "table", "grad_rec", "grad_flag", "item_grad")]
+        "item_frame", "table", "grad_rec", "grad_flag", "item_grad", "tile_key_bytes")]
 
 
 class W2WCaps(ctypes.Structure):

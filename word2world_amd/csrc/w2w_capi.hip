@@ -116,6 +116,7 @@ struct Plan {
   W2WCaps c;
   uint32_t dtiles, stiles, ctiles;
   int tpasses, tlast_bits;
+  bool key16;  // tile keys of 16 bits (at most 65536 tiles)
   size_t status, table, dkey0, dkey1, id1, dhist, dstatus, slb;  // geom
   size_t tkey0, tkey1, tid0, tid1, thist, tstatus;               // binning
   // depth-sliced binning (c.depth_split in 1..999)
@@ -161,6 +162,7 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   const int tb = tile_bits((uint64_t)c.tiles);
   p.tpasses = (tb + 7) / 8;
   p.tlast_bits = tb - 8 * (p.tpasses - 1);
+  p.key16 = tb <= 16;
   size_t o = 0;
   auto take = [&](size_t bytes) { size_t a = o; o = align_up(o + bytes); return a; };
   const size_t I = (size_t)c.items;
@@ -213,6 +215,7 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   if (p.sliced) p.tid_far = take(4 * C);
   L.binning_bytes = o;
   L.sorted_tiles = (p.tpasses & 1) ? p.tkey1 : p.tkey0;
+  L.tile_key_bytes = p.key16 ? 2 : 4;
   L.sorted_ids = (p.tpasses & 1) ? p.tid1 : p.tid0;
   o = 0;
   L.ranges = take(8 * (size_t)c.tiles);
@@ -394,6 +397,81 @@ struct Params {
 };
 
 // the forward launch sequence over a table already in device memory
+// Near lists (or the one pass): instance emission in depth order, the
+// stable tile sort and the per-tile ranges, with KT-wide tile keys.
+template <typename KT>
+void bin_near(const Plan& p, const void* table, void* G, void* B, void* I, uint32_t* status, uint32_t items_cap,
+              uint32_t cap, uint32_t tiles_cap, const uint32_t* d_near_end, const uint32_t* d_kn, hipStream_t s) {
+  const W2WCaps& c = p.c;
+  {
+    ProfSpan ps(W2W_KID_EMIT, s);
+    hipLaunchKernelGGL(k_emit<KT>, dim3(std::min<uint32_t>(cdiv(items_cap, 256), kEmitGrid)),
+                       dim3(256), 0, s, table, (uint32_t)c.scenes, at<uint32_t>(G, p.L.perm),
+                       at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.L.counts),
+                       at<uint2>(G, p.L.rect), at<uint32_t>(G, p.L.item_frame), items_cap, cap,
+                       d_near_end, status + ST_EMIT_TICKET,
+                       at<KT>(B, p.tkey0), at<uint32_t>(B, p.tid0), at<uint32_t>(B, p.thist),
+                       p.tpasses, p.tlast_bits);
+  }
+  if (p.sliced) {
+    ProfSpan ps(W2W_KID_EMIT_FAR, s);
+    hipLaunchKernelGGL(k_emit_far<KT>, dim3(std::min<uint32_t>(cdiv(items_cap, 256), kEmitFarGrid)), dim3(256), 0, s,
+                       table, (uint32_t)c.scenes, tiles_cap, items_cap, cap, status, at<uint32_t>(G, p.L.perm),
+                       at<uint32_t>(G, p.L.offsets), status + ST_KNEAR_RAW, at<uint2>(G, p.L.rect),
+                       at<uint32_t>(G, p.L.item_frame), at<uint32_t>(G, p.hint_bm), nullptr, nullptr,
+                       at<KT>(B, p.tkey0), at<uint32_t>(B, p.tid0), at<uint32_t>(B, p.thist),
+                       p.tpasses, p.tlast_bits, nullptr, at<uint32_t>(G, p.big), status + ST_BIG_H);
+    hipLaunchKernelGGL(k_emit_far_big<KT>, dim3(1024), dim3(256), 0, s, table, (uint32_t)c.scenes, cap,
+                       at<uint32_t>(G, p.L.perm), at<uint32_t>(G, p.L.offsets), at<uint2>(G, p.L.rect),
+                       at<uint32_t>(G, p.L.item_frame), at<uint32_t>(G, p.hint_bm), nullptr, nullptr,
+                       at<uint32_t>(G, p.big), status + ST_BIG_H, items_cap, at<KT>(B, p.tkey0),
+                       at<uint32_t>(B, p.tid0), at<uint32_t>(B, p.thist), p.tpasses, p.tlast_bits, nullptr);
+  }
+  sort_passes<KT>(at<KT>(B, p.tkey0), at<uint32_t>(B, p.tid0),
+                  at<KT>(B, p.L.sorted_tiles), at<uint32_t>(B, p.L.sorted_ids),
+                  at<KT>(B, p.scratch_keys()), at<uint32_t>(B, p.scratch_ids()), d_kn,
+                  cap, p.tpasses, p.tlast_bits, true, at<uint32_t>(B, p.thist),
+                  at<uint32_t>(B, p.tstatus), status + ST_TTICKET, W2W_KID_TSORT_HIST,
+                  W2W_KID_TSORT_PASS, s, /*hist_ready=*/true);
+  const uint32_t rgrid = std::min<uint32_t>(cdiv(cap, 256), 4096);
+  ProfSpan ps(W2W_KID_RANGES, s);
+  hipLaunchKernelGGL(k_ranges<KT>, dim3(rgrid), dim3(256), 0, s, at<KT>(B, p.L.sorted_tiles),
+                     d_kn, table, cap, tiles_cap, at<uint2>(I, p.L.ranges));
+}
+
+// Far lists of depth-sliced binning (the open tiles the hint missed):
+// emission at the compact offsets, the tile sort and the ranges.
+template <typename KT>
+void bin_far(const Plan& p, const void* table, void* G, void* B, void* I, uint32_t* status, uint32_t items_cap,
+             uint32_t cap, uint32_t tiles_cap, const uint32_t* open_bm, const uint32_t* hint_bm,
+             uint2* ranges_far, hipStream_t s) {
+  const W2WCaps& c = p.c;
+  {
+    ProfSpan ps(W2W_KID_EMIT_FAR, s);
+    hipLaunchKernelGGL(k_emit_far<KT>, dim3(std::min<uint32_t>(cdiv(items_cap, 256), kEmitFarGrid)), dim3(256), 0, s,
+                       table, (uint32_t)c.scenes, tiles_cap, items_cap, cap, status, at<uint32_t>(G, p.L.perm),
+                       at<uint32_t>(G, p.boff), status + ST_KB, at<uint2>(G, p.L.rect),
+                       at<uint32_t>(G, p.L.item_frame), open_bm, hint_bm, nullptr,
+                       at<KT>(B, p.scratch_keys()), at<uint32_t>(B, p.scratch_ids()), at<uint32_t>(B, p.thist),
+                       p.tpasses, p.tlast_bits, status + ST_OPEN_UNITS, at<uint32_t>(G, p.big), status + ST_BIG_F);
+    hipLaunchKernelGGL(k_emit_far_big<KT>, dim3(1024), dim3(256), 0, s, table, (uint32_t)c.scenes, cap,
+                       at<uint32_t>(G, p.L.perm), at<uint32_t>(G, p.boff), at<uint2>(G, p.L.rect),
+                       at<uint32_t>(G, p.L.item_frame), open_bm, hint_bm, nullptr, at<uint32_t>(G, p.big),
+                       status + ST_BIG_F, items_cap, at<KT>(B, p.scratch_keys()),
+                       at<uint32_t>(B, p.scratch_ids()), at<uint32_t>(B, p.thist), p.tpasses, p.tlast_bits,
+                       status + ST_OPEN_UNITS);
+  }
+  sort_passes<KT>(at<KT>(B, p.scratch_keys()), at<uint32_t>(B, p.scratch_ids()),
+                  at<KT>(B, p.far_keys()), at<uint32_t>(B, p.far_ids()),
+                  at<KT>(B, p.far_scratch_keys()), at<uint32_t>(B, p.far_scratch_ids()),
+                  status + ST_KB, cap, p.tpasses, p.tlast_bits, true, at<uint32_t>(B, p.thist),
+                  at<uint32_t>(B, p.tstatus), status + ST_TTICKET, W2W_KID_TSORT_HIST,
+                  W2W_KID_TSORT_PASS, s, /*hist_ready=*/true);
+  ProfSpan ps(W2W_KID_RANGES, s);
+  hipLaunchKernelGGL(k_ranges<KT>, dim3(std::min<uint32_t>(cdiv(cap, 256), 4096)), dim3(256), 0, s,
+                     at<KT>(B, p.far_keys()), status + ST_KB, table, cap, tiles_cap, ranges_far);
+}
+
 int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const Params& prm,
                 float* out_color, float* out_depth, float* out_alpha, int32_t* out_radii,
                 hipStream_t s) {
@@ -520,40 +598,10 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
         d_near_end = status + ST_SPLIT;
         d_kn = status + ST_KNEAR;
       }
-      {
-        ProfSpan ps(W2W_KID_EMIT, s);
-        hipLaunchKernelGGL(k_emit, dim3(std::min<uint32_t>(cdiv(items_cap, 256), kEmitGrid)),
-                           dim3(256), 0, s, table, (uint32_t)c.scenes, at<uint32_t>(G, p.L.perm),
-                           at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.L.counts),
-                           at<uint2>(G, p.L.rect), at<uint32_t>(G, p.L.item_frame), items_cap, cap,
-                           d_near_end, status + ST_EMIT_TICKET,
-                           at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0), at<uint32_t>(B, p.thist),
-                           p.tpasses, p.tlast_bits);
-      }
-      if (p.sliced) {
-        ProfSpan ps(W2W_KID_EMIT_FAR, s);
-        hipLaunchKernelGGL(k_emit_far, dim3(std::min<uint32_t>(cdiv(items_cap, 256), kEmitFarGrid)), dim3(256), 0, s,
-                           table, (uint32_t)c.scenes, tiles_cap, items_cap, cap, status, at<uint32_t>(G, p.L.perm),
-                           at<uint32_t>(G, p.L.offsets), status + ST_KNEAR_RAW, at<uint2>(G, p.L.rect),
-                           at<uint32_t>(G, p.L.item_frame), at<uint32_t>(G, p.hint_bm), nullptr, nullptr,
-                           at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0), at<uint32_t>(B, p.thist),
-                           p.tpasses, p.tlast_bits, nullptr, at<uint32_t>(G, p.big), status + ST_BIG_H);
-        hipLaunchKernelGGL(k_emit_far_big, dim3(1024), dim3(256), 0, s, table, (uint32_t)c.scenes, cap,
-                           at<uint32_t>(G, p.L.perm), at<uint32_t>(G, p.L.offsets), at<uint2>(G, p.L.rect),
-                           at<uint32_t>(G, p.L.item_frame), at<uint32_t>(G, p.hint_bm), nullptr, nullptr,
-                           at<uint32_t>(G, p.big), status + ST_BIG_H, items_cap, at<uint32_t>(B, p.tkey0),
-                           at<uint32_t>(B, p.tid0), at<uint32_t>(B, p.thist), p.tpasses, p.tlast_bits, nullptr);
-      }
-      sort_passes<uint32_t>(at<uint32_t>(B, p.tkey0), at<uint32_t>(B, p.tid0),
-                            at<uint32_t>(B, p.L.sorted_tiles), at<uint32_t>(B, p.L.sorted_ids),
-                            at<uint32_t>(B, p.scratch_keys()), at<uint32_t>(B, p.scratch_ids()), d_kn,
-                            cap, p.tpasses, p.tlast_bits, true, at<uint32_t>(B, p.thist),
-                            at<uint32_t>(B, p.tstatus), status + ST_TTICKET, W2W_KID_TSORT_HIST,
-                            W2W_KID_TSORT_PASS, s, /*hist_ready=*/true);
-      const uint32_t rgrid = std::min<uint32_t>(cdiv(cap, 256), 4096);
-      ProfSpan ps(W2W_KID_RANGES, s);
-      hipLaunchKernelGGL(k_ranges, dim3(rgrid), dim3(256), 0, s, at<uint32_t>(B, p.L.sorted_tiles),
-                         d_kn, table, cap, tiles_cap, at<uint2>(I, p.L.ranges));
+      if (p.key16)
+        bin_near<uint16_t>(p, table, G, B, I, status, items_cap, cap, tiles_cap, d_near_end, d_kn, s);
+      else
+        bin_near<uint32_t>(p, table, G, B, I, status, items_cap, cap, tiles_cap, d_near_end, d_kn, s);
     }
     if (out_radii)
       (void)hipMemcpyAsync(out_radii, at<int32_t>(G, p.L.radii), 4 * (size_t)items_cap,
@@ -615,32 +663,10 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
                        at<uint32_t>(G, p.slb_far) + 1, status + ST_SCANB_TICKET, status + ST_KB, nullptr,
                        d_items, items_cap, cap, nullptr, nullptr, status + ST_OPEN_UNITS);
   }
-  {
-    ProfSpan ps(W2W_KID_EMIT_FAR, s);
-    hipLaunchKernelGGL(k_emit_far, dim3(std::min<uint32_t>(cdiv(items_cap, 256), kEmitFarGrid)), dim3(256), 0, s,
-                       table, (uint32_t)c.scenes, tiles_cap, items_cap, cap, status, at<uint32_t>(G, p.L.perm),
-                       at<uint32_t>(G, p.boff), status + ST_KB, at<uint2>(G, p.L.rect),
-                       at<uint32_t>(G, p.L.item_frame), open_bm, hint_bm, nullptr,
-                       at<uint32_t>(B, p.scratch_keys()), at<uint32_t>(B, p.scratch_ids()), at<uint32_t>(B, p.thist),
-                       p.tpasses, p.tlast_bits, status + ST_OPEN_UNITS, at<uint32_t>(G, p.big), status + ST_BIG_F);
-    hipLaunchKernelGGL(k_emit_far_big, dim3(1024), dim3(256), 0, s, table, (uint32_t)c.scenes, cap,
-                       at<uint32_t>(G, p.L.perm), at<uint32_t>(G, p.boff), at<uint2>(G, p.L.rect),
-                       at<uint32_t>(G, p.L.item_frame), open_bm, hint_bm, nullptr, at<uint32_t>(G, p.big),
-                       status + ST_BIG_F, items_cap, at<uint32_t>(B, p.scratch_keys()),
-                       at<uint32_t>(B, p.scratch_ids()), at<uint32_t>(B, p.thist), p.tpasses, p.tlast_bits,
-                       status + ST_OPEN_UNITS);
-  }
-  sort_passes<uint32_t>(at<uint32_t>(B, p.scratch_keys()), at<uint32_t>(B, p.scratch_ids()),
-                        at<uint32_t>(B, p.far_keys()), at<uint32_t>(B, p.far_ids()),
-                        at<uint32_t>(B, p.far_scratch_keys()), at<uint32_t>(B, p.far_scratch_ids()),
-                        status + ST_KB, cap, p.tpasses, p.tlast_bits, true, at<uint32_t>(B, p.thist),
-                        at<uint32_t>(B, p.tstatus), status + ST_TTICKET, W2W_KID_TSORT_HIST,
-                        W2W_KID_TSORT_PASS, s, /*hist_ready=*/true);
-  {
-    ProfSpan ps(W2W_KID_RANGES, s);
-    hipLaunchKernelGGL(k_ranges, dim3(std::min<uint32_t>(cdiv(cap, 256), 4096)), dim3(256), 0, s,
-                       at<uint32_t>(B, p.far_keys()), status + ST_KB, table, cap, tiles_cap, ranges_far);
-  }
+  if (p.key16)
+    bin_far<uint16_t>(p, table, G, B, I, status, items_cap, cap, tiles_cap, open_bm, hint_bm, ranges_far, s);
+  else
+    bin_far<uint32_t>(p, table, G, B, I, status, items_cap, cap, tiles_cap, open_bm, hint_bm, ranges_far, s);
   ba.ticket = status + ST_BLENDB_TICKET;
   ba.ranges_far = ranges_far;
   ba.ids_far = at<uint32_t>(B, p.far_ids());
@@ -952,7 +978,9 @@ int w2w_backward(const W2WFrame* f, const W2WWorkspace* ws, const float* dL_dcol
                       dL_dscales, dL_drotations, s);
 }
 
-__global__ void k_sorted_keys(const uint32_t* __restrict__ tiles, const uint32_t* __restrict__ ids,
+extern "C++" {
+template <typename KT>
+__global__ void k_sorted_keys(const KT* __restrict__ tiles, const uint32_t* __restrict__ ids,
                               const SplatRec* __restrict__ splats, const uint32_t* __restrict__ status,
                               uint32_t cap, uint32_t P, uint64_t* __restrict__ out) {
   uint32_t n = status[ST_K];
@@ -963,6 +991,7 @@ __global__ void k_sorted_keys(const uint32_t* __restrict__ tiles, const uint32_t
     out[i] = ((uint64_t)tiles[i] << 32) | (uint64_t)__float_as_uint(z);
   }
 }
+}  // extern "C++"
 
 int w2w_sorted_keys(const W2WFrame* f, const W2WWorkspace* ws, uint64_t* out_keys, void* stream) {
   if (!frame_ok(f) || !out_keys) return W2W_ERR_INVALID;
@@ -975,10 +1004,16 @@ int w2w_sorted_keys(const W2WFrame* f, const W2WWorkspace* ws, uint64_t* out_key
   hipStream_t s = (hipStream_t)stream;
   (void)hipGetLastError();
   const uint32_t grid = std::min<uint32_t>(cdiv((uint64_t)c.capacity, 256), 4096);
-  hipLaunchKernelGGL(k_sorted_keys, dim3(grid), dim3(256), 0, s,
-                     at<uint32_t>(ws->binning, p.L.sorted_tiles),
-                     at<uint32_t>(ws->binning, p.L.sorted_ids), at<SplatRec>(ws->geom, p.L.splats),
-                     at<uint32_t>(ws->geom, p.status), (uint32_t)c.capacity, (uint32_t)f->P, out_keys);
+  if (p.key16)
+    hipLaunchKernelGGL(k_sorted_keys<uint16_t>, dim3(grid), dim3(256), 0, s,
+                       at<uint16_t>(ws->binning, p.L.sorted_tiles),
+                       at<uint32_t>(ws->binning, p.L.sorted_ids), at<SplatRec>(ws->geom, p.L.splats),
+                       at<uint32_t>(ws->geom, p.status), (uint32_t)c.capacity, (uint32_t)f->P, out_keys);
+  else
+    hipLaunchKernelGGL(k_sorted_keys<uint32_t>, dim3(grid), dim3(256), 0, s,
+                       at<uint32_t>(ws->binning, p.L.sorted_tiles),
+                       at<uint32_t>(ws->binning, p.L.sorted_ids), at<SplatRec>(ws->geom, p.L.splats),
+                       at<uint32_t>(ws->geom, p.status), (uint32_t)c.capacity, (uint32_t)f->P, out_keys);
   return launch_status();
 }
 

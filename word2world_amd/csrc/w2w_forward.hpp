@@ -609,6 +609,28 @@ __global__ __launch_bounds__(256) void k_count_tiles(const void* __restrict__ ta
   }
 }
 
+// Tile keys are the global tile ids: 16-bit when the batch has at most
+// 65536 tiles (the emission, the sort and the ranges then move two bytes per
+// key), 32-bit otherwise.
+template <typename KT>
+W2W_DEV void store_keys4(KT* p, uint32_t k0, uint32_t k1, uint32_t k2, uint32_t k3) {
+  if constexpr (sizeof(KT) == 2)
+    *reinterpret_cast<uint2*>(p) = make_uint2(k0 | (k1 << 16), k2 | (k3 << 16));
+  else
+    *reinterpret_cast<uint4*>(p) = make_uint4(k0, k1, k2, k3);
+}
+
+template <typename KT>
+W2W_DEV void load_keys4(const KT* p, uint32_t& k0, uint32_t& k1, uint32_t& k2, uint32_t& k3) {
+  if constexpr (sizeof(KT) == 2) {
+    const uint2 v = *reinterpret_cast<const uint2*>(p);
+    k0 = v.x & 0xFFFFu; k1 = v.x >> 16; k2 = v.y & 0xFFFFu; k3 = v.y >> 16;
+  } else {
+    const uint4 v = *reinterpret_cast<const uint4*>(p);
+    k0 = v.x; k1 = v.y; k2 = v.z; k3 = v.w;
+  }
+}
+
 constexpr uint32_t kFarSerial = 32;
 constexpr uint32_t kSetLds = 4096;  // tile-set words k_emit_far keeps in LDS (131072 tiles)
 
@@ -621,6 +643,7 @@ constexpr uint32_t kSetLds = 4096;  // tile-set words k_emit_far keeps in LDS (1
 // An item with more than kFarSerial instances is left to k_emit_far_big
 // (its depth position appended to big[], *n_big counts them): one thread
 // would hold up its wave.
+template <typename KT>
 __global__ __launch_bounds__(256) void k_emit_far(const void* __restrict__ table, uint32_t scenes_cap,
                                                   uint32_t tiles_cap, uint32_t items_cap, uint32_t capacity,
                                                   const uint32_t* __restrict__ status,
@@ -632,7 +655,7 @@ __global__ __launch_bounds__(256) void k_emit_far(const void* __restrict__ table
                                                   const uint32_t* __restrict__ set_a,
                                                   const uint32_t* __restrict__ set_b,
                                                   const uint32_t* __restrict__ d_base,
-                                                  uint32_t* __restrict__ tile_keys,
+                                                  KT* __restrict__ tile_keys,
                                                   uint32_t* __restrict__ ids,
                                                   uint32_t* __restrict__ hist, int hist_passes,
                                                   int hist_last_bits, const uint32_t* __restrict__ gate,
@@ -675,7 +698,7 @@ __global__ __launch_bounds__(256) void k_emit_far(const void* __restrict__ table
         if (k < 32u) m &= (1u << k) - 1u;
         while (m && o < oe) {
           const uint32_t key = pos + (uint32_t)__builtin_ctz(m);
-          tile_keys[o] = key;
+          tile_keys[o] = (KT)key;
           ids[o] = g;
           o++;
           m &= m - 1u;
@@ -699,6 +722,7 @@ __global__ __launch_bounds__(256) void k_emit_far(const void* __restrict__ table
 // The far items k_emit_far left (more than kFarSerial instances): a wave per
 // item walks its rectangle 64 tiles at a time in row-major order; member
 // tiles are placed by ballot prefix counts.  Same outputs as the serial walk.
+template <typename KT>
 __global__ __launch_bounds__(256) void k_emit_far_big(const void* __restrict__ table, uint32_t scenes_cap,
                                                       uint32_t capacity, const uint32_t* __restrict__ perm,
                                                       const uint32_t* __restrict__ coff,
@@ -709,7 +733,7 @@ __global__ __launch_bounds__(256) void k_emit_far_big(const void* __restrict__ t
                                                       const uint32_t* __restrict__ d_base,
                                                       const uint32_t* __restrict__ big,
                                                       const uint32_t* __restrict__ n_big, uint32_t big_cap,
-                                                      uint32_t* __restrict__ tile_keys,
+                                                      KT* __restrict__ tile_keys,
                                                       uint32_t* __restrict__ ids,
                                                       uint32_t* __restrict__ hist, int hist_passes,
                                                       int hist_last_bits, const uint32_t* __restrict__ gate) {
@@ -741,7 +765,7 @@ __global__ __launch_bounds__(256) void k_emit_far_big(const void* __restrict__ t
       const uint64_t m = __ballot(member);
       const uint32_t at = o + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
       if (member && at < capacity) {
-        tile_keys[at] = key;
+        tile_keys[at] = (KT)key;
         ids[at] = g;
         for (int p = 0; p < hist_passes; p++) {
           const int bits = (p == hist_passes - 1) ? hist_last_bits : 8;
@@ -767,6 +791,7 @@ __global__ __launch_bounds__(256) void k_emit_far_big(const void* __restrict__ t
 // keys an extra time.
 // Depth positions [0, *d_end) (null: all; the near slice of depth-sliced
 // binning otherwise).
+template <typename KT>
 __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, uint32_t scenes_cap,
                                               const uint32_t* __restrict__ perm,
                                               const uint32_t* __restrict__ offs,
@@ -776,7 +801,7 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
                                               uint32_t items_cap, uint32_t capacity,
                                               const uint32_t* __restrict__ d_end,
                                               uint32_t* __restrict__ ticket,
-                                              uint32_t* __restrict__ tile_keys,
+                                              KT* __restrict__ tile_keys,
                                               uint32_t* __restrict__ ids,
                                               uint32_t* __restrict__ hist, int hist_passes,
                                               int hist_last_bits) {
@@ -856,13 +881,13 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
         }
       }
       if (q >= kb && q + 3 < ke) {
-        *reinterpret_cast<uint4*>(tile_keys + q) = make_uint4(key4[0], key4[1], key4[2], key4[3]);
+        store_keys4(tile_keys + q, key4[0], key4[1], key4[2], key4[3]);
         *reinterpret_cast<uint4*>(ids + q) = make_uint4(id4[0], id4[1], id4[2], id4[3]);
       } else {
 #pragma unroll
         for (int m = 0; m < 4; m++) {
           if (q + m >= kb && q + m < ke) {
-            tile_keys[q + m] = key4[m];
+            tile_keys[q + m] = (KT)key4[m];
             ids[q + m] = id4[m];
           }
         }
@@ -877,8 +902,9 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
 }
 
 // Per-tile [start, end) of the sorted tile keys; four keys per thread with
-// one 16-byte load (the key array is 256-byte aligned).
-__global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ keys,
+// one 8- or 16-byte load (the key array is 256-byte aligned).
+template <typename KT>
+__global__ __launch_bounds__(256) void k_ranges(const KT* __restrict__ keys,
                                                 const uint32_t* __restrict__ d_n,
                                                 const void* __restrict__ table, uint32_t capacity,
                                                 uint32_t tiles_cap, uint2* __restrict__ ranges) {
@@ -890,15 +916,14 @@ __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ key
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += stride) {
     const uint32_t i0 = 4 * q;
     uint32_t k[6];  // keys i0-1 .. i0+4; out of range = all ones (no tile)
-    k[0] = i0 > 0 ? keys[i0 - 1] : 0xFFFFFFFFu;
+    k[0] = i0 > 0 ? (uint32_t)keys[i0 - 1] : 0xFFFFFFFFu;
     if (i0 + 4 <= n) {
-      const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
-      k[1] = v.x; k[2] = v.y; k[3] = v.z; k[4] = v.w;
+      load_keys4(keys + i0, k[1], k[2], k[3], k[4]);
     } else {
 #pragma unroll
-      for (int j = 0; j < 4; j++) k[1 + j] = i0 + j < n ? keys[i0 + j] : 0xFFFFFFFFu;
+      for (int j = 0; j < 4; j++) k[1 + j] = i0 + j < n ? (uint32_t)keys[i0 + j] : 0xFFFFFFFFu;
     }
-    k[5] = i0 + 4 < n ? keys[i0 + 4] : 0xFFFFFFFFu;
+    k[5] = i0 + 4 < n ? (uint32_t)keys[i0 + 4] : 0xFFFFFFFFu;
 #pragma unroll
     for (int j = 1; j <= 4; j++) {
       const uint32_t t = k[j];

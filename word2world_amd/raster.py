@@ -113,6 +113,10 @@ class Workspace:
         return self._view(self.binning, self.layout.sorted_ids, n, torch.int32)
 
     def sorted_tiles(self, n: int):
+        """The first n sorted tile keys, as int32 (stored 16-bit when the view
+        has at most 65536 tiles)."""
+        if self.layout.tile_key_bytes == 2:
+            return self._view(self.binning, self.layout.sorted_tiles, n, torch.int16).to(torch.int32) & 0xFFFF
         return self._view(self.binning, self.layout.sorted_tiles, n, torch.int32)
 
     def ranges(self):
