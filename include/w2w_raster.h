@@ -209,6 +209,25 @@ int w2w_backward_batch(const void* table, const W2WCaps* caps, const W2WWorkspac
                        float* dL_dopacity, float* dL_dmeans3D, float* dL_dcov3D, float* dL_dsh,
                        float* dL_dscales, float* dL_drotations, void* stream);
 
+/* w2w_backward_batch with the SH coefficients' Adam step taken in the SH
+ * backward instead of writing dL_dsh (no counterpart in the reference, whose
+ * trainer runs torch.optim.Adam after the backward, gaussians_handler.py:40;
+ * for a trainer that keeps its parameters and Adam state in device memory):
+ * sh is the parameter store's SH block and is updated in place, sh_exp_avg
+ * and sh_exp_avg_sq are packed like it; step size hyper[hyper_dc] for the
+ * first three coefficients of each row and hyper[hyper_ac] for the rest
+ * (-lr / (1 - b1^t)), hyper[16..20] = 1/sqrt(1 - b2^t), 1 - b1, b2, 1 - b2,
+ * eps (the w2w_adam_step_hyper layout); no SH update while *skip_if != 0.
+ * The update of each coefficient is the one w2w_adam_step_hyper applies with
+ * the gradient w2w_backward_batch would write (bit for bit). */
+int w2w_backward_batch_sh_adam(const void* table, const W2WCaps* caps, const W2WWorkspace* ws,
+                               const float* means, const float* scales, const float* rotations, float* sh,
+                               const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
+                               float* dL_dmeans2D, float* dL_dcolors, float* dL_dopacity, float* dL_dmeans3D,
+                               float* dL_dcov3D, float* sh_exp_avg, float* sh_exp_avg_sq,
+                               float* dL_dscales, float* dL_drotations, const float* hyper,
+                               int32_t hyper_dc, int32_t hyper_ac, const uint32_t* skip_if, void* stream);
+
 /* 64-bit keys (tile << 32 | depth bits) of the sorted instances, i.e. the
  * reference's sorted keysArray (prepareSort.cu:431-434), for parity checks. */
 int w2w_sorted_keys(const W2WFrame* f, const W2WWorkspace* ws, uint64_t* out_keys, void* stream);

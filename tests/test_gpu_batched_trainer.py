@@ -43,7 +43,9 @@ def _plain_grads(p):
 def test_batched_matches_plain_trainer(gpu):
     jobs = _jobs()
     tg = _targets(jobs, gpu)
-    bt = BatchedTrainer([j[0] for j in jobs], [j[2] for j in jobs], tg, gpu)
+    # the gradients themselves are compared: the SH gradient is written
+    # (the trainer's default takes the SH step without storing it)
+    bt = BatchedTrainer([j[0] for j in jobs], [j[2] for j in jobs], tg, gpu, fused_sh_adam=False)
     plains = [PlainTrainer(sc, [gpu_camera(c, gpu) for c in cams], t, TrainConfig(), gpu)
               for (sc, _, cams), t in zip(jobs, tg)]
     st = bt.store
@@ -113,6 +115,41 @@ def test_graph_replay_matches_eager(gpu):
             assert torch.equal(c1, c2) and torch.equal(d1, d2) and torch.equal(a1, a2)
         for name in ("grad", "flat", "exp_avg", "exp_avg_sq"):
             assert torch.equal(getattr(graph.store, name), getattr(eager.store, name)), (it, name)
+
+
+@pytest.mark.parametrize("n_coeffs", [16, 20])
+def test_fused_sh_adam_matches_separate_adam(gpu, n_coeffs):
+    """The SH backward taking the SH coefficients' Adam step (the trainer's
+    default) against the SH gradient written out and the store-wide Adam
+    kernel: parameters and Adam state bit for bit over eager steps and graph
+    replays, SH degrees 0-3 in one batch (rows wider than the degree's
+    coefficients; 20 coefficients per row: rows too wide for the LDS block,
+    stepped by their own thread), and the other groups' gradients
+    unchanged."""
+    jobs = []
+    for k, (n, res, deg) in enumerate([(1500, 96, 3), (700, 64, 1), (900, 80, 0), (1200, 72, 2)]):
+        cams = [orbit_camera(90.0 * v + 15 * k, 5.0 * v, 2.5, res, res) for v in range(2)]
+        sc, tsc = make_scene(n, 3, 300 + k, n_coeffs=n_coeffs), make_scene(n, 3, 400 + k)
+        sc.sh_degree = deg
+        jobs.append((sc, tsc, cams))
+    tg = _targets(jobs, gpu)
+    sep = BatchedTrainer([j[0] for j in jobs], [j[2] for j in jobs], tg, gpu, fused_sh_adam=False)
+    fus = BatchedTrainer([j[0] for j in jobs], [j[2] for j in jobs], tg, gpu)
+    assert fus.fused_sh_adam and not sep.fused_sh_adam
+    st = fus.store
+    sh0 = st.group(st.flat, "sh").clone()
+    for it in range(6):
+        if it == 3:
+            fus.capture()
+            sep.capture()
+        ls, lf = sep.step().clone(), fus.step().clone()
+        torch.cuda.synchronize()
+        assert torch.equal(ls, lf), it
+        for name in ("flat", "exp_avg", "exp_avg_sq"):
+            assert torch.equal(getattr(sep.store, name), getattr(fus.store, name)), (it, name)
+        for name in ("means", "opacity", "svec", "quaternion"):
+            assert torch.equal(st.view(name, st.grad), sep.store.view(name, sep.store.grad)), (it, name)
+    assert not torch.equal(st.group(st.flat, "sh"), sh0)  # the SH coefficients did move
 
 
 def test_depth_sliced_training_is_exact(gpu):

@@ -201,12 +201,14 @@ def _single_view_check(bt, means, sh, scenes, cameras, dev, grad_scenes=None):
 
 
 def test_scene1m_sliced_steps_equal_one_pass_and_single_view(gpu):
-    """Tight emission + depth split 0.3 (the benchmark's path) against one
-    binning pass for two steps (the second with the first's tile hint), then
-    the second step's frames and gradients against the one-view path."""
+    """Tight emission + depth split 0.3 with the SH step fused into the SH
+    backward (the benchmark's path) against one binning pass with the SH
+    gradient written and the store-wide Adam, for two steps (the second with
+    the first's tile hint), then the second step's frames and gradients
+    against the one-view path."""
     job = _scene_1m()
     tg = [_targets(job, gpu)]
-    ref = BatchedTrainer([job.scene], [job.cameras], tg, gpu, depth_split=0.0)
+    ref = BatchedTrainer([job.scene], [job.cameras], tg, gpu, depth_split=0.0, fused_sh_adam=False)
     sl = BatchedTrainer([job.scene], [job.cameras], tg, gpu, depth_split=0.3)
     assert sl.caps.depth_split == 300
     for it in range(2):
@@ -218,8 +220,10 @@ def test_scene1m_sliced_steps_equal_one_pass_and_single_view(gpu):
         assert torch.equal(lr_, ls_), it
         for (c1, d1, a1), (c2, d2, a2) in zip(ref.frame_images(), sl.frame_images()):
             assert torch.equal(c1, c2) and torch.equal(d1, d2) and torch.equal(a1, a2), it
-        for name in ("grad", "flat", "exp_avg", "exp_avg_sq"):
+        for name in ("flat", "exp_avg", "exp_avg_sq"):
             assert torch.equal(getattr(ref.store, name), getattr(sl.store, name)), (it, name)
+        for name in ("means", "opacity", "svec", "quaternion"):  # the fused path stores no SH gradient
+            assert torch.equal(ref.store.view(name, ref.store.grad), sl.store.view(name, sl.store.grad)), (it, name)
     S, KA, KB, n_open, KH, KN = (int(v) for v in sl.ws.sliced_counts().cpu())
     K = int(ref.ws.status()[0])
     print(f"K {K}: near {KA} hinted {KH} far {KB} open blocks {n_open}", flush=True)
@@ -321,7 +325,7 @@ def test_batched64_graph_step_equals_single_view(gpu):
     jobs = workloads.batched_64()
     scenes = [j.scene for j in jobs]
     cameras = [j.cameras for j in jobs]
-    bt = BatchedTrainer(scenes, cameras, None, gpu)
+    bt = BatchedTrainer(scenes, cameras, None, gpu, fused_sh_adam=False)  # the SH gradient is checked
     bt.step()
     bt.capture()
     means, sh = _snapshot_inputs(bt)

@@ -61,7 +61,7 @@ class W2WDensifyConfig(ctypes.Structure):
 
 
 EXPORTS = ["w2w_version", "w2w_splat_extents", "w2w_block_touches", "w2w_workspace_layout", "w2w_batch_workspace_layout",
-           "w2w_batch_table_bytes", "w2w_batch_encode", "w2w_forward_batch", "w2w_backward_batch", "w2w_forward", "w2w_backward", "w2w_sorted_keys",
+           "w2w_batch_table_bytes", "w2w_batch_encode", "w2w_forward_batch", "w2w_backward_batch", "w2w_backward_batch_sh_adam", "w2w_forward", "w2w_backward", "w2w_sorted_keys",
            "w2w_sort_temp_bytes", "w2w_sort_pairs_u64", "w2w_scan_temp_bytes",
            "w2w_exclusive_scan_u32", "w2w_exclusive_scan_u32_noalloc", "w2w_prof_enable", "w2w_prof_reset", "w2w_prof_read", "w2w_stats_enable", "w2w_adam_step", "w2w_activate", "w2w_activate_backward",
            "w2w_l2_loss", "w2w_adam_step_hyper", "w2w_activate_rows", "w2w_activate_backward_rows",
@@ -107,6 +107,9 @@ def load(path: str = LIB_PATH):
     L.w2w_forward_batch.argtypes = [_vp, P(W2WCaps), P(W2WWorkspace)] + [_vp] * 10
     L.w2w_backward_batch.restype = ctypes.c_int
     L.w2w_backward_batch.argtypes = [_vp, P(W2WCaps), P(W2WWorkspace)] + [_vp] * 16
+    L.w2w_backward_batch_sh_adam.restype = ctypes.c_int
+    L.w2w_backward_batch_sh_adam.argtypes = ([_vp, P(W2WCaps), P(W2WWorkspace)] + [_vp] * 17
+                                             + [ctypes.c_int32, ctypes.c_int32, _vp, _vp])
     L.w2w_forward.restype = ctypes.c_int
     L.w2w_forward.argtypes = [P(W2WFrame), P(W2WWorkspace), _vp, _vp, _vp, _vp, _vp]
     L.w2w_backward.restype = ctypes.c_int

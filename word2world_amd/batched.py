@@ -5,8 +5,10 @@ All scenes' parameters live in one flat HBM store, group-major:
 [G,4]`` with the scenes back to back inside each group, so the SH group *is*
 the packed SH array the rasterizer reads and the means group its packed
 means.  One step is a fixed sequence of native launches -- activation,
-batched forward, L2 loss, batched backward (writing the means and SH
-gradients straight into the flat gradient), activation backward, Adam -- with
+batched forward, L2 loss, batched backward (writing the means gradient
+straight into the flat gradient; the SH coefficients take their Adam step
+inside the SH backward, so their gradient is never stored), activation
+backward, Adam over the other groups -- with
 no autograd and no host synchronisation, so it can be captured once in a HIP
 graph and replayed; the per-step learning rates reach the captured Adam
 kernel through a small device buffer refreshed before each replay.
@@ -58,10 +60,13 @@ class BatchedStore:
         self.g_opac = torch.empty(G, **f32)
         self.g_rots = torch.empty(G, 4, **f32)
         self.g_cov = torch.empty(G, 6, **f32)
-        self.hyper = torch.zeros(32, **f32)
+        # Adam hyper-parameters (w2w_adam_step_hyper layout) in [0, 32), and in
+        # [32, 64) the same for a launch over the groups after the SH block
+        # (opacity, svec, quaternion as groups 0-2)
+        self.hyper = torch.zeros(64, **f32)
         # ring of pinned staging buffers: a buffer is rewritten only after the
         # copy that last read it has completed
-        self._ring = [torch.zeros(32, dtype=torch.float32).pin_memory() for _ in range(4)]
+        self._ring = [torch.zeros(64, dtype=torch.float32).pin_memory() for _ in range(4)]
         self._ring_ev = [None] * 4
         self._ring_i = 0
 
@@ -108,7 +113,7 @@ class BatchedStore:
     def hyper_values(self, step: int, lrs: dict, b1=0.9, b2=0.999, eps=1e-15) -> np.ndarray:
         bc1 = 1.0 - b1 ** step
         bc2 = 1.0 - b2 ** step
-        h = np.zeros(32, np.float32)
+        h = np.zeros(64, np.float32)
         h[0] = -lrs["means"] / bc1
         h[1] = -lrs["sh_dc"] / bc1
         h[8 + 1] = -lrs["sh_ac"] / bc1
@@ -117,6 +122,8 @@ class BatchedStore:
         h[4] = -lrs["quaternion"] / bc1
         h[16] = 1.0 / np.sqrt(bc2)
         h[17], h[18], h[19], h[20] = 1.0 - b1, b2, 1.0 - b2, eps
+        h[32:35] = h[2:5]
+        h[48:53] = h[16:21]
         return h
 
 
@@ -127,7 +134,7 @@ class BatchedTrainer:
                  capacity: int | None = None, tight: bool = True, densify_every: int = 0,
                  densify_grad: float = 2e-4, densify_min_opacity: float = 0.005,
                  generator: torch.Generator | None = None, depth_split: float = 0.0,
-                 densify_options: dict | None = None, chunk: int = 0):
+                 densify_options: dict | None = None, chunk: int = 0, fused_sh_adam: bool = True):
         """cameras[k]: word2world_amd.camera.Camera views of scene k;
         targets[k][v]: [3,H,W] target image (or None for render-only);
         tight: emit instances only where a Gaussian's cull extent reaches
@@ -143,7 +150,10 @@ class BatchedTrainer:
         split_scale_min, prune_scale_max, split_factor, split_shrink of
         word2world_amd.densify.config; the reference trainer's values when
         absent); chunk: instances per forward-blend chunk (64, 128 or 256;
-        0 = 64), a speed knob with the same outputs."""
+        0 = 64), a speed knob with the same outputs; fused_sh_adam: the SH
+        backward takes the SH coefficients' Adam step itself (their gradient
+        is never written; the same parameters bit for bit)."""
+        self.fused_sh_adam = bool(fused_sh_adam)
         self.store = BatchedStore(scenes, device)
         self.chunk = int(chunk)
         self.depth_split = float(depth_split)
@@ -244,13 +254,24 @@ class BatchedTrainer:
         _lib.check(L.w2w_l2_loss(ws.table.data_ptr(), ctypes.byref(self.caps), ws.color.data_ptr(),
                                  self.target.data_ptr(), self.dL_dcolor.data_ptr(),
                                  self.loss.data_ptr(), self.tile_loss.data_ptr(), s), "w2w_l2_loss")
-        _lib.check(L.w2w_backward_batch(
-            ws.table.data_ptr(), ctypes.byref(self.caps), ctypes.byref(ws.struct), means.data_ptr(),
-            st.scales.data_ptr(), st.rots.data_ptr(), sh.data_ptr(), self.dL_dcolor.data_ptr(), None,
-            None, g.means2D.data_ptr(), g.colors.data_ptr(), st.g_opac.data_ptr(),
-            st.view("means", st.grad).data_ptr(), st.g_cov.data_ptr(),
-            st.group(st.grad, "sh").data_ptr(), st.g_scales.data_ptr(), st.g_rots.data_ptr(), s),
-            "w2w_backward_batch")
+        skip = ws.geom.data_ptr() + 4  # the forward's overflow word: an overflowed step updates nothing
+        if self.fused_sh_adam:
+            _lib.check(L.w2w_backward_batch_sh_adam(
+                ws.table.data_ptr(), ctypes.byref(self.caps), ctypes.byref(ws.struct), means.data_ptr(),
+                st.scales.data_ptr(), st.rots.data_ptr(), sh.data_ptr(), self.dL_dcolor.data_ptr(), None,
+                None, g.means2D.data_ptr(), g.colors.data_ptr(), st.g_opac.data_ptr(),
+                st.view("means", st.grad).data_ptr(), st.g_cov.data_ptr(),
+                st.group(st.exp_avg, "sh").data_ptr(), st.group(st.exp_avg_sq, "sh").data_ptr(),
+                st.g_scales.data_ptr(), st.g_rots.data_ptr(), st.hyper.data_ptr(), 1, 9, skip, s),
+                "w2w_backward_batch_sh_adam")
+        else:
+            _lib.check(L.w2w_backward_batch(
+                ws.table.data_ptr(), ctypes.byref(self.caps), ctypes.byref(ws.struct), means.data_ptr(),
+                st.scales.data_ptr(), st.rots.data_ptr(), sh.data_ptr(), self.dL_dcolor.data_ptr(), None,
+                None, g.means2D.data_ptr(), g.colors.data_ptr(), st.g_opac.data_ptr(),
+                st.view("means", st.grad).data_ptr(), st.g_cov.data_ptr(),
+                st.group(st.grad, "sh").data_ptr(), st.g_scales.data_ptr(), st.g_rots.data_ptr(), s),
+                "w2w_backward_batch")
         if self.densify_every:
             _lib.check(L.w2w_densify_stats(ws.table.data_ptr(), ctypes.byref(self.caps),
                                            g.means2D.data_ptr(), ws.radii.data_ptr(), self.accum.data_ptr(),
@@ -260,12 +281,28 @@ class BatchedTrainer:
             st.g_scales.data_ptr(), st.g_opac.data_ptr(), st.g_rots.data_ptr(), G,
             st.view("svec", st.grad).data_ptr(), st.view("opacity", st.grad).data_ptr(),
             st.view("quaternion", st.grad).data_ptr(), s), "w2w_activate_backward")
+        if not self.fused_sh_adam:
+            _lib.check(L.w2w_adam_step_hyper(st.flat.data_ptr(), st.grad.data_ptr(),
+                                             st.exp_avg.data_ptr(), st.exp_avg_sq.data_ptr(),
+                                             int(st.ends[-1]), len(GROUPS), st.ends.ctypes.data,
+                                             self._period.ctypes.data, self._split.ctypes.data,
+                                             st.hyper.data_ptr(), skip, self.sticky.data_ptr(), s),
+                       "w2w_adam_step_hyper")
+            return
+        # the SH block took its step in the backward: the means, then the
+        # groups after the SH block (their rates at hyper[32 + k])
         _lib.check(L.w2w_adam_step_hyper(st.flat.data_ptr(), st.grad.data_ptr(),
                                          st.exp_avg.data_ptr(), st.exp_avg_sq.data_ptr(),
-                                         int(st.ends[-1]), len(GROUPS), st.ends.ctypes.data,
-                                         self._period.ctypes.data, self._split.ctypes.data,
-                                         st.hyper.data_ptr(), ws.geom.data_ptr() + 4,
-                                         self.sticky.data_ptr(), s), "w2w_adam_step_hyper")
+                                         int(st.ends[0]), 1, st.ends.ctypes.data, None, None,
+                                         st.hyper.data_ptr(), skip, self.sticky.data_ptr(), s),
+                   "w2w_adam_step_hyper")
+        b = int(st.ends[1])
+        self._tail_ends = (st.ends[2:5] - b).astype(np.int64)
+        _lib.check(L.w2w_adam_step_hyper(st.flat.data_ptr() + 4 * b, st.grad.data_ptr() + 4 * b,
+                                         st.exp_avg.data_ptr() + 4 * b, st.exp_avg_sq.data_ptr() + 4 * b,
+                                         int(st.ends[-1]) - b, 3, self._tail_ends.ctypes.data, None, None,
+                                         st.hyper.data_ptr() + 4 * 32, skip, self.sticky.data_ptr(), s),
+                   "w2w_adam_step_hyper")
 
     def _upload_hyper(self):
         self.it += 1

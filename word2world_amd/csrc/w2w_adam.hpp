@@ -28,6 +28,17 @@ struct AdamArgs {
   float one_m_b1, b2, one_m_b2, eps;
 };
 
+// One element's update with the step size ns = -lr / (1 - b1^t) and
+// inv_bc2_sqrt = 1 / sqrt(1 - b2^t) (the store-wide kernels and the SH
+// backward's fused step share it).
+W2W_DEV void adam_one(float& p, float g, float& m, float& v, float ns, float inv_bc2_sqrt, float one_m_b1,
+                      float b2, float one_m_b2, float eps) {
+  m = m + one_m_b1 * (g - m);
+  v = v * b2 + (one_m_b2 * g) * g;
+  const float denom = __builtin_sqrtf(v) * inv_bc2_sqrt + eps;
+  p = p + ns * (m / denom);
+}
+
 __global__ __launch_bounds__(256) void k_adam(AdamArgs a) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {

@@ -18,6 +18,7 @@
 //                     item's instance records, which lie contiguously from the
 //                     item's emission offset, in emission order.
 #pragma once
+#include "w2w_adam.hpp"
 #include "w2w_device.hpp"
 #include "w2w_forward.hpp"
 
@@ -1260,6 +1261,18 @@ struct ShBwdArgs {
   const float* dL_dcolors;  // per item [3], written by k_preprocess_bwd
   float* dL_dsh;            // packed like sh
   float* dL_dmeans3D;       // per Gaussian [3]: the view-direction term is added
+  // optional (adam_m non-null): Adam on the SH coefficients in place of
+  // writing dL_dsh -- sh_out (= sh, the parameter store's SH block), adam_m
+  // and adam_v packed like sh; step sizes hyper[hyper_dc] for the first three
+  // coefficients of a row (the DC term), hyper[hyper_ac] for the rest,
+  // hyper[16..20] = 1/sqrt(1 - b2^t), 1 - b1, b2, 1 - b2, eps; no update
+  // while *skip_if != 0
+  float* sh_out;
+  float* adam_m;
+  float* adam_v;
+  const float* hyper;
+  int hyper_dc, hyper_ac;
+  const uint32_t* skip_if;
 };
 
 // SH part of the colour backward (backward.cu:291-444 semantics, with our
@@ -1267,9 +1280,12 @@ struct ShBwdArgs {
 // dL/dSH accumulates in registers (the degree is a template parameter, so the
 // 3 (D+1)^2 accumulators and the basis are straight-line code) and is written
 // once; the gradient through the view direction is added to dL/dmean3D.
+// Coefficients per row staged in LDS by k_sh_bwd_adam (3 x 16: degree 3).
+constexpr int SHA_W = 48;
+
 template <int D>
 W2W_DEV void sh_bwd_one(const ShBwdArgs& a, const SceneDesc& sd, const FrameDesc* frames, uint32_t g,
-                        uint32_t gg) {
+                        uint32_t gg, float* stage = nullptr) {
   constexpr int NB = (D + 1) * (D + 1);
   float acc[NB][3];
 #pragma unroll
@@ -1319,6 +1335,46 @@ W2W_DEV void sh_bwd_one(const ShBwdArgs& a, const SceneDesc& sd, const FrameDesc
     gm1 += (gdy - pd * y) / len;
     gm2 += (gdz - pd * z) / len;
   }
+  a.dL_dmeans3D[3 * (size_t)gg] += gm0;
+  a.dL_dmeans3D[3 * (size_t)gg + 1] += gm1;
+  a.dL_dmeans3D[3 * (size_t)gg + 2] += gm2;
+  if (a.adam_m && stage && 3 * sd.M <= SHA_W) {
+    // the row's gradient to the workgroup's LDS block (zero above the
+    // degree): k_sh_bwd_adam applies the step with coalesced accesses
+#pragma unroll
+    for (int f = 0; f < NF; f++) stage[f] = acc[f / 3][f % 3];
+    for (int f = NF; f < 3 * sd.M; f++) stage[f] = 0.0f;
+    return;
+  }
+  if (a.adam_m) {
+    // the optimizer step on this row (every coefficient of the row, those
+    // above the degree with a zero gradient), from the coefficients held in
+    // cf: the same arithmetic as the store-wide Adam kernel.  Rows too wide
+    // for the LDS block only: a thread walking its own row touches a
+    // different cache line per lane and access.
+    if (a.skip_if && *a.skip_if) return;
+    const size_t row = sd.sh_off + (size_t)g * sd.M * 3;
+    const float inv_bc2_sqrt = a.hyper[16], one_m_b1 = a.hyper[17], b2 = a.hyper[18];
+    const float one_m_b2 = a.hyper[19], eps = a.hyper[20];
+    const float ns_dc = a.hyper[a.hyper_dc], ns_ac = a.hyper[a.hyper_ac];
+    const int nrow = sd.M * 3;
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      float pv = cf[f], m = a.adam_m[row + f], v = a.adam_v[row + f];
+      adam_one(pv, acc[f / 3][f % 3], m, v, f < 3 ? ns_dc : ns_ac, inv_bc2_sqrt, one_m_b1, b2, one_m_b2, eps);
+      a.sh_out[row + f] = pv;
+      a.adam_m[row + f] = m;
+      a.adam_v[row + f] = v;
+    }
+    for (int f = NF; f < nrow; f++) {
+      float pv = a.sh_out[row + f], m = a.adam_m[row + f], v = a.adam_v[row + f];
+      adam_one(pv, 0.0f, m, v, f < 3 ? ns_dc : ns_ac, inv_bc2_sqrt, one_m_b1, b2, one_m_b2, eps);
+      a.sh_out[row + f] = pv;
+      a.adam_m[row + f] = m;
+      a.adam_v[row + f] = v;
+    }
+    return;
+  }
   float* dsh = a.dL_dsh + sd.sh_off + (size_t)g * sd.M * 3;
   if ((reinterpret_cast<uintptr_t>(dsh) & 15u) == 0) {
 #pragma unroll
@@ -1339,9 +1395,6 @@ W2W_DEV void sh_bwd_one(const ShBwdArgs& a, const SceneDesc& sd, const FrameDesc
     }
   }
   for (int k = NF; k < 3 * (int)sd.M; k++) dsh[k] = 0.0f;
-  a.dL_dmeans3D[3 * (size_t)gg] += gm0;
-  a.dL_dmeans3D[3 * (size_t)gg + 1] += gm1;
-  a.dL_dmeans3D[3 * (size_t)gg + 2] += gm2;
 }
 
 __global__ __launch_bounds__(256) void k_sh_bwd(ShBwdArgs a) {
@@ -1362,6 +1415,62 @@ __global__ __launch_bounds__(256) void k_sh_bwd(ShBwdArgs a) {
       case 2: sh_bwd_one<2>(a, sd, frames, g, gg); break;
       default: sh_bwd_one<3>(a, sd, frames, g, gg); break;
     }
+  }
+}
+
+// k_sh_bwd with the SH coefficients' Adam step (a.adam_m set): a workgroup
+// takes 256 consecutive Gaussians, each thread stages its row's gradient in
+// LDS, then the threads walk the 256 rows coefficient by coefficient (the
+// rows of a store's consecutive Gaussians are contiguous: coalesced) and
+// apply the step to the parameter and its two moments.
+__global__ __launch_bounds__(256) void k_sh_bwd_adam(ShBwdArgs a) {
+  __shared__ float s_g[256][SHA_W + 1];
+  __shared__ uint64_t s_row[256];  // first float of each thread's row
+  __shared__ int s_w[256];         // staged coefficients of the row (0: none)
+  const BatchHeader* hdr = static_cast<const BatchHeader*>(a.table);
+  const uint32_t G = min(hdr->n_gaussians, a.gaussians_cap);
+  const uint32_t n_scenes = min(hdr->n_scenes, a.scenes_cap);
+  const SceneDesc* scenes = table_scenes(a.table);
+  const FrameDesc* frames = table_frames(a.table, a.scenes_cap);
+  const int tid = (int)threadIdx.x;
+  const bool skip = a.skip_if && *a.skip_if;
+  const float inv_bc2_sqrt = a.hyper[16], one_m_b1 = a.hyper[17], b2 = a.hyper[18];
+  const float one_m_b2 = a.hyper[19], eps = a.hyper[20];
+  const float ns_dc = a.hyper[a.hyper_dc], ns_ac = a.hyper[a.hyper_ac];
+  for (uint32_t base = blockIdx.x * 256u; base < G; base += gridDim.x * 256u) {  // uniform
+    const uint32_t bg = base + (uint32_t)tid;
+    s_w[tid] = 0;
+    if (bg < G) {
+      const uint32_t si = upper_index(n_scenes, bg, [&](uint32_t k) { return scenes[k].g_off; });
+      const SceneDesc& sd = scenes[si];
+      const uint32_t g = bg - sd.g_off;
+      const uint32_t gg = sd.g_base + g;
+      switch (sd.D) {
+        case 0: sh_bwd_one<0>(a, sd, frames, g, gg, s_g[tid]); break;
+        case 1: sh_bwd_one<1>(a, sd, frames, g, gg, s_g[tid]); break;
+        case 2: sh_bwd_one<2>(a, sd, frames, g, gg, s_g[tid]); break;
+        default: sh_bwd_one<3>(a, sd, frames, g, gg, s_g[tid]); break;
+      }
+      if (3 * sd.M <= SHA_W) {
+        s_row[tid] = sd.sh_off + (uint64_t)g * sd.M * 3;
+        s_w[tid] = 3 * sd.M;
+      }
+    }
+    __syncthreads();
+    if (!skip) {
+      for (int e = tid; e < 256 * SHA_W; e += 256) {
+        const int r = e / SHA_W, f = e - r * SHA_W;
+        if (f < s_w[r]) {
+          const uint64_t i = s_row[r] + (uint64_t)f;
+          float pv = a.sh_out[i], m = a.adam_m[i], v = a.adam_v[i];
+          adam_one(pv, s_g[r][f], m, v, f < 3 ? ns_dc : ns_ac, inv_bc2_sqrt, one_m_b1, b2, one_m_b2, eps);
+          a.sh_out[i] = pv;
+          a.adam_m[i] = m;
+          a.adam_v[i] = v;
+        }
+      }
+    }
+    __syncthreads();  // the block's rows are rewritten by the next round
   }
 }
 

@@ -674,11 +674,22 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   return launch_status();
 }
 
+// The SH coefficients' optimizer step taken by the SH backward in place of
+// writing dL/dSH (the batched trainer's step; ShBwdArgs has the meaning).
+struct ShAdam {
+  float* sh;
+  float* m;
+  float* v;
+  const float* hyper;
+  int hyper_dc, hyper_ac;
+  const uint32_t* skip_if;
+};
+
 int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const Params& prm,
                  const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
                  float* dL_dmeans2D, float* dL_dcolors, float* dL_dopacity, float* dL_dmeans3D,
                  float* dL_dcov3D, float* dL_dsh, float* dL_dscales, float* dL_drotations,
-                 hipStream_t s) {
+                 hipStream_t s, const ShAdam* sh_adam = nullptr) {
   void* G = ws->geom;
   void* B = ws->binning;
   void* I = ws->image;
@@ -762,9 +773,20 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
     sa.gaussians_cap = (uint32_t)c.gaussians;
     sa.means = prm.means; sa.sh = prm.sh; sa.radii = at<int32_t>(G, p.L.radii);
     sa.dL_dcolors = dL_dcolors; sa.dL_dsh = dL_dsh; sa.dL_dmeans3D = dL_dmeans3D;
+    sa.sh_out = sh_adam ? sh_adam->sh : nullptr;
+    sa.adam_m = sh_adam ? sh_adam->m : nullptr;
+    sa.adam_v = sh_adam ? sh_adam->v : nullptr;
+    sa.hyper = sh_adam ? sh_adam->hyper : nullptr;
+    sa.hyper_dc = sh_adam ? sh_adam->hyper_dc : 0;
+    sa.hyper_ac = sh_adam ? sh_adam->hyper_ac : 0;
+    sa.skip_if = sh_adam ? sh_adam->skip_if : nullptr;
     ProfSpan ps(W2W_KID_SH_BWD, s);
-    hipLaunchKernelGGL(k_sh_bwd, dim3(std::min<uint32_t>(cdiv((uint64_t)c.gaussians, 256), 8192)), dim3(256), 0,
-                       s, sa);
+    if (sh_adam)
+      hipLaunchKernelGGL(k_sh_bwd_adam, dim3(std::min<uint32_t>(cdiv((uint64_t)c.gaussians, 256), 8192)),
+                         dim3(256), 0, s, sa);
+    else
+      hipLaunchKernelGGL(k_sh_bwd, dim3(std::min<uint32_t>(cdiv((uint64_t)c.gaussians, 256), 8192)), dim3(256), 0,
+                         s, sa);
   }
   return launch_status();
 }
@@ -940,6 +962,30 @@ int w2w_backward_batch(const void* table, const W2WCaps* caps, const W2WWorkspac
   return run_backward(p, table, ws, prm, dL_dcolor, dL_ddepth, dL_dalpha, dL_dmeans2D, dL_dcolors,
                       dL_dopacity, dL_dmeans3D, dL_dcov3D, dL_dsh, dL_dscales, dL_drotations,
                       (hipStream_t)stream);
+}
+
+int w2w_backward_batch_sh_adam(const void* table, const W2WCaps* caps, const W2WWorkspace* ws,
+                               const float* means, const float* scales, const float* rotations, float* sh,
+                               const float* dL_dcolor, const float* dL_ddepth, const float* dL_dalpha,
+                               float* dL_dmeans2D, float* dL_dcolors, float* dL_dopacity, float* dL_dmeans3D,
+                               float* dL_dcov3D, float* sh_exp_avg, float* sh_exp_avg_sq,
+                               float* dL_dscales, float* dL_drotations, const float* hyper,
+                               int32_t hyper_dc, int32_t hyper_ac, const uint32_t* skip_if, void* stream) {
+  Plan p;
+  int err;
+  if (!table || !caps_ws(caps, ws, p, err)) return table ? err : W2W_ERR_INVALID;
+  if (!dL_dcolor || !hyper || hyper_dc < 0 || hyper_dc >= 16 || hyper_ac < 0 || hyper_ac >= 16)
+    return W2W_ERR_INVALID;
+  if (caps->items > 0 && (!means || !scales || !rotations || !sh || !dL_dmeans2D || !dL_dcolors ||
+                          !dL_dopacity || !dL_dmeans3D || !dL_dcov3D || !sh_exp_avg || !sh_exp_avg_sq ||
+                          !dL_dscales || !dL_drotations))
+    return W2W_ERR_INVALID;
+  (void)hipGetLastError();
+  Params prm{means, scales, rotations, nullptr, sh};
+  const ShAdam sa{sh, sh_exp_avg, sh_exp_avg_sq, hyper, hyper_dc, hyper_ac, skip_if};
+  return run_backward(p, table, ws, prm, dL_dcolor, dL_ddepth, dL_dalpha, dL_dmeans2D, dL_dcolors,
+                      dL_dopacity, dL_dmeans3D, dL_dcov3D, nullptr, dL_dscales, dL_drotations,
+                      (hipStream_t)stream, &sa);
 }
 
 int w2w_forward(const W2WFrame* f, const W2WWorkspace* ws, float* out_color, float* out_depth,

@@ -183,14 +183,6 @@ struct AdamGraphArgs {
   uint32_t* sticky;         // optional: ORed with 1 when the update was skipped
 };
 
-W2W_DEV void adam_one(float& p, float g, float& m, float& v, float ns, float inv_bc2_sqrt, float one_m_b1,
-                      float b2, float one_m_b2, float eps) {
-  m = m + one_m_b1 * (g - m);
-  v = v * b2 + (one_m_b2 * g) * g;
-  const float denom = __builtin_sqrtf(v) * inv_bc2_sqrt + eps;
-  p = p + ns * (m / denom);
-}
-
 // Step size of element i of the store: its group's rate, or the group's
 // second rate past `split` within each period.  `r` carries (i - group
 // start) mod period from the previous element of the same thread.
