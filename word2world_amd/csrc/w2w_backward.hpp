@@ -922,8 +922,15 @@ W2W_DEV void bwd_tile_coop(const BlendBwdArgs& a, BwdShared& sh, BwdCoopShared& 
   __syncthreads();  // cs.wmax is rewritten by the next tile
 }
 
+// Without the alpha/depth terms (the training step) at least 4 waves per
+// SIMD (128 VGPRs; the compiler otherwise takes 154 and 3 waves, and the
+// walk's dependent chains are then hidden less well: 0.866 -> 0.808 ms per
+// launch at the benchmark size, with 8 bytes per lane spilled).  The AUX
+// variants would spill 116-124 bytes per lane at 4 waves and keep their own
+// choice.
 template <int TS, bool AUX, bool STATS>
-__global__ __launch_bounds__(256) void k_blend_bwd_tile(BlendBwdArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AUX ? 1 : 4, AUX ? 8 : 4)))
+void k_blend_bwd_tile(BlendBwdArgs a) {
   __shared__ BwdShared sh;
   __shared__ BwdCoopShared cs;
   __shared__ uint32_t s_claim;

@@ -1065,8 +1065,11 @@ W2W_DEV bool wave_all(bool p) { return __ballot(!p) == 0ull; }
 // sets how many gathers each lane keeps in flight.  A workgroup claims four
 // consecutive blocks (a quad) at a time and its waves take them one by one,
 // each moving on to the next quad on its own (no workgroup barrier).
+// Occupancy: CW 1 runs 5 waves per SIMD (at most 102 VGPRs; 4 waves: 0.457
+// ms per launch at the benchmark size, 5 waves: 0.427 ms), CW 2 up to 4 (it
+// would spill at 5), CW 4 up to 2 (it holds four records per lane).
 template <int TS, int MODE, int CW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, CW == 4 ? 2 : 4))) void k_blend(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CW == 1 ? 5 : 1, CW == 1 ? 5 : (CW == 2 ? 4 : 2)))) void k_blend(
     BlendArgs a) {
   __shared__ Splat s_chunk[4][64];  // each wave's current sub-chunk, for broadcast reads
   constexpr int CH = 64 * CW;
