@@ -1288,7 +1288,7 @@ int w2w_l2_loss(const void* table, const W2WCaps* caps, const float* color, cons
   hipLaunchKernelGGL(k_l2_loss_tiles, dim3(std::min<uint32_t>(cdiv((uint64_t)caps->tiles, 4), 8192)), dim3(256), 0,
                      s, table, (uint32_t)caps->scenes, (uint32_t)caps->tiles, (int)caps->tile, color,
                      target, dL_dcolor, tile_scratch);
-  hipLaunchKernelGGL(k_loss_frames, dim3(std::min<uint32_t>(cdiv((uint64_t)caps->frames, 4), 1024)),
+  hipLaunchKernelGGL(k_loss_frames, dim3(std::min<uint32_t>((uint32_t)caps->frames, 1024)),
                      dim3(256), 0, s, table, (uint32_t)caps->scenes, (uint32_t)caps->frames,
                      tile_scratch, loss_per_frame);
   return launch_status();

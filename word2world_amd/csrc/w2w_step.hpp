@@ -142,23 +142,33 @@ __global__ __launch_bounds__(256) void k_l2_loss_tiles(const void* __restrict__ 
   }
 }
 
-// Per-frame loss: one wave per frame sums its tiles' shares in a fixed order.
+// Per-frame loss: a workgroup per frame sums its tiles' shares in a fixed
+// order (thread t the tiles t, t + 256, ... in turn, then the threads' sums
+// in a fixed tree through LDS).  A single wave walking 4096 tiles was a chain
+// of dependent loads (0.017 ms for four frames).
 __global__ __launch_bounds__(256) void k_loss_frames(const void* __restrict__ table,
                                                      uint32_t scenes_cap, uint32_t frames_cap,
                                                      const float* __restrict__ tile_loss,
                                                      float* __restrict__ loss) {
+  __shared__ float s_part[256];
   const BatchHeader* hdr = static_cast<const BatchHeader*>(table);
   const FrameDesc* frames = table_frames(table, scenes_cap);
   const uint32_t nf = min(hdr->n_frames, frames_cap);
-  const int lane = (int)(threadIdx.x & 63u);
-  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-  for (uint32_t f = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); f < nf; f += waves) {
+  const int tid = (int)threadIdx.x;
+  for (uint32_t f = blockIdx.x; f < nf; f += gridDim.x) {  // uniform
     const FrameDesc& fd = frames[f];
     const uint32_t n = (uint32_t)(fd.tiles_x * fd.tiles_y);
     float part = 0.0f;
-    for (uint32_t k = (uint32_t)lane; k < n; k += 64) part += tile_loss[fd.tile_off + k];
-    part = wave_reduce_dpp(part);
-    if (lane == 63) loss[f] = part;
+    for (uint32_t k = (uint32_t)tid; k < n; k += 256) part += tile_loss[fd.tile_off + k];
+    s_part[tid] = part;
+    __syncthreads();
+#pragma unroll
+    for (int w = 128; w >= 1; w >>= 1) {
+      if (tid < w) s_part[tid] += s_part[tid + w];
+      __syncthreads();
+    }
+    if (tid == 0) loss[f] = s_part[0];
+    __syncthreads();  // s_part is rewritten for the next frame
   }
 }
 
