@@ -103,7 +103,8 @@ typedef struct W2WLayout {
   size_t table;      /* geom: the one-frame table the single-view entry points write */
   size_t grad_rec;   /* binning: float[capacity][10] per-instance backward partials */
   size_t grad_flag;  /* binning: uint8[capacity] 1 where grad_rec holds a partial */
-  size_t item_grad;  /* geom: float[items][8] per-item record sums of the backward */
+  size_t item_grad;  /* geom: float[5][items]: per item the backward's summed records
+                        (dL/dconic a, b, c, dL/dopacity, dL/ddepth), one plane per field */
   size_t tile_key_bytes; /* 2 when the batch has at most 65536 tiles (uint16 keys), else 4 */
 } W2WLayout;
 

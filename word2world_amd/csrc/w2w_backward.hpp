@@ -1020,7 +1020,7 @@ struct PreBwdArgs {
   const SplatRec* splats;    // per item: mean2D, conic and opacity of the forward
   const uint32_t* rec_base;  // per PRE_TILE items: first record position (plus the splat record's offset)
   const uint32_t* item_frame;  // per item: its frame
-  float* item_grad;          // per item [8]: dL/dconic a, b, c, dL/dopacity, dL/ddepth, 3 unused
+  float* item_grad;          // five planes of items_cap floats: dL/dconic a, b, c, dL/dopacity, dL/ddepth
   float* dL_dmeans2D;    // per item [3]  (NDC units, z = 0)
   float* dL_dcolors;     // per item [3]
   float* dL_dopacity;    // per Gaussian
@@ -1087,9 +1087,14 @@ __global__ __launch_bounds__(256) void k_item_grads(PreBwdArgs a) {
     a.dL_dcolors[3 * (size_t)i] = ga[GA_R];
     a.dL_dcolors[3 * (size_t)i + 1] = ga[GA_G];
     a.dL_dcolors[3 * (size_t)i + 2] = ga[GA_B];
-    float4* ig = reinterpret_cast<float4*>(a.item_grad + 8 * (size_t)i);
-    ig[0] = make_float4(ga[GA_CA], ga[GA_CB], ga[GA_CC], ga[GA_OP]);
-    ig[1] = make_float4(ga[GA_DEPTH], 0.0f, 0.0f, 0.0f);
+    // one plane per field: the per-Gaussian kernel's lanes (a Gaussian's
+    // views are consecutive items) then read 16 bytes apart instead of 128
+    const size_t pl = a.items_cap;
+    a.item_grad[i] = ga[GA_CA];
+    a.item_grad[pl + i] = ga[GA_CB];
+    a.item_grad[2 * pl + i] = ga[GA_CC];
+    a.item_grad[3 * pl + i] = ga[GA_OP];
+    a.item_grad[4 * pl + i] = ga[GA_DEPTH];
   }
 }
 
@@ -1130,13 +1135,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       if (vis) {
         ga[GA_MX] = a.dL_dmeans2D[3 * (size_t)i];
         ga[GA_MY] = a.dL_dmeans2D[3 * (size_t)i + 1];
-        const float4* ig = reinterpret_cast<const float4*>(a.item_grad + 8 * (size_t)i);
-        const float4 t0 = ig[0], t1 = ig[1];
-        ga[GA_CA] = t0.x;
-        ga[GA_CB] = t0.y;
-        ga[GA_CC] = t0.z;
-        ga[GA_OP] = t0.w;
-        ga[GA_DEPTH] = t1.x;
+        const size_t pl = a.items_cap;
+        ga[GA_CA] = a.item_grad[i];
+        ga[GA_CB] = a.item_grad[pl + i];
+        ga[GA_CC] = a.item_grad[2 * pl + i];
+        ga[GA_OP] = a.item_grad[3 * pl + i];
+        ga[GA_DEPTH] = a.item_grad[4 * pl + i];
       }
       if (!vis) continue;
       gop += ga[GA_OP];

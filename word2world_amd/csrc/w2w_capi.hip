@@ -187,7 +187,7 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   p.rtot = take((size_t)p.rtiles * 4);
   L.item_off = take((size_t)p.rtiles * 4);
   p.rlb = take((size_t)cdiv(p.rtiles, SCAN_TILE) * 4 + 4);
-  L.item_grad = take(32 * I);
+  L.item_grad = take(20 * I);
   p.sliced = c.depth_split > 0 && c.depth_split < 1000;
   p.ocount = p.boff = p.slb_far = p.open_bm = p.hint_bm = p.open_units = p.hint_sat = p.big = 0;
   p.tid_far = p.ranges_far = 0;
