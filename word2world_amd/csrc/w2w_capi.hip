@@ -354,14 +354,20 @@ void launch_blend_bwd(const BlendBwdArgs& a, uint32_t tiles_cap, hipStream_t s) 
   }
 }
 
+// the sort histogram's grid: its workgroups flush their digit counts with
+// global atomics, one per nonzero bin, so keys with few distinct digits
+// (depth bits; the far lists' few open tiles) take fewer workgroups
+constexpr uint32_t kHistGrid = 1024;
+constexpr uint32_t kShortHistGrid = 256;
+
 template <typename KT>
 void sort_passes(const KT* kin, const uint32_t* vin, KT* kA, uint32_t* vA, KT* kB, uint32_t* vB,
                  const uint32_t* d_n, uint32_t n_cap, int npasses, int last_bits, bool write_last_keys,
                  uint32_t* hist, uint32_t* status, uint32_t* tickets, int kid_hist, int kid_pass,
-                 hipStream_t s, bool hist_ready = false) {
+                 hipStream_t s, bool hist_ready = false, uint32_t hist_grid = kHistGrid) {
   const uint32_t ntiles = cdiv(n_cap, SORT_TILE);
   if (ntiles == 0) return;
-  const uint32_t hgrid = std::min<uint32_t>(cdiv(n_cap, 256), 1024);
+  const uint32_t hgrid = std::min<uint32_t>(cdiv(n_cap, 256), hist_grid);
   {
     ProfSpan ps(kid_hist, s);
     hipLaunchKernelGGL(k_sort_hist<KT>, dim3(hgrid), dim3(256), 0, s, hist_ready ? nullptr : kin, d_n,
@@ -409,7 +415,7 @@ void bin_near(const Plan& p, const void* table, void* G, void* B, void* I, uint3
                        dim3(256), 0, s, table, (uint32_t)c.scenes, at<uint32_t>(G, p.L.perm),
                        at<uint32_t>(G, p.L.offsets), at<uint32_t>(G, p.L.counts),
                        at<uint2>(G, p.L.rect), at<uint32_t>(G, p.L.item_frame), items_cap, cap,
-                       d_near_end, status + ST_EMIT_TICKET,
+                       d_near_end,
                        at<KT>(B, p.tkey0), at<uint32_t>(B, p.tid0), at<uint32_t>(B, p.thist),
                        p.tpasses, p.tlast_bits);
   }
@@ -440,7 +446,10 @@ void bin_near(const Plan& p, const void* table, void* G, void* B, void* I, uint3
 }
 
 // Far lists of depth-sliced binning (the open tiles the hint missed):
-// emission at the compact offsets, the tile sort and the ranges.
+// emission at the compact offsets, the tile sort and the ranges.  The sort
+// counts the keys itself: they fall on a few open tiles, whose histogram
+// bins every emitting workgroup would otherwise flush with contending
+// atomics.
 template <typename KT>
 void bin_far(const Plan& p, const void* table, void* G, void* B, void* I, uint32_t* status, uint32_t items_cap,
              uint32_t cap, uint32_t tiles_cap, const uint32_t* open_bm, const uint32_t* hint_bm,
@@ -453,12 +462,12 @@ void bin_far(const Plan& p, const void* table, void* G, void* B, void* I, uint32
                        at<uint32_t>(G, p.boff), status + ST_KB, at<uint2>(G, p.L.rect),
                        at<uint32_t>(G, p.L.item_frame), open_bm, hint_bm, nullptr,
                        at<KT>(B, p.scratch_keys()), at<uint32_t>(B, p.scratch_ids()), at<uint32_t>(B, p.thist),
-                       p.tpasses, p.tlast_bits, status + ST_OPEN_UNITS, at<uint32_t>(G, p.big), status + ST_BIG_F);
+                       0, p.tlast_bits, status + ST_OPEN_UNITS, at<uint32_t>(G, p.big), status + ST_BIG_F);
     hipLaunchKernelGGL(k_emit_far_big<KT>, dim3(1024), dim3(256), 0, s, table, (uint32_t)c.scenes, cap,
                        at<uint32_t>(G, p.L.perm), at<uint32_t>(G, p.boff), at<uint2>(G, p.L.rect),
                        at<uint32_t>(G, p.L.item_frame), open_bm, hint_bm, nullptr, at<uint32_t>(G, p.big),
                        status + ST_BIG_F, items_cap, at<KT>(B, p.scratch_keys()),
-                       at<uint32_t>(B, p.scratch_ids()), at<uint32_t>(B, p.thist), p.tpasses, p.tlast_bits,
+                       at<uint32_t>(B, p.scratch_ids()), at<uint32_t>(B, p.thist), 0, p.tlast_bits,
                        status + ST_OPEN_UNITS);
   }
   sort_passes<KT>(at<KT>(B, p.scratch_keys()), at<uint32_t>(B, p.scratch_ids()),
@@ -466,7 +475,7 @@ void bin_far(const Plan& p, const void* table, void* G, void* B, void* I, uint32
                   at<KT>(B, p.far_scratch_keys()), at<uint32_t>(B, p.far_scratch_ids()),
                   status + ST_KB, cap, p.tpasses, p.tlast_bits, true, at<uint32_t>(B, p.thist),
                   at<uint32_t>(B, p.tstatus), status + ST_TTICKET, W2W_KID_TSORT_HIST,
-                  W2W_KID_TSORT_PASS, s, /*hist_ready=*/true);
+                  W2W_KID_TSORT_PASS, s, /*hist_ready=*/false, kShortHistGrid);
   ProfSpan ps(W2W_KID_RANGES, s);
   hipLaunchKernelGGL(k_ranges<KT>, dim3(std::min<uint32_t>(cdiv(cap, 256), 4096)), dim3(256), 0, s,
                      at<KT>(B, p.far_keys()), status + ST_KB, table, cap, tiles_cap, ranges_far);
@@ -574,7 +583,8 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
                           at<uint32_t>(G, p.dkey0), at<uint32_t>(G, p.L.perm),
                           at<uint32_t>(G, p.dkey1), at<uint32_t>(G, p.id1), d_items, items_cap, 4,
                           8, false, at<uint32_t>(G, p.dhist), at<uint32_t>(G, p.dstatus),
-                          status + ST_DTICKET, W2W_KID_DSORT_HIST, W2W_KID_DSORT_PASS, s);
+                          status + ST_DTICKET, W2W_KID_DSORT_HIST, W2W_KID_DSORT_PASS, s, false,
+                          kShortHistGrid);
     if (cap > 0) {
       // emission offsets: the tile counts gathered in depth order and
       // scanned.  Depth-sliced binning: the near slice is binned here, the

@@ -53,7 +53,6 @@ enum StatusWord : int {
   ST_DTICKET = 8,      // 4 depth-sort tickets
   ST_TTICKET = 16,     // up to 4 tile-sort tickets
   ST_SCAN_TICKET = 24,
-  ST_EMIT_TICKET = 25,
   ST_BLEND_TICKET = 26,
   ST_BWD_TICKET = 27,
   ST_SCANB_TICKET = 28,
@@ -784,8 +783,10 @@ __global__ __launch_bounds__(256) void k_emit_far_big(const void* __restrict__ t
 
 // Instance emission in depth order.  A workgroup owns depth-sorted positions
 // [256b, 256b+256) and the contiguous instance range they cover; each thread
-// strides over instances and finds its owner by binary search in LDS.  Chunks
-// are claimed through a ticket (any grid size).
+// strides over instances and finds its owner by binary search in LDS.
+// Workgroups take chunks in a fixed order (any grid size): a ticket would
+// cost a contended atomic per chunk, which at a few thousand chunks per
+// launch costs more than the imbalance it would even out.
 // Emission also histograms the tile keys for the tile sort (every digit of
 // every pass, hist[pass][256], pre-zeroed), so the sort does not read the
 // keys an extra time.
@@ -800,13 +801,12 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
                                               const uint32_t* __restrict__ item_frame,
                                               uint32_t items_cap, uint32_t capacity,
                                               const uint32_t* __restrict__ d_end,
-                                              uint32_t* __restrict__ ticket,
                                               KT* __restrict__ tile_keys,
                                               uint32_t* __restrict__ ids,
                                               uint32_t* __restrict__ hist, int hist_passes,
                                               int hist_last_bits) {
   __shared__ uint32_t s_start[256], s_g[256], s_x0[256], s_y0[256], s_w[256], s_tx[256], s_toff[256];
-  __shared__ uint32_t s_end, s_chunk;
+  __shared__ uint32_t s_end;
   __shared__ uint32_t s_hist[4][256];
   const BatchHeader* hdr = static_cast<const BatchHeader*>(table);
   const FrameDesc* frames = table_frames(table, scenes_cap);
@@ -815,11 +815,7 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
   const uint32_t nchunks = (end + 255) / 256;
   const int tid = threadIdx.x;
   for (int i = tid; i < 4 * 256; i += 256) (&s_hist[0][0])[i] = 0u;
-  for (;;) {
-    if (tid == 0) s_chunk = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t chunk = s_chunk;
-    if (chunk >= nchunks) break;
+  for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
     const uint32_t cbase = chunk * 256;
     const uint32_t i = cbase + tid;
     const uint32_t last = min(end, cbase + 256) - 1;
