@@ -90,12 +90,14 @@ HBM_STAGES_NULL = ("fill",)  # buffer clears: no algorithmic bytes
 SPLIT_CANDIDATES = (0.0, 0.2, 0.25, 0.3, 0.35, 0.4)
 
 
-def stage_cost(name, P, M, V, K, T, HW, stats_view, tpasses, dpasses=4):
+def stage_cost(name, P, M, V, K, T, HW, stats_view, tpasses, dpasses=4, fused_sh=True):
     """Algorithmic bytes (or flops) of a stage over a whole step (all its
     launches): V views, K tile instances sorted (over both slices when the
     binning is depth-sliced), T tiles and HW pixels summed over the views,
     tile sorts of `tpasses` passes.  Adam, activation, the SH backward and
-    the statistics are per Gaussian or per item."""
+    the statistics are per Gaussian or per item.  fused_sh: the SH backward
+    takes the SH coefficients' Adam step (their parameter and both moments
+    read and written, no gradient), which the Adam stage then leaves out."""
     p_in = 44 + 12 * M
     I = V * P  # items
     table = {
@@ -111,9 +113,9 @@ def stage_cost(name, P, M, V, K, T, HW, stats_view, tpasses, dpasses=4):
         "ranges": ("hbm", 4 * K + 8 * T),
         "loss": ("hbm", 36 * HW),               # render and target in, dL/dcolor out
         "preprocess_bwd": ("hbm", I * (2 * p_in + 36)),
-        "sh_bwd": ("hbm", P * (24 * M + 24) + 12 * I),
+        "sh_bwd": ("hbm", P * ((72 if fused_sh else 24) * M + 24) + 12 * I),
         "densify": ("hbm", 16 * I + 16 * P),
-        "adam": ("hbm", 28 * (11 + 3 * M) * P),
+        "adam": ("hbm", 28 * (11 + (0 if fused_sh else 3 * M)) * P),
     }
     if name == "blend":
         return "valu", V * (18 * stats_view[0] + 12 * stats_view[1])
