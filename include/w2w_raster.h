@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define W2W_ABI_VERSION 8
+#define W2W_ABI_VERSION 9
 
 enum {
   W2W_OK = 0,
@@ -102,7 +102,7 @@ typedef struct W2WLayout {
   size_t item_frame; /* geom: uint32[items] frame of each item */
   size_t table;      /* geom: the one-frame table the single-view entry points write */
   size_t grad_rec;   /* binning: float[capacity][10] per-instance backward partials */
-  size_t grad_flag;  /* binning: uint8[capacity] 1 where grad_rec holds a partial */
+  size_t grad_flag;  /* binning: uint32[ceil(capacity / 32)], bit k set where grad_rec holds a partial */
   size_t item_grad;  /* geom: float[5][items]: per item the backward's summed records
                         (dL/dconic a, b, c, dL/dopacity, dL/ddepth), one plane per field */
   size_t tile_key_bytes; /* 2 when the batch has at most 65536 tiles (uint16 keys), else 4 */

@@ -23,7 +23,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
     assert set(_lib.EXPORTS) <= set(names)
-    assert L.w2w_version() == 8
+    assert L.w2w_version() == 9
 
 
 def test_layout_is_pure_host_logic():

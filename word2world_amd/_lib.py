@@ -76,7 +76,7 @@ KERNEL_IDS = {0: "fill", 1: "preprocess", 2: "depth_sort_hist", 3: "depth_sort_p
               24: "tile_order"}
 
 _lib = None
-ABI_VERSION = 8  # W2W_ABI_VERSION of include/w2w_raster.h
+ABI_VERSION = 9  # W2W_ABI_VERSION of include/w2w_raster.h
 
 
 def load(path: str = LIB_PATH):

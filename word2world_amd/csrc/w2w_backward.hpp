@@ -141,7 +141,7 @@ struct BlendBwdArgs {
   const float* dL_dalpha;   // per frame [H,W] at pix_off; may be null (zero)
   const float* dL_ddepth;   // per frame [H,W] at pix_off; may be null (zero)
   float* rec;               // [capacity][GR_FIELDS] per-instance partials
-  uint8_t* flag;            // [capacity], zeroed: 1 where rec was written
+  uint32_t* flag;           // [capacity / 32] bits, zeroed: bit k set where record k was written
   // depth-sliced binning: a tile's list is its near list followed by its far
   // list (null: one list)
   const uint2* ranges_far;
@@ -410,7 +410,7 @@ void k_blend_bwd(BlendBwdArgs a) {
             }
           }
           *reinterpret_cast<float2*>(a.rec + (size_t)pos * GR_FIELDS + 2 * fp) = make_float2(s0, s1);
-          if (fp == 0) a.flag[pos] = 1;
+          if (fp == 0) atomicOr(a.flag + (pos >> 5), 1u << (pos & 31u));
         }
       }
       __syncthreads();  // the slots and masks are rewritten by the next chunk
@@ -759,7 +759,7 @@ W2W_DEV void bwd_tile(const BlendBwdArgs& a, BwdShared& sh, const FrameDesc& fd,
               r[4 + fa] = s1;
               if ((row & 1) == 0) r[8 + (row >> 1)] = s2;
             }
-            if (lane == 0) a.flag[pos] = 1;
+            if (lane == 0) atomicOr(a.flag + (pos >> 5), 1u << (pos & 31u));
           }
         }
         if (jn < 0) break;
@@ -914,7 +914,7 @@ W2W_DEV void bwd_tile_coop(const BlendBwdArgs& a, BwdShared& sh, BwdCoopShared& 
           }
         }
         *reinterpret_cast<float2*>(a.rec + (size_t)pos * GR_FIELDS + 2 * fp) = make_float2(v0, v1);
-        if (fp == 0) a.flag[pos] = 1;
+        if (fp == 0) atomicOr(a.flag + (pos >> 5), 1u << (pos & 31u));
       }
     }
     __syncthreads();  // the sums and masks are rewritten by the next chunk
@@ -1015,7 +1015,7 @@ struct PreBwdArgs {
   const int32_t* radii;  // per item
   const uint32_t* counts;    // per item: instances emitted
   const float* rec;          // [capacity][GR_FIELDS] per-instance partials
-  const uint8_t* flag;       // [capacity]
+  const uint32_t* flag;      // [capacity / 32] bits
   uint32_t capacity;
   const SplatRec* splats;    // per item: mean2D, conic and opacity of the forward
   const uint32_t* rec_base;  // per PRE_TILE items: first record position (plus the splat record's offset)
@@ -1052,13 +1052,13 @@ __global__ __launch_bounds__(256) void k_item_grads(PreBwdArgs a) {
     if (vis) {
       const uint32_t k0 = a.rec_base[i / PRE_TILE] + a.splats[i].item_off;  // the item's first record
       const uint32_t k1 = min(k0 + a.counts[i], a.capacity);
-      for (uint32_t kw = k0 & ~7u; kw < k1; kw += 8) {
-        uint64_t f8 = *reinterpret_cast<const uint64_t*>(a.flag + kw);
-        if (kw < k0) f8 &= ~0ull << (8 * (k0 - kw));
-        if (k1 - kw < 8) f8 &= ~(~0ull << (8 * (k1 - kw)));
-        while (f8) {
-          const uint32_t b = (uint32_t)__builtin_ctzll(f8) >> 3;
-          f8 &= ~(0xFFull << (8 * b));
+      for (uint32_t kw = k0 & ~31u; kw < k1; kw += 32) {
+        uint32_t f32 = a.flag[kw >> 5];
+        if (kw < k0) f32 &= ~0u << (k0 - kw);
+        if (k1 - kw < 32) f32 &= (1u << (k1 - kw)) - 1u;
+        while (f32) {
+          const uint32_t b = (uint32_t)__builtin_ctz(f32);
+          f32 &= f32 - 1u;
           const float2* r = reinterpret_cast<const float2*>(a.rec + (size_t)(kw + b) * GR_FIELDS);
 #pragma unroll
           for (int f = 0; f < GR_FIELDS / 2; f++) {

@@ -211,7 +211,7 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   p.thist = take(4 * 4 * RADIX);
   p.tstatus = take((size_t)p.tpasses * p.ctiles * RADIX * 4);
   L.grad_rec = take(sizeof(float) * GR_FIELDS * C);
-  L.grad_flag = take(C);  // take() rounds the end up to the 256-byte alignment
+  L.grad_flag = take(4 * cdiv((uint64_t)C, 32));  // one bit per record
   if (p.sliced) p.tid_far = take(4 * C);
   L.binning_bytes = o;
   L.sorted_tiles = (p.tpasses & 1) ? p.tkey1 : p.tkey0;
@@ -707,10 +707,10 @@ int run_backward(const Plan& p, const void* table, const W2WWorkspace* ws, const
   if (c.items == 0) return W2W_OK;
   uint32_t* status = at<uint32_t>(G, p.status);
   float* rec = at<float>(B, p.L.grad_rec);
-  uint8_t* flag = at<uint8_t>(B, p.L.grad_flag);
+  uint32_t* flag = at<uint32_t>(B, p.L.grad_flag);
   {
     FillBatch fb;
-    fb.add(reinterpret_cast<uint32_t*>(flag), cdiv((uint64_t)c.capacity, 4), 0u, s);
+    fb.add(flag, cdiv((uint64_t)c.capacity, 32), 0u, s);
     fb.add(status + ST_BWD_TICKET, 1, 0u, s);
     fb.add(status + ST_ORDER_N, 1 + 2 * kCostBuckets, 0u, s);  // count, histogram, cursors
     fb.launch(s);
