@@ -117,7 +117,7 @@ struct Plan {
   uint32_t dtiles, stiles, ctiles;
   int tpasses, tlast_bits;
   bool key16;  // tile keys of 16 bits (at most 65536 tiles)
-  size_t status, table, dkey0, dkey1, id1, dhist, dstatus, slb;  // geom
+  size_t status, table, dkey0, dkey1, dkey2, id1, id2, dhist, dstatus, slb;  // geom
   size_t tkey0, tkey1, tid0, tid1, thist, tstatus;               // binning
   // depth-sliced binning (c.depth_split in 1..999)
   bool sliced;
@@ -176,6 +176,8 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   p.dkey1 = take(4 * I);
   L.perm = take(4 * I);
   p.id1 = take(4 * I);
+  p.dkey2 = take(4 * I);  // the depth sort's third pair (its top pass skipped)
+  p.id2 = take(4 * I);
   L.offsets = take(4 * I);
   L.item_frame = take(4 * I);
   p.dhist = take(4 * 4 * RADIX);
@@ -364,7 +366,8 @@ template <typename KT>
 void sort_passes(const KT* kin, const uint32_t* vin, KT* kA, uint32_t* vA, KT* kB, uint32_t* vB,
                  const uint32_t* d_n, uint32_t n_cap, int npasses, int last_bits, bool write_last_keys,
                  uint32_t* hist, uint32_t* status, uint32_t* tickets, int kid_hist, int kid_pass,
-                 hipStream_t s, bool hist_ready = false, uint32_t hist_grid = kHistGrid) {
+                 hipStream_t s, bool hist_ready = false, uint32_t hist_grid = kHistGrid, KT* kC = nullptr,
+                 uint32_t* vC = nullptr) {
   const uint32_t ntiles = cdiv(n_cap, SORT_TILE);
   if (ntiles == 0) return;
   const uint32_t hgrid = std::min<uint32_t>(cdiv(n_cap, 256), hist_grid);
@@ -376,19 +379,37 @@ void sort_passes(const KT* kin, const uint32_t* vin, KT* kA, uint32_t* vA, KT* k
   const uint32_t pgrid = std::min<uint32_t>(ntiles, 1024);  // 2 workgroups of 512 per CU (LDS)
   const KT* ks = kin;
   const uint32_t* vs = vin;
+  // with a third buffer pair C and four passes, a top pass found to be the
+  // identity on the device is skipped: the other three go kin -> C -> B -> A
+  const bool skippable = kC && vC && npasses == 4;
+  const KT* const skin[3] = {kin, kC, kB};
+  const uint32_t* const svin[3] = {vin, vC, vB};
+  KT* const skout[3] = {kC, kB, kA};
+  uint32_t* const svout[3] = {vC, vB, vA};
   for (int p = 0; p < npasses; p++) {
     const bool toA = ((npasses - 1 - p) % 2) == 0;
     KT* kd = toA ? kA : kB;
     uint32_t* vd = toA ? vA : vB;
     const int nbits = (p == npasses - 1) ? last_bits : 8;
     uint32_t* st = status + (size_t)p * ntiles * RADIX;
+    TopSkip<KT> sk{nullptr, 0, nullptr, nullptr, nullptr, nullptr};
+    if (skippable) {
+      sk.top_hist = hist + (npasses - 1) * RADIX;
+      sk.is_top = p == npasses - 1;
+      if (p < npasses - 1) {
+        sk.kin = skin[p];
+        sk.vin = svin[p];
+        sk.kout = skout[p];
+        sk.vout = svout[p];
+      }
+    }
     ProfSpan ps(kid_pass, s);
     if (p < npasses - 1 || write_last_keys)
       hipLaunchKernelGGL((k_sort_pass<KT, true>), dim3(pgrid), dim3(SORT_THREADS), 0, s, ks, vs, kd,
-                         vd, d_n, n_cap, 8 * p, nbits, hist + p * RADIX, st, tickets + p);
+                         vd, d_n, n_cap, 8 * p, nbits, hist + p * RADIX, st, tickets + p, sk);
     else
       hipLaunchKernelGGL((k_sort_pass<KT, false>), dim3(pgrid), dim3(SORT_THREADS), 0, s, ks, vs,
-                         kd, vd, d_n, n_cap, 8 * p, nbits, hist + p * RADIX, st, tickets + p);
+                         kd, vd, d_n, n_cap, 8 * p, nbits, hist + p * RADIX, st, tickets + p, sk);
     ks = kd;
     vs = vd;
   }
@@ -578,13 +599,14 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
                          at<uint32_t>(G, p.L.item_off), at<uint32_t>(G, p.rlb) + 1, status + ST_REC_TICKET,
                          status + ST_K, status + ST_OVERFLOW, nullptr, p.rtiles, cap);
     }
-    // depth sort: (depth bits, item) over the items, 4 passes, result in dkey0 / perm
+    // depth sort: (depth bits, item) over the items, 4 passes (3 when every
+    // key has the same top byte), result in perm
     sort_passes<uint32_t>(at<uint32_t>(G, p.dkey0), at<uint32_t>(G, p.L.perm),
                           at<uint32_t>(G, p.dkey0), at<uint32_t>(G, p.L.perm),
                           at<uint32_t>(G, p.dkey1), at<uint32_t>(G, p.id1), d_items, items_cap, 4,
                           8, false, at<uint32_t>(G, p.dhist), at<uint32_t>(G, p.dstatus),
                           status + ST_DTICKET, W2W_KID_DSORT_HIST, W2W_KID_DSORT_PASS, s, false,
-                          kShortHistGrid);
+                          kShortHistGrid, at<uint32_t>(G, p.dkey2), at<uint32_t>(G, p.id2));
     if (cap > 0) {
       // emission offsets: the tile counts gathered in depth order and
       // scanned.  Depth-sliced binning: the near slice is binned here, the

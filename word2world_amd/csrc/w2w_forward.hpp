@@ -36,7 +36,12 @@
 
 namespace w2w {
 
-constexpr uint32_t CULLED_DEPTH_KEY = 0xffffffffu;
+// Culled items' depth key: the largest key below 2.0f, so culled items sort
+// behind every kept item of depth bits under 2.0f (ndc depth in front of the
+// far plane) and share the top byte with them, which lets the depth sort skip
+// its top pass.  Where culled items land among kept ones changes nothing:
+// they have no instances.
+constexpr uint32_t CULLED_DEPTH_KEY = 0x3fffffffu;
 
 // status block words (first 256 bytes of the geometry buffer)
 enum StatusWord : int {
@@ -868,13 +873,28 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
             ty++;
           }
         }
-        const uint32_t key = s_toff[lo] + ty * s_tx[lo] + tx;
-        key4[m] = key;
+        key4[m] = s_toff[lo] + ty * s_tx[lo] + tx;
         id4[m] = s_g[lo];
-        for (int p = 0; p < hist_passes; p++) {
-          const int bits = (p == hist_passes - 1) ? hist_last_bits : 8;
-          atomicAdd(&s_hist[p][(key >> (8 * p)) & ((1u << bits) - 1u)], 1u);
+      }
+      // the group's digits, runs of equal digits counted with one atomic
+      // (the upper digits of a row's consecutive tiles mostly agree)
+      for (int p = 0; p < hist_passes; p++) {
+        const int bits = (p == hist_passes - 1) ? hist_last_bits : 8;
+        const uint32_t dmask = (1u << bits) - 1u;
+        uint32_t rd = 0u, rc = 0u;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          if (q + m < kb || q + m >= ke) continue;
+          const uint32_t d = (key4[m] >> (8 * p)) & dmask;
+          if (rc && d == rd) {
+            rc++;
+          } else {
+            if (rc) atomicAdd(&s_hist[p][rd], rc);
+            rd = d;
+            rc = 1u;
+          }
         }
+        if (rc) atomicAdd(&s_hist[p][rd], rc);
       }
       if (q >= kb && q + 3 < ke) {
         store_keys4(tile_keys + q, key4[0], key4[1], key4[2], key4[3]);

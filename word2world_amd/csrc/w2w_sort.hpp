@@ -50,14 +50,44 @@ __global__ __launch_bounds__(256) void k_sort_hist(const KT* __restrict__ keys,
   __syncthreads();
   const uint32_t stride = gridDim.x * blockDim.x;
   if (keys) {  // null: the histogram was accumulated by the producer of the keys
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-      KT k = keys[i];
+    auto count = [&](KT k) {
       for (int p = 0; p < npasses; p++) {
-        int bits = (p == npasses - 1) ? last_bits : 8;
-        uint32_t d = (uint32_t)(k >> (begin_bit + 8 * p)) & ((1u << bits) - 1u);
+        const int bits = (p == npasses - 1) ? last_bits : 8;
+        const uint32_t d = (uint32_t)(k >> (begin_bit + 8 * p)) & ((1u << bits) - 1u);
         atomicAdd(&sh[p][d], 1u);
       }
+    };
+    // 16-byte loads, two per thread in flight, for keys of up to 32 bits at
+    // a 16-byte aligned address: the grid is small (its workgroups flush
+    // with global atomics), so a load per key at a time would leave the
+    // kernel waiting on memory latency
+    uint32_t head = 0;
+    if constexpr (sizeof(KT) <= 4) {
+      if ((reinterpret_cast<uintptr_t>(keys) & 15u) == 0) {
+        constexpr uint32_t VEC = 16 / sizeof(KT);
+        const uint32_t nv = n / VEC;
+        head = nv * VEC;
+        const uint4* kv = reinterpret_cast<const uint4*>(keys);
+        auto count_word = [&](uint32_t w) {
+          if constexpr (sizeof(KT) == 2) {
+            count((KT)(w & 0xFFFFu));
+            count((KT)(w >> 16));
+          } else {
+            count((KT)w);
+          }
+        };
+        for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nv; j += 2 * stride) {
+          const bool two = j + stride < nv;
+          const uint4 va = kv[j];
+          const uint4 vb = two ? kv[j + stride] : make_uint4(0u, 0u, 0u, 0u);
+          count_word(va.x); count_word(va.y); count_word(va.z); count_word(va.w);
+          if (two) {
+            count_word(vb.x); count_word(vb.y); count_word(vb.z); count_word(vb.w);
+          }
+        }
+      }
     }
+    for (uint32_t i = head + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) count(keys[i]);
     __syncthreads();
     for (int i = threadIdx.x; i < npasses * RADIX; i += blockDim.x) {
       uint32_t v = (&sh[0][0])[i];
@@ -108,12 +138,24 @@ W2W_DEV uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp /*[T/64]*/) {
   return wpre + inc - v;
 }
 
+// Skipping the top pass of a sort whose keys share their top digit (null
+// top_hist: never): the pass's buffers when it is skipped.
+template <typename KT>
+struct TopSkip {
+  const uint32_t* top_hist;  // hist[npasses - 1][256]
+  int is_top;
+  const KT* kin;
+  const uint32_t* vin;
+  KT* kout;
+  uint32_t* vout;
+};
+
 template <typename KT, bool WRITE_KEYS>
 __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(
     const KT* __restrict__ kin, const uint32_t* __restrict__ vin, KT* __restrict__ kout,
     uint32_t* __restrict__ vout, const uint32_t* __restrict__ d_n, uint32_t n_cap, int shift,
     int nbits, const uint32_t* __restrict__ hist, uint32_t* __restrict__ status,
-    uint32_t* __restrict__ ticket) {
+    uint32_t* __restrict__ ticket, TopSkip<KT> skip) {
   __shared__ KT s_k[SORT_TILE];
   __shared__ uint32_t s_v[SORT_TILE];
   __shared__ uint32_t s_wcnt[SORT_WAVES][RADIX];
@@ -127,6 +169,18 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(
   const uint32_t ntiles = (n + SORT_TILE - 1) / SORT_TILE;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t dmask = (1u << nbits) - 1u;
+  if (skip.top_hist) {
+    // every key has the same top digit (one bin holds all n): the top pass
+    // would keep the order as it is, so it returns at once and the other
+    // passes take the chain that ends in the output without it
+    if (__syncthreads_or(tid < RADIX && skip.top_hist[tid] == n)) {
+      if (skip.is_top) return;
+      kin = skip.kin;
+      vin = skip.vin;
+      kout = skip.kout;
+      vout = skip.vout;
+    }
+  }
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
   // exclusive prefix of the global digit histogram (thread d < RADIX: digit d)
