@@ -210,7 +210,7 @@ bool make_plan(const W2WCaps& c, Plan& p) {
   p.tkey1 = take(4 * C);
   p.tid0 = take(4 * C);
   p.tid1 = take(4 * C);
-  p.thist = take(4 * 4 * RADIX);
+  p.thist = take(4 * (size_t)kHistCopies * kHistCopyStride);
   p.tstatus = take((size_t)p.tpasses * p.ctiles * RADIX * 4);
   L.grad_rec = take(sizeof(float) * GR_FIELDS * C);
   L.grad_flag = take(4 * cdiv((uint64_t)C, 32));  // one bit per record
@@ -367,14 +367,14 @@ void sort_passes(const KT* kin, const uint32_t* vin, KT* kA, uint32_t* vA, KT* k
                  const uint32_t* d_n, uint32_t n_cap, int npasses, int last_bits, bool write_last_keys,
                  uint32_t* hist, uint32_t* status, uint32_t* tickets, int kid_hist, int kid_pass,
                  hipStream_t s, bool hist_ready = false, uint32_t hist_grid = kHistGrid, KT* kC = nullptr,
-                 uint32_t* vC = nullptr) {
+                 uint32_t* vC = nullptr, uint32_t copies = 1) {
   const uint32_t ntiles = cdiv(n_cap, SORT_TILE);
   if (ntiles == 0) return;
   const uint32_t hgrid = std::min<uint32_t>(cdiv(n_cap, 256), hist_grid);
   {
     ProfSpan ps(kid_hist, s);
     hipLaunchKernelGGL(k_sort_hist<KT>, dim3(hgrid), dim3(256), 0, s, hist_ready ? nullptr : kin, d_n,
-                       n_cap, 0, npasses, last_bits, hist, status, ntiles, tickets);
+                       n_cap, 0, npasses, last_bits, hist, status, ntiles, tickets, copies);
   }
   const uint32_t pgrid = std::min<uint32_t>(ntiles, 1024);  // 2 workgroups of 512 per CU (LDS)
   const KT* ks = kin;
@@ -459,7 +459,7 @@ void bin_near(const Plan& p, const void* table, void* G, void* B, void* I, uint3
                   at<KT>(B, p.scratch_keys()), at<uint32_t>(B, p.scratch_ids()), d_kn,
                   cap, p.tpasses, p.tlast_bits, true, at<uint32_t>(B, p.thist),
                   at<uint32_t>(B, p.tstatus), status + ST_TTICKET, W2W_KID_TSORT_HIST,
-                  W2W_KID_TSORT_PASS, s, /*hist_ready=*/true);
+                  W2W_KID_TSORT_PASS, s, /*hist_ready=*/true, kHistGrid, nullptr, nullptr, kHistCopies);
   const uint32_t rgrid = std::min<uint32_t>(cdiv(cap, 256), 4096);
   ProfSpan ps(W2W_KID_RANGES, s);
   hipLaunchKernelGGL(k_ranges<KT>, dim3(rgrid), dim3(256), 0, s, at<KT>(B, p.L.sorted_tiles),
@@ -532,7 +532,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
     fb.add(at<uint32_t>(G, p.rlb), cdiv(p.rtiles, SCAN_TILE) + 1, 0u, s);
     fb.add(at<uint32_t>(G, p.dhist), 4 * RADIX, 0u, s);
     fb.add(at<uint32_t>(G, p.slb), p.stiles + 1, 0u, s);
-    fb.add(at<uint32_t>(B, p.thist), 4 * RADIX, 0u, s);
+    fb.add(at<uint32_t>(B, p.thist), kHistCopies * kHistCopyStride, 0u, s);
   }
   if (hinted) fb.add(at<uint32_t>(G, p.open_bm), tile_words, 0u, s);
   fb.launch(s);
@@ -684,7 +684,7 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
   {
     FillBatch fb;
     fb.add(at<uint32_t>(G, p.slb_far), p.stiles + 1, 0u, s);
-    fb.add(at<uint32_t>(B, p.thist), 4 * RADIX, 0u, s);  // the near sort is done with it
+    fb.add(at<uint32_t>(B, p.thist), kHistCopies * kHistCopyStride, 0u, s);  // the near sort is done with it
     fb.add(reinterpret_cast<uint32_t*>(ranges_far), 2 * (uint64_t)c.tiles, 0u, s);
     fb.launch(s);
   }

@@ -719,7 +719,7 @@ __global__ __launch_bounds__(256) void k_emit_far(const void* __restrict__ table
   __syncthreads();
   for (int k = threadIdx.x; k < hist_passes * 256; k += blockDim.x) {
     const uint32_t v = (&s_hist[0][0])[k];
-    if (v) atomicAdd(hist + k, v);
+    if (v) atomicAdd(hist + (blockIdx.x % kHistCopies) * kHistCopyStride + k, v);
   }
 }
 
@@ -782,7 +782,7 @@ __global__ __launch_bounds__(256) void k_emit_far_big(const void* __restrict__ t
   __syncthreads();
   for (int k = threadIdx.x; k < hist_passes * 256; k += blockDim.x) {
     const uint32_t v = (&s_hist[0][0])[k];
-    if (v) atomicAdd(hist + k, v);
+    if (v) atomicAdd(hist + (blockIdx.x % kHistCopies) * kHistCopyStride + k, v);
   }
 }
 
@@ -913,7 +913,7 @@ __global__ __launch_bounds__(256) void k_emit(const void* __restrict__ table, ui
   }
   for (int i = tid; i < hist_passes * 256; i += 256) {
     const uint32_t v = (&s_hist[0][0])[i];
-    if (v) atomicAdd(hist + i, v);
+    if (v) atomicAdd(hist + (blockIdx.x % kHistCopies) * kHistCopyStride + i, v);
   }
 }
 

@@ -23,6 +23,13 @@ constexpr int SORT_WAVES = SORT_THREADS / 64;
 constexpr int SORT_ITEMS = 12;
 constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;  // 6144
 constexpr int RADIX = 256;
+// A histogram accumulated by the producer of the keys may come in several
+// copies, hist[copy][4][256]: each producing workgroup flushes its counts
+// into copy blockIdx.x % copies, so a word takes 1/copies of the flush
+// atomics (same-address atomics from many workgroups serialise).
+// k_sort_hist sums the copies into the first before the passes read it.
+constexpr uint32_t kHistCopyStride = 4 * RADIX;
+constexpr uint32_t kHistCopies = 16;  // the tile sort's, from the emission
 constexpr uint32_t LB_AGG = 1u << 30;
 constexpr uint32_t LB_INC = 2u << 30;
 constexpr uint32_t LB_VAL = (1u << 30) - 1u;
@@ -33,7 +40,8 @@ W2W_DEV uint32_t sort_count(const uint32_t* d_n, uint32_t n_cap) {
   return n < n_cap ? n : n_cap;
 }
 
-// Histogram of every digit of every pass (hist[pass][256], pre-zeroed), and
+// Histogram of every digit of every pass (hist[pass][256], pre-zeroed; with
+// null keys the producer's copies, at most kHistCopies, summed), and
 // zeroing of the look-back words status[pass][tile][256] (pass stride
 // tiles_stride tiles) and the per-pass tickets.
 template <typename KT>
@@ -43,7 +51,7 @@ __global__ __launch_bounds__(256) void k_sort_hist(const KT* __restrict__ keys,
                                                    uint32_t* __restrict__ hist,
                                                    uint32_t* __restrict__ status,
                                                    uint32_t tiles_stride,
-                                                   uint32_t* __restrict__ tickets) {
+                                                   uint32_t* __restrict__ tickets, uint32_t copies) {
   __shared__ uint32_t sh[8][RADIX];
   const uint32_t n = sort_count(d_n, n_cap);
   for (int i = threadIdx.x; i < 8 * RADIX; i += blockDim.x) (&sh[0][0])[i] = 0;
@@ -92,6 +100,16 @@ __global__ __launch_bounds__(256) void k_sort_hist(const KT* __restrict__ keys,
     for (int i = threadIdx.x; i < npasses * RADIX; i += blockDim.x) {
       uint32_t v = (&sh[0][0])[i];
       if (v) atomicAdd(&hist[i], v);
+    }
+  } else if (copies > 1 && blockIdx.x == 0) {
+    for (int i = threadIdx.x; i < npasses * RADIX; i += blockDim.x) {
+      uint32_t v[kHistCopies];
+#pragma unroll
+      for (uint32_t c = 0; c < kHistCopies; c++) v[c] = c < copies ? hist[c * kHistCopyStride + i] : 0u;
+      uint32_t sum = 0;
+#pragma unroll
+      for (uint32_t c = 0; c < kHistCopies; c++) sum += v[c];
+      hist[i] = sum;
     }
   }
   // pass p owns status[p * tiles_stride * RADIX ...]; zero the tiles this n uses
