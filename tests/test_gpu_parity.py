@@ -11,7 +11,7 @@ from helpers import (GRAD_TOL, IMG_TOL, cam_args, compare_forward as _compare_fo
                      oracle_forward, rel_err as _rel, run_gpu as _run_gpu, scene_np, to_dev)
 from oracle import oracle, torch_ref
 from word2world_amd import raster
-from word2world_amd.camera import orbit_views
+from word2world_amd.camera import orbit_camera, orbit_views
 
 pytestmark = pytest.mark.gpu
 
@@ -129,6 +129,23 @@ def test_empty_and_culled(gpu):
         f = oracle_forward(a, 0, cam, 16)
         g = _run_gpu(a, 0, cam, gpu, 16)
         _compare_forward(f, g, 16)
+
+
+def test_depth_keys_of_mixed_top_bytes(gpu):
+    """A near plane at 1.0 puts the ndc depth of Gaussians nearer than it
+    below zero and the others in [0, 0.5): the depth keys' top bytes differ,
+    so the depth sort counts and runs its top pass.  Bit-exact against the
+    oracle, whose keys order the same float bits as unsigned integers."""
+    cam = orbit_camera(30.0, 10.0, 2.5, 128, 128, z_near=1.0)
+    arrs = scene_np(3000, 1, 21)
+    ndcz = []
+    for k in range(2):
+        c = cam if k == 0 else orbit_camera(200.0, -20.0, 2.2, 96, 128, z_near=1.0)
+        print(f"case mixed top bytes view={k}", flush=True)
+        f = oracle_forward(arrs, 1, c, 16)
+        g = _run_gpu(arrs, 1, c, gpu, 16)
+        _compare_forward(f, g, 16)
+        assert g["K"] > 0
 
 
 def test_dropin_signature(gpu):

@@ -367,14 +367,16 @@ void sort_passes(const KT* kin, const uint32_t* vin, KT* kA, uint32_t* vA, KT* k
                  const uint32_t* d_n, uint32_t n_cap, int npasses, int last_bits, bool write_last_keys,
                  uint32_t* hist, uint32_t* status, uint32_t* tickets, int kid_hist, int kid_pass,
                  hipStream_t s, bool hist_ready = false, uint32_t hist_grid = kHistGrid, KT* kC = nullptr,
-                 uint32_t* vC = nullptr, uint32_t copies = 1) {
+                 uint32_t* vC = nullptr, uint32_t copies = 1, const uint32_t* top_differs = nullptr,
+                 uint32_t top_digit = 0) {
   const uint32_t ntiles = cdiv(n_cap, SORT_TILE);
   if (ntiles == 0) return;
   const uint32_t hgrid = std::min<uint32_t>(cdiv(n_cap, 256), hist_grid);
   {
     ProfSpan ps(kid_hist, s);
     hipLaunchKernelGGL(k_sort_hist<KT>, dim3(hgrid), dim3(256), 0, s, hist_ready ? nullptr : kin, d_n,
-                       n_cap, 0, npasses, last_bits, hist, status, ntiles, tickets, copies);
+                       n_cap, 0, npasses, last_bits, hist, status, ntiles, tickets, copies, top_differs,
+                       top_digit);
   }
   const uint32_t pgrid = std::min<uint32_t>(ntiles, 1024);  // 2 workgroups of 512 per CU (LDS)
   const KT* ks = kin;
@@ -606,7 +608,8 @@ int run_forward(const Plan& p, const void* table, const W2WWorkspace* ws, const 
                           at<uint32_t>(G, p.dkey1), at<uint32_t>(G, p.id1), d_items, items_cap, 4,
                           8, false, at<uint32_t>(G, p.dhist), at<uint32_t>(G, p.dstatus),
                           status + ST_DTICKET, W2W_KID_DSORT_HIST, W2W_KID_DSORT_PASS, s, false,
-                          kShortHistGrid, at<uint32_t>(G, p.dkey2), at<uint32_t>(G, p.id2));
+                          kShortHistGrid, at<uint32_t>(G, p.dkey2), at<uint32_t>(G, p.id2), 1,
+                          status + ST_DTOP, CULLED_DEPTH_KEY >> 24);
     if (cap > 0) {
       // emission offsets: the tile counts gathered in depth order and
       // scanned.  Depth-sliced binning: the near slice is binned here, the

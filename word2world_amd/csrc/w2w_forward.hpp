@@ -67,6 +67,7 @@ enum StatusWord : int {
   ST_COST_CURSOR = 67, //   slots taken per cost bucket (kCostBuckets words)
   ST_REC_TICKET = 100, // the scan of k_preprocess's tile sums
   ST_KNEAR_RAW = 101,  // depth-sliced binning: the near lists' instances before the capacity cap
+  ST_DTOP = 102,       // nonzero when a depth key's top byte differs from CULLED_DEPTH_KEY's
   ST_WORDS = 128,
 };
 
@@ -357,6 +358,12 @@ __global__ __launch_bounds__(PRE_TILE) void k_preprocess(PreArgs a) {
     if (a.hint_sat) hcnt = cnt ? sat_rect(a.hint_sat, fd.tile_off, (uint32_t)fd.tiles_x, x0, x1, y0, y1) : 0u;
     dkey = keep ? __float_as_uint(g.ndcz) : CULLED_DEPTH_KEY;
     }
+    // the depth sort counts its top digit only when some key's top byte
+    // differs from the culled key's (an atomic per such wave until the flag
+    // reads as set; none in the usual case)
+    if (__ballot((dkey >> 24) != (CULLED_DEPTH_KEY >> 24)) && (tid & 63) == 0 &&
+        __hip_atomic_load(a.status + ST_DTOP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)
+      atomicOr(a.status + ST_DTOP, 1u);
     uint32_t btot;
     const uint32_t bexcl = block_excl_scan_256(cnt, s_tmp, &btot);
     if (tid == 0) a.tile_tot[tile] = min(btot, LB_VAL);

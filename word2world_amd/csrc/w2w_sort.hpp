@@ -41,7 +41,9 @@ W2W_DEV uint32_t sort_count(const uint32_t* d_n, uint32_t n_cap) {
 }
 
 // Histogram of every digit of every pass (hist[pass][256], pre-zeroed; with
-// null keys the producer's copies, at most kHistCopies, summed), and
+// null keys the producer's copies, at most kHistCopies, summed; the top pass
+// filled in without counting when the producer found every key's top digit
+// equal to top_digit), and
 // zeroing of the look-back words status[pass][tile][256] (pass stride
 // tiles_stride tiles) and the per-pass tickets.
 template <typename KT>
@@ -51,15 +53,24 @@ __global__ __launch_bounds__(256) void k_sort_hist(const KT* __restrict__ keys,
                                                    uint32_t* __restrict__ hist,
                                                    uint32_t* __restrict__ status,
                                                    uint32_t tiles_stride,
-                                                   uint32_t* __restrict__ tickets, uint32_t copies) {
+                                                   uint32_t* __restrict__ tickets, uint32_t copies,
+                                                   const uint32_t* __restrict__ top_differs,
+                                                   uint32_t top_digit) {
   __shared__ uint32_t sh[8][RADIX];
   const uint32_t n = sort_count(d_n, n_cap);
   for (int i = threadIdx.x; i < 8 * RADIX; i += blockDim.x) (&sh[0][0])[i] = 0;
   __syncthreads();
   const uint32_t stride = gridDim.x * blockDim.x;
   if (keys) {  // null: the histogram was accumulated by the producer of the keys
+    // top_differs (null: not known) == 0: every key's top digit is
+    // top_digit, so its bin holds n and the top digits are not counted
+    int cpasses = npasses;
+    if (top_differs && *top_differs == 0u) {
+      cpasses = npasses - 1;
+      if (blockIdx.x == 0 && threadIdx.x == 0) hist[(npasses - 1) * RADIX + top_digit] = n;
+    }
     auto count = [&](KT k) {
-      for (int p = 0; p < npasses; p++) {
+      for (int p = 0; p < cpasses; p++) {
         const int bits = (p == npasses - 1) ? last_bits : 8;
         const uint32_t d = (uint32_t)(k >> (begin_bit + 8 * p)) & ((1u << bits) - 1u);
         atomicAdd(&sh[p][d], 1u);
