@@ -23,6 +23,7 @@ constexpr int SORT_WAVES = SORT_THREADS / 64;
 constexpr int SORT_ITEMS = 12;
 constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;  // 6144
 constexpr int RADIX = 256;
+constexpr int kLookAhead = 8;  // look-back words loaded per step
 // A histogram accumulated by the producer of the keys may come in several
 // copies, hist[copy][4][256]: each producing workgroup flushes its counts
 // into copy blockIdx.x % copies, so a word takes 1/copies of the flush
@@ -253,8 +254,10 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(
       if (valid && rank == 0) s_wcnt[wave][d] = pre + cnt;
     }
     __syncthreads();
-    // digit totals of this tile and the per-wave exclusive offsets
+    // digit totals of this tile and the per-wave exclusive offsets; the
+    // totals are published for the tiles behind at once
     uint32_t total = 0;
+    uint32_t* st = status + (size_t)tile * RADIX;
     if (tid < RADIX) {
 #pragma unroll
       for (int w = 0; w < SORT_WAVES; w++) {
@@ -262,21 +265,32 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(
         s_wcnt[w][tid] = total;
         total += c;
       }
+      st_release_dev(st + tid, (tile == 0 ? LB_INC : LB_AGG) | total);
     }
     const uint32_t dstart = block_excl_scan<SORT_THREADS>(total, s_tmp);
+    if (tid < RADIX) s_dstart[tid] = dstart;
+    __syncthreads();
+    // the tile reordered by digit in LDS before the look-back, which then
+    // waits with the keys and values out of registers
+#pragma unroll
+    for (int j = 0; j < SORT_ITEMS; j++) {
+      uint32_t idx = wbase + j * 64 + lane;
+      if (idx < n) {
+        uint32_t d = (uint32_t)(keys[j] >> shift) & dmask;
+        uint32_t pos = s_dstart[d] + s_wcnt[wave][d] + woff[j];
+        if (pos < (uint32_t)SORT_TILE) {
+          s_k[pos] = keys[j];
+          s_v[pos] = vals[j];
+        }
+      }
+    }
     if (tid < RADIX) {
-      s_dstart[tid] = dstart;
-      // decoupled look-back for digit tid
-      uint32_t* st = status + (size_t)tile * RADIX;
+      // decoupled look-back for digit tid: the preceding tiles' words,
+      // kLookAhead at a time (independent loads), consumed in order up to
+      // the first inclusive one; a word not yet published is read again
+      // from there
       uint32_t excl = 0;
-      if (tile == 0) {
-        st_release_dev(st + tid, LB_INC | total);
-      } else {
-        st_release_dev(st + tid, LB_AGG | total);
-        // the preceding tiles' words, kLookAhead at a time (independent
-        // loads), consumed in order up to the first inclusive one; a word not
-        // yet published is read again from there
-        constexpr int kLookAhead = 8;
+      if (tile > 0) {
         int p = (int)tile - 1;
         for (;;) {
           uint32_t w[kLookAhead];
@@ -300,19 +314,6 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(
         st_release_dev(st + tid, LB_INC | (excl + total));
       }
       s_gbase[tid] = hexcl + excl - dstart;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < SORT_ITEMS; j++) {
-      uint32_t idx = wbase + j * 64 + lane;
-      if (idx < n) {
-        uint32_t d = (uint32_t)(keys[j] >> shift) & dmask;
-        uint32_t pos = s_dstart[d] + s_wcnt[wave][d] + woff[j];
-        if (pos < (uint32_t)SORT_TILE) {
-          s_k[pos] = keys[j];
-          s_v[pos] = vals[j];
-        }
-      }
     }
     __syncthreads();
     const uint32_t tile_n = (n - base) < (uint32_t)SORT_TILE ? (n - base) : (uint32_t)SORT_TILE;
