@@ -293,7 +293,7 @@ constexpr uint32_t kBlendGrid = 256 * 8;
 // the sort's global one at its end (device-scope atomics on a few hundred
 // words, executed memory-side), so the grids stay near one resident round.
 constexpr uint32_t kEmitGrid = 2048;
-constexpr uint32_t kEmitFarGrid = 1024;
+constexpr uint32_t kEmitFarGrid = 4096;
 
 // workgroups of four independent waves, one 8x8 pixel block per wave at a time
 template <int TS>
