@@ -196,6 +196,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(
   static_assert(SORT_THREADS >= RADIX, "one thread per digit");
 
   const uint32_t n = sort_count(d_n, n_cap);
+  if (n == 0) return;  // e.g. the far lists of a step with no open tile: no ticket, no histogram scan
   const uint32_t ntiles = (n + SORT_TILE - 1) / SORT_TILE;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t dmask = (1u << nbits) - 1u;
